@@ -1,0 +1,201 @@
+/*
+ * wds.h -- C ABI of the MI355X batched GGA library (libwds.so).
+ *
+ * Drop-in boundary for dgga's batched forward GGA and its implicit adjoint.
+ * The reference exposes this path through Python keyword APIs, not an FFI;
+ * each entry point below names the reference function it replaces.  The
+ * Python host layer (wdsgpu_amd) binds these with ctypes; see INTEGRATION.md
+ * for the binding a dgga maintainer would add.
+ *
+ * Conventions: every function returns 0 on success or a negative wds_status;
+ * wds_last_error() gives the message of the last failure on this thread.
+ * Device pointers are borrowed (the caller owns them, e.g. torch tensors);
+ * the plan is owned by the library and released with wds_plan_destroy.
+ * Arrays are row-major [B][n] with B scenarios.  Units are EPANET internal
+ * units (ft, cfs).  All arithmetic is IEEE float64.
+ */
+#ifndef WDS_H
+#define WDS_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct wds_plan wds_plan;
+
+enum wds_status {
+  WDS_OK = 0,
+  WDS_EINVAL = -1,      /* bad argument            -> ValueError */
+  WDS_EUNSUPPORTED = -2,/* unsupported feature      -> NotImplementedError */
+  WDS_ENOTSPD = -3,     /* non-positive pivot       -> RuntimeError naming samples */
+  WDS_EUNCONVERGED = -4,/* adjoint on unconverged   -> RuntimeError */
+  WDS_EDEVICE = -5,     /* HIP runtime failure      -> RuntimeError */
+  WDS_ESTALE = -6       /* factor buffer not produced by this plan */
+};
+
+/* Static network description (file order, EPANET internal units).  Mirrors
+ * the host precompute of GGASolver.__init__ (dgga/solver.py:160-495,
+ * 1272-1390): internal status codes (CLOSED=2, OPEN=3, ACTIVE=4), internal
+ * valve settings (MISSING = -1e10 for fixed valves), initial flows. */
+typedef struct wds_network_desc {
+  int32_t N, L;
+  const int32_t* node_type;    /* [N] 0 junction, 1 reservoir, 2 tank */
+  const double* elev;          /* [N] ft */
+  const int32_t* link_n1;      /* [L] */
+  const int32_t* link_n2;      /* [L] */
+  const int32_t* link_type;    /* [L] EN_LinkType: 0 CVPIPE 1 PIPE 2 PUMP 3 PRV 7 TCV */
+  const int32_t* init_status;  /* [L] internal status code */
+  const double* diam;          /* [L] ft */
+  const double* km;            /* [L] pipe minor loss 0.02517 K / D^4 */
+  const double* km_tcv;        /* [L] TCV loss coefficient */
+  const double* km_valve;      /* [L] valve minor loss */
+  const double* kc;            /* [L] Darcy-Weisbach roughness (ft) */
+  const double* q0;            /* [L] initial flow (cfs) */
+  const double* setting;       /* [L] internal setting; pumps: speed */
+  const int32_t* pump_kind;    /* [L] -1 not a pump, 0 CONST_HP, 1 POWER_FUNC, 3 NOCURVE */
+  const double* pump_h0;       /* [L] */
+  const double* pump_r;        /* [L] */
+  const double* pump_n;        /* [L] (N snapped to 1 when |N-1| < 1e-6) */
+  const double* pump_hmax;     /* [L] pumpstatus cut-off head (BIG for CONST_HP) */
+  int32_t ntank;
+  const int32_t* tank_node;    /* [ntank] */
+  const double* tank_hmin;     /* [ntank] ft */
+  const double* tank_hmax;     /* [ntank] ft */
+  const int32_t* tank_overflow;/* [ntank] */
+  int32_t headloss;            /* 0 Hazen-Williams, 1 Darcy-Weisbach */
+  double hexp, qexp, rqtol, viscos, htol, qtol;
+  int32_t checkfreq, maxcheck, extra_iter;
+} wds_network_desc;
+
+typedef struct wds_plan_stats {
+  int32_t N, L, Nj, nnzL, nlevels, npcv;
+  int64_t nnz_csr;             /* reference CSR pattern size (full symmetric) */
+  int64_t nnz_lower;           /* lower pattern of A incl. diagonal */
+  int64_t factor_updates;      /* multiply-adds of one numeric factorisation */
+  int32_t factor_steps, fwd_steps, bwd_steps;
+  int64_t factor_instr, fwd_instr, bwd_instr;
+  int64_t lds_bytes;           /* LDS per scenario of the fused kernels */
+} wds_plan_stats;
+
+/* Counters (GGASolver.cudss_counters, dgga/solver.py:2580-2593): batched
+ * numeric factorisations, triangular solve pairs, backward solves, backward
+ * reuse of the forward's terminal factor, backward refactorisations. */
+typedef struct wds_counters {
+  int64_t factorize, solve, bwd_solve, bwd_reuse, bwd_refactorize, launches;
+} wds_counters;
+
+/* Symbolic analysis, once per topology (replaces cuDSS plan(),
+ * dgga/solver.py:2474-2480, and models genmmd/etree/col_counts,
+ * dgga/smatrix.py:352-547, dgga/symbolic.py:101-230).  Host only: no GPU
+ * call.  ordering: 0 minimum degree, 1 natural. */
+int wds_plan_create(const wds_network_desc* net, int32_t ordering, wds_plan** out);
+void wds_plan_destroy(wds_plan* plan);
+int wds_plan_stats_get(const wds_plan* plan, wds_plan_stats* out);
+
+/* Copy a host-side plan array for inspection (tests): what = 0 perm,
+ * 1 colptr, 2 rowidx, 3 parent, 4 node_slot, 5 csr indptr, 6 csr indices,
+ * 7 csr_to_L, 8.. programs (8+3*p: instr as int32 x4, +1 step (off,cnt),
+ * +2 level_end; p = 0 factor, 1 fwd, 2 bwd).  Returns the element count
+ * (int32 units) or a negative status; copies min(count, cap). */
+int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* dst, int64_t cap);
+
+/* Upload the plan's static data to the current HIP device (idempotent). */
+int wds_plan_upload(wds_plan* plan);
+
+/* Device workspace the solver entry points need for B scenarios. */
+int64_t wds_workspace_bytes(const wds_plan* plan, int32_t B);
+
+/* Batched factor + solve with refinement on the reference's CSR values
+ * (replaces _cudss_forward / dense cholesky + cholesky_solve + refinement,
+ * dgga/solver.py:2630-2651, 3037-3064).  csr [B][nnz_csr] in the CSR pattern
+ * of GGASolver (dgga/solver.py:336-369), F [B][Nj] and H [B][Nj] in original
+ * junction order.  factor_out [B][nnzL] (optional) receives the factor.
+ * info [B] (int32) gets 0 or the first non-positive pivot (1-based). */
+int wds_factor_solve(wds_plan* plan, void* stream, int32_t B, const double* csr,
+                     const double* F, double* H, int32_t n_refine,
+                     double* factor_out, int32_t* info, void* workspace);
+
+/* Per-scenario inputs and outputs of the forward solve. */
+typedef struct wds_solve_args {
+  int32_t B;
+  const double* demand;        /* [B][N] cfs */
+  const double* res_head;      /* [B][N] ft at fixed-head nodes (junction entries ignored) */
+  const double* ke;            /* [B][N] or [N] emitter coefficients */
+  int32_t ke_stride;           /* N (per scenario) or 0 (shared) */
+  const double* r_hw;          /* [B][L] or [L] resistance */
+  int32_t r_stride;            /* L or 0 */
+  const double* q0;            /* [B][L] warm start or NULL (initial flow) */
+  const double* e0;            /* [B][N] warm start or NULL (1 at emitters) */
+  int32_t max_iter;
+  double accuracy;
+  int32_t status_machine;      /* 1: batched status machine (dgga/solver.py:3126-3179) */
+  int32_t n_refine;            /* refinement steps per linear solve (reference: 2) */
+  int32_t polish_steps;        /* >0: reduced Newton polish after convergence
+                                  (dgga/autodiff.py:1053-1104) */
+  /* outputs */
+  double* head;                /* [B][N] */
+  double* flow;                /* [B][L] */
+  double* emitter;             /* [B][N] */
+  int32_t* iters;              /* [B] */
+  double* relerr;              /* [B] */
+  int8_t* status;              /* [B][L] internal status codes */
+  double* diag_ratio;          /* [B] or NULL */
+  int32_t* info;               /* [B] 0 ok, >0 non-SPD at that iteration, <0 polish failure */
+  double* resid;               /* [B] polish residual inf-norm or NULL */
+  double* factor;              /* [B][nnzL] terminal factor for the adjoint, or NULL */
+  void* workspace;             /* wds_workspace_bytes(plan, B) */
+} wds_solve_args;
+
+/* Fused forward: the whole Newton loop of every scenario in one persistent
+ * launch, one wavefront per scenario with its factor resident in LDS
+ * (replaces GGASolver.solve mode='dense', dgga/solver.py:2703-3212, and
+ * ImplicitGGASolveGPU.forward's polish + terminal factor,
+ * dgga/autodiff.py:1171-1243). */
+int wds_solve(wds_plan* plan, void* stream, const wds_solve_args* args);
+
+/* One Newton iteration of every active scenario with the state kept in
+ * global memory (the unfused path: one launch per iteration, host loop).
+ * state: q [B][L], e [B][N] (by node), head [B][N], status [B][L],
+ * ctrl [B][4] int32 (active, iters, nextcheck, info), relerr [B].
+ * it is the 1-based iteration number. */
+int wds_newton_iter(wds_plan* plan, void* stream, const wds_solve_args* args,
+                    int32_t it, int32_t* ctrl);
+
+/* Backward of the implicit solve (replaces ImplicitGGASolveGPU.backward,
+ * dgga/autodiff.py:1245-1312): reuses the forward's terminal factor.  gH/gE
+ * [B][N], gQ [B][L]; outputs g_demand/g_res_head/g_ke [B][N], g_r [B][L]. */
+typedef struct wds_adjoint_args {
+  int32_t B;
+  const double* demand;        /* [B][N] */
+  const double* ke;            /* [B][N] or [N] */
+  int32_t ke_stride;
+  const double* r_hw;          /* [B][L] or [L] */
+  int32_t r_stride;
+  const double* head;          /* [B][N] converged heads */
+  const double* flow;          /* [B][L] */
+  const double* emitter;       /* [B][N] */
+  const int8_t* status;        /* [B][L] */
+  const double* factor;        /* [B][nnzL] from wds_solve */
+  const double* gH;            /* [B][N] */
+  const double* gQ;            /* [B][L] */
+  const double* gE;            /* [B][N] */
+  int32_t n_refine;
+  double* g_demand;            /* [B][N] */
+  double* g_res_head;          /* [B][N] */
+  double* g_ke;                /* [B][N] */
+  double* g_r;                 /* [B][L] */
+  int32_t* info;               /* [B] */
+  void* workspace;
+} wds_adjoint_args;
+
+int wds_adjoint(wds_plan* plan, void* stream, const wds_adjoint_args* args);
+
+int wds_counters_get(wds_plan* plan, wds_counters* out, int32_t reset);
+const char* wds_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WDS_H */

@@ -1,0 +1,209 @@
+# -*- coding: utf-8 -*-
+"""Oracle: the reference's CPU implicit adjoint (full Jacobian + SuperLU).
+
+TEST INFRASTRUCTURE ONLY (see oracle/__init__.py).
+
+Restates dgga/autodiff.py:221-603 and 609-732 (``ImplicitGGASolve``): the
+residual system F(z; theta) = 0 over z = (Q[L], qE[emitters], H[Nj]), its
+explicit sparse Jacobian, a full-Newton polish with ``scipy.sparse.linalg
+.splu`` and the adjoint solve J^T lambda = dL/dz followed by the theta
+contraction (demands, reservoir heads, emitter Ke, pipe resistance r).
+ACTIVE PRVs contribute the constraint row H[n2] - hset (prvcoeff fixed
+point).  Pumps and Darcy-Weisbach are not on this oracle (the GPU adjoint
+refuses them too).
+"""
+from __future__ import annotations
+
+import numpy as np
+import scipy.sparse as sp
+from scipy.sparse.linalg import splu
+
+from wdsgpu_amd.network import CSMALL, CBIG, MISSING, ST_CLOSED, ST_ACTIVE, PRV, TCV
+
+from . import gga
+
+
+def link_coeffs(m, q, r_hw, status):
+    """(hloss, hgrad, dhloss/dr, act_prv) per link (dgga/autodiff.py:239-362,
+    221-236), H-W / TCV / valves / closed links."""
+    if m.headloss != "H-W" or m.n_pumps:
+        raise NotImplementedError("oracle adjoint: H-W without pumps")
+    qa = np.abs(q)
+    sgn = np.where(q < 0, -1.0, 1.0)
+    hg = m.hexp * r_hw * qa ** (m.hexp - 1.0)
+    lin = hg < m.rqtol
+    hg = np.where(lin, m.rqtol, hg)
+    hl = np.where(lin, hg * qa, hg * qa / m.hexp)
+    dr = np.where(lin, 0.0, qa ** m.hexp)
+    mlp = m.km_pipe > 0.0
+    hl = np.where(mlp, hl + m.km_pipe * qa * qa, hl)
+    hg = np.where(mlp, hg + 2.0 * m.km_pipe * qa, hg)
+    hl = hl * sgn
+    dr = dr * sgn
+    km = m.km_tcv
+    hgv = 2.0 * km * qa
+    linv = hgv < m.rqtol
+    hgv = np.where(linv, m.rqtol, hgv)
+    hlv = np.where(linv, q * hgv, q * hgv / 2.0)
+    kmp = km > 0.0
+    hgv = np.where(kmp, hgv, CSMALL)
+    hlv = np.where(kmp, hlv, CSMALL * q)
+    hl = np.where(m.is_tcv, hlv, hl)
+    hg = np.where(m.is_tcv, hgv, hg)
+    dr = np.where(m.is_tcv, 0.0, dr)
+    vmask = m.is_valve & ~m.is_tcv
+    act_prv = (np.asarray(status) == ST_ACTIVE) & (m.init_setting != MISSING) & (m.lt == PRV)
+    if vmask.any():
+        kmv = m.km_valve_ml
+        hgv2 = 2.0 * kmv * qa
+        linv2 = hgv2 < m.rqtol
+        hgv2 = np.where(linv2, m.rqtol, hgv2)
+        hlv2 = np.where(linv2, q * hgv2, q * hgv2 / 2.0)
+        kmp2 = kmv > 0.0
+        hgv2 = np.where(kmp2, hgv2, CSMALL)
+        hlv2 = np.where(kmp2, hlv2, CSMALL * q)
+        hl = np.where(vmask, hlv2, hl)
+        hg = np.where(vmask, hgv2, hg)
+        dr = np.where(vmask, 0.0, dr)
+        hl = np.where(act_prv, 0.0, hl)
+        hg = np.where(act_prv, 0.0, hg)
+    cl = np.asarray(status) <= ST_CLOSED
+    hl = np.where(cl, CBIG * q, hl)
+    hg = np.where(cl, CBIG, hg)
+    dr = np.where(cl, 0.0, dr)
+    return hl, hg, dr, act_prv
+
+
+def emitter_coeffs(m, e, ke):
+    """dgga/autodiff.py:365-377."""
+    ke_adj = np.maximum(CSMALL, ke)
+    ea = np.abs(e)
+    hg = m.qexp * ke_adj * ea ** (m.qexp - 1.0)
+    lin = hg < m.rqtol
+    hg = np.where(lin, m.rqtol, hg)
+    hl = np.where(lin, hg * e, hg * e / m.qexp)
+    dke = np.where(lin | (ke < CSMALL), 0.0, ea ** (m.qexp - 1.0) * e)
+    return hl, hg, dke
+
+
+def residual(m, q, e_j, Hj, d, rh, ke_j, r_hw, status):
+    """F(z; theta) (dgga/autodiff.py:395-428)."""
+    hl, hg, dr, act = link_coeffs(m, q, r_hw, status)
+    Hf = np.empty(m.N)
+    Hf[m.junc_nodes] = Hj
+    Hf[m.fixed_nodes] = rh[m.fixed_nodes]
+    r_link = hl - (Hf[m.n1] - Hf[m.n2])
+    for k in np.where(act)[0]:
+        n2 = m.n2[k]
+        r_link[k] = Hf[n2] - (m.elev[n2] + m.init_setting[k])
+    em = ke_j > 0.0
+    hle, hge, dke = emitter_coeffs(m, e_j, ke_j)
+    r_em = (hle - (Hj - m.el_junc))[em]
+    mass = np.zeros(m.Nj)
+    np.add.at(mass, m.j2[m.m2], q[m.m2])
+    np.add.at(mass, m.j1[m.m1], -q[m.m1])
+    mass -= np.where(em, e_j, 0.0)
+    mass -= d[m.junc_nodes]
+    return np.concatenate([r_link, r_em, mass]), (hl, hg, dr, hle, hge, dke, em, act)
+
+
+def build_J(m, hg, hge, em, act):
+    """Sparse Jacobian over (Q, qE, H) (dgga/autodiff.py:431-470)."""
+    L, Nj = m.L, m.Nj
+    em_rows = np.where(em)[0]
+    ne = em_rows.size
+    off_e, off_h = L, L + ne
+    rows, cols, vals = [], [], []
+    ar = np.arange(L)
+    rows.append(ar); cols.append(ar); vals.append(hg)
+    k1 = ar[m.m1]
+    v1 = np.full(k1.size, -1.0)
+    v1[act[m.m1]] = 0.0
+    rows.append(k1); cols.append(off_h + m.j1[m.m1]); vals.append(v1)
+    k2 = ar[m.m2]
+    rows.append(k2); cols.append(off_h + m.j2[m.m2]); vals.append(np.full(k2.size, 1.0))
+    if ne:
+        ae = np.arange(ne)
+        rows.append(off_e + ae); cols.append(off_e + ae); vals.append(hge[em_rows])
+        rows.append(off_e + ae); cols.append(off_h + em_rows); vals.append(np.full(ne, -1.0))
+        rows.append(off_h + em_rows); cols.append(off_e + ae); vals.append(np.full(ne, -1.0))
+    rows.append(off_h + m.j1[m.m1]); cols.append(k1); vals.append(np.full(k1.size, -1.0))
+    rows.append(off_h + m.j2[m.m2]); cols.append(k2); vals.append(np.full(k2.size, 1.0))
+    M = L + ne + Nj
+    J = sp.coo_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))),
+                      shape=(M, M)).tocsc()
+    return J, em_rows
+
+
+def polish(m, d, rh, ke, r_hw, q, e_j, Hj, status, steps=3):
+    """Full Newton J dz = -F with SuperLU (dgga/autodiff.py:473-491)."""
+    ke_j = ke[m.junc_nodes]
+    q, e_j, Hj = q.copy(), e_j.copy(), Hj.copy()
+    for _ in range(int(steps)):
+        F, (hl, hg, dr, hle, hge, dke, em, act) = residual(m, q, e_j, Hj, d, rh, ke_j, r_hw, status)
+        J, em_rows = build_J(m, hg, hge, em, act)
+        dz = splu(J).solve(-F)
+        L, ne = m.L, em_rows.size
+        q += dz[:L]
+        e_j[em_rows] += dz[L:L + ne]
+        Hj += dz[L + ne:]
+    F, _ = residual(m, q, e_j, Hj, d, rh, ke_j, r_hw, status)
+    return q, e_j, Hj, float(np.abs(F).max())
+
+
+def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_steps=3):
+    """Dense GGA with the status machine, then per-scenario polish with the
+    converged status frozen (dgga/autodiff.py:539-603)."""
+    d = np.atleast_2d(d)
+    B = d.shape[0]
+    rh = np.broadcast_to(np.atleast_2d(rh), (B, m.N))
+    ke = np.broadcast_to(np.atleast_2d(ke), (B, m.N))
+    out = gga.solve(m, d, rh, ke=ke, r_hw=r_hw, accuracy=accuracy, max_iter=max_iter,
+                    status_machine=True)
+    q_all = out["flow_cfs"].copy()
+    e_all = out["emitter_cfs"][:, m.junc_nodes].copy()
+    H_all = out["head_ft"].copy()
+    S_all = out["status"].copy()
+    resid = np.zeros(B)
+    for b in range(B):
+        qb, eb, Hjb, rb = polish(m, d[b], rh[b], ke[b], r_hw, q_all[b], e_all[b],
+                                 H_all[b, m.junc_nodes], S_all[b], polish_steps)
+        q_all[b], e_all[b], H_all[b, m.junc_nodes], resid[b] = qb, eb, Hjb, rb
+    emitter = np.zeros((B, m.N))
+    emitter[:, m.junc_nodes] = np.where(ke[:, m.junc_nodes] > 0, e_all, 0.0)
+    return dict(q=q_all, e_j=e_all, head=H_all, emitter=emitter, status=S_all,
+                resid_inf=resid, iters=out["iters"], d=d, rh=rh, ke=ke, r_hw=r_hw)
+
+
+def cpu_adjoint(m, sol, gH, gQ, gE):
+    """ImplicitGGASolve.backward (dgga/autodiff.py:658-732): per scenario
+    splu(J).solve(v, trans='T') and the theta contraction.  Returns
+    (g_demand [B,N], g_res_head [B,N], g_ke [B,N], g_r [B,L])."""
+    B = sol["q"].shape[0]
+    gH, gQ, gE = np.atleast_2d(gH), np.atleast_2d(gQ), np.atleast_2d(gE)
+    gd = np.zeros((B, m.N))
+    grh = np.zeros((B, m.N))
+    gke = np.zeros((B, m.N))
+    gr = np.zeros((B, m.L))
+    L = m.L
+    f1 = ~m.m1
+    f2 = ~m.m2
+    for b in range(B):
+        q, e_j = sol["q"][b], sol["e_j"][b]
+        ke_j = sol["ke"][b, m.junc_nodes]
+        hl, hg, dr, act = link_coeffs(m, q, sol["r_hw"], sol["status"][b])
+        _, hge, dke = emitter_coeffs(m, e_j, ke_j)
+        em = ke_j > 0.0
+        J, em_rows = build_J(m, hg, hge, em, act)
+        ne = em_rows.size
+        v = np.concatenate([gQ[b], gE[b, m.junc_nodes][em_rows], gH[b, m.junc_nodes]])
+        lam = splu(J).solve(v, trans="T")
+        lam_l, lam_e, lam_m = lam[:L], lam[L:L + ne], lam[L + ne:]
+        gd[b, m.junc_nodes] = lam_m
+        gke[b, m.junc_nodes[em_rows]] = -lam_e * dke[em_rows]
+        f1e = f1 & ~act
+        np.add.at(grh[b], m.n1[f1e], lam_l[f1e])
+        np.add.at(grh[b], m.n2[f2], -lam_l[f2])
+        grh[b, m.fixed_nodes] += gH[b, m.fixed_nodes]
+        gr[b] = -lam_l * dr
+    return gd, grh, gke, gr

@@ -1,0 +1,43 @@
+# -*- coding: utf-8 -*-
+"""Tiny hand-built networks for closed-form and brute-force checks."""
+import numpy as np
+
+from wdsgpu_amd.network import (Net, JUNCTION, RESERVOIR, PIPE, PRV, TCV, CVPIPE,
+                                hw_resistance)
+
+
+def make_net(nodes, links, accuracy=1e-3, ke=None):
+    """nodes: list of (type, elev_ft, demand_cfs); links: list of dicts with
+    n1, n2 and type PIPE (len_ft, diam_ft, C, [km]) / PRV (diam_ft,
+    setting_ft, [status]) / TCV (diam_ft, setting) / CVPIPE."""
+    N = len(nodes)
+    nt = np.array([n[0] for n in nodes], dtype=np.int8)
+    elev = np.array([n[1] for n in nodes], dtype=np.float64)
+    dem = np.array([n[2] for n in nodes], dtype=np.float64)
+    L = len(links)
+    lt = np.array([lk.get("type", PIPE) for lk in links], dtype=np.int8)
+    ln = np.array([lk.get("len_ft", 0.0) for lk in links])
+    dm = np.array([lk.get("diam_ft", 1.0) for lk in links])
+    C = np.array([lk.get("C", 0.0) for lk in links])
+    km = np.array([lk.get("km", 0.0) for lk in links])
+    r = np.zeros(L)
+    pipe = lt <= PIPE
+    r[pipe] = hw_resistance(C[pipe], ln[pipe], dm[pipe])
+    st = np.array([lk.get("status", 2 if lk.get("type", PIPE) in (PRV, TCV) else 1)
+                   for lk in links], dtype=np.int8)
+    # PRV settings are in user pressure units; flow units CFS -> US units (psi):
+    # setting_ft is converted with Ucf[PRESSURE] = 0.4333 psi/ft.
+    setting = np.array([lk.get("setting_ft", 0.0) * 0.4333 if lk.get("type") == PRV
+                        else lk.get("setting", 0.0) for lk in links])
+    meta = dict(flow_units="CFS", headloss="H-W", accuracy=accuracy, trials=200)
+    return Net(node_id=["N%d" % i for i in range(N)], node_type=nt, elev_ft=elev,
+               node_ke=np.zeros(N) if ke is None else np.asarray(ke, dtype=np.float64),
+               demand_cfs=dem, link_id=["L%d" % k for k in range(L)], link_type=lt,
+               link_n1=np.array([lk["n1"] for lk in links], dtype=np.int32),
+               link_n2=np.array([lk["n2"] for lk in links], dtype=np.int32),
+               diam_ft=dm, len_ft=ln, roughness=C, km_int=km, r_hw=r,
+               init_status=st, valve_setting_user=setting, meta=meta)
+
+
+def res_heads(net):
+    return np.nan_to_num(net.reservoir_head_ft())

@@ -1,0 +1,58 @@
+# -*- coding: utf-8 -*-
+"""Build libwds.so (HIP kernels for gfx950 + host symbolic analysis) in-tree.
+
+    python -m wdsgpu_amd.build
+
+Every compile uses hipcc directly: the library is a plain C-ABI shared object
+loaded with ctypes, not a torch extension.
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libwds.so")
+SOURCES = ["kernels.hip", "plan.cpp", "symbolic.cpp"]
+HEADERS = ["devplan.h", "device_common.h", "kernels_impl.h", "kernels_polish.h", "plan.h",
+           "symbolic.h"]
+
+
+def hipcc():
+    for p in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc"):
+        if p and os.path.exists(p):
+            return p
+    return "hipcc"
+
+
+def build(verbose=False, arch="gfx950"):
+    """Compile every source and link libwds.so; raises on failure."""
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    common = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
+              "-I", os.path.join(HERE, "..", "include")]
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, src + ".o")
+        cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".hip"):
+            cmd[1:1] = ["-x", "hip", "--offload-arch=" + arch,
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        subprocess.run(cmd, check=True)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + arch, "-o", tmp] + objs
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose="-v" in sys.argv)
+    print(LIB)

@@ -1,0 +1,313 @@
+// Device building blocks shared by the GGA kernels (one wavefront = one
+// scenario; every function below is called by all 64 lanes of the wave).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "devplan.h"
+#include "symbolic.h"
+
+namespace wds {
+
+constexpr double CSMALL = 1.0e-6;
+constexpr double CBIG = 1.0e8;
+constexpr int8_t ST_XHEAD = 0, ST_TEMPCLOSED = 1, ST_CLOSED = 2, ST_OPEN = 3, ST_ACTIVE = 4,
+                 ST_XPRESSURE = 7;
+// Darcy-Weisbach friction-factor constants (EPANET hydcoeffs.c; quoted in
+// dgga/solver.py:40-45)
+constexpr double DW_A1 = 3.14159265358979323850e+03;
+constexpr double DW_A2 = 1.57079632679489661930e+03;
+constexpr double DW_A8 = 4.61841319859066668690e+00;
+constexpr double DW_A9 = -8.68588963806503655300e-01;
+constexpr double DW_AB = 3.28895476345399058690e-03;
+constexpr double DW_AC = -5.14214965799093883760e-03;
+constexpr double PI_EPANET = 3.141592654;
+
+__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+
+// Order LDS traffic between program levels of one wavefront: wait for the
+// wave's outstanding LDS operations only (lgkmcnt(0); vector-memory loads of
+// prefetched instructions stay in flight) and stop the compiler from moving
+// memory operations across.
+__device__ __forceinline__ void lds_sync() {
+  __builtin_amdgcn_s_waitcnt(0xc07f);
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Make this wave's global stores visible to its own later loads.
+__device__ __forceinline__ void mem_sync() {
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmax(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+  for (int m = 32; m >= 1; m >>= 1) v = fmin(v, __shfl_xor(v, m, 64));
+  return v;
+}
+__device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
+
+// ---------------------------------------------------------------------------
+// Level-scheduled program executor.  MODE 0: LDL^T factorisation in place in
+// V (unscaled columns w, reciprocal pivots on the diagonal); MODE 1: forward
+// solve z = D^-1 L^-1 x in place in X; MODE 2: backward solve x = L^-T z.
+// Instructions of chunk g+1 are fetched from global memory while chunk g runs.
+template <int MODE>
+__device__ __forceinline__ void load_chunk(const ProgDev& pg, int g, int lane, int4* buf) {
+#pragma unroll
+  for (int u = 0; u < kUnroll; ++u) {
+    const int s = g + u;
+    int4 in = make_int4(0, 0, 0, kNop);
+    if (s < pg.nsteps) {
+      const int2 st = pg.step[s];
+      if (lane < st.y) in = pg.instr[st.x + lane];
+    }
+    buf[u] = in;
+  }
+}
+
+template <int MODE>
+__device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
+  const int lane = lane_id();
+  const int ns = pg.nsteps;
+  if (ns == 0) return;
+  int lev = 0;
+  int lev_end = pg.level_end[0];
+  int4 cur[kUnroll];
+  load_chunk<MODE>(pg, 0, lane, cur);
+  double acc = 0.0;
+  for (int g = 0; g < ns; g += kUnroll) {
+    int4 nxt[kUnroll];
+    load_chunk<MODE>(pg, g + kUnroll, lane, nxt);
+    double va[kUnroll], vb[kUnroll], vc[kUnroll], vt[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int4 in = cur[u];
+      const int t = in.w & kTMask;
+      va[u] = V[in.x];
+      vc[u] = V[in.z];
+      if (MODE == 0) {
+        vb[u] = V[in.y];
+        vt[u] = V[t];
+      } else {
+        vb[u] = X[in.y];
+        vt[u] = X[t];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const int w = cur[u].w;
+      if (w & kNop) continue;
+      const int t = w & kTMask;
+      if (MODE == 0) {
+        if (w & kFirst) acc = vt[u];
+        acc = fma(-va[u], vb[u] * vc[u], acc);
+        if (w & kLast) {
+          if (w & kDiag) {
+            if (!(acc > 0.0)) bad = 1;
+            V[t] = 1.0 / acc;
+          } else {
+            V[t] = acc;
+          }
+        }
+      } else if (MODE == 1) {
+        if (w & kFirst) acc = vt[u];
+        acc = fma(-va[u], vb[u], acc);
+        if (w & kLast) X[t] = acc * vc[u];
+      } else {
+        if (w & kFirst) acc = 0.0;
+        acc = fma(va[u], vb[u], acc);
+        if (w & kLast) X[t] = vt[u] - vc[u] * acc;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) cur[u] = nxt[u];
+    if (g + kUnroll == lev_end) {
+      lds_sync();
+      ++lev;
+      if (lev < pg.nlevels) lev_end = pg.level_end[lev];
+    }
+  }
+}
+
+// y = A x over the symmetric rows (A values by L position, global memory)
+__device__ __forceinline__ double spmv_row(const DevPlan& pl, const double* A, const double* x,
+                                           int i) {
+  double s = 0.0;
+  const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int2 en = pl.spmv_ent[e];
+    s = fma(A[en.x], x[en.y], s);
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Link coefficients (headlosscoeffs, dgga/solver.py:2924-2956 and the
+// pipe/TCV/pump/valve helpers 2176-2326, 834-848; D-W 1404-1450).
+struct LinkPY {
+  double P, Y;
+};
+
+__device__ __forceinline__ void hw_pipe(const DevPlan& pl, double q, double r, double km,
+                                        double& hloss, double& hgrad) {
+  const double sgn = q < 0.0 ? -1.0 : 1.0;
+  const double qa = q * sgn;
+  double hg = pl.hexp * r * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+  const bool lin = hg < pl.rqtol;
+  if (lin) hg = pl.rqtol;
+  double hl = lin ? hg * qa : hg * qa / pl.hexp;
+  if (km > 0.0) {
+    hl = hl + km * qa * qa;
+    hg = hg + 2.0 * km * qa;
+  }
+  hloss = hl * sgn;
+  hgrad = hg;
+}
+
+__device__ __forceinline__ void dw_pipe(const DevPlan& pl, double q, double r, double km,
+                                        double diam, double kc, double& hloss, double& hgrad) {
+  const double qa = fabs(q);
+  const double s = pl.viscos * diam;
+  const double e = kc / diam;
+  if (qa <= DW_A2 * s) {
+    const double r16 = 16.0 * PI_EPANET * s;
+    hloss = q * (r16 * r + km * qa);
+    hgrad = r16 * r + 2.0 * km * qa;
+    return;
+  }
+  const double w = qa / s;
+  double f, dfdq;
+  if (w >= DW_A1) {
+    const double y1 = DW_A8 / pow(w, 0.9);
+    const double y2 = e / 3.7 + y1;
+    const double y3 = DW_A9 * log(y2);
+    f = 1.0 / (y3 * y3);
+    dfdq = 1.8 * f * y1 * DW_A9 / y2 / y3 / qa;
+  } else {
+    const double y2 = e / 3.7 + DW_AB;
+    const double y3 = DW_A9 * log(y2);
+    const double fa = 1.0 / (y3 * y3);
+    const double fb = (2.0 + DW_AC / (y2 * y3)) * fa;
+    const double r2 = w / DW_A2;
+    const double x1 = 7.0 * fa - fb;
+    const double x2 = 0.128 - 17.0 * fa + 2.5 * fb;
+    const double x3 = -0.128 + 13.0 * fa - (fb + fb);
+    const double x4 = 0.032 - 3.0 * fa + 0.5 * fb;
+    f = x1 + r2 * (x2 + r2 * (x3 + r2 * x4));
+    dfdq = (x2 + r2 * (2.0 * x3 + r2 * 3.0 * x4)) / s / DW_A2;
+  }
+  const double r1 = f * r + km;
+  hloss = r1 * qa * q;
+  hgrad = (2.0 * r1 * qa) + (dfdq * r * qa * qa);
+}
+
+__device__ __forceinline__ void valve_open(const DevPlan& pl, double q, double km, double& P,
+                                           double& Y) {
+  if (km > 0.0) {
+    const double qa = fabs(q);
+    double hg = 2.0 * km * qa;
+    const bool lin = hg < pl.rqtol;
+    if (lin) hg = pl.rqtol;
+    const double hl = lin ? q * hg : q * hg / 2.0;
+    P = 1.0 / hg;
+    Y = hl / hg;
+  } else {
+    P = 1.0 / CSMALL;
+    Y = q;
+  }
+}
+
+__device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double q, double& P,
+                                        double& Y) {
+  if (f & LF_PUMP_NOCURVE) {
+    P = 1.0 / CSMALL;
+    Y = q;
+    return;
+  }
+  const double sgn = q < 0.0 ? -1.0 : 1.0;
+  const double qa = q * sgn;
+  const double spc = fmax(pl.pu_speed[k], 1e-12);
+  const double n = pl.pu_n[k];
+  const double h0e = spc * spc * pl.pu_h0[k];
+  const double re = pl.pu_r[k] * pow(spc, 2.0 - n);
+  double hg, hl;
+  if (f & LF_PUMP_CHP) {
+    const double qa_c = fmax(qa, 1e-30);
+    const double raw = -re / qa_c / qa_c;
+    const bool big = raw > CBIG, sml = raw < pl.rqtol;
+    hg = big ? CBIG : (sml ? pl.rqtol : raw);
+    hl = (big || sml) ? -hg * q : re / (sgn * qa_c);
+  } else if (f & LF_PUMP_LIN) {
+    hg = re;
+    hl = h0e + re * q;
+  } else {
+    const double nm1 = n - 1.0;
+    double pw;
+    if (qa == 0.0)
+      pw = nm1 > 0.0 ? 0.0 : (nm1 < 0.0 ? __builtin_huge_val() : 1.0);
+    else
+      pw = pow(qa, nm1);
+    const double raw = n * re * pw;
+    const bool sml = raw < pl.rqtol;
+    hg = sml ? pl.rqtol : raw;
+    hl = sml ? h0e + hg * q : h0e + hg * q / fmax(n, 1e-30);
+  }
+  P = 1.0 / hg;
+  Y = hl / hg;
+}
+
+// P, Y of link k at flow q and status st (pcv PRVs -> 0, handled later)
+__device__ __forceinline__ LinkPY link_PY(const DevPlan& pl, int k, double q, double r,
+                                          int8_t st) {
+  const int f = pl.l_flags[k];
+  double P, Y;
+  if (f & LF_PIPE) {
+    double hl, hg;
+    if (pl.headloss == 0)
+      hw_pipe(pl, q, r, pl.l_km[k], hl, hg);
+    else
+      dw_pipe(pl, q, r, pl.l_km[k], pl.l_diam[k], pl.l_kc[k], hl, hg);
+    P = 1.0 / hg;
+    Y = hl / hg;
+  } else if (f & LF_TCV) {
+    valve_open(pl, q, pl.l_km_tcv[k], P, Y);
+  } else if (f & LF_PUMP) {
+    pump_PY(pl, k, f, q, P, Y);
+  } else if (f & LF_PRV_FIXED) {
+    valve_open(pl, q, pl.l_km_valve[k], P, Y);
+  } else {
+    P = 0.0;
+    Y = 0.0;
+  }
+  if (f & LF_PCV) {
+    P = 0.0;
+    Y = 0.0;
+  } else if (st <= ST_CLOSED || (f & LF_ZERO_SPEED)) {
+    P = 1.0 / CBIG;
+    Y = q;
+  }
+  return LinkPY{P, Y};
+}
+
+// emitterheadloss (dgga/solver.py:2368-2377)
+__device__ __forceinline__ void emitter_hl(const DevPlan& pl, double e, double ke, double& hloss,
+                                           double& hgrad) {
+  const double ke_adj = fmax(ke, CSMALL);
+  double hg = pl.qexp * ke_adj * pow(fmax(fabs(e), 1e-30), pl.qexp - 1.0);
+  const bool lin = hg < pl.rqtol;
+  if (lin) hg = pl.rqtol;
+  hloss = lin ? hg * e : hg * e / pl.qexp;
+  hgrad = hg;
+}
+
+}  // namespace wds

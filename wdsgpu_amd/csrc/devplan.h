@@ -1,0 +1,105 @@
+// Device-side view of a plan: plain pointers into one uploaded blob plus the
+// scalars every kernel needs.  Shared by the host builder and the kernels.
+#pragma once
+#include <cstdint>
+
+namespace wds {
+
+constexpr int kUnroll = 4;     // program steps processed per chunk
+constexpr int kMaxPrv = 8;     // pressure-reducing valves with a setting
+
+// per-link integer attributes (bit field in link_flags)
+enum LinkFlag : int32_t {
+  LF_PIPE = 1 << 0,        // CVPIPE or PIPE
+  LF_CV = 1 << 1,
+  LF_TCV = 1 << 2,
+  LF_PUMP = 1 << 3,
+  LF_PRV_FIXED = 1 << 4,   // PRV whose setting is MISSING (fixed open/closed)
+  LF_PCV = 1 << 5,         // PRV with a setting: coefficients from prvcoeff
+  LF_PUMP_CHP = 1 << 6,    // constant-horsepower pump
+  LF_PUMP_LIN = 1 << 7,    // power-function pump with N == 1
+  LF_PUMP_PF = 1 << 8,     // power-function pump
+  LF_PUMP_NOCURVE = 1 << 9,
+  LF_ZERO_SPEED = 1 << 10, // pump with speed 0 (always closed)
+  LF_PUMP_LIVE = 1 << 11,  // pump with speed > 0 (pumpstatus applies)
+  LF_J1 = 1 << 12,         // n1 is a junction
+  LF_J2 = 1 << 13,         // n2 is a junction
+};
+
+struct ProgDev {
+  const int4* instr;        // compacted step-major instructions
+  const int2* step;         // per step: (offset into instr, active lanes)
+  const int32_t* level_end; // cumulative step count at the end of each level
+  int32_t nsteps;
+  int32_t nlevels;
+};
+
+struct DevPlan {
+  // sizes
+  int32_t N, L, Nj, Nf, nnzL, p, ntl;
+  int32_t headloss;         // 0 Hazen-Williams, 1 Darcy-Weisbach
+  double hexp, qexp, rqtol, viscos, htol, qtol;
+  int32_t checkfreq, maxcheck, extra_iter;
+  int32_t has_cv, has_pump;
+  // nodes
+  const int32_t* node_slot;   // [N] original node -> slot (junction slots
+                              //     0..Nj-1 in elimination order, fixed Nj..)
+  const int32_t* slot_node;   // [Nj+Nf]
+  const double* el_slot;      // [Nj] junction elevation by slot
+  // links
+  const int32_t* l_s1;        // [L] slot of n1
+  const int32_t* l_s2;        // [L] slot of n2
+  const int32_t* l_flags;     // [L]
+  const int8_t* l_st0;        // [L] initial internal status
+  const double* l_diam;       // [L]
+  const double* l_km;         // [L] pipe minor loss 0.02517 K / D^4
+  const double* l_km_tcv;     // [L] TCV loss coefficient
+  const double* l_km_valve;   // [L] valve minor loss (fixed PRVs)
+  const double* l_kc;         // [L] D-W roughness
+  const double* l_q0;         // [L] initial flow
+  const double* pu_h0;        // [L] pump shutoff-head coefficient
+  const double* pu_r;         // [L]
+  const double* pu_n;         // [L]
+  const double* pu_speed;     // [L]
+  const double* pu_hmax;      // [L] pumpstatus cut-off head
+  // junction gather lists, reference assembly order: links with n1 == j
+  // (link order) then links with n2 == j; entry = link | (side << 31)
+  const int32_t* jadj_ptr;    // [Nj+1]
+  const int32_t* jadj;        // [..]
+  // fixed-head node incidence (slot Nj+f): links with n1 == f then n2 == f
+  const int32_t* fadj_ptr;    // [Nf+1]
+  const int32_t* fadj;        // [..] link | (side << 31)
+  // off-diagonal A entries: L position and contributing links (segment order)
+  int32_t noff;
+  const int32_t* off_pos;     // [noff]
+  const int32_t* off_ptr;     // [noff+1]
+  const int32_t* off_link;    // [..]
+  int32_t nfill;
+  const int32_t* fill_pos;    // [nfill] positions of L not in the pattern of A
+  // symmetric SpMV rows over A (slot order): (L position, column slot)
+  const int32_t* spmv_ptr;    // [Nj+1]
+  const int2* spmv_ent;       // [..]
+  // PRVs with a setting (file order)
+  const int32_t* prv_k;       // [p]
+  const int32_t* prv_s1;      // [p] slot of n1
+  const int32_t* prv_s2;      // [p]
+  const int32_t* prv_pos_off; // [p] L position of (j1,j2) lower
+  const double* prv_hset;     // [p]
+  const double* prv_km;       // [p]
+  // tank links (tankstatus)
+  const int32_t* tl_k;        // [ntl]
+  const int32_t* tl_tank;     // [ntl] slot of the tank
+  const int32_t* tl_other;    // [ntl] slot of the other end
+  const int32_t* tl_bits;     // [ntl] bit0 use_n2 (qsign -1), bit1 ovf, bit2 is_pump, bit3 pump_in, bit4 pump_out
+  const double* tl_hmax;      // [ntl]
+  const double* tl_hmin;      // [ntl]
+  // reference CSR (full symmetric pattern, original junction order) -> L
+  int32_t nnz_csr;
+  const int32_t* csr_to_L;    // [nnz_csr] L position for lower-in-elimination-order entries, else -1
+  const int32_t* diag_pos;    // [Nj] L position of slot j's diagonal (colptr)
+  const int32_t* row_slot;    // [Nj] original junction row -> slot
+  // programs
+  ProgDev factor, fwd, bwd;
+};
+
+}  // namespace wds

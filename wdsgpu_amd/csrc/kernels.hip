@@ -1,0 +1,324 @@
+// MI355X (gfx950) kernels of the batched GGA forward solve.
+//
+// Mapping: one 64-lane wavefront (one workgroup) per scenario.  The scenario's
+// numeric factor, its heads and one work vector live in LDS for the whole
+// Newton loop; per-link coefficient arrays and the assembled matrix values
+// sit in a per-scenario global workspace that stays L2-resident.  Every
+// Newton iteration (dgga/solver.py:2922-3179) runs inside the one launch:
+//   K1 link coefficients  -> K2 gather assembly (reference order)
+//   -> K3 LDL^T factor    -> K4 solve + refinement
+//   -> K5 flow update + relerr (wave reduction) -> K6 status machine.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "plan.h"
+#include "device_common.h"
+#include "kernels_impl.h"
+
+using namespace wds;
+
+// ---------------------------------------------------------------------------
+// plan upload
+namespace {
+
+struct BlobBuilder {
+  std::vector<char> bytes;
+  template <class T>
+  size_t add(const std::vector<T>& v) {
+    size_t off = (bytes.size() + 255) & ~size_t(255);
+    bytes.resize(off + std::max<size_t>(v.size() * sizeof(T), 8));
+    if (!v.empty()) std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(T));
+    return off;
+  }
+};
+
+int hip_fail(hipError_t e, const char* what) {
+  set_error(std::string(what) + ": " + hipGetErrorString(e));
+  return WDS_EDEVICE;
+}
+
+template <class T>
+const T* at(void* base, size_t off) {
+  return reinterpret_cast<const T*>(static_cast<char*>(base) + off);
+}
+
+}  // namespace
+
+extern "C" int wds_plan_upload(wds_plan* P) {
+  if (!P) return WDS_EINVAL;
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  if (P->d_blob && P->device == dev) return WDS_OK;
+  if (P->d_blob) {
+    set_error("plan already uploaded to another device");
+    return WDS_EINVAL;
+  }
+  const Symbolic& S = P->sym;
+  BlobBuilder bb;
+  auto prog_vectors = [&](const Program& pr, size_t& oi, size_t& os, size_t& ol) {
+    std::vector<int32_t> st(pr.step_off.size() * 2);
+    for (size_t i = 0; i < pr.step_off.size(); ++i) {
+      st[2 * i] = pr.step_off[i];
+      st[2 * i + 1] = pr.step_cnt[i];
+    }
+    oi = bb.add(pr.instr);
+    os = bb.add(st);
+    ol = bb.add(pr.level_end);
+  };
+  size_t o_ns = bb.add(P->node_slot), o_sn = bb.add(P->slot_node), o_el = bb.add(P->el_slot);
+  size_t o_s1 = bb.add(P->l_s1), o_s2 = bb.add(P->l_s2), o_fl = bb.add(P->l_flags),
+         o_st0 = bb.add(P->l_st0);
+  size_t o_di = bb.add(P->l_diam), o_km = bb.add(P->l_km), o_kt = bb.add(P->l_km_tcv),
+         o_kv = bb.add(P->l_km_valve), o_kc = bb.add(P->l_kc), o_q0 = bb.add(P->l_q0);
+  size_t o_ph = bb.add(P->pu_h0), o_pr = bb.add(P->pu_r), o_pn = bb.add(P->pu_n),
+         o_ps = bb.add(P->pu_speed), o_pm = bb.add(P->pu_hmax);
+  size_t o_jp = bb.add(P->jadj_ptr), o_ja = bb.add(P->jadj);
+  size_t o_fap = bb.add(P->fadj_ptr), o_fa = bb.add(P->fadj);
+  size_t o_op = bb.add(P->off_pos), o_optr = bb.add(P->off_ptr), o_ol = bb.add(P->off_link);
+  size_t o_fp = bb.add(P->fill_pos), o_sp = bb.add(P->spmv_ptr), o_se = bb.add(P->spmv_ent);
+  size_t o_pk = bb.add(P->prv_k), o_p1 = bb.add(P->prv_s1), o_p2 = bb.add(P->prv_s2),
+         o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
+  size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
+         o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
+  size_t o_c2l = bb.add(P->csr_to_L), o_dp = bb.add(P->diag_pos), o_rs = bb.add(P->row_slot);
+  size_t fi, fs, fl, wi, ws, wl, bi, bs, bl;
+  prog_vectors(S.factor, fi, fs, fl);
+  prog_vectors(S.fwd, wi, ws, wl);
+  prog_vectors(S.bwd, bi, bs, bl);
+  void* d = nullptr;
+  e = hipMalloc(&d, bb.bytes.size());
+  if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan)");
+  e = hipMemcpy(d, bb.bytes.data(), bb.bytes.size(), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(d);
+    return hip_fail(e, "hipMemcpy(plan)");
+  }
+  DevPlan& D = P->dev;
+  D.N = P->N;
+  D.L = P->L;
+  D.Nj = P->Nj;
+  D.Nf = P->Nf;
+  D.nnzL = (int32_t)S.nnzL;
+  D.p = P->p;
+  D.ntl = (int32_t)P->tl_k.size();
+  D.headloss = P->headloss;
+  D.hexp = P->hexp;
+  D.qexp = P->qexp;
+  D.rqtol = P->rqtol;
+  D.viscos = P->viscos;
+  D.htol = P->htol;
+  D.qtol = P->qtol;
+  D.checkfreq = P->checkfreq;
+  D.maxcheck = P->maxcheck;
+  D.extra_iter = P->extra_iter;
+  D.has_cv = P->has_cv;
+  D.has_pump = P->has_pump;
+  D.node_slot = at<int32_t>(d, o_ns);
+  D.slot_node = at<int32_t>(d, o_sn);
+  D.el_slot = at<double>(d, o_el);
+  D.l_s1 = at<int32_t>(d, o_s1);
+  D.l_s2 = at<int32_t>(d, o_s2);
+  D.l_flags = at<int32_t>(d, o_fl);
+  D.l_st0 = at<int8_t>(d, o_st0);
+  D.l_diam = at<double>(d, o_di);
+  D.l_km = at<double>(d, o_km);
+  D.l_km_tcv = at<double>(d, o_kt);
+  D.l_km_valve = at<double>(d, o_kv);
+  D.l_kc = at<double>(d, o_kc);
+  D.l_q0 = at<double>(d, o_q0);
+  D.pu_h0 = at<double>(d, o_ph);
+  D.pu_r = at<double>(d, o_pr);
+  D.pu_n = at<double>(d, o_pn);
+  D.pu_speed = at<double>(d, o_ps);
+  D.pu_hmax = at<double>(d, o_pm);
+  D.jadj_ptr = at<int32_t>(d, o_jp);
+  D.jadj = at<int32_t>(d, o_ja);
+  D.fadj_ptr = at<int32_t>(d, o_fap);
+  D.fadj = at<int32_t>(d, o_fa);
+  D.noff = (int32_t)P->off_pos.size();
+  D.off_pos = at<int32_t>(d, o_op);
+  D.off_ptr = at<int32_t>(d, o_optr);
+  D.off_link = at<int32_t>(d, o_ol);
+  D.nfill = (int32_t)P->fill_pos.size();
+  D.fill_pos = at<int32_t>(d, o_fp);
+  D.spmv_ptr = at<int32_t>(d, o_sp);
+  D.spmv_ent = at<int2>(d, o_se);
+  D.prv_k = at<int32_t>(d, o_pk);
+  D.prv_s1 = at<int32_t>(d, o_p1);
+  D.prv_s2 = at<int32_t>(d, o_p2);
+  D.prv_pos_off = at<int32_t>(d, o_po);
+  D.prv_hset = at<double>(d, o_hs);
+  D.prv_km = at<double>(d, o_pkm);
+  D.tl_k = at<int32_t>(d, o_tk);
+  D.tl_tank = at<int32_t>(d, o_tt);
+  D.tl_other = at<int32_t>(d, o_to);
+  D.tl_bits = at<int32_t>(d, o_tb);
+  D.tl_hmax = at<double>(d, o_tx);
+  D.tl_hmin = at<double>(d, o_tn);
+  D.nnz_csr = (int32_t)P->csr_to_L.size();
+  D.csr_to_L = at<int32_t>(d, o_c2l);
+  D.diag_pos = at<int32_t>(d, o_dp);
+  D.row_slot = at<int32_t>(d, o_rs);
+  auto set_prog = [&](ProgDev& pg, const Program& pr, size_t oi, size_t os, size_t ol) {
+    pg.instr = at<int4>(d, oi);
+    pg.step = at<int2>(d, os);
+    pg.level_end = at<int32_t>(d, ol);
+    pg.nsteps = pr.nsteps();
+    pg.nlevels = (int32_t)pr.level_end.size();
+  };
+  set_prog(D.factor, S.factor, fi, fs, fl);
+  set_prog(D.fwd, S.fwd, wi, ws, wl);
+  set_prog(D.bwd, S.bwd, bi, bs, bl);
+  P->d_blob = d;
+  P->device = dev;
+  return WDS_OK;
+}
+
+extern "C" void wds_plan_destroy(wds_plan* P) {
+  if (!P) return;
+  if (P->d_blob) (void)hipFree(P->d_blob);
+  delete P;
+}
+
+extern "C" int64_t wds_workspace_bytes(const wds_plan* P, int32_t B) {
+  if (!P || B <= 0) return WDS_EINVAL;
+  return (int64_t)B * ws_doubles(P->L, P->Nj, (int32_t)P->sym.nnzL) * 8;
+}
+
+extern "C" int wds_counters_get(wds_plan* P, wds_counters* out, int32_t reset) {
+  if (!P || !out) return WDS_EINVAL;
+  *out = P->cnt;
+  if (reset) P->cnt = wds_counters{};
+  return WDS_OK;
+}
+
+static int check_ready(wds_plan* P) {
+  if (!P) {
+    set_error("null plan");
+    return WDS_EINVAL;
+  }
+  if (!P->d_blob) {
+    int r = wds_plan_upload(P);
+    if (r) return r;
+  }
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != P->device) {
+    set_error("plan was uploaded to another device");
+    return WDS_EINVAL;
+  }
+  if (P->stats.lds_bytes > kMaxLds) {
+    set_error("network needs " + std::to_string(P->stats.lds_bytes) +
+              " bytes of LDS per scenario (> " + std::to_string(kMaxLds) + ")");
+    return WDS_EUNSUPPORTED;
+  }
+  return WDS_OK;
+}
+
+static int launch_check(const char* what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return hip_fail(e, what);
+  return WDS_OK;
+}
+
+template <class K>
+static void set_lds(K kernel, int64_t bytes) {
+  if (bytes > 65536)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+}
+
+extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const double* csr,
+                                const double* F, double* H, int32_t n_refine, double* factor_out,
+                                int32_t* info, void* workspace) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (B <= 0 || !csr || !F || !H || !info || !workspace || n_refine < 0) {
+    set_error("wds_factor_solve: bad argument");
+    return WDS_EINVAL;
+  }
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_factor_solve, lds);
+  hipLaunchKernelGGL(k_factor_solve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr, F,
+                     H, n_refine, factor_out, info, (double*)workspace);
+  P->cnt.factorize += 1;
+  P->cnt.solve += 1 + n_refine;
+  P->cnt.launches += 1;
+  return launch_check("k_factor_solve");
+}
+
+extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (!a || a->B <= 0 || !a->demand || !a->res_head || !a->ke || !a->r_hw || !a->head ||
+      !a->flow || !a->emitter || !a->iters || !a->relerr || !a->status || !a->info ||
+      !a->workspace || a->max_iter < 0 || a->n_refine < 0) {
+    set_error("wds_solve: bad argument");
+    return WDS_EINVAL;
+  }
+  if (P->p > 0 && !a->status_machine) {
+    set_error("networks with PRVs need status_machine=1 (dgga/solver.py:2786-2797)");
+    return WDS_EUNSUPPORTED;
+  }
+  if (P->has_cv && !a->status_machine) {
+    set_error("networks with CV pipes need status_machine=1");
+    return WDS_EUNSUPPORTED;
+  }
+  SolveParams sp = make_params(a);
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_solve, lds);
+  hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
+  P->cnt.launches += 1;
+  return launch_check("k_solve");
+}
+
+extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* a, int32_t it,
+                               int32_t* ctrl) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (!a || a->B <= 0 || !ctrl || it < 1 || !a->workspace) {
+    set_error("wds_newton_iter: bad argument");
+    return WDS_EINVAL;
+  }
+  if ((P->p > 0 || P->has_cv) && !a->status_machine) {
+    set_error("networks with PRVs or CV pipes need status_machine=1");
+    return WDS_EUNSUPPORTED;
+  }
+  SolveParams sp = make_params(a);
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_newton_iter, lds);
+  hipLaunchKernelGGL(k_newton_iter, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp,
+                     it, ctrl);
+  P->cnt.launches += 1;
+  return launch_check("k_newton_iter");
+}
+
+extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (!a || a->B <= 0 || !a->factor || !a->gH || !a->gQ || !a->gE || !a->g_demand ||
+      !a->g_res_head || !a->g_ke || !a->g_r || !a->info || !a->workspace || !a->head ||
+      !a->flow || !a->emitter || !a->status || !a->ke || !a->r_hw || a->n_refine < 0) {
+    set_error("wds_adjoint: bad argument");
+    return WDS_EINVAL;
+  }
+  if (P->has_pump || P->headloss != 0) {
+    set_error("the batched adjoint covers Hazen-Williams pipes, TCVs and PRVs "
+              "(pump and Darcy-Weisbach gradients are not on this path)");
+    return WDS_EUNSUPPORTED;
+  }
+  AdjParams ap = make_adj_params(a);
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_adjoint, lds);
+  hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, ap);
+  P->cnt.bwd_solve += 1;
+  P->cnt.bwd_reuse += 1;
+  P->cnt.launches += 1;
+  return launch_check("k_adjoint");
+}

@@ -1,0 +1,723 @@
+// Kernel bodies of the batched GGA forward solve (see kernels.hip).
+#pragma once
+#include "device_common.h"
+
+namespace wds {
+
+constexpr int64_t kMaxLds = 160 * 1024;
+
+// per-scenario global workspace, in doubles
+__host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t nnzL) {
+  int64_t n = 2 * (int64_t)L + nnzL + 12 * (int64_t)Nj + 2 * (int64_t)kMaxPrv * Nj +
+              ((int64_t)L + 7) / 8 + 512;
+  return (n + 31) / 32 * 32;
+}
+
+struct SolveParams {
+  const double* d;
+  const double* rh;
+  const double* ke;
+  int32_t ke_stride;
+  const double* r;
+  int32_t r_stride;
+  const double* q0;
+  const double* e0;
+  int32_t max_iter;
+  double hacc;
+  int32_t sm;
+  int32_t n_refine;
+  int32_t polish_steps;
+  double* head;
+  double* flow;
+  double* emitter;
+  int32_t* iters;
+  double* relerr;
+  int8_t* status;
+  double* diag_ratio;
+  int32_t* info;
+  double* resid;
+  double* factor;
+  double* ws;
+};
+
+inline SolveParams make_params(const wds_solve_args* a) {
+  SolveParams s;
+  s.d = a->demand;
+  s.rh = a->res_head;
+  s.ke = a->ke;
+  s.ke_stride = a->ke_stride;
+  s.r = a->r_hw;
+  s.r_stride = a->r_stride;
+  s.q0 = a->q0;
+  s.e0 = a->e0;
+  s.max_iter = a->max_iter;
+  s.hacc = a->accuracy;
+  s.sm = a->status_machine;
+  s.n_refine = a->n_refine;
+  s.polish_steps = a->polish_steps;
+  s.head = a->head;
+  s.flow = a->flow;
+  s.emitter = a->emitter;
+  s.iters = a->iters;
+  s.relerr = a->relerr;
+  s.status = a->status;
+  s.diag_ratio = a->diag_ratio;
+  s.info = a->info;
+  s.resid = a->resid;
+  s.factor = a->factor;
+  s.ws = (double*)a->workspace;
+  return s;
+}
+
+struct AdjParams {
+  const double* d;
+  const double* ke;
+  int32_t ke_stride;
+  const double* r;
+  int32_t r_stride;
+  const double* head;
+  const double* flow;
+  const double* emitter;
+  const int8_t* status;
+  const double* factor;
+  const double* gH;
+  const double* gQ;
+  const double* gE;
+  int32_t n_refine;
+  double* gd;
+  double* grh;
+  double* gke;
+  double* gr;
+  int32_t* info;
+  double* ws;
+};
+
+inline AdjParams make_adj_params(const wds_adjoint_args* a) {
+  AdjParams s;
+  s.d = a->demand;
+  s.ke = a->ke;
+  s.ke_stride = a->ke_stride;
+  s.r = a->r_hw;
+  s.r_stride = a->r_stride;
+  s.head = a->head;
+  s.flow = a->flow;
+  s.emitter = a->emitter;
+  s.status = a->status;
+  s.factor = a->factor;
+  s.gH = a->gH;
+  s.gQ = a->gQ;
+  s.gE = a->gE;
+  s.n_refine = a->n_refine;
+  s.gd = a->g_demand;
+  s.grh = a->g_res_head;
+  s.gke = a->g_ke;
+  s.gr = a->g_r;
+  s.info = a->info;
+  s.ws = (double*)a->workspace;
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Per-scenario context: LDS arrays and global workspace slices.
+struct Scn {
+  // LDS
+  double* V;    // [nnzL + 1] factor (V[nnzL] == 0)
+  double* Hn;   // [Nj + Nf] heads by slot
+  double* R;    // [Nj] work vector
+  double* X2;   // [Nj] second vector (solves of the polish / adjoint)
+  double* sh;   // [2*kMaxPrv] small shared scratch
+  // global workspace
+  double* P;    // [L]
+  double* Y;    // [L]
+  double* A;    // [nnzL] assembled values by L position
+  double* F;    // [Nj]
+  double* e;    // [Nj] emitter flow by slot
+  double* hle;  // [Nj]
+  double* hge;  // [Nj]
+  double* kej;  // [Nj] emitter coefficient by slot
+  double* dj;   // [Nj] demand by slot
+  double* v1;   // [Nj] extra vectors (polish / adjoint)
+  double* v2;   // [Nj]
+  double* v3;   // [Nj]
+  double* v4;   // [Nj]
+  double* v5;   // [Nj]
+  double* Sm;   // [2*kMaxPrv][Nj]
+  int8_t* st0;  // [L]
+  double* misc; // [512] small dense systems (Woodbury)
+  // per-scenario inputs / state
+  double* q;    // [L] flows (output array)
+  int8_t* S;    // [L] statuses (output array)
+  const double* r;  // [L] resistance
+};
+
+__device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, int b,
+                               double* flow_all, int8_t* status_all, const double* r_all,
+                               int r_stride) {
+  Scn c;
+  const int Nj = pl.Nj, L = pl.L;
+  c.V = lds;
+  c.Hn = lds + pl.nnzL + 1;
+  c.R = c.Hn + Nj + pl.Nf;
+  c.X2 = c.R + Nj;
+  c.sh = c.X2 + Nj;
+  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.nnzL);
+  c.P = w;
+  w += L;
+  c.Y = w;
+  w += L;
+  c.A = w;
+  w += pl.nnzL;
+  c.F = w;
+  w += Nj;
+  c.e = w;
+  w += Nj;
+  c.hle = w;
+  w += Nj;
+  c.hge = w;
+  w += Nj;
+  c.kej = w;
+  w += Nj;
+  c.dj = w;
+  w += Nj;
+  c.v1 = w;
+  w += Nj;
+  c.v2 = w;
+  w += Nj;
+  c.v3 = w;
+  w += Nj;
+  c.v4 = w;
+  w += Nj;
+  c.v5 = w;
+  w += Nj;
+  c.Sm = w;
+  w += 2 * kMaxPrv * Nj;
+  c.st0 = reinterpret_cast<int8_t*>(w);
+  w += ((int64_t)L + 7) / 8;
+  c.misc = w;
+  c.q = flow_all + (int64_t)b * L;
+  c.S = status_all + (int64_t)b * L;
+  c.r = r_all + (int64_t)b * r_stride;
+  return c;
+}
+
+// ---------------------------------------------------------------------------
+// K1: link and emitter coefficients at the current state
+__device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
+  const int lane = lane_id();
+  for (int k = lane; k < pl.L; k += 64) {
+    const LinkPY py = link_PY(pl, k, c.q[k], c.r[k], c.S[k]);
+    c.P[k] = py.P;
+    c.Y[k] = py.Y;
+  }
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const double ke = c.kej[j];
+    if (ke > 0.0) {
+      double hl, hg;
+      emitter_hl(pl, c.e[j], ke, hl, hg);
+      c.hle[j] = hl;
+      c.hge[j] = hg;
+    }
+  }
+  mem_sync();
+}
+
+// K2: gather assembly in the reference's order (dgga/solver.py:2958-3007).
+// Writes A values (global and LDS V), F (global) and Xflow (LDS R).
+// polish: pcv PRV links carry their prvcoeff P (adjoint/polish matrix,
+// dgga/autodiff.py:928-962) instead of being skipped.
+__device__ inline void k2_assemble(const DevPlan& pl, Scn& c, bool want_F) {
+  const int lane = lane_id();
+  for (int j = lane; j < pl.Nj; j += 64) {
+    double dg = 0.0, F = 0.0, X = 0.0;
+    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
+    for (int e = e0; e < e1; ++e) {
+      const int ent = pl.jadj[e];
+      const int k = ent & 0x7fffffff;
+      const double P = c.P[k];
+      if (P == 0.0) continue;
+      dg += P;
+      if (want_F) {
+        const double Y = c.Y[k], q = c.q[k];
+        if (ent >= 0) {
+          F += Y;
+          X += -q;
+        } else {
+          F += -Y;
+          X += q;
+        }
+      }
+    }
+    if (want_F) {
+      // ground terms: links with a fixed-head n1 (g1), then fixed-head n2 (g2)
+      for (int e = e0; e < e1; ++e) {
+        const int ent = pl.jadj[e];
+        const int k = ent & 0x7fffffff;
+        if (ent < 0 && !(pl.l_flags[k] & LF_J1)) {
+          const double P = c.P[k];
+          if (P != 0.0) F += P * c.Hn[pl.l_s1[k]];
+        }
+      }
+      for (int e = e0; e < e1; ++e) {
+        const int ent = pl.jadj[e];
+        const int k = ent & 0x7fffffff;
+        if (ent >= 0 && !(pl.l_flags[k] & LF_J2)) {
+          const double P = c.P[k];
+          if (P != 0.0) F += P * c.Hn[pl.l_s2[k]];
+        }
+      }
+    }
+    const double ke = c.kej[j];
+    if (ke > 0.0) {
+      const double hg = c.hge[j];
+      dg = dg + 1.0 / hg;
+      if (want_F) {
+        F = F + (c.hle[j] + pl.el_slot[j]) / hg;
+        X = X - c.e[j];
+      }
+    }
+    const int dp = pl.diag_pos[j];
+    c.A[dp] = dg;
+    c.V[dp] = dg;
+    if (want_F) {
+      X = X - c.dj[j];
+      c.F[j] = F + X;
+      c.R[j] = X;
+    }
+  }
+  for (int o = lane; o < pl.noff; o += 64) {
+    double v = 0.0;
+    for (int e = pl.off_ptr[o]; e < pl.off_ptr[o + 1]; ++e) {
+      const double P = c.P[pl.off_link[e]];
+      if (P != 0.0) v += -P;
+    }
+    const int pos = pl.off_pos[o];
+    c.A[pos] = v;
+    c.V[pos] = v;
+  }
+  for (int f = lane; f < pl.nfill; f += 64) c.V[pl.fill_pos[f]] = 0.0;
+  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  mem_sync();
+  lds_sync();
+}
+
+// prvcoeff for every PRV with a setting, in valve order, by one lane
+// (dgga/solver.py:850-912).  Must follow k2_assemble (reads Xflow[j2]).
+__device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
+  if (pl.p == 0) return;
+  if (lane_id() == 0) {
+    for (int i = 0; i < pl.p; ++i) {
+      const int k = pl.prv_k[i], j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
+      const double qk = c.q[k];
+      const int8_t sk = c.S[k];
+      const bool act = sk == ST_ACTIVE, cl = sk <= ST_CLOSED;
+      double pk_o, yk_o;
+      valve_open(pl, qk, pl.prv_km[i], pk_o, yk_o);
+      const double pk_na = cl ? 1.0 / CBIG : pk_o;
+      const double yk_na = cl ? qk : yk_o;
+      const double mna = act ? 0.0 : 1.0, mact = act ? 1.0 : 0.0;
+      const double xf2 = c.R[j2];
+      const double pna = pk_na * mna;
+      const int po = pl.prv_pos_off[i], d1 = pl.diag_pos[j1], d2 = pl.diag_pos[j2];
+      double v;
+      v = c.A[po] + (-pna);
+      c.A[po] = v;
+      c.V[po] = v;
+      v = c.A[d1] + pna;
+      c.A[d1] = v;
+      c.V[d1] = v;
+      v = c.A[d2] + pna;
+      v = v + CBIG * mact;
+      c.A[d2] = v;
+      c.V[d2] = v;
+      const double dy = (yk_na - qk) * mna;
+      const double f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
+      const double f2 = -dy + pl.prv_hset[i] * CBIG * mact;
+      c.F[j1] = c.F[j1] + f1;
+      c.F[j2] = c.F[j2] + f2;
+      c.P[k] = pna;
+      c.Y[k] = act ? qk + xf2 : yk_na;
+    }
+  }
+  mem_sync();
+  lds_sync();
+}
+
+// K3 + K4: factor V in place, then x = A^-1 b with n_refine refinement steps.
+// b is read from global (bsrc, slot order), x is left in X (LDS).
+__device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, double* X,
+                                int n_refine) {
+  const int lane = lane_id();
+  int dummy = 0;
+  for (int j = lane; j < pl.Nj; j += 64) X[j] = bsrc[j];
+  lds_sync();
+  run_program<1>(pl.fwd, c.V, X, dummy);
+  run_program<2>(pl.bwd, c.V, X, dummy);
+  for (int it = 0; it < n_refine; ++it) {
+    for (int i = lane; i < pl.Nj; i += 64) c.R[i] = bsrc[i] - spmv_row(pl, c.A, X, i);
+    lds_sync();
+    run_program<1>(pl.fwd, c.V, c.R, dummy);
+    run_program<2>(pl.bwd, c.V, c.R, dummy);
+    for (int j = lane; j < pl.Nj; j += 64) X[j] += c.R[j];
+    lds_sync();
+  }
+}
+
+__device__ inline bool k3_factor(const DevPlan& pl, Scn& c) {
+  int bad = 0;
+  run_program<0>(pl.factor, c.V, nullptr, bad);
+  return wave_any(bad != 0);
+}
+
+// K5: newflows and relerr (dgga/solver.py:3092-3110); returns relerr.
+__device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
+  const int lane = lane_id();
+  double dqs = 0.0, qs = 0.0, dqe = 0.0, qse = 0.0;
+  for (int k = lane; k < pl.L; k += 64) {
+    const double dh = c.Hn[pl.l_s1[k]] - c.Hn[pl.l_s2[k]];
+    const double q = c.q[k];
+    double dq = c.Y[k] - c.P[k] * dh;
+    if ((pl.l_flags[k] & LF_PUMP_CHP) && dq > q) dq = q / 2.0;
+    const double qn = q - dq;
+    dqs += fabs(dq);
+    qs += fabs(qn);
+    c.q[k] = qn;
+  }
+  for (int j = lane; j < pl.Nj; j += 64) {
+    if (c.kej[j] > 0.0) {
+      const double dh = c.Hn[j] - pl.el_slot[j];
+      const double dq = (c.hle[j] - dh) / c.hge[j];
+      const double en = c.e[j] - dq;
+      dqe += fabs(dq);
+      qse += fabs(en);
+      c.e[j] = en;
+    }
+  }
+  const double dqsum = wave_sum(dqs) + wave_sum(dqe);
+  const double qsum = wave_sum(qs) + wave_sum(qse);
+  mem_sync();
+  return qsum > hacc ? dqsum / qsum : dqsum;
+}
+
+// K6a: prvstatus for every PRV with a setting (dgga/solver.py:914-947)
+__device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c) {
+  const int i = lane_id();
+  bool chg = false;
+  if (i < pl.p) {
+    const int k = pl.prv_k[i];
+    const double h1 = c.Hn[pl.prv_s1[i]], h2 = c.Hn[pl.prv_s2[i]];
+    const double qk = c.q[k];
+    const int8_t sk = c.S[k];
+    const double hset = pl.prv_hset[i];
+    const double hml = pl.prv_km[i] * (qk * qk);
+    const double htol = pl.htol;
+    const bool neg = qk < -pl.qtol;
+    int8_t sn = sk;
+    if (sk == ST_ACTIVE)
+      sn = neg ? ST_CLOSED : (h1 - hml < hset - htol ? ST_OPEN : ST_ACTIVE);
+    else if (sk == ST_OPEN)
+      sn = neg ? ST_CLOSED : (h2 >= hset + htol ? ST_ACTIVE : ST_OPEN);
+    else if (sk == ST_CLOSED)
+      sn = (h1 >= hset + htol && h2 < hset - htol)
+               ? ST_ACTIVE
+               : ((h1 < hset - htol && h1 > h2 + htol) ? ST_OPEN : ST_CLOSED);
+    else if (sk == ST_XPRESSURE)
+      sn = neg ? ST_CLOSED : sk;
+    chg = sn != sk;
+    c.S[k] = sn;
+  }
+  const bool any = wave_any(chg);
+  mem_sync();
+  return any;
+}
+
+__device__ inline int8_t cvstatus(double htol, double qtol, int8_t s, double dh, double q) {
+  const bool big = fabs(dh) > htol;
+  const bool rev = q < -qtol;
+  if (big) return (dh < -htol || rev) ? ST_CLOSED : ST_OPEN;
+  return rev ? ST_CLOSED : s;
+}
+__device__ inline bool cv_closed(double htol, double qtol, bool s_is_open, double dh, double q) {
+  const bool big = fabs(dh) > htol;
+  if (big) return (dh < -htol) || (q < -qtol);
+  return s_is_open ? (q < -qtol) : true;
+}
+
+// K6b: linkstatus (dgga/solver.py:697-739); returns "changed"
+__device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c) {
+  const int lane = lane_id();
+  bool chg = false;
+  if (!pl.has_cv && !pl.has_pump && pl.ntl == 0) return false;
+  for (int k = lane; k < pl.L; k += 64) {
+    const int8_t st0 = c.S[k];
+    c.st0[k] = st0;
+    int8_t s = (st0 == ST_XHEAD || st0 == ST_TEMPCLOSED) ? ST_OPEN : st0;
+    const int f = pl.l_flags[k];
+    const double dh = c.Hn[pl.l_s1[k]] - c.Hn[pl.l_s2[k]];
+    if (f & LF_CV) s = cvstatus(pl.htol, pl.qtol, s, dh, c.q[k]);
+    if ((f & LF_PUMP_LIVE) && s >= ST_OPEN)
+      s = (-dh > pl.pu_hmax[k] + pl.htol) ? ST_XHEAD : ST_OPEN;
+    c.S[k] = s;
+  }
+  mem_sync();
+  for (int t = lane; t < pl.ntl; t += 64) {
+    const int k = pl.tl_k[t], bits = pl.tl_bits[t];
+    const int8_t s = c.S[k];
+    const bool live = s > ST_CLOSED;
+    const double ht = c.Hn[pl.tl_tank[t]];
+    const double dht = ht - c.Hn[pl.tl_other[t]];
+    const double qk = c.q[k] * ((bits & 1) ? -1.0 : 1.0);
+    const bool full = (ht >= pl.tl_hmax[t] - pl.htol) && !(bits & 2);
+    const bool empty = ht <= pl.tl_hmin[t] + pl.htol;
+    const bool is_pump = bits & 4;
+    const bool cut_f = is_pump ? (bool)(bits & 8) : cv_closed(pl.htol, pl.qtol, true, dht, qk);
+    const bool cut_e = is_pump ? (bool)(bits & 16) : !cv_closed(pl.htol, pl.qtol, false, dht, qk);
+    if (live && ((full && cut_f) || (empty && cut_e))) c.S[k] = ST_TEMPCLOSED;
+  }
+  mem_sync();
+  for (int k = lane; k < pl.L; k += 64) chg |= c.S[k] != c.st0[k];
+  return wave_any(chg);
+}
+
+// diag ratio proxy max|diag| / min|diag| (dgga/solver.py:3026-3032)
+__device__ inline double diag_ratio_now(const DevPlan& pl, Scn& c) {
+  double mx = 0.0, mn = __builtin_huge_val();
+  for (int j = lane_id(); j < pl.Nj; j += 64) {
+    const double v = fabs(c.V[pl.diag_pos[j]]);
+    mx = fmax(mx, v);
+    mn = fmin(mn, v);
+  }
+  mx = wave_max(mx);
+  mn = wave_min(mn);
+  return mx / fmax(mn, 1e-300);
+}
+
+// state load shared by the fused and the per-iteration kernels
+__device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams& sp, int b) {
+  const int lane = lane_id();
+  const double* d = sp.d + (int64_t)b * pl.N;
+  const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
+  const double* rh = sp.rh + (int64_t)b * pl.N;
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const int node = pl.slot_node[j];
+    c.dj[j] = d[node];
+    c.kej[j] = ke[node];
+  }
+  for (int f = lane; f < pl.Nf; f += 64) c.Hn[pl.Nj + f] = rh[pl.slot_node[pl.Nj + f]];
+  for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
+}
+
+// one Newton iteration; returns relerr, sets bad on a non-positive pivot
+__device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const SolveParams& sp,
+                                          bool& bad, double* dratio) {
+  k1_coeffs(pl, c);
+  k2_assemble(pl, c, true);
+  k2_prv(pl, c);
+  if (pl.p > 0 && dratio) *dratio = diag_ratio_now(pl, c);
+  bad = k3_factor(pl, c);
+  if (bad) return 0.0;
+  k4_solve(pl, c, c.F, c.Hn, sp.n_refine);
+  return k5_update(pl, c, sp.hacc);
+}
+
+struct Ctrl {
+  int active, iters, nextcheck, info;
+};
+
+// status-machine bookkeeping after an iteration (dgga/solver.py:3113-3179);
+// returns true when the scenario is done
+__device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams& sp, int it,
+                                    double relerr, Ctrl& st) {
+  if (!sp.sm) return !(relerr > sp.hacc);
+  bool vchg = false;
+  if (pl.p > 0) vchg = k6_prvstatus(pl, c);
+  const bool conv = relerr <= sp.hacc;
+  const bool br_conv = conv && it <= sp.max_iter;
+  const bool br_per = !conv && it <= pl.maxcheck && st.nextcheck == it;
+  bool stat_change = false;
+  if (br_conv || br_per) {
+    const bool chg = k6_linkstatus(pl, c);
+    stat_change = chg && br_conv;
+  }
+  if (pl.p > 0) stat_change = stat_change || (vchg && br_conv);
+  if (br_conv || br_per) {
+    if (br_conv && stat_change) st.nextcheck = it + pl.checkfreq;
+    if (br_per) st.nextcheck = st.nextcheck + pl.checkfreq;
+  }
+  return conv && ((it > sp.max_iter) || !stat_change);
+}
+
+__device__ inline int maxtrials_of(const DevPlan& pl, const SolveParams& sp) {
+  return sp.max_iter + ((sp.sm && pl.extra_iter > 0) ? pl.extra_iter : 0);
+}
+
+__device__ inline void write_outputs(const DevPlan& pl, Scn& c, const SolveParams& sp, int b) {
+  const int lane = lane_id();
+  double* H = sp.head + (int64_t)b * pl.N;
+  double* E = sp.emitter + (int64_t)b * pl.N;
+  for (int n = lane; n < pl.N; n += 64) {
+    const int s = pl.node_slot[n];
+    H[n] = c.Hn[s];
+    E[n] = (s < pl.Nj && c.kej[s] > 0.0) ? c.e[s] : 0.0;
+  }
+}
+
+}  // namespace wds
+
+#include "kernels_polish.h"
+
+namespace wds {
+
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(64) void k_factor_solve(DevPlan pl, const double* csr,
+                                                     const double* F, double* H, int n_refine,
+                                                     double* factor_out, int32_t* info,
+                                                     double* ws) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
+  const double* a = csr + (int64_t)b * pl.nnz_csr;
+  for (int f = lane; f < pl.nfill; f += 64) {
+    c.V[pl.fill_pos[f]] = 0.0;
+    c.A[pl.fill_pos[f]] = 0.0;
+  }
+  for (int e = lane; e < pl.nnz_csr; e += 64) {
+    const int t = pl.csr_to_L[e];
+    if (t >= 0) {
+      c.V[t] = a[e];
+      c.A[t] = a[e];
+    }
+  }
+  const double* Fb = F + (int64_t)b * pl.Nj;
+  for (int r = lane; r < pl.Nj; r += 64) c.F[pl.row_slot[r]] = Fb[r];
+  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  mem_sync();
+  lds_sync();
+  const bool bad = k3_factor(pl, c);
+  if (bad) {
+    if (lane == 0) info[b] = 1;
+    return;
+  }
+  if (factor_out)
+    for (int t = lane; t < pl.nnzL; t += 64) factor_out[(int64_t)b * pl.nnzL + t] = c.V[t];
+  k4_solve(pl, c, c.F, c.Hn, n_refine);
+  double* Hb = H + (int64_t)b * pl.Nj;
+  for (int r = lane; r < pl.Nj; r += 64) Hb[r] = c.Hn[pl.row_slot[r]];
+  if (lane == 0) info[b] = 0;
+}
+
+// Fused forward: the whole Newton loop of scenario blockIdx.x.
+__global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
+  load_inputs(pl, c, sp, b);
+  const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
+  for (int k = lane; k < pl.L; k += 64) {
+    c.q[k] = sp.q0 ? sp.q0[(int64_t)b * pl.L + k] : pl.l_q0[k];
+    c.S[k] = pl.l_st0[k];
+  }
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const int node = pl.slot_node[j];
+    c.e[j] = sp.e0 ? sp.e0[(int64_t)b * pl.N + node] : (ke[node] > 0.0 ? 1.0 : 0.0);
+    c.Hn[j] = 0.0;
+  }
+  mem_sync();
+  lds_sync();
+  const int maxtrials = maxtrials_of(pl, sp);
+  Ctrl st{1, 0, pl.checkfreq, 0};
+  double relerr_out = 0.0, dratio = 0.0;
+  for (int it = 1; it <= maxtrials; ++it) {
+    bool bad = false;
+    const double relerr = newton_iteration(pl, c, sp, bad, &dratio);
+    if (bad) {
+      st.info = it;
+      break;
+    }
+    st.iters = it;
+    relerr_out = relerr;
+    if (control_step(pl, c, sp, it, relerr, st)) {
+      st.active = 0;
+      break;
+    }
+  }
+  if (st.active && st.info == 0) st.iters = maxtrials + 1;
+  double resid = 0.0;
+  if (st.info == 0 && sp.polish_steps >= 0 && sp.factor) {
+    const int pinfo = polish_and_factor(pl, c, sp, b, resid);
+    if (pinfo) st.info = pinfo;
+  }
+  write_outputs(pl, c, sp, b);
+  if (lane == 0) {
+    sp.iters[b] = st.iters;
+    sp.relerr[b] = relerr_out;
+    sp.info[b] = st.info;
+    if (sp.diag_ratio) sp.diag_ratio[b] = dratio;
+    if (sp.resid) sp.resid[b] = resid;
+  }
+}
+
+// One Newton iteration per launch (the unfused path).  ctrl[b] = {active,
+// iters, nextcheck, info}; heads/flows/emitters/statuses persist in the
+// output arrays between launches (it == 1 initialises them).
+__global__ __launch_bounds__(64) void k_newton_iter(DevPlan pl, SolveParams sp, int it,
+                                                    int32_t* ctrl) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  int32_t* cb = ctrl + 4 * b;
+  Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
+  load_inputs(pl, c, sp, b);
+  const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
+  double* H = sp.head + (int64_t)b * pl.N;
+  double* E = sp.emitter + (int64_t)b * pl.N;
+  if (it == 1) {
+    for (int k = lane; k < pl.L; k += 64) {
+      c.q[k] = sp.q0 ? sp.q0[(int64_t)b * pl.L + k] : pl.l_q0[k];
+      c.S[k] = pl.l_st0[k];
+    }
+    for (int j = lane; j < pl.Nj; j += 64) {
+      const int node = pl.slot_node[j];
+      c.e[j] = sp.e0 ? sp.e0[(int64_t)b * pl.N + node] : (ke[node] > 0.0 ? 1.0 : 0.0);
+      c.Hn[j] = 0.0;
+    }
+    if (lane == 0) {
+      cb[0] = 1;
+      cb[1] = 0;
+      cb[2] = pl.checkfreq;
+      cb[3] = 0;
+    }
+  } else {
+    for (int j = lane; j < pl.Nj; j += 64) {
+      const int node = pl.slot_node[j];
+      c.Hn[j] = H[node];
+      c.e[j] = E[node];
+    }
+  }
+  mem_sync();
+  lds_sync();
+  Ctrl st{cb[0], cb[1], cb[2], cb[3]};
+  if (!st.active || st.info) return;
+  const int maxtrials = maxtrials_of(pl, sp);
+  bool bad = false;
+  double dratio = 0.0;
+  const double relerr = newton_iteration(pl, c, sp, bad, &dratio);
+  if (bad) {
+    st.info = it;
+  } else {
+    st.iters = it;
+    if (control_step(pl, c, sp, it, relerr, st)) st.active = 0;
+    if (st.active && it == maxtrials) st.iters = maxtrials + 1;
+  }
+  write_outputs(pl, c, sp, b);
+  if (lane == 0) {
+    cb[0] = st.active;
+    cb[1] = st.iters;
+    cb[2] = st.nextcheck;
+    cb[3] = st.info;
+    if (!bad) sp.relerr[b] = relerr;
+    sp.iters[b] = st.iters;
+    sp.info[b] = st.info;
+    if (sp.diag_ratio && pl.p > 0) sp.diag_ratio[b] = dratio;
+  }
+}
+
+}  // namespace wds

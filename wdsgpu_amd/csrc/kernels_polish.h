@@ -1,0 +1,518 @@
+// Reduced full-Newton polish, terminal factor and implicit adjoint.
+//
+// Mathematics of dgga/autodiff.py:735-1312 (the GPU batched reduced
+// adjoint): eliminating the link and emitter rows of the full residual
+// Jacobian leaves the converged GGA node matrix A; ACTIVE PRVs (big-M rows in
+// A) are handled by Woodbury row replacement so the symmetric factor of A is
+// reused for the primal polish system and for the adjoint node system.
+#pragma once
+#include "device_common.h"
+
+namespace wds {
+
+// Coefficients of the polish / adjoint at the current state
+// (_adjgpu_coeffs, dgga/autodiff.py:786-875): link P/Y with pcv PRVs taking
+// their prvcoeff values (ACTIVE -> P = 0), emitter hloss/hgrad.  Writes P, Y
+// (global), hle/hge and the per-valve ACTIVE mask into sh[0..p).
+__device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
+  const int lane = lane_id();
+  for (int k = lane; k < pl.L; k += 64) {
+    const double q = c.q[k];
+    LinkPY py = link_PY(pl, k, q, c.r[k], c.S[k]);
+    c.P[k] = py.P;
+    c.Y[k] = py.Y;
+  }
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const double ke = c.kej[j];
+    if (ke > 0.0) {
+      double hl, hg;
+      emitter_hl(pl, c.e[j], ke, hl, hg);
+      c.hle[j] = hl;
+      c.hge[j] = hg;
+    }
+  }
+  mem_sync();
+  const int i = lane;
+  if (i < pl.p) {
+    const int k = pl.prv_k[i];
+    const double qk = c.q[k];
+    const int8_t sk = c.S[k];
+    const bool act = sk == ST_ACTIVE, cl = sk <= ST_CLOSED;
+    double pk_o, yk_o;
+    valve_open(pl, qk, pl.prv_km[i], pk_o, yk_o);
+    const double pk_na = cl ? 1.0 / CBIG : pk_o;
+    const double yk_na = cl ? qk : yk_o;
+    c.P[k] = pk_na * (act ? 0.0 : 1.0);
+    c.Y[k] = act ? qk : yk_na;
+    c.sh[i] = act ? 1.0 : 0.0;
+  }
+  mem_sync();
+  lds_sync();
+}
+
+// Assemble A-hat = A(P) + diag(em/hge) + CBIG * ACTIVE at j2 into A and V
+__device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
+  k2_assemble(pl, c, false);
+  if (lane_id() == 0) {
+    for (int i = 0; i < pl.p; ++i) {
+      if (c.sh[i] != 0.0) {
+        const int d2 = pl.diag_pos[pl.prv_s2[i]];
+        const double v = c.A[d2] + CBIG;
+        c.A[d2] = v;
+        c.V[d2] = v;
+      }
+    }
+  }
+  mem_sync();
+  lds_sync();
+}
+
+// x = A-hat^-1 b (b global, slot order) with n_ref refinements, into dst
+// (global).  The factor in V is reused.
+__device__ inline void solveA(const DevPlan& pl, Scn& c, const double* b, double* dst,
+                              int n_ref) {
+  k4_solve(pl, c, b, c.X2, n_ref);
+  for (int j = lane_id(); j < pl.Nj; j += 64) dst[j] = c.X2[j];
+  mem_sync();
+}
+
+// unit right-hand side e_u into dst
+__device__ inline void unit_vec(const DevPlan& pl, double* dst, int u) {
+  for (int j = lane_id(); j < pl.Nj; j += 64) dst[j] = (j == u) ? 1.0 : 0.0;
+  mem_sync();
+}
+
+// (A-hat x)_i for all rows into dst (global)
+__device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double* dst) {
+  for (int i = lane_id(); i < pl.Nj; i += 64) dst[i] = spmv_row(pl, c.A, x, i);
+  mem_sync();
+}
+
+// Solve the n x n system M y = w (row-major in misc) by Gaussian elimination
+// with partial pivoting on lane 0; y overwrites w.  Returns false if singular.
+__device__ inline bool small_solve(double* M, double* w, int n) {
+  bool ok = true;
+  if (lane_id() == 0) {
+    for (int col = 0; col < n; ++col) {
+      int piv = col;
+      double best = fabs(M[col * n + col]);
+      for (int r = col + 1; r < n; ++r)
+        if (fabs(M[r * n + col]) > best) {
+          best = fabs(M[r * n + col]);
+          piv = r;
+        }
+      if (best == 0.0) {
+        ok = false;
+        break;
+      }
+      if (piv != col) {
+        for (int cc = 0; cc < n; ++cc) {
+          const double t = M[col * n + cc];
+          M[col * n + cc] = M[piv * n + cc];
+          M[piv * n + cc] = t;
+        }
+        const double t = w[col];
+        w[col] = w[piv];
+        w[piv] = t;
+      }
+      for (int r = col + 1; r < n; ++r) {
+        const double f = M[r * n + col] / M[col * n + col];
+        if (f == 0.0) continue;
+        for (int cc = col; cc < n; ++cc) M[r * n + cc] -= f * M[col * n + cc];
+        w[r] -= f * w[col];
+      }
+    }
+    if (ok)
+      for (int r = n - 1; r >= 0; --r) {
+        double s = w[r];
+        for (int cc = r + 1; cc < n; ++cc) s -= M[r * n + cc] * w[cc];
+        w[r] = s / M[r * n + r];
+      }
+  }
+  mem_sync();
+  return __shfl(ok ? 1 : 0, 0, 64) != 0;
+}
+
+// Primal reduced Newton system M_p dH = r_p (_adjgpu_primal_solve,
+// dgga/autodiff.py:965-1029).  rhs_p in c.v1, r_prv in misc[256..]; result
+// in dst.  Uses v3 (rp), v4 (work), Sm (2p columns).
+__device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
+  const int lane = lane_id();
+  const int p = pl.p;
+  bool any_act = false;
+  for (int i = 0; i < p; ++i) any_act |= c.sh[i] != 0.0;
+  if (!any_act) {
+    solveA(pl, c, c.v1, dst, 2);
+    return true;
+  }
+  double* rp = c.v3;
+  double* wk = c.v4;
+  const double* rprv = c.misc + 256;
+  double* M = c.misc;           // [2p x 2p]
+  double* w = c.misc + 272;     // [2p]
+  for (int j = lane; j < pl.Nj; j += 64) rp[j] = c.v1[j];
+  mem_sync();
+  if (lane == 0) {
+    for (int i = 0; i < p; ++i)
+      if (c.sh[i] != 0.0) rp[pl.prv_s1[i]] += c.v1[pl.prv_s2[i]];
+    for (int i = 0; i < p; ++i)
+      rp[pl.prv_s2[i]] = (c.sh[i] != 0.0) ? -rprv[i] : c.v1[pl.prv_s2[i]];
+  }
+  mem_sync();
+  // Woodbury columns S = A-hat^-1 [e_j2a..., e_j1a...]
+  for (int col = 0; col < 2 * p; ++col) {
+    const int u = col < p ? pl.prv_s2[col] : pl.prv_s1[col - p];
+    unit_vec(pl, wk, u);
+    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, 2);
+  }
+  auto wood = [&](double* x, const double* rr) -> bool {
+    // capacitance matrix C_eff (rebuilt each call: small_solve destroys it)
+    if (lane == 0) {
+      const int n = 2 * p;
+      for (int a = 0; a < p; ++a) {
+        const double act = c.sh[a];
+        const int j2 = pl.prv_s2[a];
+        for (int col = 0; col < n; ++col) {
+          const double r2 = c.Sm[(int64_t)col * pl.Nj + j2];
+          const double eye = (col == a || col == a + p) ? 1.0 : 0.0;
+          M[a * n + col] = r2 * act + ((a == col) ? 1.0 - act : 0.0);
+          M[(a + p) * n + col] = (eye - CBIG * r2) * act + ((a + p == col) ? 1.0 - act : 0.0);
+        }
+        w[a] = (x[j2] - rr[j2]) * act;
+        w[a + p] = (rr[j2] - CBIG * x[j2]) * act;
+      }
+    }
+    mem_sync();
+    if (!small_solve(M, w, 2 * p)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) {
+      double s = x[j];
+      for (int col = 0; col < 2 * p; ++col) s -= c.Sm[(int64_t)col * pl.Nj + j] * w[col];
+      x[j] = s;
+    }
+    mem_sync();
+    return true;
+  };
+  solveA(pl, c, rp, dst, 2);
+  if (!wood(dst, rp)) return false;
+  for (int ref = 0; ref < 2; ++ref) {
+    matvec(pl, c, dst, wk);  // base = A-hat dH
+    double* res = c.v5;
+    for (int j = lane; j < pl.Nj; j += 64) res[j] = rp[j] - wk[j];
+    mem_sync();
+    if (lane == 0) {
+      for (int i = 0; i < p; ++i) {
+        if (c.sh[i] == 0.0) continue;
+        const int j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
+        const double row1 = wk[j1] + wk[j2] - CBIG * dst[j2];
+        res[j1] = rp[j1] - row1;
+        res[j2] = rp[j2] - dst[j2];
+      }
+    }
+    mem_sync();
+    solveA(pl, c, res, wk, 2);
+    if (!wood(wk, res)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
+    mem_sync();
+  }
+  return true;
+}
+
+// link residuals r_l (into Y), PRV residuals (misc[256..]), emitter residual
+// r_e (into v4 when want_re) and node mass m -> rhs_p in v1
+// (_adjgpu_residuals + the right-hand side of _adjgpu_polish,
+// dgga/autodiff.py:1032-1050, 1066-1075).  Returns the inf-norm residual.
+__device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_rhs) {
+  const int lane = lane_id();
+  double rmax = 0.0;
+  for (int k = lane; k < pl.L; k += 64) {
+    const double P = c.P[k];
+    const double hl = (P != 0.0) ? c.Y[k] / P : 0.0;
+    const double rl = hl - (c.Hn[pl.l_s1[k]] - c.Hn[pl.l_s2[k]]);
+    c.Y[k] = rl;
+    if (P != 0.0) rmax = fmax(rmax, fabs(rl));
+  }
+  double* rprv = c.misc + 256;
+  if (lane < pl.p) {
+    const double rp = (c.Hn[pl.prv_s2[lane]] - pl.prv_hset[lane]) * c.sh[lane];
+    rprv[lane] = rp;
+    rmax = fmax(rmax, fabs(rp));
+  }
+  mem_sync();
+  for (int j = lane; j < pl.Nj; j += 64) {
+    double m = 0.0, bt = 0.0;
+    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
+    for (int e = e0; e < e1; ++e) {  // n2 == j first (reference scatter order)
+      const int ent = pl.jadj[e];
+      if (ent >= 0) continue;
+      const int k = ent & 0x7fffffff;
+      m += c.q[k];
+      bt += c.P[k] * c.Y[k];
+    }
+    for (int e = e0; e < e1; ++e) {
+      const int ent = pl.jadj[e];
+      if (ent < 0) continue;
+      const int k = ent;
+      m += -c.q[k];
+      bt += -(c.P[k] * c.Y[k]);
+    }
+    double re = 0.0, emh = 0.0;
+    if (c.kej[j] > 0.0) {
+      re = c.hle[j] - (c.Hn[j] - pl.el_slot[j]);
+      emh = 1.0 / c.hge[j];
+      m = m - c.e[j];
+      rmax = fmax(rmax, fabs(re));
+    }
+    m = m - c.dj[j];
+    rmax = fmax(rmax, fabs(m));
+    if (build_rhs) {
+      c.v1[j] = m - bt + emh * re;
+      c.v4[j] = re;
+    }
+  }
+  mem_sync();
+  return wave_max(rmax);
+}
+
+// Polish steps + terminal factor (_adjgpu_polish, dgga/autodiff.py:1053-1104).
+// Returns 0, or a negative info code on failure.
+__device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolveParams& sp, int b,
+                                        double& resid) {
+  const int lane = lane_id();
+  double* dH = c.v2;
+  for (int step = 0; step < sp.polish_steps; ++step) {
+    polish_coeffs(pl, c);
+    polish_assemble(pl, c);
+    if (k3_factor(pl, c)) return -(step + 1);
+    polish_residuals(pl, c, true);
+    // v4 holds r_e; keep it in F (free during the polish) before primal_solve
+    for (int j = lane; j < pl.Nj; j += 64) c.F[j] = c.v4[j];
+    mem_sync();
+    if (!primal_solve(pl, c, dH)) return -100 - step;
+    // dq = -P (r_l + dH[j2] - dH[j1]); ACTIVE PRVs from the downstream row
+    for (int k = lane; k < pl.L; k += 64) {
+      const int f = pl.l_flags[k];
+      const double b1 = (f & LF_J1) ? dH[pl.l_s1[k]] : 0.0;
+      const double b2 = (f & LF_J2) ? dH[pl.l_s2[k]] : 0.0;
+      c.P[k] = -c.P[k] * (c.Y[k] + (b2 - b1));  // P now holds dq
+    }
+    mem_sync();
+    bool any_act = false;
+    for (int i = 0; i < pl.p; ++i) any_act |= c.sh[i] != 0.0;
+    if (any_act) {
+      matvec(pl, c, dH, c.v3);
+      if (lane == 0)
+        for (int i = 0; i < pl.p; ++i) {
+          if (c.sh[i] == 0.0) continue;
+          const int j2 = pl.prv_s2[i];
+          const double abar2 = c.v3[j2] - CBIG * dH[j2];
+          c.P[pl.prv_k[i]] = c.v1[j2] - abar2;
+        }
+      mem_sync();
+    }
+    for (int k = lane; k < pl.L; k += 64) c.q[k] = c.q[k] + c.P[k];
+    for (int j = lane; j < pl.Nj; j += 64) {
+      if (c.kej[j] > 0.0) c.e[j] = c.e[j] + (-c.F[j] + dH[j]) / c.hge[j];
+      c.Hn[j] = c.Hn[j] + dH[j];
+    }
+    mem_sync();
+    lds_sync();
+  }
+  // terminal state: coefficients, factor kept for the adjoint, residual
+  polish_coeffs(pl, c);
+  polish_assemble(pl, c);
+  if (k3_factor(pl, c)) return -99;
+  double* fo = sp.factor + (int64_t)b * pl.nnzL;
+  for (int t = lane; t < pl.nnzL; t += 64) fo[t] = c.V[t];
+  resid = polish_residuals(pl, c, false);
+  return 0;
+}
+
+// Node adjoint M lam = r (_adjgpu_node_solve, dgga/autodiff.py:1107-1154):
+// r in v1, gQ of the valves in misc[256..]; result in dst.
+__device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_ref) {
+  const int lane = lane_id();
+  const int p = pl.p;
+  bool any_act = false;
+  for (int i = 0; i < p; ++i) any_act |= c.sh[i] != 0.0;
+  if (!any_act) {
+    solveA(pl, c, c.v1, dst, n_ref);
+    return true;
+  }
+  double* r2 = c.v3;
+  double* wk = c.v4;
+  const double* gqp = c.misc + 256;
+  double* M = c.misc;
+  double* w = c.misc + 272;
+  for (int j = lane; j < pl.Nj; j += 64) r2[j] = c.v1[j];
+  mem_sync();
+  if (lane == 0)
+    for (int i = 0; i < p; ++i)
+      if (c.sh[i] != 0.0) r2[pl.prv_s2[i]] = gqp[i];
+  mem_sync();
+  for (int col = 0; col < p; ++col) {
+    unit_vec(pl, wk, pl.prv_s2[col]);
+    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, n_ref);
+  }
+  auto wood = [&](double* x, const double* rr) -> bool {
+    if (lane == 0) {
+      for (int a = 0; a < p; ++a) {
+        const double acta = c.sh[a];
+        const int j1 = pl.prv_s1[a], j2 = pl.prv_s2[a];
+        for (int col = 0; col < p; ++col) {
+          const double cw = c.Sm[(int64_t)col * pl.Nj + j2] - c.Sm[(int64_t)col * pl.Nj + j1];
+          M[a * p + col] = cw * (acta * c.sh[col]) + ((a == col) ? 1.0 - acta : 0.0);
+        }
+        w[a] = (x[j2] - x[j1] - rr[j2]) * acta;
+      }
+    }
+    mem_sync();
+    if (!small_solve(M, w, p)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) {
+      double s = x[j];
+      for (int col = 0; col < p; ++col) s -= c.Sm[(int64_t)col * pl.Nj + j] * w[col];
+      x[j] = s;
+    }
+    mem_sync();
+    return true;
+  };
+  solveA(pl, c, r2, dst, n_ref);
+  if (!wood(dst, r2)) return false;
+  for (int ref = 0; ref < 2; ++ref) {
+    matvec(pl, c, dst, wk);
+    double* res = c.v5;
+    for (int j = lane; j < pl.Nj; j += 64) res[j] = r2[j] - wk[j];
+    mem_sync();
+    if (lane == 0)
+      for (int i = 0; i < p; ++i) {
+        if (c.sh[i] == 0.0) continue;
+        const int j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
+        res[j2] = r2[j2] - (dst[j2] - dst[j1]);
+      }
+    mem_sync();
+    solveA(pl, c, res, wk, n_ref);
+    if (!wood(wk, res)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
+    mem_sync();
+  }
+  return true;
+}
+
+// Backward of the implicit solve (ImplicitGGASolveGPU.backward,
+// dgga/autodiff.py:1245-1312), one wavefront per scenario.
+__global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  Scn c = make_scn(pl, lds, ap.ws, b, const_cast<double*>(ap.flow),
+                   const_cast<int8_t*>(ap.status), ap.r, ap.r_stride);
+  const double* ke = ap.ke + (int64_t)b * ap.ke_stride;
+  const double* H = ap.head + (int64_t)b * pl.N;
+  const double* E = ap.emitter + (int64_t)b * pl.N;
+  const double* gH = ap.gH + (int64_t)b * pl.N;
+  const double* gE = ap.gE + (int64_t)b * pl.N;
+  const double* gQ = ap.gQ + (int64_t)b * pl.L;
+  for (int s = lane; s < pl.Nj + pl.Nf; s += 64) c.Hn[s] = H[pl.slot_node[s]];
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const int node = pl.slot_node[j];
+    c.kej[j] = ke[node];
+    c.e[j] = E[node];
+  }
+  for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
+  mem_sync();
+  lds_sync();
+  // coefficients at the converged state; assemble A-hat (values for the
+  // refinement mat-vec) and load the forward's terminal factor into V
+  polish_coeffs(pl, c);
+  polish_assemble(pl, c);
+  const double* fac = ap.factor + (int64_t)b * pl.nnzL;
+  for (int t = lane; t < pl.nnzL; t += 64) c.V[t] = fac[t];
+  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  lds_sync();
+  // r = B^T P gQ - em gE / hge - gH (n2 side first, reference scatter order)
+  for (int j = lane; j < pl.Nj; j += 64) {
+    double r = 0.0;
+    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
+    for (int e = e0; e < e1; ++e) {
+      const int ent = pl.jadj[e];
+      if (ent >= 0) continue;
+      const int k = ent & 0x7fffffff;
+      r += c.P[k] * gQ[k];
+    }
+    for (int e = e0; e < e1; ++e) {
+      const int ent = pl.jadj[e];
+      if (ent < 0) continue;
+      r += -(c.P[ent] * gQ[ent]);
+    }
+    const int node = pl.slot_node[j];
+    const double emh = c.kej[j] > 0.0 ? 1.0 / c.hge[j] : 0.0;
+    r = r - emh * gE[node];
+    r = r - gH[node];
+    c.v1[j] = r;
+  }
+  if (lane < pl.p) c.misc[256 + lane] = gQ[pl.prv_k[lane]];
+  mem_sync();
+  double* lam = c.F;  // F is free in the backward
+  if (!node_solve(pl, c, lam, ap.n_refine)) {
+    if (lane == 0) ap.info[b] = -1;
+    return;
+  }
+  // back-substitution and theta contraction
+  double* gd = ap.gd + (int64_t)b * pl.N;
+  double* gke = ap.gke + (int64_t)b * pl.N;
+  double* grh = ap.grh + (int64_t)b * pl.N;
+  double* gr = ap.gr + (int64_t)b * pl.L;
+  double* laml = c.Y;  // lambda_l per link
+  for (int k = lane; k < pl.L; k += 64) {
+    const int f = pl.l_flags[k];
+    const double l1 = (f & LF_J1) ? lam[pl.l_s1[k]] : 0.0;
+    const double l2 = (f & LF_J2) ? lam[pl.l_s2[k]] : 0.0;
+    const double ll = c.P[k] * (gQ[k] - (l2 - l1));
+    laml[k] = ll;
+    // dr = d hloss / d r (H-W pipe, open, unclamped)
+    double dr = 0.0;
+    const int8_t s = c.S[k];
+    const bool closed_now = (s <= ST_CLOSED || (f & LF_ZERO_SPEED)) && !(f & LF_PCV);
+    if ((f & LF_PIPE) && !closed_now) {
+      const double q = c.q[k];
+      const double sgn = q < 0.0 ? -1.0 : 1.0;
+      const double qa = q * sgn;
+      const double hgr = pl.hexp * c.r[k] * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+      if (hgr >= pl.rqtol) dr = pow(qa, pl.hexp) * sgn;
+    }
+    gr[k] = -(ll * dr);
+  }
+  for (int j = lane; j < pl.Nj; j += 64) {
+    const int node = pl.slot_node[j];
+    gd[node] = lam[j];
+    double g = 0.0;
+    const double kv = c.kej[j];
+    if (kv > 0.0) {
+      const double lame = (gE[node] + lam[j]) / c.hge[j];
+      const double ea = fabs(c.e[j]);
+      const double pw = pow(fmax(ea, 1e-30), pl.qexp - 1.0);
+      const double hraw = pl.qexp * fmax(kv, CSMALL) * pw;
+      const double dke = (hraw < pl.rqtol || kv < CSMALL) ? 0.0 : pw * c.e[j];
+      g = -(lame * dke);
+    }
+    gke[node] = g;
+    grh[node] = 0.0;
+  }
+  mem_sync();
+  for (int f = lane; f < pl.Nf; f += 64) {
+    double g = 0.0;
+    for (int e = pl.fadj_ptr[f]; e < pl.fadj_ptr[f + 1]; ++e) {
+      const int ent = pl.fadj[e];
+      if (ent >= 0) g += laml[ent];
+    }
+    for (int e = pl.fadj_ptr[f]; e < pl.fadj_ptr[f + 1]; ++e) {
+      const int ent = pl.fadj[e];
+      if (ent < 0) g += -laml[ent & 0x7fffffff];
+    }
+    const int node = pl.slot_node[pl.Nj + f];
+    grh[node] = g + gH[node];
+    gd[node] = 0.0;
+    gke[node] = 0.0;
+  }
+  if (lane == 0) ap.info[b] = 0;
+}
+
+}  // namespace wds

@@ -1,0 +1,361 @@
+// Plan construction: junction numbering, elimination order, gather lists in
+// the reference's assembly order, PRV/tank tables and the reference CSR map.
+//
+// Assembly order (must match the reference's scatter_add source order so the
+// assembled values are reproducible, dgga/solver.py:283-334, 2958-3017):
+//   off-diagonal (lo,hi): links whose off1 slot is (lo,hi) in link order, then
+//     links whose off2 slot is (lo,hi); pairs of >=3 links in mixed directions
+//     put every link in the first segment (the (min,max) rule, 316-325);
+//   diagonal of junction j: links with n1 == j, then links with n2 == j.
+#include <algorithm>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+
+#include "plan.h"
+
+namespace wds {
+
+thread_local std::string g_err;
+void set_error(const std::string& msg) { g_err = msg; }
+
+static constexpr double kMissing = -1.0e10;
+
+static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
+  const int N = d->N, L = d->L;
+  if (N <= 0 || L <= 0) throw std::invalid_argument("empty network");
+  P->N = N;
+  P->L = L;
+  std::vector<int32_t> junc_row(N, -1), fixed_idx(N, -1);
+  int Nj = 0, Nf = 0;
+  for (int i = 0; i < N; ++i) {
+    if (d->node_type[i] == 0)
+      junc_row[i] = Nj++;
+    else
+      fixed_idx[i] = Nf++;
+  }
+  if (Nj == 0) throw std::invalid_argument("network has no junctions");
+  P->Nj = Nj;
+  P->Nf = Nf;
+  // supported link types: CVPIPE 0, PIPE 1, PUMP 2, PRV 3, TCV 7
+  for (int k = 0; k < L; ++k) {
+    const int t = d->link_type[k];
+    if (!(t == 0 || t == 1 || t == 2 || t == 3 || t == 7))
+      throw std::domain_error("link type " + std::to_string(t) + " is not on the GGA dense path");
+    if (d->link_n1[k] == d->link_n2[k]) throw std::invalid_argument("self-loop link");
+    if (d->link_n1[k] < 0 || d->link_n1[k] >= N || d->link_n2[k] < 0 || d->link_n2[k] >= N)
+      throw std::invalid_argument("link end out of range");
+  }
+  // ---- symbolic analysis on the junction graph ----
+  std::vector<std::vector<int32_t>> adj(Nj);
+  for (int k = 0; k < L; ++k) {
+    const int a = junc_row[d->link_n1[k]], b = junc_row[d->link_n2[k]];
+    if (a >= 0 && b >= 0) adj[a].push_back(b);
+  }
+  analyse(Nj, adj, ordering, P->sym);
+  const Symbolic& S = P->sym;
+  auto slot_of_row = [&](int r) { return S.iperm[r]; };
+  P->node_slot.assign(N, -1);
+  P->slot_node.assign(Nj + Nf, -1);
+  for (int i = 0; i < N; ++i) {
+    const int s = junc_row[i] >= 0 ? slot_of_row(junc_row[i]) : Nj + fixed_idx[i];
+    P->node_slot[i] = s;
+    P->slot_node[s] = i;
+  }
+  P->el_slot.assign(Nj, 0.0);
+  for (int i = 0; i < N; ++i)
+    if (junc_row[i] >= 0) P->el_slot[P->node_slot[i]] = d->elev[i];
+  // ---- per-link tables ----
+  P->l_s1.resize(L);
+  P->l_s2.resize(L);
+  P->l_flags.assign(L, 0);
+  P->l_st0.resize(L);
+  P->l_diam.assign(d->diam, d->diam + L);
+  P->l_km.assign(d->km, d->km + L);
+  P->l_km_tcv.assign(d->km_tcv, d->km_tcv + L);
+  P->l_km_valve.assign(d->km_valve, d->km_valve + L);
+  P->l_kc.assign(d->kc, d->kc + L);
+  P->l_q0.assign(d->q0, d->q0 + L);
+  P->pu_h0.assign(L, 0.0);
+  P->pu_r.assign(L, 0.0);
+  P->pu_n.assign(L, 1.0);
+  P->pu_speed.assign(L, 0.0);
+  P->pu_hmax.assign(L, 0.0);
+  for (int k = 0; k < L; ++k) {
+    const int n1 = d->link_n1[k], n2 = d->link_n2[k], t = d->link_type[k];
+    P->l_s1[k] = P->node_slot[n1];
+    P->l_s2[k] = P->node_slot[n2];
+    P->l_st0[k] = (int8_t)d->init_status[k];
+    int32_t f = 0;
+    if (t <= 1) f |= LF_PIPE;
+    if (t == 0) {
+      f |= LF_CV;
+      P->has_cv = 1;
+    }
+    if (t == 7) f |= LF_TCV;
+    if (t == 3) f |= (d->setting[k] == kMissing) ? LF_PRV_FIXED : LF_PCV;
+    if (t == 2) {
+      P->has_pump = 1;
+      f |= LF_PUMP;
+      const int kind = d->pump_kind[k];
+      const double sp = d->setting[k];
+      if (kind == 0) f |= LF_PUMP_CHP;
+      else if (kind == 1) f |= LF_PUMP_PF;
+      else if (kind == 3) f |= LF_PUMP_NOCURVE;
+      else throw std::domain_error("pump curve kind " + std::to_string(kind) + " not supported");
+      if (kind == 1 && d->pump_n[k] == 1.0) f |= LF_PUMP_LIN;
+      if (sp == 0.0) f |= LF_ZERO_SPEED;
+      if (sp > 0.0) f |= LF_PUMP_LIVE;
+      P->pu_h0[k] = d->pump_h0[k];
+      P->pu_r[k] = d->pump_r[k];
+      P->pu_n[k] = d->pump_n[k];
+      P->pu_speed[k] = sp;
+      P->pu_hmax[k] = d->pump_hmax[k];
+    }
+    if (junc_row[n1] >= 0) f |= LF_J1;
+    if (junc_row[n2] >= 0) f |= LF_J2;
+    P->l_flags[k] = f;
+  }
+  // ---- junction gather lists ----
+  std::vector<std::vector<int32_t>> ja(Nj);
+  for (int k = 0; k < L; ++k)
+    if (junc_row[d->link_n1[k]] >= 0) ja[P->node_slot[d->link_n1[k]]].push_back(k);
+  for (int k = 0; k < L; ++k)
+    if (junc_row[d->link_n2[k]] >= 0)
+      ja[P->node_slot[d->link_n2[k]]].push_back(k | (int32_t)(1u << 31));
+  P->jadj_ptr.assign(Nj + 1, 0);
+  for (int s = 0; s < Nj; ++s) P->jadj_ptr[s + 1] = P->jadj_ptr[s] + (int32_t)ja[s].size();
+  P->jadj.clear();
+  for (int s = 0; s < Nj; ++s) P->jadj.insert(P->jadj.end(), ja[s].begin(), ja[s].end());
+  std::vector<std::vector<int32_t>> fa(Nf);
+  for (int k = 0; k < L; ++k)
+    if (fixed_idx[d->link_n1[k]] >= 0) fa[fixed_idx[d->link_n1[k]]].push_back(k);
+  for (int k = 0; k < L; ++k)
+    if (fixed_idx[d->link_n2[k]] >= 0)
+      fa[fixed_idx[d->link_n2[k]]].push_back(k | (int32_t)(1u << 31));
+  P->fadj_ptr.assign(Nf + 1, 0);
+  for (int f = 0; f < Nf; ++f) P->fadj_ptr[f + 1] = P->fadj_ptr[f] + (int32_t)fa[f].size();
+  P->fadj.clear();
+  for (int f = 0; f < Nf; ++f) P->fadj.insert(P->fadj.end(), fa[f].begin(), fa[f].end());
+  // ---- off-diagonal entries: unordered junction pairs ----
+  std::map<std::pair<int, int>, std::vector<int>> pairs;  // (lo,hi) rows -> links
+  for (int k = 0; k < L; ++k) {
+    const int a = junc_row[d->link_n1[k]], b = junc_row[d->link_n2[k]];
+    if (a < 0 || b < 0) continue;
+    pairs[{std::min(a, b), std::max(a, b)}].push_back(k);
+  }
+  P->row_slot.resize(Nj);
+  for (int r = 0; r < Nj; ++r) P->row_slot[r] = slot_of_row(r);
+  P->diag_pos.resize(Nj);
+  for (int s = 0; s < Nj; ++s) P->diag_pos[s] = S.colptr[s];
+  std::vector<char> is_a(S.nnzL, 0);
+  for (int s = 0; s < Nj; ++s) is_a[S.colptr[s]] = 1;
+  P->off_pos.clear();
+  P->off_ptr.assign(1, 0);
+  P->off_link.clear();
+  std::vector<std::vector<std::pair<int32_t, int32_t>>> rows(Nj);
+  for (auto& kv : pairs) {
+    const int lo = kv.first.first, hi = kv.first.second;
+    const auto& ks = kv.second;
+    int nfwd = 0;
+    for (int k : ks) nfwd += (junc_row[d->link_n1[k]] == lo);
+    const bool trig = ks.size() >= 3 && nfwd != 0 && nfwd != (int)ks.size();
+    std::vector<int32_t> seq;
+    for (int k : ks)
+      if (trig || junc_row[d->link_n1[k]] == lo) seq.push_back(k);
+    if (!trig)
+      for (int k : ks)
+        if (junc_row[d->link_n1[k]] == hi) seq.push_back(k);
+    const int sa = slot_of_row(lo), sb = slot_of_row(hi);
+    const int64_t pos = S.pos(std::max(sa, sb), std::min(sa, sb));
+    if (pos < 0) throw std::logic_error("pattern of A not contained in L");
+    P->off_pos.push_back((int32_t)pos);
+    P->off_link.insert(P->off_link.end(), seq.begin(), seq.end());
+    P->off_ptr.push_back((int32_t)P->off_link.size());
+    is_a[pos] = 1;
+    rows[sa].push_back({(int32_t)pos, sb});
+    rows[sb].push_back({(int32_t)pos, sa});
+  }
+  P->fill_pos.clear();
+  for (int64_t t = 0; t < S.nnzL; ++t)
+    if (!is_a[t]) P->fill_pos.push_back((int32_t)t);
+  P->spmv_ptr.assign(Nj + 1, 0);
+  P->spmv_ent.clear();
+  for (int s = 0; s < Nj; ++s) {
+    auto& r = rows[s];
+    r.push_back({S.colptr[s], s});
+    std::sort(r.begin(), r.end(), [](const std::pair<int32_t, int32_t>& x,
+                                     const std::pair<int32_t, int32_t>& y) {
+      return x.second < y.second;
+    });
+    for (auto& e : r) {
+      P->spmv_ent.push_back(e.first);
+      P->spmv_ent.push_back(e.second);
+    }
+    P->spmv_ptr[s + 1] = P->spmv_ptr[s] + (int32_t)r.size();
+  }
+  // ---- PRVs with a setting ----
+  P->prv_k.clear();
+  for (int k = 0; k < L; ++k) {
+    if (!(P->l_flags[k] & LF_PCV)) continue;
+    const int n1 = d->link_n1[k], n2 = d->link_n2[k];
+    if (junc_row[n1] < 0 || junc_row[n2] < 0)
+      throw std::domain_error("PRV with a fixed-head end (the batched PRV path needs junction ends)");
+    const int s1 = P->node_slot[n1], s2 = P->node_slot[n2];
+    P->prv_k.push_back(k);
+    P->prv_s1.push_back(s1);
+    P->prv_s2.push_back(s2);
+    P->prv_pos_off.push_back((int32_t)S.pos(std::max(s1, s2), std::min(s1, s2)));
+    P->prv_hset.push_back(d->elev[n2] + d->setting[k]);
+    P->prv_km.push_back(d->km_valve[k]);
+  }
+  P->p = (int32_t)P->prv_k.size();
+  if (P->p > kMaxPrv)
+    throw std::domain_error("more than " + std::to_string(kMaxPrv) + " PRVs with a setting");
+  // ---- tank links (dgga/solver.py:536-568) ----
+  std::vector<char> is_tank(N, 0), ovf(N, 0);
+  std::vector<double> thmin(N, 0.0), thmax(N, 0.0);
+  for (int t = 0; t < d->ntank; ++t) {
+    const int n = d->tank_node[t];
+    is_tank[n] = 1;
+    thmin[n] = d->tank_hmin[t];
+    thmax[n] = d->tank_hmax[t];
+    ovf[n] = d->tank_overflow[t] != 0;
+  }
+  for (int k = 0; k < L; ++k) {
+    const int n1 = d->link_n1[k], n2 = d->link_n2[k];
+    const bool f1 = d->node_type[n1] != 0, f2 = d->node_type[n2] != 0;
+    const bool use_n2 = !f1 && f2;
+    const int cand = f1 ? n1 : (f2 ? n2 : -1);
+    if (cand < 0 || !is_tank[cand]) continue;
+    const bool is_pump = d->link_type[k] == 2;
+    int32_t bits = (use_n2 ? 1 : 0) | (ovf[cand] ? 2 : 0) | (is_pump ? 4 : 0) |
+                   ((is_pump && n2 == cand) ? 8 : 0) | ((is_pump && n1 == cand) ? 16 : 0);
+    P->tl_k.push_back(k);
+    P->tl_tank.push_back(P->node_slot[cand]);
+    P->tl_other.push_back(P->node_slot[use_n2 ? n1 : n2]);
+    P->tl_bits.push_back(bits);
+    P->tl_hmax.push_back(thmax[cand]);
+    P->tl_hmin.push_back(thmin[cand]);
+  }
+  // ---- reference CSR pattern (dgga/solver.py:336-369) and its map to L ----
+  std::vector<int64_t> flat;
+  for (auto& kv : pairs) {
+    flat.push_back((int64_t)kv.first.first * Nj + kv.first.second);
+    flat.push_back((int64_t)kv.first.second * Nj + kv.first.first);
+  }
+  for (int r = 0; r < Nj; ++r) flat.push_back((int64_t)r * Nj + r);
+  std::sort(flat.begin(), flat.end());
+  flat.erase(std::unique(flat.begin(), flat.end()), flat.end());
+  P->csr_indptr.assign(Nj + 1, 0);
+  P->csr_indices.resize(flat.size());
+  P->csr_to_L.resize(flat.size());
+  for (size_t e = 0; e < flat.size(); ++e) {
+    const int x = (int)(flat[e] / Nj), y = (int)(flat[e] % Nj);
+    P->csr_indptr[x + 1]++;
+    P->csr_indices[e] = y;
+    const int sx = slot_of_row(x), sy = slot_of_row(y);
+    P->csr_to_L[e] = (x == y) ? S.colptr[sx] : (sx > sy ? (int32_t)S.pos(sx, sy) : -1);
+  }
+  for (int r = 0; r < Nj; ++r) P->csr_indptr[r + 1] += P->csr_indptr[r];
+  // ---- scalars ----
+  P->headloss = d->headloss;
+  P->hexp = d->hexp;
+  P->qexp = d->qexp;
+  P->rqtol = d->rqtol;
+  P->viscos = d->viscos;
+  P->htol = d->htol;
+  P->qtol = d->qtol;
+  P->checkfreq = d->checkfreq;
+  P->maxcheck = d->maxcheck;
+  P->extra_iter = d->extra_iter;
+  // ---- stats ----
+  wds_plan_stats& st = P->stats;
+  st.N = N;
+  st.L = L;
+  st.Nj = Nj;
+  st.nnzL = (int32_t)S.nnzL;
+  st.nlevels = S.nlevels;
+  st.npcv = P->p;
+  st.nnz_csr = (int64_t)flat.size();
+  st.nnz_lower = Nj + (int64_t)P->off_pos.size();
+  st.factor_updates = S.factor_updates;
+  st.factor_steps = S.factor.nsteps();
+  st.fwd_steps = S.fwd.nsteps();
+  st.bwd_steps = S.bwd.nsteps();
+  st.factor_instr = (int64_t)S.factor.instr.size();
+  st.fwd_instr = (int64_t)S.fwd.instr.size();
+  st.bwd_instr = (int64_t)S.bwd.instr.size();
+  st.lds_bytes = 8 * ((int64_t)S.nnzL + 1 + (Nj + Nf) + 2 * (int64_t)Nj + 2 * kMaxPrv);
+}
+
+}  // namespace wds
+
+extern "C" int wds_plan_create(const wds_network_desc* net, int32_t ordering, wds_plan** out) {
+  if (!net || !out) {
+    wds::set_error("wds_plan_create: null argument");
+    return WDS_EINVAL;
+  }
+  wds_plan* P = new wds_plan();
+  try {
+    wds::build(net, ordering, P);
+  } catch (const std::domain_error& e) {
+    wds::set_error(e.what());
+    delete P;
+    return WDS_EUNSUPPORTED;
+  } catch (const std::exception& e) {
+    wds::set_error(e.what());
+    delete P;
+    return WDS_EINVAL;
+  }
+  *out = P;
+  return WDS_OK;
+}
+
+extern "C" int wds_plan_stats_get(const wds_plan* plan, wds_plan_stats* out) {
+  if (!plan || !out) return WDS_EINVAL;
+  *out = plan->stats;
+  return WDS_OK;
+}
+
+static int64_t copy_out(const std::vector<int32_t>& v, int32_t* dst, int64_t cap) {
+  if (dst && cap > 0) std::memcpy(dst, v.data(), sizeof(int32_t) * std::min<int64_t>(cap, v.size()));
+  return (int64_t)v.size();
+}
+
+extern "C" int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* dst, int64_t cap) {
+  if (!plan) return WDS_EINVAL;
+  const wds::Symbolic& S = plan->sym;
+  switch (what) {
+    case 0: return copy_out(S.perm, dst, cap);
+    case 1: return copy_out(S.colptr, dst, cap);
+    case 2: return copy_out(S.rowidx, dst, cap);
+    case 3: return copy_out(S.parent, dst, cap);
+    case 4: return copy_out(plan->node_slot, dst, cap);
+    case 5: return copy_out(plan->csr_indptr, dst, cap);
+    case 6: return copy_out(plan->csr_indices, dst, cap);
+    case 7: return copy_out(plan->csr_to_L, dst, cap);
+    default: break;
+  }
+  if (what >= 8 && what < 17) {
+    const int p = (what - 8) / 3, part = (what - 8) % 3;
+    const wds::Program& pr = p == 0 ? S.factor : (p == 1 ? S.fwd : S.bwd);
+    if (part == 0) {
+      std::vector<int32_t> v(pr.instr.size() * 4);
+      std::memcpy(v.data(), pr.instr.data(), v.size() * sizeof(int32_t));
+      return copy_out(v, dst, cap);
+    }
+    if (part == 1) {
+      std::vector<int32_t> v(pr.step_off.size() * 2);
+      for (size_t i = 0; i < pr.step_off.size(); ++i) {
+        v[2 * i] = pr.step_off[i];
+        v[2 * i + 1] = pr.step_cnt[i];
+      }
+      return copy_out(v, dst, cap);
+    }
+    return copy_out(pr.level_end, dst, cap);
+  }
+  return WDS_EINVAL;
+}
+
+extern "C" const char* wds_last_error(void) { return wds::g_err.c_str(); }

@@ -1,0 +1,40 @@
+// Host-side plan object behind the opaque wds_plan handle.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/wds.h"
+#include "devplan.h"
+#include "symbolic.h"
+
+struct wds_plan {
+  wds::Symbolic sym;
+  int32_t N = 0, L = 0, Nj = 0, Nf = 0, p = 0;
+  // host copies of everything that goes to the device
+  std::vector<int32_t> node_slot, slot_node, l_s1, l_s2, l_flags;
+  std::vector<int8_t> l_st0;
+  std::vector<double> el_slot, l_diam, l_km, l_km_tcv, l_km_valve, l_kc, l_q0,
+      pu_h0, pu_r, pu_n, pu_speed, pu_hmax;
+  std::vector<int32_t> jadj_ptr, jadj, fadj_ptr, fadj, off_pos, off_ptr, off_link, fill_pos,
+      spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, tl_k, tl_tank, tl_other,
+      tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;
+  std::vector<int32_t> spmv_ent;  // pairs (pos, col)
+  std::vector<double> prv_hset, prv_km, tl_hmax, tl_hmin;
+  // scalars
+  int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
+  int32_t has_cv = 0, has_pump = 0;
+  double hexp = 1.852, qexp = 2.0, rqtol = 1e-7, viscos = 1.1e-5, htol = 5e-4, qtol = 1e-4;
+  // device state
+  wds::DevPlan dev{};
+  void* d_blob = nullptr;
+  int device = -1;
+  wds_counters cnt{};
+  wds_plan_stats stats{};
+};
+
+namespace wds {
+void set_error(const std::string& msg);
+}

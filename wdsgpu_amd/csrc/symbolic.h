@@ -1,0 +1,64 @@
+// Host-side symbolic analysis for the batched sparse LDL^T factorisation of
+// the GGA node matrix A (junction x junction, symmetric positive definite).
+//
+// Once per topology: minimum-degree ordering, elimination tree, the pattern
+// of L, tree levels, and the per-lane instruction streams that a 64-lane
+// wavefront executes to factor and solve with L resident in LDS.
+#pragma once
+#include <cstdint>
+#include <vector>
+
+namespace wds {
+
+constexpr int kLanes = 64;
+
+// One instruction of a level-scheduled sparse dot-product program.  Each lane
+// of the wavefront owns its own stream.  Within a level the lanes are sorted
+// by decreasing work, so at step s the active lanes are 0..cnt[s]-1 and lane
+// l reads instr[off[s] + l]: the loads of a step are contiguous and padding
+// costs no memory traffic.
+//   factor: acc -= V[a] * V[b] * V[c]     (w_ik * w_jk * invD_k)
+//   fwd:    acc -= V[a] * X[b]            (w_jk * z_k)
+//   bwd:    acc += V[a] * X[b]            (w_ij * x_i)
+// flags live in the top bits of t.
+struct Instr {
+  int32_t a, b, c, t;
+};
+constexpr int32_t kFirst = 1 << 30;   // acc starts from the target (or 0)
+constexpr int32_t kLast = 1 << 29;    // store acc into the target after this op
+constexpr int32_t kDiag = 1 << 28;    // factor: target is a diagonal -> store 1/acc
+constexpr int32_t kNop = 1 << 27;     // padding, no load/store
+constexpr int32_t kTMask = (1 << 27) - 1;
+
+struct Program {
+  std::vector<Instr> instr;            // compacted, step-major
+  std::vector<int32_t> step_off;       // per step: first instruction
+  std::vector<int32_t> step_cnt;       // per step: active lanes (prefix)
+  std::vector<int32_t> level_end;      // cumulative steps at each level end
+  int64_t useful = 0;                  // instructions (== instr.size())
+  int32_t nsteps() const { return level_end.empty() ? 0 : level_end.back(); }
+};
+// steps of every level are padded to a multiple of this so an unrolled
+// chunk never straddles a level boundary
+constexpr int kStepAlign = 4;
+
+struct Symbolic {
+  int n = 0;                           // junction count
+  std::vector<int32_t> perm, iperm;    // perm[new] = old, iperm[old] = new
+  std::vector<int32_t> parent;         // elimination tree (new order), -1 root
+  std::vector<int32_t> colptr, rowidx; // L pattern, CSC, diagonal first
+  std::vector<int32_t> height;         // 0 at leaves
+  int nlevels = 0;
+  int64_t nnzL = 0;
+  int64_t factor_updates = 0;          // sum over targets of update terms
+  Program factor, fwd, bwd;
+  // position of L(i,j) for a pattern entry (binary search in column j)
+  int64_t pos(int i, int j) const;
+};
+
+// adjacency: for each vertex (original index) its neighbours (no self loops,
+// duplicates allowed).  ordering: 0 = minimum degree, 1 = natural.
+void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
+             Symbolic& s);
+
+}  // namespace wds

@@ -1,0 +1,261 @@
+# -*- coding: utf-8 -*-
+"""Seeded synthetic networks and scenario batches for the GGA workloads.
+
+Every generator takes an integer seed and is deterministic for it (numpy
+``default_rng``).  Quantities are drawn in SI user units (m, mm, L/s) and
+converted to EPANET internal units (ft, cfs) exactly as the reference's INP
+reader does (dgga/parse.py:557-611): lengths / 0.3048, diameters / 304.8,
+flows / 28.317, emitter C -> Ke via (28.317^Qexp / 0.3048) / C^Qexp.
+
+Workloads:
+  * ``grid_network``    -- 10x10 grid, 180 grid + 20 chord pipes, one
+    reservoir at 100 m joined to corner (0,0).
+  * ``planar_network``  -- planar random network: Delaunay graph of random
+    points, a random breadth-first spanning tree grown from the reservoir
+    mains, chords to loop density ~0.15, 3 PRVs on trunk mains, 10 leaks.
+  * ``scenario_batch``  -- per-scenario i.i.d. lognormal demand multipliers.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from .network import (Net, JUNCTION, RESERVOIR, PIPE, PRV, M_PER_FT,
+                      LPS_PER_CFS, hw_resistance, emitter_ke)
+
+DIAMS_MM = np.array([100.0, 150.0, 200.0, 250.0, 300.0, 400.0])
+
+
+def _mk_net(node_type, elev_m, demand_lps, links, meta, ke=None):
+    """links: list of dicts with n1, n2, type, len_m, diam_mm, C, setting_m."""
+    N = len(node_type)
+    L = len(links)
+    lt = np.array([lk["type"] for lk in links], dtype=np.int8)
+    ln_ft = np.array([lk.get("len_m", 0.0) for lk in links]) / M_PER_FT
+    d_ft = np.array([lk["diam_mm"] for lk in links]) / (1000.0 * M_PER_FT)
+    C = np.array([lk.get("C", 0.0) for lk in links])
+    r = np.zeros(L)
+    pipe = lt == PIPE
+    r[pipe] = hw_resistance(C[pipe], ln_ft[pipe], d_ft[pipe])
+    st = np.array([2 if lk["type"] == PRV else 1 for lk in links], dtype=np.int8)
+    setting = np.array([lk.get("setting_m", 0.0) for lk in links])
+    nid = [("R%d" % i if t != JUNCTION else "J%d" % i) for i, t in enumerate(node_type)]
+    return Net(node_id=nid, node_type=np.asarray(node_type, dtype=np.int8),
+               elev_ft=np.asarray(elev_m, dtype=np.float64) / M_PER_FT,
+               node_ke=np.zeros(N) if ke is None else ke,
+               demand_cfs=np.asarray(demand_lps, dtype=np.float64) / LPS_PER_CFS,
+               link_id=["P%d" % k for k in range(L)], link_type=lt,
+               link_n1=np.array([lk["n1"] for lk in links], dtype=np.int32),
+               link_n2=np.array([lk["n2"] for lk in links], dtype=np.int32),
+               diam_ft=d_ft, len_ft=ln_ft, roughness=C, km_int=np.zeros(L),
+               r_hw=r, init_status=st, valve_setting_user=setting, meta=meta)
+
+
+def grid_network(seed=0, nx=10, ny=10, n_chords=20):
+    """10x10 grid network (100 junctions, elevation U[0,50] m, demand
+    U[0.1,2.0] L/s), one reservoir at 100 m joined to corner junction (0,0),
+    180 grid pipes plus ``n_chords`` cell-diagonal chords; pipe length
+    U[100,1000] m, diameter from {100..400} mm, C U[90,140].  The reservoir
+    main is a 100 m, 400 mm, C=130 pipe."""
+    g = np.random.default_rng(seed)
+    nj = nx * ny
+    elev = np.concatenate([g.uniform(0.0, 50.0, nj), [100.0]])
+    dem = np.concatenate([g.uniform(0.1, 2.0, nj), [0.0]])
+    ntype = [JUNCTION] * nj + [RESERVOIR]
+    res = nj
+    jid = lambda i, j: i * ny + j
+    pairs = []
+    for i in range(nx):
+        for j in range(ny):
+            if i + 1 < nx:
+                pairs.append((jid(i, j), jid(i + 1, j)))
+            if j + 1 < ny:
+                pairs.append((jid(i, j), jid(i, j + 1)))
+    diag = [(jid(i, j), jid(i + 1, j + 1)) for i in range(nx - 1) for j in range(ny - 1)]
+    diag += [(jid(i, j + 1), jid(i + 1, j)) for i in range(nx - 1) for j in range(ny - 1)]
+    pick = g.choice(len(diag), size=n_chords, replace=False)
+    pairs += [diag[int(p)] for p in pick]
+    links = [dict(n1=res, n2=jid(0, 0), type=PIPE, len_m=100.0, diam_mm=400.0, C=130.0)]
+    for a, b in pairs:
+        links.append(dict(n1=a, n2=b, type=PIPE, len_m=g.uniform(100.0, 1000.0),
+                          diam_mm=float(g.choice(DIAMS_MM)), C=g.uniform(90.0, 140.0)))
+    meta = dict(flow_units="LPS", headloss="H-W", accuracy=1e-3, trials=200,
+                seed=int(seed), workload="grid10x10")
+    return _mk_net(ntype, elev, dem, links, meta)
+
+
+def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,
+                   n_leaks=10, demand_lps=(0.05, 0.5), prv_setting_m=(30.0, 50.0),
+                   res_head_m=(60.0, 90.0)):
+    """Seeded random planar network.
+
+    * junctions: uniform points in a square of side 120 m * sqrt(n_junc),
+      elevation U[0,50] m, base demand U[demand_lps] L/s;
+    * graph: Delaunay triangulation (planar); each reservoir (head
+      U[res_head_m] m) sits on the boundary and feeds its nearest junction
+      through a 400 mm main;
+    * spanning tree: multi-source breadth-first search from the reservoir
+      mains with a random neighbour order;
+    * chords: random non-tree Delaunay edges until the link count reaches
+      n_pipes + n_prv (loop density (links - tree links) / tree links ~0.16);
+    * lengths lognormal (median 120 m, sigma 0.8); C U[90,140]; diameters drawn
+      from {100..400} mm, the tree draws assigned by downstream-demand rank so
+      mains carry the large sizes;
+    * n_prv trunk tree links become PRVs (setting U[prv_setting_m] m),
+      oriented downstream, both ends junctions, distinct downstream nodes and
+      no cascade (no PRV's downstream node is another PRV's upstream node);
+    * n_leaks junctions get emitter C U[0,0.5] L/s/m^0.5 (Qexp 2).
+    """
+    from scipy.spatial import Delaunay
+
+    g = np.random.default_rng(seed)
+    side = 120.0 * np.sqrt(n_junc)
+    pts = g.uniform(0.0, side, (n_junc, 2))
+    tri = Delaunay(pts)
+    adj = [set() for _ in range(n_junc)]
+    for s in tri.simplices:
+        for a in range(3):
+            u, v = int(s[a]), int(s[(a + 1) % 3])
+            adj[u].add(v)
+            adj[v].add(u)
+    adj = [sorted(a) for a in adj]
+    # reservoir attachment junctions: nearest junction to random boundary points
+    roots = []
+    for r in range(n_res):
+        edge = g.integers(4)
+        t = g.uniform(0.1, 0.9) * side
+        bp = [(t, 0.0), (side, t), (t, side), (0.0, t)][edge]
+        d2 = ((pts - np.asarray(bp)) ** 2).sum(1)
+        for j in np.argsort(d2):
+            if int(j) not in roots:
+                roots.append(int(j))
+                break
+    # multi-source BFS tree with random neighbour order
+    parent = np.full(n_junc, -2, dtype=np.int64)
+    depth = np.zeros(n_junc, dtype=np.int64)
+    frontier = list(roots)
+    for r in roots:
+        parent[r] = -1
+    while frontier:
+        nxt = []
+        for u in frontier:
+            nb = list(adj[u])
+            g.shuffle(nb)
+            for v in nb:
+                if parent[v] == -2:
+                    parent[v] = u
+                    depth[v] = depth[u] + 1
+                    nxt.append(v)
+        g.shuffle(nxt)
+        frontier = nxt
+    assert (parent != -2).all(), "Delaunay graph is connected"
+    tree = [(int(parent[v]), v) for v in range(n_junc) if parent[v] >= 0]
+    # subtree demand (for trunk ranking and diameter assignment)
+    dem = g.uniform(demand_lps[0], demand_lps[1], n_junc)
+    order = np.argsort(-depth)
+    sub = dem.copy()
+    subn = np.ones(n_junc, dtype=np.int64)
+    for v in order:
+        p = parent[v]
+        if p >= 0:
+            sub[p] += sub[v]
+            subn[p] += subn[v]
+    # node arrays: junctions 0..n_junc-1, reservoirs after
+    elev = np.concatenate([g.uniform(0.0, 50.0, n_junc),
+                           g.uniform(res_head_m[0], res_head_m[1], n_res)])
+    ntype = [JUNCTION] * n_junc + [RESERVOIR] * n_res
+    demand = np.concatenate([dem, np.zeros(n_res)])
+    # PRV trunk selection: tree links with a meaningful downstream zone
+    trunk = [i for i, (p, v) in enumerate(tree) if 15 <= subn[v] <= 150 and parent[p] >= 0]
+    g.shuffle(trunk)
+    prv_idx, used_up, used_dn = [], set(), set()
+    for i in trunk:
+        p, v = tree[i]
+        if v in used_dn or v in used_up or p in used_dn or p in used_up:
+            continue
+        prv_idx.append(i)
+        used_up.add(p)
+        used_dn.add(v)
+        if len(prv_idx) == n_prv:
+            break
+    assert len(prv_idx) == n_prv
+    prv_set = set(prv_idx)
+    # pressure zones: a junction belongs to the zone of its nearest PRV
+    # ancestor (0 = the main zone); chords never cross zones, so every PRV
+    # zone is fed through its valve only (a district metered area)
+    zone = np.zeros(n_junc, dtype=np.int64)
+    zone_of_dn = {tree[i][1]: z + 1 for z, i in enumerate(prv_idx)}
+    for v in np.argsort(depth):
+        p = parent[v]
+        if v in zone_of_dn:
+            zone[v] = zone_of_dn[v]
+        elif p >= 0:
+            zone[v] = zone[p]
+    tset = set((min(a, b), max(a, b)) for a, b in tree)
+    cand = sorted(set((min(u, v), max(u, v)) for u in range(n_junc) for v in adj[u]
+                      if zone[u] == zone[v]) - tset)
+    n_links_total = n_pipes + n_prv
+    n_tree = len(tree) + n_res
+    n_ch = n_links_total - n_tree
+    pick = g.choice(len(cand), size=n_ch, replace=False)
+    chords = [cand[int(i)] for i in sorted(pick)]
+    # diameters: tree draws sorted by subtree demand rank, chords random
+    dt_draw = np.sort(g.choice(DIAMS_MM, size=len(tree)))
+    rank = np.argsort(np.argsort([sub[v] for _, v in tree]))
+    tree_diam = dt_draw[rank]
+    links = []
+    for r in range(n_res):
+        links.append(dict(n1=n_junc + r, n2=roots[r], type=PIPE,
+                          len_m=float(np.exp(np.log(120.0) + 0.8 * g.standard_normal())),
+                          diam_mm=400.0, C=float(g.uniform(90.0, 140.0))))
+    for i, (p, v) in enumerate(tree):
+        lk = dict(n1=p, n2=v, type=PIPE,
+                  len_m=float(np.exp(np.log(120.0) + 0.8 * g.standard_normal())),
+                  diam_mm=float(tree_diam[i]), C=float(g.uniform(90.0, 140.0)))
+        if i in prv_set:
+            lk["type"] = PRV
+            lk["setting_m"] = float(g.uniform(prv_setting_m[0], prv_setting_m[1]))
+            lk["C"] = 0.0
+            lk["len_m"] = 0.0
+        links.append(lk)
+    for a, b in chords:
+        if g.random() < 0.5:
+            a, b = b, a
+        links.append(dict(n1=a, n2=b, type=PIPE,
+                          len_m=float(np.exp(np.log(120.0) + 0.8 * g.standard_normal())),
+                          diam_mm=float(g.choice(DIAMS_MM)), C=float(g.uniform(90.0, 140.0))))
+    leaks = g.choice(n_junc, size=n_leaks, replace=False)
+    c_leak = np.zeros(n_junc + n_res)
+    c_leak[leaks] = g.uniform(0.0, 0.5, n_leaks)
+    ke = emitter_ke(c_leak, "LPS", 2.0, M_PER_FT)
+    meta = dict(flow_units="LPS", headloss="H-W", accuracy=1e-3, trials=200,
+                seed=int(seed), workload="planar%d" % n_junc,
+                leak_nodes=np.sort(leaks).tolist(), emitter_c=c_leak)
+    return _mk_net(ntype, elev, demand, links, meta, ke=ke)
+
+
+def scenario_batch(net, B, seed, sigma=0.3):
+    """Per-scenario demands: every junction base demand times an independent
+    lognormal(0, sigma) factor.  Returns (demand_cfs[B,N], res_head_ft[B,N])
+    float64; reservoir heads are the network's fixed heads."""
+    g = np.random.default_rng(seed)
+    base = np.asarray(net.demand_cfs, dtype=np.float64)
+    mult = np.exp(sigma * g.standard_normal((B, net.N)))
+    d = base[None, :] * mult
+    rh = np.broadcast_to(np.nan_to_num(net.reservoir_head_ft()), (B, net.N)).copy()
+    return d, rh
+
+
+def roughness_truth(net, seed, sd=5.0):
+    """'True' Hazen-Williams C field: C + N(0, sd) on pipes (other links 0)."""
+    g = np.random.default_rng(seed)
+    C = np.asarray(net.roughness, dtype=np.float64).copy()
+    pipe = np.asarray(net.link_type) == PIPE
+    C[pipe] = C[pipe] + sd * g.standard_normal(int(pipe.sum()))
+    return C
+
+
+def sensor_nodes(net, n, seed):
+    """n distinct random junction indices (sorted)."""
+    g = np.random.default_rng(seed)
+    junc = np.where(np.asarray(net.node_type) == JUNCTION)[0]
+    return np.sort(g.choice(junc, size=n, replace=False))

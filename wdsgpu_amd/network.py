@@ -1,0 +1,322 @@
+# -*- coding: utf-8 -*-
+"""Water-network data model in EPANET internal units (ft, cfs).
+
+The field names and meanings follow dgga's ``Net`` dataclass
+(dgga/parse.py:205-270) so that code written against the reference's network
+object reads the same here.  Only the fields the GGA hot path needs are kept.
+Arrays are in file order, 0-based.
+
+EPANET constants used by the solver live here too; their values are fixed by
+EPANET 2.2 (dgga/solver.py:34-56 quotes them from hydcoeffs.c/types.h) and
+therefore must be identical, not merely similar.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ---------------------------------------------------------------- constants
+# EPANET 2.2 values; dgga/solver.py:34-56.
+CSMALL = 1.0e-6
+CBIG = 1.0e8
+BIG = 1.0e10
+TINY = 1.0e-6
+MISSING = -1.0e10
+QZERO = 1.0e-6
+RQTOL_DEFAULT = 1.0e-7
+HEXP_HW = 1.852
+PI_EPANET = 3.141592654          # types.h fallback literal (dgga/solver.py:53)
+VISCOS_DEFAULT = 1.1e-5
+# Darcy-Weisbach friction-factor constants (dgga/solver.py:40-45)
+DW_A1 = 3.14159265358979323850e+03
+DW_A2 = 1.57079632679489661930e+03
+DW_A8 = 4.61841319859066668690e+00
+DW_A9 = -8.68588963806503655300e-01
+DW_AB = 3.28895476345399058690e-03
+DW_AC = -5.14214965799093883760e-03
+
+# unit factors (dgga/units.py:12-24)
+LPS_PER_CFS = 28.317
+GPM_PER_CFS = 448.831
+M_PER_FT = 0.3048
+PSI_PER_FT = 0.4333
+FLOW_UCF = {"LPS": LPS_PER_CFS, "GPM": GPM_PER_CFS, "CFS": 1.0,
+            "LPM": 1699.0, "MLD": 2.4466, "CMH": 101.94, "CMD": 2446.6,
+            "MGD": 0.64632, "IMGD": 0.5382, "AFD": 1.9837}
+SI_FLOW_UNITS = ("LPS", "LPM", "MLD", "CMH", "CMD")
+
+# link types (EN_LinkType; dgga/solver.py:72)
+CVPIPE, PIPE, PUMP, PRV, PSV, PBV, FCV, TCV, GPV = range(9)
+# node types
+JUNCTION, RESERVOIR, TANK = 0, 1, 2
+# internal status codes (dgga/solver.py:1278-1281)
+ST_XHEAD, ST_TEMPCLOSED, ST_CLOSED, ST_OPEN, ST_ACTIVE = 0, 1, 2, 3, 4
+ST_XFLOW, ST_XFCV, ST_XPRESSURE = 5, 6, 7
+# pump curve types (dgga/parse.py Net.pump_ptype)
+PUMP_CONST_HP, PUMP_POWER_FUNC, PUMP_CUSTOM, PUMP_NOCURVE = 0, 1, 2, 3
+
+
+@dataclass
+class Net:
+    """Network in EPANET internal units.  Mirrors dgga/parse.py:205-270."""
+
+    node_id: list
+    node_type: np.ndarray          # int8[N] 0 junction, 1 reservoir, 2 tank
+    elev_ft: np.ndarray            # float64[N] (reservoir: its fixed head)
+    node_ke: np.ndarray            # float64[N] emitter coefficient (internal)
+    demand_cfs: np.ndarray         # float64[N] base demand (cfs)
+    link_id: list
+    link_type: np.ndarray          # int8[L]
+    link_n1: np.ndarray            # int32[L]
+    link_n2: np.ndarray            # int32[L]
+    diam_ft: np.ndarray            # float64[L]
+    len_ft: np.ndarray             # float64[L]
+    roughness: np.ndarray          # float64[L] H-W C, or D-W Kc (ft)
+    km_int: np.ndarray             # float64[L] 0.02517*K/D^4
+    r_hw: np.ndarray               # float64[L] resistance coefficient
+    init_status: np.ndarray        # int8[L] 0 closed, 1 open, 2 active
+    valve_setting_user: np.ndarray  # float64[L] valve setting (user units) / pump speed
+    meta: dict = field(default_factory=dict)
+    # tanks (steady state: fixed-head nodes with level bounds)
+    tank_node: np.ndarray = None
+    tank_h0: np.ndarray = None
+    tank_hmin: np.ndarray = None
+    tank_hmax: np.ndarray = None
+    tank_area: np.ndarray = None
+    tank_overflow: np.ndarray = None
+    # pumps (file order of pump links; internal units)
+    pump_link: np.ndarray = None
+    pump_ptype: np.ndarray = None
+    pump_h0: np.ndarray = None
+    pump_r: np.ndarray = None
+    pump_n: np.ndarray = None
+    pump_q0: np.ndarray = None
+    pump_qmax: np.ndarray = None
+    pump_hmax: np.ndarray = None
+
+    def __post_init__(self):
+        empty = {"tank_node": np.int32, "tank_h0": np.float64,
+                 "tank_hmin": np.float64, "tank_hmax": np.float64,
+                 "tank_area": np.float64, "tank_overflow": np.int8,
+                 "pump_link": np.int32, "pump_ptype": np.int8,
+                 "pump_h0": np.float64, "pump_r": np.float64,
+                 "pump_n": np.float64, "pump_q0": np.float64,
+                 "pump_qmax": np.float64, "pump_hmax": np.float64}
+        for k, dt in empty.items():
+            if getattr(self, k) is None:
+                setattr(self, k, np.zeros(0, dtype=dt))
+        defaults = dict(flow_units="LPS", headloss="H-W", trials=200,
+                        accuracy=1e-3, demand_model="DDA", qexp=2.0,
+                        emitter_exponent=0.5, extra_iter=-1, checkfreq=2,
+                        maxcheck=10, damp_limit=0.0, spgrav=1.0,
+                        press_units="", htol=0.0005, qtol=0.0001,
+                        rqtol=RQTOL_DEFAULT, viscosity=VISCOS_DEFAULT)
+        for k, v in defaults.items():
+            self.meta.setdefault(k, v)
+
+    @property
+    def N(self):
+        return len(self.node_id)
+
+    @property
+    def L(self):
+        return len(self.link_id)
+
+    @property
+    def Nj(self):
+        return int(np.count_nonzero(np.asarray(self.node_type) == JUNCTION))
+
+    def reservoir_head_ft(self):
+        """float64[N]: fixed heads at reservoirs/tanks, nan at junctions
+        (the res_head_ft convention of dgga/solver.py:2703)."""
+        h = np.full(self.N, np.nan)
+        nt = np.asarray(self.node_type)
+        h[nt == RESERVOIR] = np.asarray(self.elev_ft)[nt == RESERVOIR]
+        if self.tank_node.size:
+            h[self.tank_node] = self.tank_h0
+        return h
+
+    def ucf_pressure(self):
+        """Ucf[PRESSURE] (dgga/solver.py:1291-1303)."""
+        spg = float(self.meta.get("spgrav", 1.0))
+        if str(self.meta["flow_units"]).upper() in SI_FLOW_UNITS:
+            press = str(self.meta.get("press_units", "") or "METERS").upper()
+            if press == "KPA":
+                return 6.895 * PSI_PER_FT * spg
+            return M_PER_FT * spg
+        return PSI_PER_FT * spg
+
+    def ucf_flow(self):
+        return FLOW_UCF[str(self.meta["flow_units"]).upper()]
+
+
+def hw_resistance(c, len_ft, diam_ft):
+    """Hazen-Williams resistance r = 4.727 L / C^1.852 / D^4.871 in EPANET
+    internal units; operation order of dgga/calib.py:64-86 (parse.py:603)."""
+    return 4.727 * np.asarray(len_ft) / np.asarray(c) ** 1.852 \
+        / np.asarray(diam_ft) ** 4.871
+
+
+def dw_resistance(len_ft, diam_ft):
+    """Darcy-Weisbach resistance factor (friction factor applied at solve
+    time): r = L / 2 / 32.2 / D / (pi D^2 / 4)^2 (dgga/parse.py:605-606)."""
+    d = np.asarray(diam_ft)
+    return np.asarray(len_ft) / 2.0 / 32.2 / d / ((PI_EPANET * (d * d) / 4.0) ** 2)
+
+
+def emitter_ke(c_user, flow_units="LPS", qexp=2.0, ucf_pressure=M_PER_FT):
+    """User emitter coefficient C -> internal Ke = (Ucf_Q^Qexp / Ucf_P) / C^Qexp
+    for C>0, else 0 (dgga/parse.py:557-568)."""
+    c = np.asarray(c_user, dtype=np.float64)
+    qcf = FLOW_UCF[flow_units]
+    out = np.zeros_like(c)
+    pos = c > 0.0
+    out[pos] = (qcf ** qexp / ucf_pressure) / c[pos] ** qexp
+    return out
+
+
+def emitter_ke_dC(c_user, ke, qexp=2.0):
+    """dKe/dC = -Qexp * Ke / C (0 where C==0)."""
+    c = np.asarray(c_user, dtype=np.float64)
+    out = np.zeros_like(c)
+    pos = c > 0.0
+    out[pos] = -qexp * np.asarray(ke)[pos] / c[pos]
+    return out
+
+
+# ---------------------------------------------------------------- INP reader
+def _sections(path):
+    sec = None
+    out = {}
+    with open(path, "r", encoding="utf-8", errors="replace") as f:
+        for raw in f:
+            line = raw.split(";", 1)[0].strip()
+            if not line:
+                continue
+            if line.startswith("["):
+                sec = line.upper()
+                out.setdefault(sec, [])
+                continue
+            if sec is not None:
+                out[sec].append(line.split())
+    return out
+
+
+def read_inp(path):
+    """Read the subset of EPANET INP text used by the GGA hot path:
+    [JUNCTIONS] [RESERVOIRS] [TANKS] [PIPES] [VALVES] [DEMANDS] [EMITTERS]
+    [STATUS] [OPTIONS].  Pumps and patterns are refused (NotImplementedError)
+    rather than silently dropped.  Unit conversion follows dgga/parse.py
+    (lengths and diameters to ft, flows to cfs, emitter C to Ke, H-W r)."""
+    s = _sections(path)
+    opts = {}
+    for tk in s.get("[OPTIONS]", []):
+        key = tk[0].upper()
+        if key in ("UNITS", "HEADLOSS") and len(tk) > 1:
+            opts[key] = tk[1].upper()
+        elif key in ("TRIALS", "ACCURACY", "CHECKFREQ", "MAXCHECK") and len(tk) > 1:
+            opts[key] = float(tk[1])
+        elif key == "EMITTER" and len(tk) > 2:
+            opts["EMITTER"] = float(tk[2])
+    units = opts.get("UNITS", "GPM")
+    si = units in SI_FLOW_UNITS
+    qcf = FLOW_UCF[units]
+    hcf = M_PER_FT if si else 1.0
+    dcf = 1000.0 * M_PER_FT if si else 12.0
+    headloss = opts.get("HEADLOSS", "H-W")
+    if s.get("[PUMPS]"):
+        raise NotImplementedError(f"{path}: [PUMPS] are not read by read_inp")
+    node_id, node_type, elev = [], [], []
+    for tk in s.get("[JUNCTIONS]", []):
+        node_id.append(tk[0]); node_type.append(JUNCTION); elev.append(float(tk[1]) / hcf)
+    for tk in s.get("[RESERVOIRS]", []):
+        node_id.append(tk[0]); node_type.append(RESERVOIR); elev.append(float(tk[1]) / hcf)
+    tanks = []
+    for tk in s.get("[TANKS]", []):
+        node_id.append(tk[0]); node_type.append(TANK); elev.append(float(tk[1]) / hcf)
+        tanks.append(tk)
+    idx = {n: i for i, n in enumerate(node_id)}
+    N = len(node_id)
+    demand = np.zeros(N)
+    dem_rows = s.get("[DEMANDS]", [])
+    if dem_rows:
+        for tk in dem_rows:
+            demand[idx[tk[0]]] += float(tk[1]) / qcf
+    else:
+        for tk in s.get("[JUNCTIONS]", []):
+            if len(tk) > 2:
+                demand[idx[tk[0]]] += float(tk[2]) / qcf
+    qexp = 1.0 / opts.get("EMITTER", 0.5)
+    ke = np.zeros(N)
+    pcf = M_PER_FT if si else PSI_PER_FT
+    for tk in s.get("[EMITTERS]", []):
+        i = idx[tk[0]]
+        c = float(tk[1])
+        if node_type[i] == JUNCTION and c > 0:
+            ke[i] = (qcf ** qexp / pcf) / c ** qexp
+    link_id, lt, n1, n2, dia, ln, rough, km, st, setting = ([] for _ in range(10))
+    for tk in s.get("[PIPES]", []):
+        link_id.append(tk[0]); n1.append(idx[tk[1]]); n2.append(idx[tk[2]])
+        ln.append(float(tk[3]) / hcf); dia.append(float(tk[4]) / dcf)
+        rough.append(float(tk[5]) if headloss != "D-W" else float(tk[5]) / (1000.0 * hcf))
+        mlk = float(tk[6]) if len(tk) > 6 else 0.0
+        km.append(0.02517 * mlk / dia[-1] ** 4)
+        status = tk[7].upper() if len(tk) > 7 else "OPEN"
+        lt.append(CVPIPE if status == "CV" else PIPE)
+        st.append(0 if status == "CLOSED" else 1)
+        setting.append(0.0)
+    vt = {"PRV": PRV, "PSV": PSV, "PBV": PBV, "FCV": FCV, "TCV": TCV, "GPV": GPV}
+    for tk in s.get("[VALVES]", []):
+        link_id.append(tk[0]); n1.append(idx[tk[1]]); n2.append(idx[tk[2]])
+        dia.append(float(tk[3]) / dcf); ln.append(0.0); rough.append(0.0)
+        lt.append(vt[tk[4].upper()]); setting.append(float(tk[5]))
+        mlk = float(tk[6]) if len(tk) > 6 else 0.0
+        km.append(0.02517 * mlk / dia[-1] ** 4)
+        st.append(2)
+    lidx = {n: i for i, n in enumerate(link_id)}
+    for tk in s.get("[STATUS]", []):
+        k = lidx[tk[0]]
+        word = tk[1].upper()
+        if word == "CLOSED":
+            st[k] = 0
+        elif word == "OPEN":
+            st[k] = 1
+    ln_a, dia_a = np.asarray(ln), np.asarray(dia)
+    rough_a = np.asarray(rough)
+    lt_a = np.asarray(lt, dtype=np.int8)
+    r = np.zeros(len(link_id))
+    pipe = lt_a <= PIPE
+    if headloss == "H-W":
+        r[pipe] = hw_resistance(rough_a[pipe], ln_a[pipe], dia_a[pipe])
+    elif headloss == "D-W":
+        r[pipe] = dw_resistance(ln_a[pipe], dia_a[pipe])
+    else:
+        raise NotImplementedError(f"headloss {headloss}")
+    meta = dict(flow_units=units, headloss=headloss,
+                trials=int(opts.get("TRIALS", 200)),
+                accuracy=float(opts.get("ACCURACY", 1e-3)), qexp=qexp,
+                emitter_exponent=1.0 / qexp)
+    if "CHECKFREQ" in opts:
+        meta["checkfreq"] = int(opts["CHECKFREQ"])
+    if "MAXCHECK" in opts:
+        meta["maxcheck"] = int(opts["MAXCHECK"])
+    net = Net(node_id=node_id, node_type=np.asarray(node_type, dtype=np.int8),
+              elev_ft=np.asarray(elev), node_ke=ke, demand_cfs=demand,
+              link_id=link_id, link_type=lt_a,
+              link_n1=np.asarray(n1, dtype=np.int32),
+              link_n2=np.asarray(n2, dtype=np.int32), diam_ft=dia_a,
+              len_ft=ln_a, roughness=rough_a, km_int=np.asarray(km), r_hw=r,
+              init_status=np.asarray(st, dtype=np.int8),
+              valve_setting_user=np.asarray(setting), meta=meta)
+    if tanks:
+        tn = np.asarray([idx[t[0]] for t in tanks], dtype=np.int32)
+        el = net.elev_ft[tn]
+        net.tank_node = tn
+        net.tank_h0 = el + np.asarray([float(t[2]) for t in tanks]) / hcf
+        net.tank_hmin = el + np.asarray([float(t[3]) for t in tanks]) / hcf
+        net.tank_hmax = el + np.asarray([float(t[4]) for t in tanks]) / hcf
+        dd = np.asarray([float(t[5]) for t in tanks]) / hcf
+        net.tank_area = PI_EPANET * dd * dd / 4.0
+        net.tank_overflow = np.zeros(len(tanks), dtype=np.int8)
+    return net

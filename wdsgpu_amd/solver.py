@@ -1,0 +1,390 @@
+# -*- coding: utf-8 -*-
+"""GGASolver: the batched GGA forward solve on MI355X.
+
+Mirrors dgga's ``GGASolver`` constructor and ``solve`` keyword API
+(dgga/solver.py:160-162, 2703-2706) so that code written against the
+reference's dense batched path can switch to this class.  All numerics run
+in libwds.so (hand-written HIP kernels, one wavefront per scenario); this
+module only prepares the static network description, allocates torch
+tensors and maps status codes to the reference's exceptions.
+
+Differences from the reference, all refusals rather than approximations:
+  * mode must be "dense" (the HIP path is the batched path; the scalar EPANET
+    replica mode="epanet" is not part of it);
+  * linear_solver "dense"/"hip" and assemble "dense"/"csr" all run the HIP
+    sparse LDL^T path; "cudss" is CUDA-only and refused;
+  * float64 only, CUDA/HIP device only.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from .network import (CSMALL, CBIG, BIG, TINY, MISSING, QZERO, HEXP_HW, PI_EPANET,
+                      CVPIPE, PIPE, PUMP, PRV, PSV, PBV, FCV, TCV, JUNCTION,
+                      ST_CLOSED, ST_OPEN, ST_ACTIVE, PUMP_CONST_HP, PUMP_CUSTOM)
+
+
+def _i32(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.int32))
+
+
+def _f64(a):
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64))
+
+
+class GGASolver:
+    """Batched GGA forward solver (dgga/solver.py:140-495 interface)."""
+
+    def __init__(self, net, device="cuda", dtype=torch.float64, mode="dense",
+                 inp_path=None, dense_tank_bound_check=True,
+                 dense_status_machine=False, ordering="mmd"):
+        if mode != "dense":
+            raise NotImplementedError(
+                "mode=%r: the MI355X path is the batched dense-mode solver "
+                "(mode='dense'); the scalar EPANET replica is not on it" % (mode,))
+        if dtype != torch.float64:
+            raise NotImplementedError("only float64 is supported (all arithmetic is FP64)")
+        self.net = net
+        self.mode = mode
+        self.device = torch.device(device)
+        self.dtype = dtype
+        self.dense_status_machine = bool(dense_status_machine)
+        self.dense_refine = 2
+        lt = np.asarray(net.link_type, dtype=np.int64)
+        nt = np.asarray(net.node_type, dtype=np.int64)
+        self.N, self.L = net.N, net.L
+        self.lt_np = lt
+        allowed = {CVPIPE, PIPE, TCV, PUMP, PRV} if self.dense_status_machine \
+            else {PIPE, TCV, PUMP}
+        bad = set(lt.tolist()) - allowed
+        if bad:
+            raise NotImplementedError(
+                f"mode=dense supports link_type in {sorted(allowed)}, got {sorted(bad)}")
+        self.headloss_form = str(net.meta.get("headloss", "H-W")).upper()
+        if self.headloss_form not in ("H-W", "D-W"):
+            raise NotImplementedError(f"headloss {self.headloss_form}")
+        if str(net.meta.get("demand_model", "DDA")).upper() != "DDA":
+            raise NotImplementedError("only demand-driven analysis (DDA)")
+        if float(net.meta.get("damp_limit", 0.0)) > 0.0:
+            raise NotImplementedError("DAMPLIMIT > 0 is not supported")
+        self.junc_nodes = np.where(nt == JUNCTION)[0]
+        self.fixed_nodes = np.where(nt != JUNCTION)[0]
+        self.Nj = int(self.junc_nodes.size)
+        self.hexp = HEXP_HW if self.headloss_form == "H-W" else 2.0
+        self.qexp = float(net.meta["qexp"])
+        self.rqtol = float(net.meta.get("rqtol", 1e-7))
+        self.max_iter_default = int(net.meta["trials"])
+        self.hacc_default = min(max(float(net.meta["accuracy"]), 1e-5), 1e-1)
+        self.htol = float(net.meta.get("htol", 0.0005))
+        self.qtol = float(net.meta.get("qtol", 0.0001))
+        self.checkfreq = int(net.meta.get("checkfreq", 2))
+        self.maxcheck = int(net.meta.get("maxcheck", 10))
+        self.extra_iter = int(net.meta.get("extra_iter", -1))
+        # ---- internal status codes and settings (dgga/solver.py:1277-1319) ----
+        st = np.asarray(net.init_status)
+        map_init = {0: ST_CLOSED, 1: ST_OPEN, 2: ST_ACTIVE}
+        self.init_status_int = np.asarray([map_init[int(s)] for s in st], dtype=np.int8)
+        setting = _f64(net.valve_setting_user).copy()
+        ucf_p, ucf_q = net.ucf_pressure(), net.ucf_flow()
+        init_setting = setting.copy()
+        for k in np.where(lt >= PRV)[0]:
+            if self.init_status_int[k] != ST_ACTIVE:
+                init_setting[k] = MISSING
+            elif lt[k] in (PRV, PSV, PBV):
+                init_setting[k] = init_setting[k] / ucf_p
+            elif lt[k] == FCV:
+                init_setting[k] = init_setting[k] / ucf_q
+        self.init_setting = init_setting
+        diam = _f64(net.diam_ft)
+        d4 = diam ** 2 * diam ** 2
+        km = _f64(net.km_int)
+        km_tcv = np.where(st == 2, np.divide(0.02517 * setting, d4, out=np.zeros_like(setting),
+                                             where=d4 > 0), km)
+        # ---- pumps (dgga/solver.py:1327-1348, 950-1011) ----
+        pump_kind = np.full(self.L, -1, dtype=np.int32)
+        ph0, pr, pn, phm, pq0 = (np.zeros(self.L) for _ in range(5))
+        pn[:] = 1.0
+        self.n_pumps = int(len(net.pump_link))
+        if self.n_pumps:
+            pl = np.asarray(net.pump_link, dtype=np.int64)
+            kinds = np.asarray(net.pump_ptype, dtype=np.int64)
+            if (kinds == PUMP_CUSTOM).any():
+                raise NotImplementedError("CUSTOM multi-point pump curves")
+            pump_kind[pl] = kinds
+            ph0[pl] = net.pump_h0
+            pr[pl] = net.pump_r
+            nn = np.asarray(net.pump_n, dtype=np.float64).copy()
+            nn[np.abs(nn - 1.0) < TINY] = 1.0
+            pn[pl] = nn
+            sp = init_setting[pl]
+            phm[pl] = np.where(kinds == PUMP_CONST_HP, BIG, sp * sp * np.asarray(net.pump_hmax))
+            pq0[pl] = net.pump_q0
+        closed = st == 0
+        q0 = PI_EPANET * diam ** 2 / 4.0
+        if self.n_pumps:
+            q0 = np.where(lt == PUMP, init_setting * pq0, q0)
+        q0 = np.where(closed, QZERO, q0)
+        self.n_tanks = int(len(net.tank_node))
+        self._keep = dict(
+            node_type=_i32(nt), elev=_f64(net.elev_ft), n1=_i32(net.link_n1),
+            n2=_i32(net.link_n2), lt=_i32(lt), st=_i32(self.init_status_int), diam=diam,
+            km=km, km_tcv=_f64(km_tcv), km_valve=km.copy(), kc=_f64(net.roughness),
+            q0=_f64(q0), setting=_f64(init_setting), pump_kind=pump_kind, ph0=_f64(ph0),
+            pr=_f64(pr), pn=_f64(pn), phm=_f64(phm),
+            tank_node=_i32(net.tank_node), tank_hmin=_f64(net.tank_hmin),
+            tank_hmax=_f64(net.tank_hmax), tank_ovf=_i32(net.tank_overflow))
+        k = self._keep
+        P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
+        desc = _lib.NetworkDesc(
+            N=self.N, L=self.L, node_type=P(k["node_type"], C.c_int32),
+            elev=P(k["elev"], C.c_double), link_n1=P(k["n1"], C.c_int32),
+            link_n2=P(k["n2"], C.c_int32), link_type=P(k["lt"], C.c_int32),
+            init_status=P(k["st"], C.c_int32), diam=P(k["diam"], C.c_double),
+            km=P(k["km"], C.c_double), km_tcv=P(k["km_tcv"], C.c_double),
+            km_valve=P(k["km_valve"], C.c_double), kc=P(k["kc"], C.c_double),
+            q0=P(k["q0"], C.c_double), setting=P(k["setting"], C.c_double),
+            pump_kind=P(k["pump_kind"], C.c_int32), pump_h0=P(k["ph0"], C.c_double),
+            pump_r=P(k["pr"], C.c_double), pump_n=P(k["pn"], C.c_double),
+            pump_hmax=P(k["phm"], C.c_double), ntank=self.n_tanks,
+            tank_node=P(k["tank_node"], C.c_int32), tank_hmin=P(k["tank_hmin"], C.c_double),
+            tank_hmax=P(k["tank_hmax"], C.c_double),
+            tank_overflow=P(k["tank_ovf"], C.c_int32),
+            headloss=0 if self.headloss_form == "H-W" else 1, hexp=self.hexp,
+            qexp=self.qexp, rqtol=self.rqtol,
+            viscos=float(net.meta.get("viscosity", 1.1e-5)), htol=self.htol,
+            qtol=self.qtol, checkfreq=self.checkfreq, maxcheck=self.maxcheck,
+            extra_iter=self.extra_iter)
+        self._lib = _lib.load()
+        h = C.c_void_p()
+        order = {"mmd": 0, "natural": 1}[ordering]
+        _lib.check(self._lib.wds_plan_create(C.byref(desc), order, C.byref(h)),
+                   "wds_plan_create")
+        self._plan = h
+        st_ = _lib.PlanStats()
+        _lib.check(self._lib.wds_plan_stats_get(self._plan, C.byref(st_)), "wds_plan_stats_get")
+        self.stats = {f: getattr(st_, f) for f, _ in _lib.PlanStats._fields_}
+        self.n_pcv = int(self.stats["npcv"])
+        self.has_cv = bool((lt == CVPIPE).any())
+        # tensors used by the autograd wrapper
+        self.r_hw = torch.as_tensor(_f64(net.r_hw), dtype=dtype, device=self.device)
+        self.node_ke_default = torch.as_tensor(_f64(net.node_ke), dtype=dtype,
+                                               device=self.device)
+        self.junc_nodes_t = torch.as_tensor(self.junc_nodes, dtype=torch.int64,
+                                            device=self.device)
+        self.prv_links = np.where((lt == PRV) & (init_setting != MISSING))[0]
+        n1, n2 = np.asarray(net.link_n1), np.asarray(net.link_n2)
+        self.prv_n1, self.prv_n2 = n1[self.prv_links], n2[self.prv_links]
+        self._uploaded = False
+
+    def __del__(self):
+        h = getattr(self, "_plan", None)
+        if h is not None and getattr(self, "_lib", None) is not None:
+            self._lib.wds_plan_destroy(h)
+            self._plan = None
+
+    # ------------------------------------------------------------------
+    def _stream(self):
+        return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def _workspace(self, B):
+        nbytes = int(_lib.check(self._lib.wds_workspace_bytes(self._plan, int(B)),
+                                "wds_workspace_bytes"))
+        return torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
+
+    def counters(self, reset=False):
+        """cudss_counters equivalent (dgga/solver.py:2580-2593)."""
+        c = _lib.Counters()
+        _lib.check(self._lib.wds_counters_get(self._plan, C.byref(c), int(bool(reset))),
+                   "wds_counters_get")
+        return {f: int(getattr(c, f)) for f, _ in _lib.Counters._fields_}
+
+    def plan_export(self, what):
+        n = int(self._lib.wds_plan_export(self._plan, int(what), None, 0))
+        buf = np.zeros(max(n, 1), dtype=np.int32)
+        self._lib.wds_plan_export(self._plan, int(what),
+                                  buf.ctypes.data_as(C.POINTER(C.c_int32)), n)
+        return buf[:n]
+
+    def _prep(self, x, B=None, width=None):
+        t = x.to(dtype=self.dtype, device=self.device) if isinstance(x, torch.Tensor) \
+            else torch.as_tensor(np.asarray(x), dtype=self.dtype, device=self.device)
+        return t
+
+    def maxtrials(self, max_iter, status_machine):
+        m = int(max_iter)
+        if status_machine and self.extra_iter > 0:
+            m += self.extra_iter
+        return m
+
+    # ------------------------------------------------------------------
+    def factor_solve(self, csr_data, F, n_refine=2, want_factor=False):
+        """Batched factor + refined solve on the reference's CSR values
+        ([B,nnz] in the CSR pattern of dgga/solver.py:336-369) and right-hand
+        sides F [B,Nj]; returns H [B,Nj] (and the factor when asked)."""
+        csr = self._prep(csr_data).contiguous()
+        Fv = self._prep(F).contiguous()
+        B = int(Fv.shape[0])
+        if csr.shape != (B, int(self.stats["nnz_csr"])) or Fv.shape != (B, self.Nj):
+            raise ValueError("factor_solve: shapes %s %s" % (tuple(csr.shape), tuple(Fv.shape)))
+        H = torch.empty_like(Fv)
+        info = torch.zeros(B, dtype=torch.int32, device=self.device)
+        fac = torch.empty(B, int(self.stats["nnzL"]), dtype=self.dtype, device=self.device) \
+            if want_factor else None
+        ws = self._workspace(B)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_factor_solve(
+                self._plan, self._stream(), B, _lib.ptr(csr), _lib.ptr(Fv), _lib.ptr(H),
+                int(n_refine), _lib.ptr(fac), _lib.ptr(info), _lib.ptr(ws)), "wds_factor_solve")
+        bad = torch.nonzero(info != 0).flatten().tolist()
+        if bad:
+            raise RuntimeError("batched LDL^T failed: samples %s are not positive definite" % bad)
+        return (H, fac) if want_factor else H
+
+    # ------------------------------------------------------------------
+    def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
+              status_machine, r_hw=None, polish_steps=-1, want_factor=False):
+        d = self._prep(demand_cfs)
+        rh = self._prep(res_head_ft)
+        single = d.dim() == 1
+        if single:
+            d = d.unsqueeze(0)
+        if rh.dim() == 1:
+            rh = rh.unsqueeze(0)
+        B = int(d.shape[0])
+        if rh.shape[0] == 1 and B > 1:
+            rh = rh.expand(B, -1)
+        if d.shape[1] != self.N or rh.shape != (B, self.N):
+            raise ValueError("demand/res_head must be [N] or [B,N] with N=%d" % self.N)
+        rh = torch.nan_to_num(rh).contiguous()
+        rh_fix = res_head_ft if not isinstance(res_head_ft, torch.Tensor) else None
+        ke = self.node_ke_default if ke_int is None else self._prep(ke_int)
+        if ke.dim() == 2 and ke.shape[0] == 1:
+            ke = ke[0]
+        ke = ke.contiguous()
+        ke_stride = self.N if ke.dim() == 2 else 0
+        if ke.dim() == 2 and ke.shape[0] != B:
+            raise ValueError("ke must be [N] or [B,N]")
+        r = self.r_hw if r_hw is None else self._prep(r_hw)
+        if r.dim() == 2 and r.shape[0] == 1:
+            r = r[0]
+        r = r.contiguous()
+        r_stride = self.L if r.dim() == 2 else 0
+        dev = self.device
+        out = dict(
+            head=torch.empty(B, self.N, dtype=self.dtype, device=dev),
+            flow=torch.empty(B, self.L, dtype=self.dtype, device=dev),
+            emitter=torch.empty(B, self.N, dtype=self.dtype, device=dev),
+            iters=torch.zeros(B, dtype=torch.int32, device=dev),
+            relerr=torch.zeros(B, dtype=self.dtype, device=dev),
+            status=torch.empty(B, self.L, dtype=torch.int8, device=dev),
+            diag_ratio=torch.zeros(B, dtype=self.dtype, device=dev),
+            info=torch.zeros(B, dtype=torch.int32, device=dev),
+            resid=torch.zeros(B, dtype=self.dtype, device=dev),
+            factor=torch.empty(B, int(self.stats["nnzL"]), dtype=self.dtype, device=dev)
+            if want_factor else None)
+        q0t = None if q0 is None else self._prep(q0).expand(B, -1).contiguous()
+        e0t = None if e0 is None else self._prep(e0).expand(B, -1).contiguous()
+        ws = self._workspace(B)
+        keep = (d.contiguous(), rh, ke, r, q0t, e0t, ws)
+        a = _lib.SolveArgs(
+            B=B, demand=_lib.ptr(keep[0]), res_head=_lib.ptr(rh), ke=_lib.ptr(ke),
+            ke_stride=ke_stride, r_hw=_lib.ptr(r), r_stride=r_stride, q0=_lib.ptr(q0t),
+            e0=_lib.ptr(e0t), max_iter=int(max_iter), accuracy=float(accuracy),
+            status_machine=int(bool(status_machine)), n_refine=int(self.dense_refine),
+            polish_steps=int(polish_steps), head=_lib.ptr(out["head"]),
+            flow=_lib.ptr(out["flow"]), emitter=_lib.ptr(out["emitter"]),
+            iters=_lib.ptr(out["iters"]), relerr=_lib.ptr(out["relerr"]),
+            status=_lib.ptr(out["status"]), diag_ratio=_lib.ptr(out["diag_ratio"]),
+            info=_lib.ptr(out["info"]), resid=_lib.ptr(out["resid"]),
+            factor=_lib.ptr(out["factor"]), workspace=_lib.ptr(ws))
+        return a, out, keep, single, B
+
+    def _validate(self, assemble, linear_solver, status_machine, link_status, link_setting):
+        if assemble not in ("dense", "csr"):
+            raise ValueError(f"assemble must be 'dense' or 'csr', got {assemble!r}")
+        if linear_solver == "cudss":
+            raise NotImplementedError("linear_solver='cudss' is CUDA-only; use 'hip'")
+        if linear_solver not in ("dense", "hip"):
+            raise ValueError(f"linear_solver must be 'dense' or 'hip', got {linear_solver!r}")
+        if not self.dense_status_machine and (link_status is not None or
+                                              link_setting is not None or status_machine):
+            raise NotImplementedError(
+                "link_status/link_setting/status_machine need dense_status_machine=True")
+        if link_status is not None or link_setting is not None:
+            raise NotImplementedError("link_status/link_setting warm starts are not supported")
+        if self.has_cv and not status_machine:
+            raise NotImplementedError("networks with CV pipes need status_machine=True")
+        if self.n_pcv and not status_machine:
+            raise NotImplementedError("networks with PRVs need status_machine=True")
+        if (self.n_pumps or self.n_tanks) and not status_machine:
+            raise NotImplementedError(
+                "networks with pumps or tanks need status_machine=True on this path "
+                "(pumpstatus/tankstatus run inside the batched status machine)")
+        if not self.device.type == "cuda":
+            raise NotImplementedError("the solver runs on a HIP device only")
+
+    def solve(self, demand_cfs, res_head_ft, ke_int=None, q0=None, e0=None,
+              max_iter=None, accuracy=None, link_status=None, link_setting=None,
+              status_machine=False, assemble="csr", linear_solver="hip",
+              record_schedule=False, r_hw=None):
+        """Batched steady-state solve (dgga/solver.py:2703-3212).  Returns the
+        reference's dict: head_ft [B,N], flow_cfs [B,L], emitter_cfs [B,N],
+        iters [B], relerr [B], converged [B], status [B,L] (status machine),
+        diag_ratio [B] (networks with PRVs).  1-D inputs give 1-D outputs."""
+        if record_schedule:
+            raise NotImplementedError("record_schedule is not recorded on the fused path")
+        self._validate(assemble, linear_solver, status_machine, link_status, link_setting)
+        max_iter = self.max_iter_default if max_iter is None else int(max_iter)
+        hacc = self.hacc_default if accuracy is None else float(accuracy)
+        a, out, keep, single, B = self._args(demand_cfs, res_head_ft, ke_int, q0, e0,
+                                             max_iter, hacc, status_machine, r_hw=r_hw)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_solve(self._plan, self._stream(), C.byref(a)), "wds_solve")
+        return self._finish(out, single, max_iter, status_machine)
+
+    def _finish(self, out, single, max_iter, status_machine):
+        info = out["info"]
+        bad = torch.nonzero(info > 0).flatten().tolist()
+        if bad:
+            its = info[bad].tolist()
+            raise RuntimeError(
+                "batched LDL^T failed: samples %s have a matrix that is not positive "
+                "definite (iterations %s); EPANET would apply badvalve, the batched "
+                "path has no counterpart (dgga/solver.py:3043-3051)" % (bad, its))
+        maxtrials = self.maxtrials(max_iter, status_machine)
+        iters = out["iters"].to(torch.int64)
+        res = dict(head_ft=out["head"], flow_cfs=out["flow"], emitter_cfs=out["emitter"],
+                   iters=iters, relerr=out["relerr"], converged=iters <= maxtrials)
+        if status_machine:
+            res["status"] = out["status"]
+        if self.n_pcv:
+            res["diag_ratio"] = out["diag_ratio"]
+        if single:
+            res = {k: v[0] for k, v in res.items()}
+        return res
+
+    # ------------------------------------------------------------------
+    def solve_unfused(self, demand_cfs, res_head_ft, ke_int=None, max_iter=None,
+                      accuracy=None, status_machine=False, sync_every=1):
+        """Plain per-iteration path: one kernel launch per Newton iteration
+        with the state in global memory and a host-side convergence check
+        after every ``sync_every`` launches (the reference's host-driven loop,
+        dgga/solver.py:2922-3179)."""
+        self._validate("csr", "hip", status_machine, None, None)
+        max_iter = self.max_iter_default if max_iter is None else int(max_iter)
+        hacc = self.hacc_default if accuracy is None else float(accuracy)
+        a, out, keep, single, B = self._args(demand_cfs, res_head_ft, ke_int, None, None,
+                                             max_iter, hacc, status_machine)
+        ctrl = torch.zeros(B, 4, dtype=torch.int32, device=self.device)
+        maxtrials = self.maxtrials(max_iter, status_machine)
+        with torch.cuda.device(self.device):
+            for it in range(1, maxtrials + 1):
+                _lib.check(self._lib.wds_newton_iter(self._plan, self._stream(), C.byref(a),
+                                                     it, _lib.ptr(ctrl)), "wds_newton_iter")
+                if it % sync_every == 0 and not bool(ctrl[:, 0].any()):
+                    break
+        return self._finish(out, single, max_iter, status_machine)
