@@ -1,0 +1,181 @@
+# -*- coding: utf-8 -*-
+"""Forward + implicit-adjoint throughput of the batched GGA on one MI355X.
+
+A step = one forward solve (fused Newton loop, polish, terminal factor) plus
+the implicit adjoint of the calibration loss for B = 1024 scenarios of the
+782-junction planar network (synthetic, seeded).  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+import torch
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "fwd+adjoint scenarios/s @B=1024 vs plain per-scenario GGA; grad rel err ≤1e-6"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--B", type=int, default=1024)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain-baseline", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=256)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: scenarios are independent and this build runs "
+              "on a single MI355X (--gpus 1); multi-GPU runs are not part of it.",
+              file=sys.stderr)
+        sys.exit(2)
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    from wdsgpu_amd.roofline import adjoint_bytes, forward_bytes
+    from wdsgpu_amd.workloads import Calibration
+
+    dev = torch.device("cuda:0")
+    B = args.B
+    log("building workload B=%d" % B)
+    wl = Calibration(B=B, device=dev)
+    s = wl.solver
+    st = s.stats
+    ne = int(np.count_nonzero(wl.net.node_ke > 0))
+    for _ in range(args.warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    s.event_log = []
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, gd, grh, gC, gCe = wl.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    value = B * args.steps / (t1 - t0)
+    # kernel durations from HIP events recorded on the launching stream
+    fw = [(e0.elapsed_time(e1), b, it) for (n, e0, e1, b, it) in s.event_log if n == "k_solve"]
+    bw = [(e0.elapsed_time(e1), b) for (n, e0, e1, b, it) in s.event_log if n == "k_adjoint"]
+    s.event_log = None
+    fw_ms = float(np.mean([x[0] for x in fw]))
+    bw_ms = float(np.mean([x[0] for x in bw]))
+    iters_sum = float(np.mean([int(x[2].sum()) for x in fw]))
+    fbytes = forward_bytes(st, iters_sum, wl.polish_steps, B, ne)
+    abytes = adjoint_bytes(st["L"], st["Nj"], st["nnz_lower"], st["nnzL"], ne,
+                           st["N"] - st["Nj"]) * B
+    achieved = fbytes / (fw_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "k_solve", "kernel_ms": round(fw_ms, 4),
+                "algorithmic_bytes": int(fbytes),
+                "adjoint_kernel_ms": round(bw_ms, 4),
+                "adjoint_achieved_GBps": round(abytes / (bw_ms * 1e-3) / 1e9, 1),
+                "mean_newton_iters": round(iters_sum / B, 3)}
+    out = {"metric": METRIC, "value": round(value, 1), "unit": "scenarios/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded planar network, lognormal demand scenarios)",
+           "config": {"workload": "planar782 calibration: forward+implicit adjoint, "
+                                  "30 sensors, grads wrt demands/C/reservoir heads/leak C",
+                      "B": B, "Nj": st["Nj"], "L": st["L"], "nnzL": st["nnzL"],
+                      "prvs": st["npcv"], "emitters": ne, "accuracy": wl.accuracy,
+                      "polish_steps": wl.polish_steps},
+           "roofline": roofline}
+    # ---- plain per-scenario GPU GGA with finite-difference gradients ----
+    if not args.no_plain_baseline:
+        out["plain_fd_baseline"] = plain_fd(wl)
+        if out["plain_fd_baseline"]["value"] > 0:
+            out["speedup_vs_plain_fd"] = round(value / out["plain_fd_baseline"]["value"], 1)
+    # ---- CPU baseline: the oracle (reference CPU path restated) ----
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sample)
+    print(json.dumps(out), flush=True)
+
+
+def plain_fd(wl, n_coords=80):
+    """Plain per-scenario GPU GGA: the unfused kernels (one launch per Newton
+    iteration, host convergence check every iteration) at B=1, with central
+    finite-difference gradients: 2P+1 forward solves per gradient scenario,
+    P = 782 demands + 905 C + 2 reservoir heads + 10 leak C.  80 coordinates
+    are timed for real; the per-scenario time is extrapolated linearly to P."""
+    s = wl.solver
+    net = wl.net
+    P = wl.solver.Nj + int((np.asarray(net.link_type) == 1).sum()) + \
+        len(s.fixed_nodes) + int(np.count_nonzero(net.node_ke > 0))
+    d = wl.d0[0].clone()
+    rh = wl.rh0.clone()
+    ke = wl.ke_of(wl.Ce0)
+    r = wl.r_of(wl.C0)
+    g = np.random.default_rng(5)
+    coords = g.choice(s.junc_nodes, n_coords, replace=False)
+
+    def fwd(dd):
+        o = s.solve_unfused(dd, rh, ke_int=ke, accuracy=wl.accuracy, status_machine=True)
+        return o["head_ft"]
+    fwd(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for j in coords:
+        h = 1e-6 * float(d[j])
+        dp = d.clone()
+        dp[j] += h
+        dm = d.clone()
+        dm[j] -= h
+        fwd(dp)
+        fwd(dm)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    per_solve = t / (2 * n_coords)
+    per_grad_scenario = per_solve * (2 * P + 1)
+    return {"value": round(1.0 / per_grad_scenario, 4), "unit": "gradient-scenarios/s",
+            "forward_ms_B1": round(per_solve * 1e3, 3), "P": int(P),
+            "coords_timed": int(n_coords), "extrapolated_to_P": True}
+
+
+def cpu_baseline(wl, n):
+    """The oracle (float64 numpy/scipy restatement of the reference's CPU
+    path: dense GGA + SuperLU polish + full-Jacobian adjoint), one thread,
+    on the first n scenarios of the same workload."""
+    from threadpoolctl import threadpool_limits
+
+    from oracle import adjoint as adj
+    from oracle import gga
+    m = gga.GGAModel(wl.net)
+    d = wl.d0[:n].cpu().numpy()
+    rh = np.broadcast_to(wl.rh0.cpu().numpy(), (n, wl.net.N)).copy()
+    ke = np.broadcast_to(wl.ke_of(wl.Ce0).cpu().numpy(), (n, wl.net.N)).copy()
+    r = wl.r_of(wl.C0).detach().cpu().numpy()
+    sens = wl.sensors.cpu().numpy()
+    with threadpool_limits(1):
+        t0 = time.perf_counter()
+        sol = adj.solve_polished(m, d, rh, ke, r, accuracy=wl.accuracy, polish_steps=3)
+        gH = np.zeros((n, wl.net.N))
+        Ht = wl.H_true[:n].cpu().numpy()
+        gH[:, sens] = 2.0 * (sol["head"][:, sens] - Ht)
+        adj.cpu_adjoint(m, sol, gH, np.zeros((n, wl.net.L)), np.zeros((n, wl.net.N)))
+        t = time.perf_counter() - t0
+    return {"value": round(n / t, 3), "unit": "scenarios/s", "cores": 1, "kind": "port",
+            "sample": "%d scenarios of the B=1024 workload (forward + polish + adjoint), "
+                      "single thread" % n, "seconds": round(t, 2),
+            "host_cpus": os.cpu_count()}
+
+
+if __name__ == "__main__":
+    main()

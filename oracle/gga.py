@@ -230,11 +230,24 @@ class GGAModel:
             q = np.where(self.is_pump, self.init_setting * self.pl_q0, q)
         return np.where(self.closed, QZERO, q)
 
+    # When set to a numpy Generator, every power-law evaluation is perturbed
+    # by -1, 0 or +1 ulp at random: the oracle's own sensitivity to one
+    # rounding of the transcendental (the reference's self-jitter acceptance
+    # practice, data/sparse_gpu_plan.md:404-445).
+    pow_jitter = None
+
+    def _pow(self, x, y):
+        p = x ** y
+        if self.pow_jitter is not None:
+            u = self.pow_jitter.integers(-1, 2, size=np.shape(p))
+            p = p * (1.0 + u * 2.0 ** -52)
+        return p
+
     def pipe_PY(self, q, r_hw):
         """H-W pipe coefficients, dgga/solver.py:2176-2201."""
         sgn = np.where(q < 0, -1.0, 1.0)
         qa = q * sgn
-        hgrad = self.hexp * r_hw * np.maximum(qa, 1e-30) ** (self.hexp - 1.0)
+        hgrad = self.hexp * r_hw * self._pow(np.maximum(qa, 1e-30), self.hexp - 1.0)
         lin = hgrad < self.rqtol
         hgrad = np.where(lin, self.rqtol, hgrad)
         hloss = np.where(lin, hgrad * qa, hgrad * qa / self.hexp)
@@ -351,7 +364,7 @@ class GGAModel:
     def emitter_hloss(self, e, ke):
         """dgga/solver.py:2368-2377."""
         ke_adj = np.maximum(ke, CSMALL)
-        hgrad = self.qexp * ke_adj * np.maximum(np.abs(e), 1e-30) ** (self.qexp - 1.0)
+        hgrad = self.qexp * ke_adj * self._pow(np.maximum(np.abs(e), 1e-30), self.qexp - 1.0)
         lin = hgrad < self.rqtol
         hgrad = np.where(lin, self.rqtol, hgrad)
         hloss = np.where(lin, hgrad * e, hgrad * e / self.qexp)
@@ -567,10 +580,15 @@ class GGAModel:
         return H
 
 
-def chol_solve_refined(A, F, n_refine=2):
+def chol_solve_refined(A, F, n_refine=2, extended=False):
     """Dense Cholesky + cholesky_solve + n_refine refinement steps with the
     residual A*H summed along the last axis (dgga/solver.py:3037-3064).
-    Raises RuntimeError naming non-SPD samples (3043-3051)."""
+    Raises RuntimeError naming non-SPD samples (3043-3051).
+
+    extended=True evaluates the refinement residual in x87 extended precision
+    (64-bit significand) and refines 4 times: the result is then the linear
+    solution to about one rounding, which separates the reference's own
+    rounding noise from a real discrepancy in a comparison."""
     B, Nj = F.shape
     H = np.empty_like(F)
     bad = []
@@ -581,9 +599,16 @@ def chol_solve_refined(A, F, n_refine=2):
             bad.append(b)
             continue
         x = sla.cho_solve(c, F[b], check_finite=False)
-        for _ in range(n_refine):
-            r = F[b] - (A[b] * x[None, :]).sum(-1)
-            x = x + sla.cho_solve(c, r, check_finite=False)
+        if extended:
+            Al = A[b].astype(np.longdouble)
+            Fl = F[b].astype(np.longdouble)
+            for _ in range(max(n_refine, 4)):
+                r = (Fl - Al @ x.astype(np.longdouble)).astype(np.float64)
+                x = x + sla.cho_solve(c, r, check_finite=False)
+        else:
+            for _ in range(n_refine):
+                r = F[b] - (A[b] * x[None, :]).sum(-1)
+                x = x + sla.cho_solve(c, r, check_finite=False)
         H[b] = x
     if bad:
         raise RuntimeError("dense Cholesky failed: samples %s are not SPD" % bad)
@@ -592,12 +617,13 @@ def chol_solve_refined(A, F, n_refine=2):
 
 def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
           max_iter=None, accuracy=None, status_machine=True, assemble="dense",
-          n_refine=2, record=False):
+          n_refine=2, record=False, extended=False):
     """Batched dense GGA (dgga/solver.py:2703-3212, mode='dense',
     dense_status_machine=True).  demand/res_head/ke: [B,N] or [N];
     r_hw: [L] or [B,L].  Returns the reference's output dict (head_ft,
     flow_cfs, emitter_cfs, iters, relerr, converged, status, diag_ratio)
-    plus, when ``record``, per-iteration traces."""
+    plus, when ``record``, per-iteration traces.  ``extended`` evaluates the
+    refinement residuals in extended precision (see chol_solve_refined)."""
     m = model
     d = np.atleast_2d(np.asarray(demand, dtype=np.float64))
     rh = np.atleast_2d(np.asarray(res_head, dtype=np.float64))
@@ -652,7 +678,7 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
             dn = np.abs(np.diagonal(A, axis1=1, axis2=2))
             dr_now = dn.max(1) / np.maximum(dn.min(1), 1e-300)
             diag_ratio = np.where(active, dr_now, diag_ratio)
-        Hj = chol_solve_refined(A, F, n_refine)
+        Hj = chol_solve_refined(A, F, n_refine, extended)
         H_junc = np.where(active[:, None], Hj, H_junc)
         H = m.full_heads(H_junc, rh_fix)
         dh = H[:, m.n1] - H[:, m.n2]

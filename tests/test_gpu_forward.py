@@ -1,0 +1,161 @@
+# -*- coding: utf-8 -*-
+"""GPU parity of the batched GGA forward solve against the CPU oracle."""
+import numpy as np
+import pytest
+import torch
+
+from oracle import gga
+from wdsgpu_amd.generators import grid_network, planar_network, scenario_batch
+from wdsgpu_amd.network import JUNCTION, RESERVOIR
+from wdsgpu_amd.solver import GGASolver
+
+from netutil import make_net
+
+# BASELINE tolerance on heads: 1e-10 m
+TOL_H_FT = 1e-10 / 0.3048
+
+pytestmark = pytest.mark.gpu
+
+
+def _case(name, B):
+    if name == "grid":
+        net = grid_network(0)
+        d = np.broadcast_to(net.demand_cfs, (B, net.N)).copy()
+        rh = np.broadcast_to(np.nan_to_num(net.reservoir_head_ft()), (B, net.N)).copy()
+        return net, d, rh, None
+    net = planar_network(1)
+    d, rh = scenario_batch(net, B, seed=5)
+    return net, d, rh, 1e-6
+
+
+@pytest.fixture(scope="module")
+def solvers(cuda):
+    out = {}
+    for name in ("grid", "planar"):
+        net = _case(name, 1)[0]
+        out[name] = (net, gga.GGAModel(net), GGASolver(net, device=cuda,
+                                                       dense_status_machine=True))
+    return out
+
+
+def test_factor_solve_matches_dense_cholesky(solvers):
+    for name in ("grid", "planar"):
+        net, m, s = solvers[name]
+        B = 4
+        _, d, rh, _ = _case(name, B)
+        q = np.broadcast_to(m.init_flow(), (B, m.L)).copy()
+        e = np.where(net.node_ke[m.junc_nodes] > 0, 1.0, 0.0)[None].repeat(B, 0)
+        H = m.full_heads(np.zeros((B, m.Nj)), rh[:, m.fixed_nodes])
+        S = np.broadcast_to(m.init_status_int, (B, m.L)).copy()
+        P, Y = m.link_PY(q, S, m.r_hw[None])
+        csr, F, *_ = m.assemble(P, Y, q, H, e, net.node_ke[m.junc_nodes][None].repeat(B, 0),
+                                d[:, m.junc_nodes], S, csr=True)
+        Hg = s.factor_solve(torch.tensor(csr), torch.tensor(F)).cpu().numpy()
+        Hx = gga.chol_solve_refined(m.csr_to_dense(csr), F, extended=True)
+        assert np.abs(Hg - Hx).max() <= 1e-9 * np.abs(Hx).max() * 1e-3
+
+
+@pytest.mark.parametrize("name,B", [("grid", 1), ("planar", 8)])
+def test_forward_matches_oracle(solvers, name, B):
+    net, m, s = solvers[name]
+    _, d, rh, acc = _case(name, B)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=acc, status_machine=True)
+    Hg = out["head_ft"].cpu().numpy()
+    ext = gga.solve(m, d, rh, ke=net.node_ke, accuracy=acc, status_machine=True,
+                    extended=True)
+    ref = gga.solve(m, d, rh, ke=net.node_ke, accuracy=acc, status_machine=True)
+    # identical Newton iteration counts and final statuses
+    assert out["iters"].cpu().numpy().tolist() == ref["iters"].tolist()
+    assert out["iters"].cpu().numpy().tolist() == ext["iters"].tolist()
+    assert np.array_equal(out["status"].cpu().numpy(), ref["status"])
+    assert bool(out["converged"].all())
+    # The last Newton iterate of a loosely converged solve (relerr ~1e-4 for
+    # accuracy 1e-3) carries every earlier rounding through the flow update:
+    # the oracle itself moves by ~1e-9 ft when its power laws are perturbed
+    # by one ulp.  Accept within 3x that self-jitter (the reference's
+    # acceptance practice, data/sparse_gpu_plan.md:404-445) or 1e-10 m.
+    m.pow_jitter = np.random.default_rng(123)
+    try:
+        jit = gga.solve(m, d, rh, ke=net.node_ke, accuracy=acc, status_machine=True,
+                        extended=True)
+    finally:
+        m.pow_jitter = None
+    assert jit["iters"].tolist() == ext["iters"].tolist()
+    noise = np.abs(jit["head_ft"] - ext["head_ft"]).max()
+    assert np.abs(Hg - ext["head_ft"]).max() <= max(3.0 * noise, TOL_H_FT)
+    assert np.abs(Hg - ref["head_ft"]).max() <= max(3.0 * noise, TOL_H_FT) + \
+        np.abs(ref["head_ft"] - ext["head_ft"]).max()
+    qn = np.abs(jit["flow_cfs"] - ext["flow_cfs"]).max()
+    assert np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max() <= max(3.0 * qn, 1e-12)
+    if m.n_pcv:
+        dr = out["diag_ratio"].cpu().numpy()
+        assert np.allclose(dr, ref["diag_ratio"], rtol=1e-9)
+
+
+def test_polished_forward_within_1e10_m(solvers):
+    # the forward of the implicit solve (GGA + full-Newton polish) is the
+    # state the adjoint differentiates; there both paths sit on the root of
+    # the residual system to ~1e-13 and the 1e-10 m bar applies directly
+    from oracle import adjoint as adj
+    from wdsgpu_amd.autodiff import implicit_solve
+    net, m, s = solvers["planar"]
+    _, d, rh, _ = _case("planar", 4)
+    ke = np.broadcast_to(net.node_ke, d.shape).copy()
+    with torch.no_grad():
+        H, Q, E = implicit_solve(s, torch.tensor(d), torch.tensor(rh), ke=torch.tensor(ke),
+                                 accuracy=1e-8, polish_steps=3, adjoint="gpu",
+                                 status_machine=True)
+    sol = adj.solve_polished(m, d, rh, ke, m.r_hw, accuracy=1e-8)
+    assert (sol["resid_inf"] < 1e-10).all()
+    assert np.abs(H.cpu().numpy() - sol["head"]).max() <= TOL_H_FT
+    # flows through open valves (P = 1/CSMALL = 1e6) turn 1e-16 head
+    # rounding into ~1e-10 cfs; every other link agrees to ~1e-15
+    assert np.abs(Q.cpu().numpy() - sol["q"]).max() <= 1e-9
+    assert float(s._last_resid.max()) < 1e-10
+
+
+def test_unfused_equals_fused_bitwise(solvers):
+    net, m, s = solvers["planar"]
+    _, d, rh, acc = _case("planar", 4)
+    args = (torch.tensor(d), torch.tensor(rh))
+    a = s.solve(*args, ke_int=torch.tensor(net.node_ke), accuracy=acc, status_machine=True)
+    b = s.solve_unfused(*args, ke_int=torch.tensor(net.node_ke), accuracy=acc,
+                        status_machine=True)
+    assert torch.equal(a["head_ft"], b["head_ft"])
+    assert torch.equal(a["flow_cfs"], b["flow_cfs"])
+    assert torch.equal(a["iters"], b["iters"])
+
+
+def test_batch_equals_single_bitwise(solvers):
+    net, m, s = solvers["planar"]
+    _, d, rh, acc = _case("planar", 5)
+    ke = torch.tensor(net.node_ke)
+    a = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=ke, accuracy=acc, status_machine=True)
+    for b in (0, 3):
+        o = s.solve(torch.tensor(d[b]), torch.tensor(rh[b]), ke_int=ke, accuracy=acc,
+                    status_machine=True)
+        assert torch.equal(o["head_ft"], a["head_ft"][b])
+        assert int(o["iters"]) == int(a["iters"][b])
+
+
+def test_unconverged_reports_maxtrials_plus_one(solvers):
+    net, m, s = solvers["grid"]
+    _, d, rh, _ = _case("grid", 2)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), max_iter=1, status_machine=True)
+    assert out["iters"].tolist() == [2, 2]
+    assert not bool(out["converged"].any())
+    ref = gga.solve(m, d, rh, max_iter=1, status_machine=True)
+    assert ref["iters"].tolist() == [2, 2]
+
+
+def test_singular_matrix_raises_naming_samples(cuda):
+    # an isolated junction with no emitter has a zero diagonal (dgga/solver.py:342-349)
+    nodes = [(RESERVOIR, 100.0, 0.0), (JUNCTION, 0.0, 0.1), (JUNCTION, 0.0, 0.0)]
+    links = [dict(n1=0, n2=1, len_ft=100.0, diam_ft=0.5, C=100.0)]
+    net = make_net(nodes, links)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    d = np.broadcast_to(net.demand_cfs, (2, 3)).copy()
+    rh = np.broadcast_to(np.nan_to_num(net.reservoir_head_ft()), (2, 3)).copy()
+    with pytest.raises(RuntimeError, match=r"samples \[0, 1\]"):
+        s.solve(torch.tensor(d), torch.tensor(rh), status_machine=True)

@@ -63,7 +63,9 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
                                           r_hw=rt, polish_steps=int(polish_steps),
                                           want_factor=True)
         with torch.cuda.device(s.device):
-            _lib.check(s._lib.wds_solve(s._plan, s._stream(), C.byref(a)), "wds_solve")
+            _lib.check(s._timed("k_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
+                                                                   C.byref(a)),
+                                B, out["iters"]), "wds_solve")
         info = out["info"]
         if bool((info != 0).any()):
             bad = torch.nonzero(info != 0).flatten().tolist()
@@ -125,7 +127,9 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             n_refine=int(s.dense_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
             g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr), info=_lib.ptr(info), workspace=_lib.ptr(ws))
         with torch.cuda.device(dev):
-            _lib.check(s._lib.wds_adjoint(s._plan, s._stream(), C.byref(a)), "wds_adjoint")
+            _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
+                                                                       C.byref(a)), B),
+                       "wds_adjoint")
         if bool((info != 0).any()):
             bad = torch.nonzero(info != 0).flatten().tolist()
             raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])

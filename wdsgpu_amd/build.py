@@ -38,7 +38,11 @@ def build(verbose=False, arch="gfx950"):
         obj = os.path.join(objdir, src + ".o")
         cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
         if src.endswith(".hip"):
-            cmd[1:1] = ["-x", "hip", "--offload-arch=" + arch,
+            # -ffp-contract=off: the coefficient, assembly and update stages
+            # round every product like the reference's CPU path does; the
+            # factor/solve loops and the double-double residual use
+            # explicit fma() where they want it
+            cmd[1:1] = ["-x", "hip", "--offload-arch=" + arch, "-ffp-contract=off",
                         "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
         if verbose:
             print(" ".join(cmd), flush=True)

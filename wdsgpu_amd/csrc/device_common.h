@@ -151,6 +151,33 @@ __device__ __forceinline__ double spmv_row(const DevPlan& pl, const double* A, c
   return s;
 }
 
+// b - (A x)_i evaluated in double-double: every product is split exactly
+// with an FMA (TwoProduct) and the running sum is carried as an unevaluated
+// pair with Knuth's TwoSum, so the residual is correct to about one rounding
+// of its own magnitude.  Iterative refinement with this residual converges
+// to the correctly rounded solution while kappa * eps < 1.
+__device__ __forceinline__ void two_sum(double a, double b, double& s, double& e) {
+  s = a + b;
+  const double bb = s - a;
+  e = (a - (s - bb)) + (b - bb);
+}
+__device__ __forceinline__ double residual_row_dd(const DevPlan& pl, const double* A,
+                                                  const double* x, double b, int i) {
+  double hi = b, lo = 0.0;
+  const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
+  for (int e = e0; e < e1; ++e) {
+    const int2 en = pl.spmv_ent[e];
+    const double a = A[en.x], xv = x[en.y];
+    const double p = a * xv;
+    const double pe = fma(a, xv, -p);
+    double s, err;
+    two_sum(hi, -p, s, err);
+    hi = s;
+    lo += err - pe;
+  }
+  return hi + lo;
+}
+
 // ---------------------------------------------------------------------------
 // Link coefficients (headlosscoeffs, dgga/solver.py:2924-2956 and the
 // pipe/TCV/pump/valve helpers 2176-2326, 834-848; D-W 1404-1450).

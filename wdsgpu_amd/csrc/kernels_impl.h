@@ -353,7 +353,7 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
   run_program<1>(pl.fwd, c.V, X, dummy);
   run_program<2>(pl.bwd, c.V, X, dummy);
   for (int it = 0; it < n_refine; ++it) {
-    for (int i = lane; i < pl.Nj; i += 64) c.R[i] = bsrc[i] - spmv_row(pl, c.A, X, i);
+    for (int i = lane; i < pl.Nj; i += 64) c.R[i] = residual_row_dd(pl, c.A, X, bsrc[i], i);
     lds_sync();
     run_program<1>(pl.fwd, c.V, c.R, dummy);
     run_program<2>(pl.bwd, c.V, c.R, dummy);

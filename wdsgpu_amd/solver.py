@@ -179,6 +179,22 @@ class GGASolver:
         n1, n2 = np.asarray(net.link_n1), np.asarray(net.link_n2)
         self.prv_n1, self.prv_n2 = n1[self.prv_links], n2[self.prv_links]
         self._uploaded = False
+        # optional kernel timing: when a list, every library launch appends
+        # (name, start_event, end_event, batch, iterations tensor) recorded on
+        # the launching stream
+        self.event_log = None
+
+    def _timed(self, name, fn, B, iters=None):
+        if self.event_log is None:
+            return fn()
+        st = torch.cuda.current_stream(self.device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        rc = fn()
+        e1.record(st)
+        self.event_log.append((name, e0, e1, int(B), iters))
+        return rc
 
     def __del__(self):
         h = getattr(self, "_plan", None)
