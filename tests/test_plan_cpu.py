@@ -1,0 +1,164 @@
+# -*- coding: utf-8 -*-
+"""CPU checks of libwds.so: exported symbols, host symbolic analysis, and the
+level-scheduled programs executed by a numpy emulation of the wavefront
+(same instruction semantics as the HIP executor) against dense solves."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import gga
+from wdsgpu_amd import _lib
+from wdsgpu_amd.generators import grid_network, planar_network, scenario_batch
+from wdsgpu_amd.network import CVPIPE, GPV, JUNCTION, PIPE, RESERVOIR
+from wdsgpu_amd.solver import GGASolver
+
+from netutil import make_net
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIRST, LAST, DIAG, NOP, LEVEL_END = 1 << 30, 1 << 29, 1 << 28, 1 << 27, 1 << 26
+WIDTH_SHIFT, ZERO_INIT = 23, 1 << 22
+TMASK = (1 << 22) - 1
+
+
+def test_library_exports_every_declared_symbol():
+    lib = _lib.load()
+    header = open(os.path.join(ROOT, "include", "wds.h")).read()
+    names = set(re.findall(r"\b(wds_[a-z_]+)\s*\(", header))
+    assert names, "no declarations found"
+    for n in sorted(names):
+        assert hasattr(lib, n), n
+    assert names <= set(_lib.SIGNATURES)
+
+
+def run_program(s, p, V, X, mode):
+    """Emulate the wavefront executor: per-lane accumulators, stores on LAST,
+    and at the end of a wide level (log2 width in bits 23-25) a group sum whose
+    first lane finalises the target of the task its slice belongs to."""
+    ins = s.plan_export(8 + 3 * p).reshape(-1, 64, 4)
+    bad = False
+    acc = np.zeros(64)
+    level_ends, wide_levels = 0, 0
+    start = True
+    fin = [None] * 64
+    for step in ins:
+        if start:
+            fin = [(int(w) & TMASK, int(c), bool(w & DIAG)) if w & FIRST else None
+                   for c, w in zip(step[:, 2], step[:, 3])]
+            start = False
+        for l in range(64):
+            a, b, c, w = (int(v) for v in step[l])
+            t = w & TMASK
+            init = 0.0 if (w & ZERO_INIT or mode == 2) else (V[t] if mode == 0 else X[t])
+            if w & FIRST:
+                acc[l] = init
+            if w & NOP:
+                continue
+            if mode == 0:
+                acc[l] -= V[a] * (V[b] * V[c])
+                if w & LAST:
+                    if w & DIAG:
+                        bad |= not acc[l] > 0
+                        V[t] = 1.0 / acc[l]
+                    else:
+                        V[t] = acc[l]
+            elif mode == 1:
+                acc[l] -= V[a] * X[b]
+                if w & LAST:
+                    X[t] = acc[l] * V[c]
+            else:
+                acc[l] += V[a] * X[b]
+                if w & LAST:
+                    X[t] = X[t] - V[c] * acc[l]
+        flags = step[:, 3] & (LEVEL_END | (7 << WIDTH_SHIFT))
+        assert (flags == flags[0]).all()   # uniform across lanes
+        if flags[0] & LEVEL_END:
+            level_ends += 1
+            lw = (int(flags[0]) >> WIDTH_SHIFT) & 7
+            if lw:
+                wide_levels += 1
+                W = 1 << lw
+                for g in range(0, 64, W):
+                    if fin[g] is None:
+                        continue
+                    t, c, diag = fin[g]
+                    red = acc[g:g + W].sum()
+                    if mode == 0:
+                        tot = V[t] + red
+                        bad |= diag and not tot > 0
+                        V[t] = 1.0 / tot if diag else tot
+                    elif mode == 1:
+                        X[t] = (X[t] + red) * V[c]
+                    else:
+                        X[t] = X[t] - V[c] * red
+            start = True
+    return bad, level_ends, wide_levels
+
+
+@pytest.mark.parametrize("name", ["grid", "planar"])
+def test_programs_factor_and_solve_like_dense(name):
+    net = grid_network(0) if name == "grid" else planar_network(1)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device="cpu", dense_status_machine=True)
+    st = s.stats
+    # symbolic sanity: permutation, etree parent above child, pattern of A in L
+    perm = s.plan_export(0)
+    assert sorted(perm.tolist()) == list(range(m.Nj))
+    par = s.plan_export(3)
+    assert all(p == -1 or p > j for j, p in enumerate(par))
+    d, rh = scenario_batch(net, 1, seed=5)
+    q = m.init_flow()[None]
+    e = np.where(net.node_ke[m.junc_nodes] > 0, 1.0, 0.0)[None]
+    H = m.full_heads(np.zeros((1, m.Nj)), rh[:, m.fixed_nodes])
+    S = m.init_status_int[None]
+    P, Y = m.link_PY(q, S, m.r_hw[None])
+    csr, F, *_ = m.assemble(P, Y, q, H, e, net.node_ke[m.junc_nodes][None], d[:, m.junc_nodes],
+                            S, csr=True)
+    assert st["nnz_csr"] == csr.shape[1]
+    c2l = s.plan_export(7)
+    slot = np.argsort(perm)          # original junction row -> elimination slot
+    V = np.zeros(st["nnzL"] + 1)
+    V[c2l[c2l >= 0]] = csr[0][c2l >= 0]
+    bad, lev, wide_f = run_program(s, 0, V, None, 0)
+    assert not bad and lev == st["nlevels"]
+    X = np.zeros(m.Nj)
+    X[slot] = F[0]
+    _, _, wide_s = run_program(s, 1, V, X, 1)
+    run_program(s, 2, V, X, 2)
+    if name == "planar":
+        # the top of the elimination tree has few long dot products: those
+        # levels must be split across lane groups
+        assert wide_f > 0 and wide_s > 0
+    A = m.csr_to_dense(csr)[0]
+    xr = np.linalg.solve(A, F[0])
+    assert np.abs(X[slot] - xr).max() <= 1e-10 * np.abs(xr).max()
+
+
+def test_minimum_degree_beats_natural_ordering():
+    net = planar_network(1)
+    a = GGASolver(net, device="cpu", dense_status_machine=True, ordering="mmd").stats
+    b = GGASolver(net, device="cpu", dense_status_machine=True, ordering="natural").stats
+    assert a["nnzL"] < b["nnzL"] and a["factor_updates"] < b["factor_updates"]
+
+
+def test_plan_errors_map_to_reference_exceptions():
+    nodes = [(RESERVOIR, 100.0, 0.0), (JUNCTION, 0.0, 0.1)]
+    with pytest.raises(ValueError, match="self-loop"):
+        GGASolver(make_net(nodes, [dict(n1=1, n2=1, len_ft=10.0, diam_ft=0.5, C=100.0)]),
+                  device="cpu")
+    with pytest.raises(NotImplementedError):
+        GGASolver(make_net(nodes, [dict(n1=0, n2=1, type=GPV, diam_ft=0.5)]), device="cpu",
+                  dense_status_machine=True)
+    with pytest.raises(NotImplementedError):
+        # CV pipes need the status machine (dgga/solver.py:2779-2785)
+        GGASolver(make_net(nodes, [dict(n1=0, n2=1, type=CVPIPE, len_ft=10.0, diam_ft=0.5,
+                                        C=100.0)]), device="cpu")
+
+
+def test_solver_refuses_cpu_execution():
+    net = grid_network(0)
+    s = GGASolver(net, device="cpu", dense_status_machine=True)
+    with pytest.raises(NotImplementedError, match="HIP device"):
+        s.solve(net.demand_cfs, np.nan_to_num(net.reservoir_head_ft()), status_machine=True)

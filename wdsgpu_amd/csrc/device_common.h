@@ -60,34 +60,58 @@ __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 // Level-scheduled program executor.  MODE 0: LDL^T factorisation in place in
 // V (unscaled columns w, reciprocal pivots on the diagonal); MODE 1: forward
 // solve z = D^-1 L^-1 x in place in X; MODE 2: backward solve x = L^-T z.
-// Instructions of chunk g+1 are fetched from global memory while chunk g runs.
-template <int MODE>
-__device__ __forceinline__ void load_chunk(const ProgDev& pg, int g, int lane, int4* buf) {
-#pragma unroll
-  for (int u = 0; u < kUnroll; ++u) {
-    const int s = g + u;
-    int4 in = make_int4(0, 0, 0, kNop);
-    if (s < pg.nsteps) {
-      const int2 st = pg.step[s];
-      if (lane < st.y) in = pg.instr[st.x + lane];
-    }
-    buf[u] = in;
-  }
+// A ring of kRing chunks (kRing * kUnroll steps) of instruction loads stays in
+// flight: a chunk is refilled with the chunk kRing ahead as soon as it is
+// consumed, and instruction addresses depend on nothing loaded, so the loads
+// never wait on each other.  The instruction array carries kRing * kUnroll
+// trailing NOP steps so the ring can over-fetch without bounds checks.
+constexpr int kRing = 4;
+
+// 1/x for a positive pivot: v_rcp_f64 refined by two Newton steps
+__device__ __forceinline__ double recip(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return r;
 }
+
+static_assert(kStepAlign % kUnroll == 0, "levels must start on a chunk boundary");
 
 template <int MODE>
 __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
   const int lane = lane_id();
   const int ns = pg.nsteps;
   if (ns == 0) return;
-  int lev = 0;
-  int lev_end = pg.level_end[0];
-  int4 cur[kUnroll];
-  load_chunk<MODE>(pg, 0, lane, cur);
+  const int4* ip = pg.instr + lane;
+  int4 ring[kRing][kUnroll];
+#pragma unroll
+  for (int d = 0; d < kRing; ++d)
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
   double acc = 0.0;
-  for (int g = 0; g < ns; g += kUnroll) {
-    int4 nxt[kUnroll];
-    load_chunk<MODE>(pg, g + kUnroll, lane, nxt);
+  // wide levels: the task whose target this lane's group finalises
+  int fin_t = 0, fin_c = 0;
+  bool fin_diag = false, fin_act = false, lvl_start = true;
+  for (int g0 = 0; g0 < ns; g0 += kRing * kUnroll) {
+#pragma unroll
+    for (int d = 0; d < kRing; ++d) {
+    const int g = g0 + d * kUnroll;
+    if (g >= ns) break;
+    int4 cur[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      cur[u] = ring[d][u];
+      ring[d][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
+    }
+    if (lvl_start) {
+      // a wide level's slices all start at its first step, which is the
+      // first step of a chunk: remember the task this lane helps finalise
+      fin_t = cur[0].w & kTMask;
+      fin_c = cur[0].z;
+      fin_diag = cur[0].w & kDiag;
+      fin_act = cur[0].w & kFirst;
+      lvl_start = false;
+    }
     double va[kUnroll], vb[kUnroll], vc[kUnroll], vt[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
@@ -103,38 +127,60 @@ __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
         vt[u] = X[t];
       }
     }
+    // branch-free: NOPs multiply by V[zero] == 0; every step stores, to its
+    // target when the instruction is a task's last, else to V[dummy]
+    double* const dummy = V + pg.dummy;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int w = cur[u].w;
-      if (w & kNop) continue;
       const int t = w & kTMask;
+      const bool first = w & kFirst, last = w & kLast;
+      // a wide slice's first instruction starts its partial sum at zero
+      const double init = (MODE == 2 || (w & kZeroInit)) ? 0.0 : vt[u];
+      double val;
+      double* dst;
       if (MODE == 0) {
-        if (w & kFirst) acc = vt[u];
-        acc = fma(-va[u], vb[u] * vc[u], acc);
-        if (w & kLast) {
-          if (w & kDiag) {
-            if (!(acc > 0.0)) bad = 1;
-            V[t] = 1.0 / acc;
+        acc = fma(-va[u], vb[u] * vc[u], first ? init : acc);
+        const bool diag = w & kDiag;
+        const double r = recip(acc);
+        if (diag && last && !(acc > 0.0)) bad = 1;
+        val = diag ? r : acc;
+        dst = last ? V + t : dummy;
+      } else if (MODE == 1) {
+        acc = fma(-va[u], vb[u], first ? init : acc);
+        val = acc * vc[u];
+        dst = last ? X + t : dummy;
+      } else {
+        acc = fma(va[u], vb[u], first ? init : acc);
+        val = vt[u] - vc[u] * acc;
+        dst = last ? X + t : dummy;
+      }
+      *dst = val;
+    }
+    // levels are padded to whole chunks: only a chunk's last step can end one
+    const int wl = __builtin_amdgcn_readfirstlane(cur[kUnroll - 1].w);
+    if (wl & kLevelEnd) {
+      const int lw = (wl >> kWidthShift) & 7;
+      if (lw) {
+        // wide level: butterfly-sum each aligned group of 2^lw lanes, then
+        // the group's first lane finalises its task's target
+        double red = acc;
+        for (int m = 1; m < (1 << lw); m <<= 1) red += __shfl_xor(red, m, 64);
+        if (fin_act && (lane & ((1 << lw) - 1)) == 0) {
+          if (MODE == 0) {
+            const double tot = V[fin_t] + red;
+            if (fin_diag && !(tot > 0.0)) bad = 1;
+            V[fin_t] = fin_diag ? recip(tot) : tot;
+          } else if (MODE == 1) {
+            X[fin_t] = (X[fin_t] + red) * V[fin_c];
           } else {
-            V[t] = acc;
+            X[fin_t] = X[fin_t] - V[fin_c] * red;
           }
         }
-      } else if (MODE == 1) {
-        if (w & kFirst) acc = vt[u];
-        acc = fma(-va[u], vb[u], acc);
-        if (w & kLast) X[t] = acc * vc[u];
-      } else {
-        if (w & kFirst) acc = 0.0;
-        acc = fma(va[u], vb[u], acc);
-        if (w & kLast) X[t] = vt[u] - vc[u] * acc;
       }
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) cur[u] = nxt[u];
-    if (g + kUnroll == lev_end) {
+      lvl_start = true;
       lds_sync();
-      ++lev;
-      if (lev < pg.nlevels) lev_end = pg.level_end[lev];
+    }
     }
   }
 }

@@ -27,11 +27,10 @@ enum LinkFlag : int32_t {
 };
 
 struct ProgDev {
-  const int4* instr;        // compacted step-major instructions
-  const int2* step;         // per step: (offset into instr, active lanes)
-  const int32_t* level_end; // cumulative step count at the end of each level
-  int32_t nsteps;
+  const int4* instr;        // step-major instructions, nsteps * 64
+  int32_t nsteps;           // multiple of kUnroll
   int32_t nlevels;
+  int32_t dummy;            // V[dummy] (= nnzL + 1) absorbs non-final stores
 };
 
 struct DevPlan {

@@ -62,14 +62,13 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   const Symbolic& S = P->sym;
   BlobBuilder bb;
   auto prog_vectors = [&](const Program& pr, size_t& oi, size_t& os, size_t& ol) {
-    std::vector<int32_t> st(pr.step_off.size() * 2);
-    for (size_t i = 0; i < pr.step_off.size(); ++i) {
-      st[2 * i] = pr.step_off[i];
-      st[2 * i + 1] = pr.step_cnt[i];
-    }
-    oi = bb.add(pr.instr);
-    os = bb.add(st);
-    ol = bb.add(pr.level_end);
+    // trailing NOP steps: the executor's prefetch ring reads up to
+    // kRing * kUnroll steps past the end
+    std::vector<Instr> padded(pr.instr);
+    padded.resize(pr.instr.size() + (size_t)kRing * kUnroll * kLanes,
+                  nop_instr((int32_t)S.nnzL));
+    oi = bb.add(padded);
+    os = ol = 0;
   };
   size_t o_ns = bb.add(P->node_slot), o_sn = bb.add(P->slot_node), o_el = bb.add(P->el_slot);
   size_t o_s1 = bb.add(P->l_s1), o_s2 = bb.add(P->l_s2), o_fl = bb.add(P->l_flags),
@@ -166,10 +165,11 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.diag_pos = at<int32_t>(d, o_dp);
   D.row_slot = at<int32_t>(d, o_rs);
   auto set_prog = [&](ProgDev& pg, const Program& pr, size_t oi, size_t os, size_t ol) {
+    (void)os;
+    (void)ol;
     pg.instr = at<int4>(d, oi);
-    pg.step = at<int2>(d, os);
-    pg.level_end = at<int32_t>(d, ol);
     pg.nsteps = pr.nsteps();
+    pg.dummy = (int32_t)S.nnzL + 1;
     pg.nlevels = (int32_t)pr.level_end.size();
   };
   set_prog(D.factor, S.factor, fi, fs, fl);

@@ -156,7 +156,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   Scn c;
   const int Nj = pl.Nj, L = pl.L;
   c.V = lds;
-  c.Hn = lds + pl.nnzL + 1;
+  c.Hn = lds + pl.nnzL + 2;  // V[nnzL] = 0 (zero slot), V[nnzL + 1] = store sink
   c.R = c.Hn + Nj + pl.Nf;
   c.X2 = c.R + Nj;
   c.sh = c.X2 + Nj;
@@ -506,17 +506,39 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
   for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
 }
 
+// Stage cycle counters for a profiling build (-DWDS_PROFILE_STAGES): lane 0
+// accumulates clock64() deltas of each stage into misc[400 + stage].
+#ifdef WDS_PROFILE_STAGES
+#define WDS_T0() long long wds_t0_ = clock64()
+#define WDS_TACC(c, i)                                              \
+  do {                                                              \
+    const long long t1_ = clock64();                                \
+    if (lane_id() == 0) (c).misc[400 + (i)] += (double)(t1_ - wds_t0_); \
+    wds_t0_ = t1_;                                                  \
+  } while (0)
+#else
+#define WDS_T0() (void)0
+#define WDS_TACC(c, i) (void)0
+#endif
+
 // one Newton iteration; returns relerr, sets bad on a non-positive pivot
 __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const SolveParams& sp,
                                           bool& bad, double* dratio) {
+  WDS_T0();
   k1_coeffs(pl, c);
+  WDS_TACC(c, 0);
   k2_assemble(pl, c, true);
   k2_prv(pl, c);
   if (pl.p > 0 && dratio) *dratio = diag_ratio_now(pl, c);
+  WDS_TACC(c, 1);
   bad = k3_factor(pl, c);
+  WDS_TACC(c, 2);
   if (bad) return 0.0;
   k4_solve(pl, c, c.F, c.Hn, sp.n_refine);
-  return k5_update(pl, c, sp.hacc);
+  WDS_TACC(c, 3);
+  const double r = k5_update(pl, c, sp.hacc);
+  WDS_TACC(c, 4);
+  return r;
 }
 
 struct Ctrl {
@@ -621,6 +643,9 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
     c.e[j] = sp.e0 ? sp.e0[(int64_t)b * pl.N + node] : (ke[node] > 0.0 ? 1.0 : 0.0);
     c.Hn[j] = 0.0;
   }
+#ifdef WDS_PROFILE_STAGES
+  if (lane < 8) c.misc[400 + lane] = 0.0;
+#endif
   mem_sync();
   lds_sync();
   const int maxtrials = maxtrials_of(pl, sp);
@@ -635,7 +660,10 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
     }
     st.iters = it;
     relerr_out = relerr;
-    if (control_step(pl, c, sp, it, relerr, st)) {
+    WDS_T0();
+    const bool done = control_step(pl, c, sp, it, relerr, st);
+    WDS_TACC(c, 5);
+    if (done) {
       st.active = 0;
       break;
     }
@@ -643,7 +671,9 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
   if (st.active && st.info == 0) st.iters = maxtrials + 1;
   double resid = 0.0;
   if (st.info == 0 && sp.polish_steps >= 0 && sp.factor) {
+    WDS_T0();
     const int pinfo = polish_and_factor(pl, c, sp, b, resid);
+    WDS_TACC(c, 6);
     if (pinfo) st.info = pinfo;
   }
   write_outputs(pl, c, sp, b);

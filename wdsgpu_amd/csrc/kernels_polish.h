@@ -10,6 +10,15 @@
 
 namespace wds {
 
+// Refinement steps of the Woodbury column solves (A-hat^-1 e_u) and of the
+// correction solves inside the outer refinement on the row-replaced system.
+// Errors of order kappa * eps in these inner solves are contracted by the
+// two outer refinement steps, which evaluate the residual of the true
+// replaced system; the first solve of the right-hand side keeps the full
+// refinement of the reference (dgga/autodiff.py:965-1029, 1107-1154).
+constexpr int kWoodColRefine = 0;
+constexpr int kWoodInnerRefine = 1;
+
 // Coefficients of the polish / adjoint at the current state
 // (_adjgpu_coeffs, dgga/autodiff.py:786-875): link P/Y with pcv PRVs taking
 // their prvcoeff values (ACTIVE -> P = 0), emitter hloss/hgrad.  Writes P, Y
@@ -163,7 +172,7 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   for (int col = 0; col < 2 * p; ++col) {
     const int u = col < p ? pl.prv_s2[col] : pl.prv_s1[col - p];
     unit_vec(pl, wk, u);
-    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, 2);
+    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, kWoodColRefine);
   }
   auto wood = [&](double* x, const double* rr) -> bool {
     // capacitance matrix C_eff (rebuilt each call: small_solve destroys it)
@@ -209,7 +218,7 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
       }
     }
     mem_sync();
-    solveA(pl, c, res, wk, 2);
+    solveA(pl, c, res, wk, kWoodInnerRefine);
     if (!wood(wk, res)) return false;
     for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
     mem_sync();
@@ -299,13 +308,20 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
     bool any_act = false;
     for (int i = 0; i < pl.p; ++i) any_act |= c.sh[i] != 0.0;
     if (any_act) {
+      // An ACTIVE valve's flow closes the linearised mass balance of its
+      // downstream junction j2:  m + dq_k - sum_{l != k} dq_l - de = 0, which
+      // with dq_l and de from dH gives dq_k = (Abar dH)[j2] - rhs_p[j2]
+      // (Abar = the node matrix without the CBIG pin).  dgga/autodiff.py:1084
+      // applies rhs_p - Abar dH, the opposite sign: that doubles the j2 mass
+      // residual every step instead of removing it; this follows the full
+      // Newton step of the CPU polish (dgga/autodiff.py:473-491).
       matvec(pl, c, dH, c.v3);
       if (lane == 0)
         for (int i = 0; i < pl.p; ++i) {
           if (c.sh[i] == 0.0) continue;
           const int j2 = pl.prv_s2[i];
           const double abar2 = c.v3[j2] - CBIG * dH[j2];
-          c.P[pl.prv_k[i]] = c.v1[j2] - abar2;
+          c.P[pl.prv_k[i]] = abar2 - c.v1[j2];
         }
       mem_sync();
     }
@@ -351,7 +367,7 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   mem_sync();
   for (int col = 0; col < p; ++col) {
     unit_vec(pl, wk, pl.prv_s2[col]);
-    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, n_ref);
+    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, kWoodColRefine);
   }
   auto wood = [&](double* x, const double* rr) -> bool {
     if (lane == 0) {
@@ -389,7 +405,7 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
         res[j2] = r2[j2] - (dst[j2] - dst[j1]);
       }
     mem_sync();
-    solveA(pl, c, res, wk, n_ref);
+    solveA(pl, c, res, wk, kWoodInnerRefine);
     if (!wood(wk, res)) return false;
     for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
     mem_sync();

@@ -286,7 +286,7 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   st.factor_instr = (int64_t)S.factor.instr.size();
   st.fwd_instr = (int64_t)S.fwd.instr.size();
   st.bwd_instr = (int64_t)S.bwd.instr.size();
-  st.lds_bytes = 8 * ((int64_t)S.nnzL + 1 + (Nj + Nf) + 2 * (int64_t)Nj + 2 * kMaxPrv);
+  st.lds_bytes = 8 * ((int64_t)S.nnzL + 2 + (Nj + Nf) + 2 * (int64_t)Nj + 2 * kMaxPrv);
 }
 
 }  // namespace wds
@@ -345,14 +345,7 @@ extern "C" int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* 
       std::memcpy(v.data(), pr.instr.data(), v.size() * sizeof(int32_t));
       return copy_out(v, dst, cap);
     }
-    if (part == 1) {
-      std::vector<int32_t> v(pr.step_off.size() * 2);
-      for (size_t i = 0; i < pr.step_off.size(); ++i) {
-        v[2 * i] = pr.step_off[i];
-        v[2 * i + 1] = pr.step_cnt[i];
-      }
-      return copy_out(v, dst, cap);
-    }
+    if (part == 1) return 0;  // reserved (no per-step table in this layout)
     return copy_out(pr.level_end, dst, cap);
   }
   return WDS_EINVAL;

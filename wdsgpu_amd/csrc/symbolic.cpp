@@ -62,15 +62,33 @@ static void minimum_degree(int n, const std::vector<std::vector<int32_t>>& adj,
   }
 }
 
-// Longest-processing-time assignment of the tasks of one level to 64
-// lanes, lanes renumbered by decreasing load, then step-major compacted
-// layout (only active lanes store an instruction).
-static void emit_level(const std::vector<std::vector<Instr>>& tasks, Program& prog) {
+static size_t align_steps(size_t s) { return (s + kStepAlign - 1) / kStepAlign * kStepAlign; }
+
+// Schedule the tasks of one level onto the 64 lanes and append the steps.
+//
+// Narrow mode (group width 1): longest-processing-time assignment of whole
+// tasks to lanes; a task's first instruction loads its target (kFirst), its
+// last stores the result (kLast).
+//
+// Wide mode (group width W = 2..64, chosen when a level has few, long tasks
+// -- the top of the elimination tree): each task owns an aligned group of W
+// lanes and its dot product is cut into W slices that start from zero
+// (kFirst | kZeroInit); at the level end the executor sums each group with a
+// butterfly of cross-lane shuffles and the group's first lane finalises the
+// target.  The level's steps shrink from the longest dot product to
+// ceil(longest / W), at the price of one reduction.
+//
+// Layout: step-major with a fixed stride of 64 (idle lanes hold NOPs); the
+// level's last step carries kLevelEnd and log2(W) in every lane.
+static void emit_level(const std::vector<Task>& tasks, Program& prog, int32_t zero) {
   if (tasks.empty()) return;
+  size_t mmax = 0;
+  for (const Task& t : tasks) mmax = std::max(mmax, t.ins.size());
+  // narrow-mode schedule
   std::vector<int> idx(tasks.size());
   for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
   std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
-    return tasks[x].size() > tasks[y].size();
+    return tasks[x].ins.size() > tasks[y].ins.size();
   });
   using Item = std::pair<int64_t, int>;  // (load, lane)
   std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
@@ -79,22 +97,63 @@ static void emit_level(const std::vector<std::vector<Instr>>& tasks, Program& pr
   for (int ti : idx) {
     Item it = heap.top();
     heap.pop();
-    for (const Instr& in : tasks[ti]) lane[it.second].push_back(in);
-    prog.useful += (int64_t)tasks[ti].size();
-    heap.push({it.first + (int64_t)tasks[ti].size(), it.second});
+    std::vector<Instr> v = tasks[ti].ins;
+    v.front().t |= kFirst;
+    v.back().t |= kLast | (tasks[ti].diag ? kDiag : 0);
+    for (const Instr& in : v) lane[it.second].push_back(in);
+    heap.push({it.first + (int64_t)v.size(), it.second});
   }
-  std::stable_sort(lane.begin(), lane.end(),
-                   [](const std::vector<Instr>& x, const std::vector<Instr>& y) {
-                     return x.size() > y.size();
-                   });
-  size_t steps = lane[0].size();
-  steps = (steps + kStepAlign - 1) / kStepAlign * kStepAlign;
+  size_t narrow_steps = 0;
+  for (auto& l : lane) narrow_steps = std::max(narrow_steps, l.size());
+  narrow_steps = align_steps(narrow_steps);
+  // best wide mode: one task per aligned group of W lanes
+  int best_lw = 0;
+  size_t best_cost = narrow_steps;
+  for (int lw = 1; lw <= 6; ++lw) {
+    const size_t W = (size_t)1 << lw;
+    if (tasks.size() * W > (size_t)kLanes) break;
+    const size_t cost = align_steps((mmax + W - 1) / W) + kWideOverheadSteps;
+    if (cost < best_cost) {
+      best_cost = cost;
+      best_lw = lw;
+    }
+  }
+  for (const Task& t : tasks) prog.useful += (int64_t)t.ins.size();
+  if (best_lw > 0) {
+    const int W = 1 << best_lw;
+    for (auto& l : lane) l.clear();
+    for (size_t g = 0; g < tasks.size(); ++g) {
+      const Task& tk = tasks[g];
+      const size_t m = tk.ins.size();
+      const size_t slice = (m + W - 1) / W;
+      const int32_t dflag = tk.diag ? kDiag : 0;
+      for (int i = 0; i < W; ++i) {
+        auto& ln = lane[g * W + i];
+        const size_t b = (size_t)i * slice, e = std::min(m, b + slice);
+        if (b >= e) {
+          // no share of the dot product: contribute an explicit zero
+          Instr in = nop_instr(zero);
+          in.t = (tk.ins.front().t & kTMask) | kFirst | kZeroInit | dflag;
+          ln.push_back(in);
+          continue;
+        }
+        for (size_t p = b; p < e; ++p) {
+          Instr in = tk.ins[p];
+          in.t |= dflag | (p == b ? (kFirst | kZeroInit) : 0);
+          ln.push_back(in);
+        }
+      }
+    }
+  }
+  size_t steps = 0;
+  for (auto& l : lane) steps = std::max(steps, l.size());
+  steps = align_steps(steps);
   for (size_t s = 0; s < steps; ++s) {
-    int cnt = 0;
-    while (cnt < kLanes && lane[cnt].size() > s) ++cnt;
-    prog.step_off.push_back((int32_t)prog.instr.size());
-    prog.step_cnt.push_back(cnt);
-    for (int l = 0; l < cnt; ++l) prog.instr.push_back(lane[l][s]);
+    for (int l = 0; l < kLanes; ++l) {
+      Instr in = s < lane[l].size() ? lane[l][s] : nop_instr(zero);
+      if (s + 1 == steps) in.t |= kLevelEnd | (best_lw << kWidthShift);
+      prog.instr.push_back(in);
+    }
   }
   const int32_t prev = prog.level_end.empty() ? 0 : prog.level_end.back();
   prog.level_end.push_back(prev + (int32_t)steps);
@@ -167,8 +226,9 @@ void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
   const int32_t ZERO = (int32_t)s.nnzL;  // V[nnzL] holds 0.0
   // ---- factor program ----
   s.factor_updates = 0;
+  if (s.nnzL + 2 > kTMask) throw std::runtime_error("factor too large for the program encoding");
   for (int lev = 0; lev < s.nlevels; ++lev) {
-    std::vector<std::vector<Instr>> tasks;
+    std::vector<Task> tasks;
     for (int32_t j : bylevel[lev]) {
       const int32_t cj = s.colptr[j];
       const int32_t nrow = s.colptr[j + 1] - cj;  // targets: diag + sub-diagonal
@@ -191,40 +251,34 @@ void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
           v.push_back(Instr{ZERO, ZERO, ZERO, cj});
         }
         s.factor_updates += (int64_t)v.size();
-        v.front().t |= kFirst;
-        v.back().t |= kLast | (diag ? kDiag : 0);
-        tasks.push_back(std::move(v));
+        tasks.push_back(Task{std::move(v), diag});
       }
     }
-    emit_level(tasks, s.factor);
+    emit_level(tasks, s.factor, ZERO);
   }
   // ---- forward solve: z_j = invD_j * (b_j - sum_k w_jk z_k) ----
   for (int lev = 0; lev < s.nlevels; ++lev) {
-    std::vector<std::vector<Instr>> tasks;
+    std::vector<Task> tasks;
     for (int32_t j : bylevel[lev]) {
       std::vector<Instr> v;
       for (int32_t k : rowst[j])
         v.push_back(Instr{(int32_t)s.pos(j, k), k, s.colptr[j], j});
       if (v.empty()) v.push_back(Instr{ZERO, j, s.colptr[j], j});
-      v.front().t |= kFirst;
-      v.back().t |= kLast;
-      tasks.push_back(std::move(v));
+      tasks.push_back(Task{std::move(v), false});
     }
-    emit_level(tasks, s.fwd);
+    emit_level(tasks, s.fwd, ZERO);
   }
   // ---- backward solve: x_j = z_j - invD_j * sum_i w_ij x_i (top level first) ----
   for (int lev = s.nlevels - 1; lev >= 0; --lev) {
-    std::vector<std::vector<Instr>> tasks;
+    std::vector<Task> tasks;
     for (int32_t j : bylevel[lev]) {
       std::vector<Instr> v;
       for (int32_t p = s.colptr[j] + 1; p < s.colptr[j + 1]; ++p)
         v.push_back(Instr{p, s.rowidx[p], s.colptr[j], j});
       if (v.empty()) continue;
-      v.front().t |= kFirst;
-      v.back().t |= kLast;
-      tasks.push_back(std::move(v));
+      tasks.push_back(Task{std::move(v), false});
     }
-    emit_level(tasks, s.bwd);
+    emit_level(tasks, s.bwd, ZERO);
   }
 }
 

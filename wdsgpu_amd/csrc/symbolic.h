@@ -13,10 +13,10 @@ namespace wds {
 constexpr int kLanes = 64;
 
 // One instruction of a level-scheduled sparse dot-product program.  Each lane
-// of the wavefront owns its own stream.  Within a level the lanes are sorted
-// by decreasing work, so at step s the active lanes are 0..cnt[s]-1 and lane
-// l reads instr[off[s] + l]: the loads of a step are contiguous and padding
-// costs no memory traffic.
+// of the wavefront owns its own stream; streams are stored step-major with a
+// fixed stride of 64, so lane l reads instr[s * 64 + l] at step s: the loads
+// of a step are one contiguous 1 KiB line set, their addresses depend on
+// nothing loaded, and a deep prefetch can run ahead.  Idle lanes hold NOPs.
 //   factor: acc -= V[a] * V[b] * V[c]     (w_ik * w_jk * invD_k)
 //   fwd:    acc -= V[a] * X[b]            (w_jk * z_k)
 //   bwd:    acc += V[a] * X[b]            (w_ij * x_i)
@@ -24,18 +24,32 @@ constexpr int kLanes = 64;
 struct Instr {
   int32_t a, b, c, t;
 };
-constexpr int32_t kFirst = 1 << 30;   // acc starts from the target (or 0)
-constexpr int32_t kLast = 1 << 29;    // store acc into the target after this op
-constexpr int32_t kDiag = 1 << 28;    // factor: target is a diagonal -> store 1/acc
-constexpr int32_t kNop = 1 << 27;     // padding, no load/store
-constexpr int32_t kTMask = (1 << 27) - 1;
+constexpr int32_t kFirst = 1 << 30;     // acc starts from the target (or 0)
+constexpr int32_t kLast = 1 << 29;      // store acc into the target after this op
+constexpr int32_t kDiag = 1 << 28;      // factor: target is a diagonal -> store 1/acc
+constexpr int32_t kNop = 1 << 27;       // padding, no load/store
+constexpr int32_t kLevelEnd = 1 << 26;  // last step of a level (set in every lane)
+constexpr int kWidthShift = 23;         // bits 23-25: log2 of the level's group width
+constexpr int32_t kZeroInit = 1 << 22;  // wide level: a slice's partial sum starts at 0
+constexpr int32_t kTMask = (1 << 22) - 1;
+// estimated cost of a wide level's group reduction and finalisation, in steps
+constexpr size_t kWideOverheadSteps = 2;
+
+// one dot-product task of a level: its updates (flags are added by the
+// scheduler) and whether its target is a diagonal pivot
+struct Task {
+  std::vector<Instr> ins;
+  bool diag;
+};
+
+// Padding instruction: multiplies by V[zero] == 0.0 (X[0] is a valid
+// operand of the solves) and stores nowhere, so the executor needs no branch.
+inline Instr nop_instr(int32_t zero) { return Instr{zero, 0, zero, kNop}; }
 
 struct Program {
-  std::vector<Instr> instr;            // compacted, step-major
-  std::vector<int32_t> step_off;       // per step: first instruction
-  std::vector<int32_t> step_cnt;       // per step: active lanes (prefix)
+  std::vector<Instr> instr;            // step-major, nsteps * 64
   std::vector<int32_t> level_end;      // cumulative steps at each level end
-  int64_t useful = 0;                  // instructions (== instr.size())
+  int64_t useful = 0;                  // non-padding instructions
   int32_t nsteps() const { return level_end.empty() ? 0 : level_end.back(); }
 };
 // steps of every level are padded to a multiple of this so an unrolled
