@@ -38,6 +38,10 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--copy-calibrate", action="store_true",
+                    help="also run one plain 512 MiB read + 512 MiB write elementwise "
+                         "kernel, the reference point that scales rocprofv3 FETCH_SIZE / "
+                         "WRITE_SIZE counters to bytes")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -98,6 +102,17 @@ def main():
                       "prvs": st["npcv"], "emitters": ne, "accuracy": wl.accuracy,
                       "polish_steps": wl.polish_steps},
            "roofline": roofline}
+    traffic = measured_traffic()
+    if traffic is not None:
+        roofline["traffic"] = traffic["bytes_per_launch"]
+        roofline["traffic_source"] = traffic["source"]
+    if args.copy_calibrate:
+        n = 512 * 2 ** 20 // 8
+        src = torch.ones(n, dtype=torch.float64, device=dev)
+        dst = torch.empty_like(src)
+        torch.mul(src, 1.0, out=dst)
+        torch.cuda.synchronize()
+        del src, dst
     # ---- plain per-scenario GPU GGA with finite-difference gradients ----
     if not args.no_plain_baseline:
         out["plain_fd_baseline"] = plain_fd(wl)
@@ -107,6 +122,24 @@ def main():
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(wl, args.cpu_sample)
     print(json.dumps(out), flush=True)
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_solve_planar782_B1024_traffic.md")
+
+
+def measured_traffic():
+    """HBM bytes per k_solve launch of this workload from the committed
+    rocprofv3 FETCH_SIZE + WRITE_SIZE summary (counter KiB scaled by what the
+    same counters read for a copy kernel of known size), or None."""
+    if not os.path.exists(TRAFFIC_FILE):
+        return None
+    import re
+    txt = open(TRAFFIC_FILE).read()
+    m = re.search(r"^traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt, re.M)
+    if not m:
+        return None
+    return {"bytes_per_launch": int(float(m.group(1))),
+            "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
 
 
 def plain_fd(wl, n_coords=80):

@@ -85,6 +85,16 @@ struct DevPlan {
   const int32_t* prv_pos_off; // [p] L position of (j1,j2) lower
   const double* prv_hset;     // [p]
   const double* prv_km;       // [p]
+  // elimination-tree paths of the valve ends (path a < p: n2 of valve a,
+  // path p + a: n1 of valve a).  y_u = L^-1 e_u is nonzero only on the path
+  // from u to its root, so unit solves with the factor cost O(height^2).
+  int32_t np, wp_tot;
+  const int32_t* wp_ptr;      // [np+1] path -> range of path nodes
+  const int32_t* wp_node;     // [wp_tot] slot (elimination index) of each path node
+  const int32_t* wp_diag;     // [wp_tot] V position of the node's reciprocal pivot
+  const int32_t* wt_ptr;      // [wp_tot+1] path node -> range of its terms
+  const int2* wt;             // [..] (V position of w(node, m), path-node index of m)
+  const int4* wpair;          // [np*np] common suffix of paths u, v: (start u, start v, length)
   // tank links (tankstatus)
   const int32_t* tl_k;        // [ntl]
   const int32_t* tl_tank;     // [ntl] slot of the tank

@@ -85,6 +85,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
          o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
          o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
+  size_t o_wpp = bb.add(P->wp_ptr), o_wpn = bb.add(P->wp_node), o_wpd = bb.add(P->wp_diag),
+         o_wtp = bb.add(P->wt_ptr), o_wt = bb.add(P->wt), o_wpr = bb.add(P->wpair);
   size_t o_c2l = bb.add(P->csr_to_L), o_dp = bb.add(P->diag_pos), o_rs = bb.add(P->row_slot);
   size_t fi, fs, fl, wi, ws, wl, bi, bs, bl;
   prog_vectors(S.factor, fi, fs, fl);
@@ -154,6 +156,14 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.prv_pos_off = at<int32_t>(d, o_po);
   D.prv_hset = at<double>(d, o_hs);
   D.prv_km = at<double>(d, o_pkm);
+  D.np = 2 * P->p;
+  D.wp_tot = (int32_t)P->wp_node.size();
+  D.wp_ptr = at<int32_t>(d, o_wpp);
+  D.wp_node = at<int32_t>(d, o_wpn);
+  D.wp_diag = at<int32_t>(d, o_wpd);
+  D.wt_ptr = at<int32_t>(d, o_wtp);
+  D.wt = at<int2>(d, o_wt);
+  D.wpair = at<int4>(d, o_wpr);
   D.tl_k = at<int32_t>(d, o_tk);
   D.tl_tank = at<int32_t>(d, o_tt);
   D.tl_other = at<int32_t>(d, o_to);
@@ -188,7 +198,8 @@ extern "C" void wds_plan_destroy(wds_plan* P) {
 
 extern "C" int64_t wds_workspace_bytes(const wds_plan* P, int32_t B) {
   if (!P || B <= 0) return WDS_EINVAL;
-  return (int64_t)B * ws_doubles(P->L, P->Nj, (int32_t)P->sym.nnzL) * 8;
+  return (int64_t)B * ws_doubles(P->L, P->Nj, (int32_t)P->sym.nnzL,
+                                   (int32_t)P->wp_node.size(), 2 * P->p) * 8;
 }
 
 extern "C" int wds_counters_get(wds_plan* P, wds_counters* out, int32_t reset) {

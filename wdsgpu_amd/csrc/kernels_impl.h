@@ -6,10 +6,11 @@ namespace wds {
 
 constexpr int64_t kMaxLds = 160 * 1024;
 
-// per-scenario global workspace, in doubles
-__host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t nnzL) {
-  int64_t n = 2 * (int64_t)L + nnzL + 12 * (int64_t)Nj + 2 * (int64_t)kMaxPrv * Nj +
-              ((int64_t)L + 7) / 8 + 512;
+// per-scenario global workspace, in doubles (wp_tot path nodes, np paths)
+__host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t nnzL,
+                                              int32_t wp_tot, int32_t np) {
+  int64_t n = 2 * (int64_t)L + nnzL + 12 * (int64_t)Nj + 2 * (int64_t)wp_tot +
+              (int64_t)np * np + ((int64_t)L + 7) / 8 + 512;
   return (n + 31) / 32 * 32;
 }
 
@@ -141,7 +142,9 @@ struct Scn {
   double* v3;   // [Nj]
   double* v4;   // [Nj]
   double* v5;   // [Nj]
-  double* Sm;   // [2*kMaxPrv][Nj]
+  double* Yp;   // [wp_tot] y_u = L^-1 e_u on the valve-end paths
+  double* Zp;   // [wp_tot] z_u = D^-1 y_u
+  double* Cm;   // [np*np] e_u^T A-hat^-1 e_v between valve ends
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
   // per-scenario inputs / state
@@ -160,7 +163,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.R = c.Hn + Nj + pl.Nf;
   c.X2 = c.R + Nj;
   c.sh = c.X2 + Nj;
-  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.nnzL);
+  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.nnzL, pl.wp_tot, pl.np);
   c.P = w;
   w += L;
   c.Y = w;
@@ -189,8 +192,12 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   w += Nj;
   c.v5 = w;
   w += Nj;
-  c.Sm = w;
-  w += 2 * kMaxPrv * Nj;
+  c.Yp = w;
+  w += pl.wp_tot;
+  c.Zp = w;
+  w += pl.wp_tot;
+  c.Cm = w;
+  w += (int64_t)pl.np * pl.np;
   c.st0 = reinterpret_cast<int8_t*>(w);
   w += ((int64_t)L + 7) / 8;
   c.misc = w;

@@ -10,14 +10,15 @@
 
 namespace wds {
 
-// Refinement steps of the Woodbury column solves (A-hat^-1 e_u) and of the
-// correction solves inside the outer refinement on the row-replaced system.
-// Errors of order kappa * eps in these inner solves are contracted by the
-// two outer refinement steps, which evaluate the residual of the true
-// replaced system; the first solve of the right-hand side keeps the full
-// refinement of the reference (dgga/autodiff.py:965-1029, 1107-1154).
-constexpr int kWoodColRefine = 0;
-constexpr int kWoodInnerRefine = 1;
+// Woodbury row replacement without column solves.  With A-hat = L D L^T,
+// e_u^T A-hat^-1 e_v = y_u^T D^-1 y_v and e_u^T A-hat^-1 r = y_u^T z, where
+// y_u = L^-1 e_u (nonzero only on the elimination-tree path from u to its
+// root) and z = D^-1 L^-1 r is what the forward program leaves in X.  The
+// capacitance matrix therefore costs O(height^2) per valve end, and the
+// correction A-hat^-1 (r - sum_c w_c e_{u_c}) is applied to z between the
+// forward and backward programs, so one Woodbury solve is one solve pair
+// (the reference solves A-hat for each column, dgga/autodiff.py:965-1029,
+// 1107-1154).
 
 // Coefficients of the polish / adjoint at the current state
 // (_adjgpu_coeffs, dgga/autodiff.py:786-875): link P/Y with pcv PRVs taking
@@ -85,12 +86,6 @@ __device__ inline void solveA(const DevPlan& pl, Scn& c, const double* b, double
   mem_sync();
 }
 
-// unit right-hand side e_u into dst
-__device__ inline void unit_vec(const DevPlan& pl, double* dst, int u) {
-  for (int j = lane_id(); j < pl.Nj; j += 64) dst[j] = (j == u) ? 1.0 : 0.0;
-  mem_sync();
-}
-
 // (A-hat x)_i for all rows into dst (global)
 __device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double* dst) {
   for (int i = lane_id(); i < pl.Nj; i += 64) dst[i] = spmv_row(pl, c.A, x, i);
@@ -142,9 +137,110 @@ __device__ inline bool small_solve(double* M, double* w, int n) {
   return __shfl(ok ? 1 : 0, 0, 64) != 0;
 }
 
+// Paths of the valve ends for the factor in V: y_u, z_u = D^-1 y_u (one
+// lane per path, in path order) and C[u][v] = y_u . z_v over the shared
+// ancestors of u and v.
+__device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
+  const int lane = lane_id();
+  if (lane < pl.np) {
+    const int i0 = pl.wp_ptr[lane], i1 = pl.wp_ptr[lane + 1];
+    for (int i = i0; i < i1; ++i) {
+      double acc = (i == i0) ? 1.0 : 0.0;
+      for (int t = pl.wt_ptr[i]; t < pl.wt_ptr[i + 1]; ++t) {
+        const int2 e = pl.wt[t];
+        acc = fma(-(c.V[e.x] * c.V[pl.wp_diag[e.y]]), c.Yp[e.y], acc);
+      }
+      c.Yp[i] = acc;
+      c.Zp[i] = acc * c.V[pl.wp_diag[i]];
+    }
+  }
+  mem_sync();
+  for (int pr = lane; pr < pl.np * pl.np; pr += 64) {
+    const int4 w = pl.wpair[pr];
+    double s = 0.0;
+    for (int t = 0; t < w.z; ++t) s = fma(c.Yp[w.x + t], c.Zp[w.y + t], s);
+    c.Cm[pr] = s;
+  }
+  mem_sync();
+}
+
+// e_u^T A-hat^-1 r for every valve end u, from z = D^-1 L^-1 r in X (LDS)
+__device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, double* x0) {
+  const int lane = lane_id();
+  if (lane < pl.np) {
+    double s = 0.0;
+    for (int i = pl.wp_ptr[lane]; i < pl.wp_ptr[lane + 1]; ++i)
+      s = fma(c.Yp[i], X[pl.wp_node[i]], s);
+    x0[lane] = s;
+  }
+  mem_sync();
+}
+
+// z -= sum_c w[c] z_{u_c} for the first ncol paths (the forward image of
+// r - sum_c w[c] e_{u_c}); paths of different ends may share nodes, so one
+// path at a time
+__device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const double* w,
+                                  int ncol) {
+  const int lane = lane_id();
+  for (int col = 0; col < ncol; ++col) {
+    const double wc = w[col];
+    for (int i = pl.wp_ptr[col] + lane; i < pl.wp_ptr[col + 1]; i += 64)
+      X[pl.wp_node[i]] -= wc * c.Zp[i];
+    lds_sync();
+  }
+}
+
+// One solve of the row-replaced system into X (LDS) for right-hand side rr
+// (global).  adjoint = false: primal M_p (rows j2 -> dH[j2] = rr[j2], rows
+// j1 merged; 2p columns e_j2, e_j1); adjoint = true: node system M (rows j2
+// -> lam[j2] - lam[j1] = rr[j2]; p columns e_j2).
+__device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, double* X,
+                                  bool adjoint) {
+  const int lane = lane_id();
+  const int p = pl.p;
+  int dummy = 0;
+  double* M = c.misc;          // [n x n], n = 2p or p
+  double* w = c.misc + 272;    // [n]
+  double* x0 = c.misc + 288;   // [2p]
+  for (int j = lane; j < pl.Nj; j += 64) X[j] = rr[j];
+  lds_sync();
+  run_program<1>(pl.fwd, c.V, X, dummy);
+  wood_probe(pl, c, X, x0);
+  const int n = adjoint ? p : 2 * p;
+  if (lane == 0) {
+    for (int a = 0; a < p; ++a) {
+      const double act = c.sh[a];
+      const int j2 = pl.prv_s2[a];
+      for (int col = 0; col < n; ++col) {
+        const double s2 = c.Cm[a * pl.np + col];          // e_j2a^T A^-1 e_col
+        if (adjoint) {
+          const double s1 = c.Cm[(p + a) * pl.np + col];  // e_j1a^T A^-1 e_col
+          M[a * n + col] = (s2 - s1) * (act * c.sh[col]) + ((a == col) ? 1.0 - act : 0.0);
+        } else {
+          const double eye = (col == a || col == a + p) ? 1.0 : 0.0;
+          M[a * n + col] = s2 * act + ((a == col) ? 1.0 - act : 0.0);
+          M[(a + p) * n + col] = (eye - CBIG * s2) * act + ((a + p == col) ? 1.0 - act : 0.0);
+        }
+      }
+      if (adjoint) {
+        w[a] = (x0[a] - x0[p + a] - rr[j2]) * act;
+      } else {
+        w[a] = (x0[a] - rr[j2]) * act;
+        w[a + p] = (rr[j2] - CBIG * x0[a]) * act;
+      }
+    }
+  }
+  mem_sync();
+  if (!small_solve(M, w, n)) return false;
+  wood_apply(pl, c, X, w, n);
+  run_program<2>(pl.bwd, c.V, X, dummy);
+  lds_sync();
+  return true;
+}
+
 // Primal reduced Newton system M_p dH = r_p (_adjgpu_primal_solve,
 // dgga/autodiff.py:965-1029).  rhs_p in c.v1, r_prv in misc[256..]; result
-// in dst.  Uses v3 (rp), v4 (work), Sm (2p columns).
+// in dst.  Uses v3 (rp), v4 (work), v5 (residual) and X2.
 __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   const int lane = lane_id();
   const int p = pl.p;
@@ -157,8 +253,6 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   double* rp = c.v3;
   double* wk = c.v4;
   const double* rprv = c.misc + 256;
-  double* M = c.misc;           // [2p x 2p]
-  double* w = c.misc + 272;     // [2p]
   for (int j = lane; j < pl.Nj; j += 64) rp[j] = c.v1[j];
   mem_sync();
   if (lane == 0) {
@@ -168,41 +262,11 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
       rp[pl.prv_s2[i]] = (c.sh[i] != 0.0) ? -rprv[i] : c.v1[pl.prv_s2[i]];
   }
   mem_sync();
-  // Woodbury columns S = A-hat^-1 [e_j2a..., e_j1a...]
-  for (int col = 0; col < 2 * p; ++col) {
-    const int u = col < p ? pl.prv_s2[col] : pl.prv_s1[col - p];
-    unit_vec(pl, wk, u);
-    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, kWoodColRefine);
-  }
-  auto wood = [&](double* x, const double* rr) -> bool {
-    // capacitance matrix C_eff (rebuilt each call: small_solve destroys it)
-    if (lane == 0) {
-      const int n = 2 * p;
-      for (int a = 0; a < p; ++a) {
-        const double act = c.sh[a];
-        const int j2 = pl.prv_s2[a];
-        for (int col = 0; col < n; ++col) {
-          const double r2 = c.Sm[(int64_t)col * pl.Nj + j2];
-          const double eye = (col == a || col == a + p) ? 1.0 : 0.0;
-          M[a * n + col] = r2 * act + ((a == col) ? 1.0 - act : 0.0);
-          M[(a + p) * n + col] = (eye - CBIG * r2) * act + ((a + p == col) ? 1.0 - act : 0.0);
-        }
-        w[a] = (x[j2] - rr[j2]) * act;
-        w[a + p] = (rr[j2] - CBIG * x[j2]) * act;
-      }
-    }
-    mem_sync();
-    if (!small_solve(M, w, 2 * p)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) {
-      double s = x[j];
-      for (int col = 0; col < 2 * p; ++col) s -= c.Sm[(int64_t)col * pl.Nj + j] * w[col];
-      x[j] = s;
-    }
-    mem_sync();
-    return true;
-  };
-  solveA(pl, c, rp, dst, 2);
-  if (!wood(dst, rp)) return false;
+  wood_paths(pl, c);
+  if (!wood_solve(pl, c, rp, c.X2, false)) return false;
+  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.X2[j];
+  mem_sync();
+  // two refinement steps on the residual of the true replaced system
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);  // base = A-hat dH
     double* res = c.v5;
@@ -218,9 +282,8 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
       }
     }
     mem_sync();
-    solveA(pl, c, res, wk, kWoodInnerRefine);
-    if (!wood(wk, res)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
+    if (!wood_solve(pl, c, res, c.X2, false)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.X2[j];
     mem_sync();
   }
   return true;
@@ -357,42 +420,16 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   double* r2 = c.v3;
   double* wk = c.v4;
   const double* gqp = c.misc + 256;
-  double* M = c.misc;
-  double* w = c.misc + 272;
   for (int j = lane; j < pl.Nj; j += 64) r2[j] = c.v1[j];
   mem_sync();
   if (lane == 0)
     for (int i = 0; i < p; ++i)
       if (c.sh[i] != 0.0) r2[pl.prv_s2[i]] = gqp[i];
   mem_sync();
-  for (int col = 0; col < p; ++col) {
-    unit_vec(pl, wk, pl.prv_s2[col]);
-    solveA(pl, c, wk, c.Sm + (int64_t)col * pl.Nj, kWoodColRefine);
-  }
-  auto wood = [&](double* x, const double* rr) -> bool {
-    if (lane == 0) {
-      for (int a = 0; a < p; ++a) {
-        const double acta = c.sh[a];
-        const int j1 = pl.prv_s1[a], j2 = pl.prv_s2[a];
-        for (int col = 0; col < p; ++col) {
-          const double cw = c.Sm[(int64_t)col * pl.Nj + j2] - c.Sm[(int64_t)col * pl.Nj + j1];
-          M[a * p + col] = cw * (acta * c.sh[col]) + ((a == col) ? 1.0 - acta : 0.0);
-        }
-        w[a] = (x[j2] - x[j1] - rr[j2]) * acta;
-      }
-    }
-    mem_sync();
-    if (!small_solve(M, w, p)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) {
-      double s = x[j];
-      for (int col = 0; col < p; ++col) s -= c.Sm[(int64_t)col * pl.Nj + j] * w[col];
-      x[j] = s;
-    }
-    mem_sync();
-    return true;
-  };
-  solveA(pl, c, r2, dst, n_ref);
-  if (!wood(dst, r2)) return false;
+  wood_paths(pl, c);
+  if (!wood_solve(pl, c, r2, c.X2, true)) return false;
+  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.X2[j];
+  mem_sync();
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);
     double* res = c.v5;
@@ -405,9 +442,8 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
         res[j2] = r2[j2] - (dst[j2] - dst[j1]);
       }
     mem_sync();
-    solveA(pl, c, res, wk, kWoodInnerRefine);
-    if (!wood(wk, res)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) dst[j] += wk[j];
+    if (!wood_solve(pl, c, res, c.X2, true)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.X2[j];
     mem_sync();
   }
   return true;

@@ -212,6 +212,53 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   P->p = (int32_t)P->prv_k.size();
   if (P->p > kMaxPrv)
     throw std::domain_error("more than " + std::to_string(kMaxPrv) + " PRVs with a setting");
+  // ---- elimination-tree paths of the valve ends ----
+  // The Woodbury corrections of the polish and adjoint need A-hat^-1 e_u only
+  // through y_u = L^-1 e_u, whose nonzeros lie on the etree path from u to its
+  // root: store each path, the pattern entries L(k_i, k_m) between its nodes,
+  // and the common suffix (shared ancestors) of every pair of paths.
+  {
+    std::vector<int32_t> ends;
+    for (int a = 0; a < P->p; ++a) ends.push_back(P->prv_s2[a]);
+    for (int a = 0; a < P->p; ++a) ends.push_back(P->prv_s1[a]);
+    P->wp_ptr.assign(1, 0);
+    P->wp_node.clear();
+    P->wp_diag.clear();
+    P->wt_ptr.assign(1, 0);
+    P->wt.clear();
+    std::vector<std::vector<int32_t>> paths;
+    for (int32_t u : ends) {
+      std::vector<int32_t> path;
+      for (int32_t k = u; k >= 0; k = S.parent[k]) path.push_back(k);
+      const int32_t base = (int32_t)P->wp_node.size();
+      for (size_t i = 0; i < path.size(); ++i) {
+        P->wp_node.push_back(path[i]);
+        P->wp_diag.push_back(S.colptr[path[i]]);
+        for (size_t m = 0; m < i; ++m) {
+          const int64_t ps = S.pos(path[i], path[m]);
+          if (ps < 0) continue;
+          P->wt.push_back((int32_t)ps);
+          P->wt.push_back(base + (int32_t)m);
+        }
+        P->wt_ptr.push_back((int32_t)(P->wt.size() / 2));
+      }
+      P->wp_ptr.push_back((int32_t)P->wp_node.size());
+      paths.push_back(std::move(path));
+    }
+    const int np = (int)ends.size();
+    P->wpair.clear();
+    for (int u = 0; u < np; ++u)
+      for (int v = 0; v < np; ++v) {
+        const auto& a = paths[u];
+        const auto& b = paths[v];
+        size_t n = 0;
+        while (n < a.size() && n < b.size() && a[a.size() - 1 - n] == b[b.size() - 1 - n]) ++n;
+        P->wpair.push_back(P->wp_ptr[u] + (int32_t)(a.size() - n));
+        P->wpair.push_back(P->wp_ptr[v] + (int32_t)(b.size() - n));
+        P->wpair.push_back((int32_t)n);
+        P->wpair.push_back(0);
+      }
+  }
   // ---- tank links (dgga/solver.py:536-568) ----
   std::vector<char> is_tank(N, 0), ovf(N, 0);
   std::vector<double> thmin(N, 0.0), thmax(N, 0.0);

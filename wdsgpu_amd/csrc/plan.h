@@ -22,6 +22,7 @@ struct wds_plan {
       spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, tl_k, tl_tank, tl_other,
       tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;
   std::vector<int32_t> spmv_ent;  // pairs (pos, col)
+  std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<double> prv_hset, prv_km, tl_hmax, tl_hmin;
   // scalars
   int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
