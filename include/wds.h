@@ -165,7 +165,8 @@ int wds_newton_iter(wds_plan* plan, void* stream, const wds_solve_args* args,
 
 /* Backward of the implicit solve (replaces ImplicitGGASolveGPU.backward,
  * dgga/autodiff.py:1245-1312): reuses the forward's terminal factor.  gH/gE
- * [B][N], gQ [B][L]; outputs g_demand/g_res_head/g_ke [B][N], g_r [B][L]. */
+ * [B][N], gQ [B][L] (each may be NULL: a zero cotangent); outputs
+ * g_demand/g_res_head/g_ke [B][N], g_r [B][L]. */
 typedef struct wds_adjoint_args {
   int32_t B;
   const double* demand;        /* [B][N] */
@@ -194,6 +195,23 @@ int wds_adjoint(wds_plan* plan, void* stream, const wds_adjoint_args* args);
 
 int wds_counters_get(wds_plan* plan, wds_counters* out, int32_t reset);
 const char* wds_last_error(void);
+
+/* Parameter maps of the calibration path, elementwise on the stream.
+ * Hazen-Williams resistance r = 4.727 L / C^1.852 / D^4.871 (replaces
+ * hw_resistance, dgga/calib.py:64-86) over n = B*L entries of C with len/diam
+ * [L] and an optional [L] mask (0 -> r = 0); its gradient
+ * g_C = g_r * (-1.852 r / C) (dr_dC, dgga/calib.py:89-108).
+ * Emitter coefficient Ke = coef / C^qexp for C > 0, else 0, with
+ * coef = Ucf_Q^qexp / Ucf_P (ke_int_of_C, dgga/parse.py:557-568) and
+ * g_C = g_ke * (-qexp Ke / C). */
+int wds_hw_resistance(void* stream, int64_t n, int32_t L, const double* C, const double* len,
+                      const double* diam, const int8_t* mask, double* r);
+int wds_hw_resistance_grad(void* stream, int64_t n, const double* C, const double* r,
+                           const double* g_r, double* g_C);
+int wds_emitter_ke(void* stream, int64_t n, const double* C, double qexp, double coef,
+                   double* ke);
+int wds_emitter_ke_grad(void* stream, int64_t n, const double* C, const double* ke,
+                        double qexp, const double* g_ke, double* g_C);
 
 #ifdef __cplusplus
 }

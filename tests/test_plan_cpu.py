@@ -8,6 +8,7 @@ import re
 
 import numpy as np
 import pytest
+import torch
 
 from oracle import gga
 from wdsgpu_amd import _lib
@@ -162,3 +163,12 @@ def test_solver_refuses_cpu_execution():
     s = GGASolver(net, device="cpu", dense_status_machine=True)
     with pytest.raises(NotImplementedError, match="HIP device"):
         s.solve(net.demand_cfs, np.nan_to_num(net.reservoir_head_ft()), status_machine=True)
+
+
+def test_parameter_maps_refuse_cpu_tensors():
+    from wdsgpu_amd.autodiff import emitter_ke, hw_resistance
+    one = torch.ones(3, dtype=torch.float64)
+    with pytest.raises(NotImplementedError, match="HIP device"):
+        hw_resistance(one, one, one)
+    with pytest.raises(NotImplementedError, match="HIP device"):
+        emitter_ke(one)

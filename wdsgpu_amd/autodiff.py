@@ -66,19 +66,21 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             _lib.check(s._timed("k_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
                                                                    C.byref(a)),
                                 B, out["iters"]), "wds_solve")
-        info = out["info"]
-        if bool((info != 0).any()):
-            bad = torch.nonzero(info != 0).flatten().tolist()
+        info, iters = s._host_status(out, B)
+        if (info != 0).any():
+            bad = np.nonzero(info != 0)[0].tolist()
             raise RuntimeError("adjoint='gpu': samples %s failed in the solve or the polish "
                                "(info %s)" % (bad[:8], info[bad[:8]].tolist()))
         maxtrials = s.maxtrials(max_iter, sm)
-        unconv = out["iters"].to(torch.int64) > maxtrials
-        if bool(unconv.any()):
-            bad = torch.nonzero(unconv).flatten().tolist()
+        if (iters > maxtrials).any():
+            bad = np.nonzero(iters > maxtrials)[0].tolist()
             raise RuntimeError(
                 "adjoint='gpu': scenarios %s did not converge (relerr %s > accuracy %g); "
                 "the adjoint is undefined there" % (bad[:8], out["relerr"][bad[:4]].tolist(),
                                                    float(accuracy)))
+        # outputs that do not reach the loss arrive in backward as None and
+        # are passed to the kernel as null pointers (zero cotangents)
+        ctx.set_materialize_grads(False)
         ctx.solver = s
         ctx.keep = keep
         ctx.out = out
@@ -102,7 +104,7 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
 
         def _g(g, ref):
             if g is None:
-                return torch.zeros_like(ref)
+                return None
             g = g.detach().to(dtype=s.dtype, device=s.device)
             return (g.unsqueeze(0).expand_as(ref) if g.dim() == 1 else g).contiguous()
 
@@ -115,7 +117,7 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         grh = torch.empty_like(gd)
         gke = torch.empty_like(gd)
         gr = torch.empty(B, s.L, dtype=s.dtype, device=dev)
-        info = torch.zeros(B, dtype=torch.int32, device=dev)
+        info = torch.empty(B, dtype=torch.int32, device=dev)
         ws = s._workspace(B)
         a = _lib.AdjointArgs(
             B=B, demand=_lib.ptr(dkeep), ke=_lib.ptr(ke),
@@ -130,8 +132,11 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
                                                                        C.byref(a)), B),
                        "wds_adjoint")
-        if bool((info != 0).any()):
-            bad = torch.nonzero(info != 0).flatten().tolist()
+        binfo = s._pinned_info(B)
+        binfo.copy_(info, non_blocking=True)
+        torch.cuda.current_stream(dev).synchronize()
+        if (binfo.numpy() != 0).any():
+            bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
             raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
         dm, dv = ctx.dims, ctx.in_devs
 
@@ -170,16 +175,84 @@ def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
                                      linear_solver, int(polish_steps))
 
 
-def hw_resistance(C_hw, len_ft, diam_ft):
+def _stream_of(t):
+    return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
+
+
+def _hip_tensor(t, what):
+    if not (isinstance(t, torch.Tensor) and t.is_cuda and t.dtype == torch.float64):
+        raise NotImplementedError("%s runs on a HIP device in float64 only" % what)
+    return t.contiguous()
+
+
+class _HWResistance(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, C_hw, len_ft, diam_ft, mask):
+        Ct = _hip_tensor(C_hw, "hw_resistance")
+        L = int(len_ft.numel())
+        lt = _hip_tensor(len_ft, "hw_resistance")
+        dt = _hip_tensor(diam_ft, "hw_resistance")
+        if Ct.shape[-1] != L or dt.numel() != L:
+            raise ValueError("hw_resistance: C [..., L] with len/diam [L]")
+        mk = None if mask is None else mask.to(device=Ct.device, dtype=torch.int8).contiguous()
+        r = torch.empty_like(Ct)
+        lib = _lib.load()
+        with torch.cuda.device(Ct.device):
+            _lib.check(lib.wds_hw_resistance(_stream_of(Ct), Ct.numel(), L, _lib.ptr(Ct),
+                                             _lib.ptr(lt), _lib.ptr(dt), _lib.ptr(mk),
+                                             _lib.ptr(r)), "wds_hw_resistance")
+        ctx.save_for_backward(Ct, r)
+        return r
+
+    @staticmethod
+    def backward(ctx, g_r):
+        Ct, r = ctx.saved_tensors
+        g = g_r.contiguous()
+        gC = torch.empty_like(Ct)
+        lib = _lib.load()
+        with torch.cuda.device(Ct.device):
+            _lib.check(lib.wds_hw_resistance_grad(_stream_of(Ct), Ct.numel(), _lib.ptr(Ct),
+                                                  _lib.ptr(r), _lib.ptr(g), _lib.ptr(gC)),
+                       "wds_hw_resistance_grad")
+        return gC, None, None, None
+
+
+class _EmitterKe(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, c_user, qexp, coef):
+        Ct = _hip_tensor(c_user, "emitter_ke")
+        ke = torch.empty_like(Ct)
+        lib = _lib.load()
+        with torch.cuda.device(Ct.device):
+            _lib.check(lib.wds_emitter_ke(_stream_of(Ct), Ct.numel(), _lib.ptr(Ct),
+                                          float(qexp), float(coef), _lib.ptr(ke)),
+                       "wds_emitter_ke")
+        ctx.save_for_backward(Ct, ke)
+        ctx.qexp = float(qexp)
+        return ke
+
+    @staticmethod
+    def backward(ctx, g_ke):
+        Ct, ke = ctx.saved_tensors
+        g = g_ke.contiguous()
+        gC = torch.empty_like(Ct)
+        lib = _lib.load()
+        with torch.cuda.device(Ct.device):
+            _lib.check(lib.wds_emitter_ke_grad(_stream_of(Ct), Ct.numel(), _lib.ptr(Ct),
+                                               _lib.ptr(ke), ctx.qexp, _lib.ptr(g),
+                                               _lib.ptr(gC)), "wds_emitter_ke_grad")
+        return gC, None, None
+
+
+def hw_resistance(C_hw, len_ft, diam_ft, mask=None):
     """Differentiable Hazen-Williams r = 4.727 L / C^1.852 / D^4.871
-    (dgga/calib.py:64-86, same operation order)."""
-    return 4.727 * len_ft / C_hw ** 1.852 / diam_ft ** 4.871
+    (dgga/calib.py:64-86, same operation order; gradient dr/dC = -1.852 r / C,
+    calib.py:89-108) as one HIP kernel each way.  C is [L] or [B, L]; mask
+    [L] (optional) zeroes r on links that are not pipes."""
+    return _HWResistance.apply(C_hw, len_ft, diam_ft, mask)
 
 
 def emitter_ke(c_user, qexp=2.0, ucf_flow=LPS_PER_CFS, ucf_pressure=M_PER_FT):
     """Differentiable emitter coefficient Ke = (Ucf_Q^Qexp / Ucf_P) / C^Qexp
-    (dgga/parse.py:557-568); zero where C == 0."""
-    pos = c_user > 0
-    safe = torch.where(pos, c_user, torch.ones_like(c_user))
-    return torch.where(pos, (ucf_flow ** qexp / ucf_pressure) / safe ** qexp,
-                       torch.zeros_like(c_user))
+    (dgga/parse.py:557-568); zero where C == 0.  One HIP kernel each way."""
+    return _EmitterKe.apply(c_user, qexp, ucf_flow ** qexp / ucf_pressure)

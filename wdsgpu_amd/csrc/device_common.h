@@ -60,12 +60,15 @@ __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 // Level-scheduled program executor.  MODE 0: LDL^T factorisation in place in
 // V (unscaled columns w, reciprocal pivots on the diagonal); MODE 1: forward
 // solve z = D^-1 L^-1 x in place in X; MODE 2: backward solve x = L^-T z.
-// A ring of kRing chunks (kRing * kUnroll steps) of instruction loads stays in
-// flight: a chunk is refilled with the chunk kRing ahead as soon as it is
-// consumed, and instruction addresses depend on nothing loaded, so the loads
-// never wait on each other.  The instruction array carries kRing * kUnroll
-// trailing NOP steps so the ring can over-fetch without bounds checks.
+// kRing chunks (kRing * kUnroll steps) of instruction loads stay in flight in
+// a ring of kRing + 1 register slots: chunk c lives in slot c % (kRing + 1),
+// and the load of chunk c + kRing goes into the slot the previous chunk just
+// released, so no slot is copied and each wait covers only the chunk about to
+// run.  Instruction addresses depend on nothing loaded; the instruction array
+// carries kRing * kUnroll trailing NOP steps so the ring can over-fetch
+// without bounds checks.
 constexpr int kRing = 4;
+constexpr int kSlots = kRing + 1;
 
 // 1/x for a positive pivot: v_rcp_f64 refined by two Newton steps
 __device__ __forceinline__ double recip(double x) {
@@ -77,13 +80,25 @@ __device__ __forceinline__ double recip(double x) {
 
 static_assert(kStepAlign % kUnroll == 0, "levels must start on a chunk boundary");
 
+// Explicit address spaces for the executor: LDS operands compile to ds_*
+// (counted by lgkmcnt only) and instruction words to global loads (vmcnt
+// only), so waiting for an operand never drains the instruction prefetch, as
+// it would with flat accesses through generic pointers.
+typedef __attribute__((address_space(3))) double lds_f64;
+typedef int i32x4 __attribute__((ext_vector_type(4)));
+typedef const __attribute__((address_space(1))) i32x4 glb_i32x4;
+
 template <int MODE>
-__device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
+__device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad) {
   const int lane = lane_id();
   const int ns = pg.nsteps;
   if (ns == 0) return;
-  const int4* ip = pg.instr + lane;
-  int4 ring[kRing][kUnroll];
+  lds_f64* const V = (lds_f64*)Vg;
+  lds_f64* const X = (lds_f64*)Xg;
+  // read once: stores through V/X could alias a generic ProgDev reference
+  lds_f64* const dummy = V + pg.dummy;
+  const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
+  i32x4 ring[kSlots][kUnroll];
 #pragma unroll
   for (int d = 0; d < kRing; ++d)
 #pragma unroll
@@ -92,16 +107,17 @@ __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
   // wide levels: the task whose target this lane's group finalises
   int fin_t = 0, fin_c = 0;
   bool fin_diag = false, fin_act = false, lvl_start = true;
-  for (int g0 = 0; g0 < ns; g0 += kRing * kUnroll) {
+  for (int g0 = 0; g0 < ns; g0 += kSlots * kUnroll) {
 #pragma unroll
-    for (int d = 0; d < kRing; ++d) {
-    const int g = g0 + d * kUnroll;
-    if (g >= ns) break;
-    int4 cur[kUnroll];
+    for (int d = 0; d < kSlots; ++d) {
+    const int g = g0 + d * kUnroll;  // ns is a multiple of kSlots * kUnroll
+    constexpr int kNextSlot[kSlots] = {4, 0, 1, 2, 3};  // (d + kRing) % kSlots
+    static_assert(kSlots == 5, "kNextSlot is written for kRing == 4");
+    i32x4 cur[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       cur[u] = ring[d][u];
-      ring[d][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
+      ring[kNextSlot[d]][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
     }
     if (lvl_start) {
       // a wide level's slices all start at its first step, which is the
@@ -115,7 +131,7 @@ __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
     double va[kUnroll], vb[kUnroll], vc[kUnroll], vt[kUnroll];
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
-      const int4 in = cur[u];
+      const i32x4 in = cur[u];
       const int t = in.w & kTMask;
       va[u] = V[in.x];
       vc[u] = V[in.z];
@@ -129,7 +145,6 @@ __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
     }
     // branch-free: NOPs multiply by V[zero] == 0; every step stores, to its
     // target when the instruction is a task's last, else to V[dummy]
-    double* const dummy = V + pg.dummy;
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const int w = cur[u].w;
@@ -138,7 +153,7 @@ __device__ void run_program(const ProgDev& pg, double* V, double* X, int& bad) {
       // a wide slice's first instruction starts its partial sum at zero
       const double init = (MODE == 2 || (w & kZeroInit)) ? 0.0 : vt[u];
       double val;
-      double* dst;
+      lds_f64* dst;
       if (MODE == 0) {
         acc = fma(-va[u], vb[u] * vc[u], first ? init : acc);
         const bool diag = w & kDiag;

@@ -28,7 +28,7 @@ enum LinkFlag : int32_t {
 
 struct ProgDev {
   const int4* instr;        // step-major instructions, nsteps * 64
-  int32_t nsteps;           // multiple of kUnroll
+  int32_t nsteps;           // multiple of kSlots * kUnroll (NOP padded)
   int32_t nlevels;
   int32_t dummy;            // V[dummy] (= nnzL + 1) absorbs non-final stores
 };

@@ -42,6 +42,12 @@ int hip_fail(hipError_t e, const char* what) {
   return WDS_EDEVICE;
 }
 
+// steps the executor runs: the program padded to whole groups of chunks
+int32_t exec_steps(const Program& pr) {
+  constexpr int32_t g = kSlots * kUnroll;
+  return (pr.nsteps() + g - 1) / g * g;
+}
+
 template <class T>
 const T* at(void* base, size_t off) {
   return reinterpret_cast<const T*>(static_cast<char*>(base) + off);
@@ -62,10 +68,10 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   const Symbolic& S = P->sym;
   BlobBuilder bb;
   auto prog_vectors = [&](const Program& pr, size_t& oi, size_t& os, size_t& ol) {
-    // trailing NOP steps: the executor's prefetch ring reads up to
-    // kRing * kUnroll steps past the end
+    // trailing NOP steps: the executor runs whole groups of kSlots chunks and
+    // its prefetch ring reads up to kRing * kUnroll steps past the end
     std::vector<Instr> padded(pr.instr);
-    padded.resize(pr.instr.size() + (size_t)kRing * kUnroll * kLanes,
+    padded.resize((size_t)(exec_steps(pr) + kRing * kUnroll) * kLanes,
                   nop_instr((int32_t)S.nnzL));
     oi = bb.add(padded);
     os = ol = 0;
@@ -178,7 +184,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
     (void)os;
     (void)ol;
     pg.instr = at<int4>(d, oi);
-    pg.nsteps = pr.nsteps();
+    pg.nsteps = exec_steps(pr);
     pg.dummy = (int32_t)S.nnzL + 1;
     pg.nlevels = (int32_t)pr.level_end.size();
   };
@@ -313,7 +319,7 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
 extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a) {
   int r = check_ready(P);
   if (r) return r;
-  if (!a || a->B <= 0 || !a->factor || !a->gH || !a->gQ || !a->gE || !a->g_demand ||
+  if (!a || a->B <= 0 || !a->factor || !a->g_demand ||
       !a->g_res_head || !a->g_ke || !a->g_r || !a->info || !a->workspace || !a->head ||
       !a->flow || !a->emitter || !a->status || !a->ke || !a->r_hw || a->n_refine < 0) {
     set_error("wds_adjoint: bad argument");
@@ -332,4 +338,101 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
   P->cnt.bwd_reuse += 1;
   P->cnt.launches += 1;
   return launch_check("k_adjoint");
+}
+
+// ---------------------------------------------------------------------------
+// Parameter maps (dgga/calib.py:64-108, dgga/parse.py:557-568): elementwise,
+// one lane per entry, grid-stride over n.
+namespace {
+
+__global__ void k_hw_resistance(int64_t n, int32_t L, const double* C, const double* len,
+                                const double* diam, const int8_t* mask, double* r) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int k = (int)(i % L);
+    const bool on = !mask || mask[k];
+    // 4.727 L / C^1.852 / D^4.871 in the reference's operation order
+    r[i] = on ? 4.727 * len[k] / pow(C[i], 1.852) / pow(diam[k], 4.871) : 0.0;
+  }
+}
+
+__global__ void k_hw_resistance_grad(int64_t n, const double* C, const double* r,
+                                     const double* g_r, double* g_C) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    g_C[i] = g_r[i] * (-1.852 * r[i] / C[i]);  // dr/dC = -1.852 r / C (calib.py:89-108)
+}
+
+__global__ void k_emitter_ke(int64_t n, const double* C, double qexp, double coef,
+                             double* ke) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double c = C[i];
+    ke[i] = c > 0.0 ? coef / pow(c, qexp) : 0.0;
+  }
+}
+
+__global__ void k_emitter_ke_grad(int64_t n, const double* C, const double* ke, double qexp,
+                                  const double* g_ke, double* g_C) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const double c = C[i];
+    g_C[i] = c > 0.0 ? g_ke[i] * (-qexp * ke[i] / c) : 0.0;
+  }
+}
+
+dim3 ew_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  return dim3((unsigned)std::min<int64_t>(std::max<int64_t>(g, 1), 4096));
+}
+
+}  // namespace
+
+extern "C" int wds_hw_resistance(void* stream, int64_t n, int32_t L, const double* C,
+                                 const double* len, const double* diam, const int8_t* mask,
+                                 double* r) {
+  if (n < 0 || L <= 0 || (n > 0 && (!C || !len || !diam || !r)) || n % L) {
+    set_error("wds_hw_resistance: bad argument");
+    return WDS_EINVAL;
+  }
+  if (n == 0) return WDS_OK;
+  hipLaunchKernelGGL(k_hw_resistance, ew_grid(n), dim3(256), 0, (hipStream_t)stream, n, L, C,
+                     len, diam, mask, r);
+  return launch_check("k_hw_resistance");
+}
+
+extern "C" int wds_hw_resistance_grad(void* stream, int64_t n, const double* C, const double* r,
+                                      const double* g_r, double* g_C) {
+  if (n < 0 || (n > 0 && (!C || !r || !g_r || !g_C))) {
+    set_error("wds_hw_resistance_grad: bad argument");
+    return WDS_EINVAL;
+  }
+  if (n == 0) return WDS_OK;
+  hipLaunchKernelGGL(k_hw_resistance_grad, ew_grid(n), dim3(256), 0, (hipStream_t)stream, n, C,
+                     r, g_r, g_C);
+  return launch_check("k_hw_resistance_grad");
+}
+
+extern "C" int wds_emitter_ke(void* stream, int64_t n, const double* C, double qexp,
+                              double coef, double* ke) {
+  if (n < 0 || (n > 0 && (!C || !ke))) {
+    set_error("wds_emitter_ke: bad argument");
+    return WDS_EINVAL;
+  }
+  if (n == 0) return WDS_OK;
+  hipLaunchKernelGGL(k_emitter_ke, ew_grid(n), dim3(256), 0, (hipStream_t)stream, n, C, qexp,
+                     coef, ke);
+  return launch_check("k_emitter_ke");
+}
+
+extern "C" int wds_emitter_ke_grad(void* stream, int64_t n, const double* C, const double* ke,
+                                   double qexp, const double* g_ke, double* g_C) {
+  if (n < 0 || (n > 0 && (!C || !ke || !g_ke || !g_C))) {
+    set_error("wds_emitter_ke_grad: bad argument");
+    return WDS_EINVAL;
+  }
+  if (n == 0) return WDS_OK;
+  hipLaunchKernelGGL(k_emitter_ke_grad, ew_grid(n), dim3(256), 0, (hipStream_t)stream, n, C,
+                     ke, qexp, g_ke, g_C);
+  return launch_check("k_emitter_ke_grad");
 }

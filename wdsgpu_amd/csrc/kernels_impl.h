@@ -509,7 +509,10 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
     c.dj[j] = d[node];
     c.kej[j] = ke[node];
   }
-  for (int f = lane; f < pl.Nf; f += 64) c.Hn[pl.Nj + f] = rh[pl.slot_node[pl.Nj + f]];
+  for (int f = lane; f < pl.Nf; f += 64) {
+    const double h = rh[pl.slot_node[pl.Nj + f]];
+    c.Hn[pl.Nj + f] = (h != h) ? 0.0 : h;  // NaN reads as 0 (torch.nan_to_num)
+  }
   for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
 }
 

@@ -459,9 +459,13 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
   const double* ke = ap.ke + (int64_t)b * ap.ke_stride;
   const double* H = ap.head + (int64_t)b * pl.N;
   const double* E = ap.emitter + (int64_t)b * pl.N;
-  const double* gH = ap.gH + (int64_t)b * pl.N;
-  const double* gE = ap.gE + (int64_t)b * pl.N;
-  const double* gQ = ap.gQ + (int64_t)b * pl.L;
+  // null cotangents are zero (outputs that do not reach the loss)
+  const double* gH = ap.gH ? ap.gH + (int64_t)b * pl.N : nullptr;
+  const double* gE = ap.gE ? ap.gE + (int64_t)b * pl.N : nullptr;
+  const double* gQ = ap.gQ ? ap.gQ + (int64_t)b * pl.L : nullptr;
+  auto GH = [&](int n) { return gH ? gH[n] : 0.0; };
+  auto GE = [&](int n) { return gE ? gE[n] : 0.0; };
+  auto GQ = [&](int k) { return gQ ? gQ[k] : 0.0; };
   for (int s = lane; s < pl.Nj + pl.Nf; s += 64) c.Hn[s] = H[pl.slot_node[s]];
   for (int j = lane; j < pl.Nj; j += 64) {
     const int node = pl.slot_node[j];
@@ -487,20 +491,20 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
       const int ent = pl.jadj[e];
       if (ent >= 0) continue;
       const int k = ent & 0x7fffffff;
-      r += c.P[k] * gQ[k];
+      r += c.P[k] * GQ(k);
     }
     for (int e = e0; e < e1; ++e) {
       const int ent = pl.jadj[e];
       if (ent < 0) continue;
-      r += -(c.P[ent] * gQ[ent]);
+      r += -(c.P[ent] * GQ(ent));
     }
     const int node = pl.slot_node[j];
     const double emh = c.kej[j] > 0.0 ? 1.0 / c.hge[j] : 0.0;
-    r = r - emh * gE[node];
-    r = r - gH[node];
+    r = r - emh * GE(node);
+    r = r - GH(node);
     c.v1[j] = r;
   }
-  if (lane < pl.p) c.misc[256 + lane] = gQ[pl.prv_k[lane]];
+  if (lane < pl.p) c.misc[256 + lane] = GQ(pl.prv_k[lane]);
   mem_sync();
   double* lam = c.F;  // F is free in the backward
   if (!node_solve(pl, c, lam, ap.n_refine)) {
@@ -517,7 +521,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
     const int f = pl.l_flags[k];
     const double l1 = (f & LF_J1) ? lam[pl.l_s1[k]] : 0.0;
     const double l2 = (f & LF_J2) ? lam[pl.l_s2[k]] : 0.0;
-    const double ll = c.P[k] * (gQ[k] - (l2 - l1));
+    const double ll = c.P[k] * (GQ(k) - (l2 - l1));
     laml[k] = ll;
     // dr = d hloss / d r (H-W pipe, open, unclamped)
     double dr = 0.0;
@@ -538,7 +542,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
     double g = 0.0;
     const double kv = c.kej[j];
     if (kv > 0.0) {
-      const double lame = (gE[node] + lam[j]) / c.hge[j];
+      const double lame = (GE(node) + lam[j]) / c.hge[j];
       const double ea = fabs(c.e[j]);
       const double pw = pow(fmax(ea, 1e-30), pl.qexp - 1.0);
       const double hraw = pl.qexp * fmax(kv, CSMALL) * pw;
@@ -560,7 +564,7 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
       if (ent < 0) g += -laml[ent & 0x7fffffff];
     }
     const int node = pl.slot_node[pl.Nj + f];
-    grh[node] = g + gH[node];
+    grh[node] = g + GH(node);
     gd[node] = 0.0;
     gke[node] = 0.0;
   }

@@ -183,6 +183,8 @@ class GGASolver:
         # (name, start_event, end_event, batch, iterations tensor) recorded on
         # the launching stream
         self.event_log = None
+        self._pinned = {}
+        self._pinned_adj = {}
 
     def _timed(self, name, fn, B, iters=None):
         if self.event_log is None:
@@ -261,8 +263,27 @@ class GGASolver:
         return (H, fac) if want_factor else H
 
     # ------------------------------------------------------------------
+    def _host_status(self, out, B):
+        """info and iters of a solve on the host: two asynchronous copies into
+        a cached pinned buffer and one stream synchronisation."""
+        buf = self._pinned.get(B)
+        if buf is None:
+            buf = torch.empty(2 * B, dtype=torch.int32, pin_memory=True)
+            self._pinned = {B: buf}
+        buf[:B].copy_(out["info"], non_blocking=True)
+        buf[B:].copy_(out["iters"], non_blocking=True)
+        torch.cuda.current_stream(self.device).synchronize()
+        return buf[:B].numpy(), buf[B:].numpy()
+
+    def _pinned_info(self, B):
+        buf = self._pinned_adj.get(B)
+        if buf is None:
+            buf = torch.empty(B, dtype=torch.int32, pin_memory=True)
+            self._pinned_adj = {B: buf}
+        return buf
+
     def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
-              status_machine, r_hw=None, polish_steps=-1, want_factor=False):
+              status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True):
         d = self._prep(demand_cfs)
         rh = self._prep(res_head_ft)
         single = d.dim() == 1
@@ -275,8 +296,7 @@ class GGASolver:
             rh = rh.expand(B, -1)
         if d.shape[1] != self.N or rh.shape != (B, self.N):
             raise ValueError("demand/res_head must be [N] or [B,N] with N=%d" % self.N)
-        rh = torch.nan_to_num(rh).contiguous()
-        rh_fix = res_head_ft if not isinstance(res_head_ft, torch.Tensor) else None
+        rh = rh.contiguous()  # NaN heads (junction entries) read as 0 in the kernel
         ke = self.node_ke_default if ke_int is None else self._prep(ke_int)
         if ke.dim() == 2 and ke.shape[0] == 1:
             ke = ke[0]
@@ -290,16 +310,19 @@ class GGASolver:
         r = r.contiguous()
         r_stride = self.L if r.dim() == 2 else 0
         dev = self.device
+        # the fused kernel writes every per-scenario scalar; the per-iteration
+        # path leaves some untouched on early exits
+        scalar = torch.empty if fused else torch.zeros
         out = dict(
             head=torch.empty(B, self.N, dtype=self.dtype, device=dev),
             flow=torch.empty(B, self.L, dtype=self.dtype, device=dev),
             emitter=torch.empty(B, self.N, dtype=self.dtype, device=dev),
-            iters=torch.zeros(B, dtype=torch.int32, device=dev),
-            relerr=torch.zeros(B, dtype=self.dtype, device=dev),
+            iters=scalar(B, dtype=torch.int32, device=dev),
+            relerr=scalar(B, dtype=self.dtype, device=dev),
             status=torch.empty(B, self.L, dtype=torch.int8, device=dev),
-            diag_ratio=torch.zeros(B, dtype=self.dtype, device=dev),
-            info=torch.zeros(B, dtype=torch.int32, device=dev),
-            resid=torch.zeros(B, dtype=self.dtype, device=dev),
+            diag_ratio=scalar(B, dtype=self.dtype, device=dev),
+            info=scalar(B, dtype=torch.int32, device=dev),
+            resid=scalar(B, dtype=self.dtype, device=dev),
             factor=torch.empty(B, int(self.stats["nnzL"]), dtype=self.dtype, device=dev)
             if want_factor else None)
         q0t = None if q0 is None else self._prep(q0).expand(B, -1).contiguous()
@@ -363,8 +386,8 @@ class GGASolver:
         return self._finish(out, single, max_iter, status_machine)
 
     def _finish(self, out, single, max_iter, status_machine):
-        info = out["info"]
-        bad = torch.nonzero(info > 0).flatten().tolist()
+        info, _ = self._host_status(out, int(out["info"].shape[0]))
+        bad = np.nonzero(info > 0)[0].tolist()
         if bad:
             its = info[bad].tolist()
             raise RuntimeError(
@@ -394,7 +417,7 @@ class GGASolver:
         max_iter = self.max_iter_default if max_iter is None else int(max_iter)
         hacc = self.hacc_default if accuracy is None else float(accuracy)
         a, out, keep, single, B = self._args(demand_cfs, res_head_ft, ke_int, None, None,
-                                             max_iter, hacc, status_machine)
+                                             max_iter, hacc, status_machine, fused=False)
         ctrl = torch.zeros(B, 4, dtype=torch.int32, device=self.device)
         maxtrials = self.maxtrials(max_iter, status_machine)
         with torch.cuda.device(self.device):

@@ -53,8 +53,7 @@ class Calibration:
             self.H_true = out["head_ft"].index_select(1, self.sensors).clone()
 
     def r_of(self, C):
-        r = hw_resistance(C, self.len_ft, self.diam_ft)
-        return torch.where(self.pipe, r, torch.zeros_like(r))
+        return hw_resistance(C, self.len_ft, self.diam_ft, mask=self.pipe)
 
     def ke_of(self, Ce):
         return emitter_ke(Ce)
