@@ -4,8 +4,7 @@
  * Drop-in boundary for dgga's batched forward GGA and its implicit adjoint.
  * The reference exposes this path through Python keyword APIs, not an FFI;
  * each entry point below names the reference function it replaces.  The
- * Python host layer (wdsgpu_amd) binds these with ctypes; see INTEGRATION.md
- * for the binding a dgga maintainer would add.
+ * Python host layer (wdsgpu_amd) binds these with ctypes.
  *
  * Conventions: every function returns 0 on success or a negative wds_status;
  * wds_last_error() gives the message of the last failure on this thread.

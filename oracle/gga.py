@@ -230,14 +230,22 @@ class GGAModel:
             q = np.where(self.is_pump, self.init_setting * self.pl_q0, q)
         return np.where(self.closed, QZERO, q)
 
-    # When set to a numpy Generator, every power-law evaluation is perturbed
-    # by -1, 0 or +1 ulp at random: the oracle's own sensitivity to one
+    # When set to a numpy Generator, every power-law and logarithm evaluation
+    # (pipe, Darcy-Weisbach friction, pump, emitter) is perturbed by -1, 0 or
+    # +1 ulp at random: the oracle's own sensitivity to one
     # rounding of the transcendental (the reference's self-jitter acceptance
     # practice, data/sparse_gpu_plan.md:404-445).
     pow_jitter = None
 
     def _pow(self, x, y):
         p = x ** y
+        if self.pow_jitter is not None:
+            u = self.pow_jitter.integers(-1, 2, size=np.shape(p))
+            p = p * (1.0 + u * 2.0 ** -52)
+        return p
+
+    def _log(self, x):
+        p = np.log(x)
         if self.pow_jitter is not None:
             u = self.pow_jitter.integers(-1, 2, size=np.shape(p))
             p = p * (1.0 + u * 2.0 ** -52)
@@ -273,13 +281,13 @@ class GGAModel:
         with np.errstate(divide="ignore", invalid="ignore"):
             w = qa / np.where(s > 0, s, 1.0)
             sj = w >= DW_A1
-            y1 = DW_A8 / np.where(sj, w, 1.0) ** 0.9
+            y1 = DW_A8 / self._pow(np.where(sj, w, 1.0), 0.9)
             y2 = e / 3.7 + y1
-            y3 = DW_A9 * np.log(y2)
+            y3 = DW_A9 * self._log(y2)
             f_sj = 1.0 / (y3 * y3)
             dfdq_sj = 1.8 * f_sj * y1 * DW_A9 / y2 / y3 / np.where(qa > 0, qa, 1.0)
             y2d = e / 3.7 + DW_AB
-            y3d = DW_A9 * np.log(y2d)
+            y3d = DW_A9 * self._log(y2d)
             fa = 1.0 / (y3d * y3d)
             fb = (2.0 + DW_AC / (y2d * y3d)) * fa
             r2 = w / DW_A2
@@ -322,7 +330,7 @@ class GGAModel:
         qa = q * sgn
         spc = np.maximum(self.pump_speed, 1e-12)
         h0e = spc * spc * self.pl_h0
-        re = self.pl_r * spc ** (2.0 - self.pump_n)
+        re = self.pl_r * self._pow(spc, 2.0 - self.pump_n)
         qa_c = np.maximum(qa, 1e-30)
         hg_chp_raw = -re / qa_c / qa_c
         big_c = hg_chp_raw > CBIG
@@ -334,7 +342,7 @@ class GGAModel:
         nm1 = self.pump_n - 1.0
         pw0 = np.where(nm1 > 0.0, 0.0, np.where((nm1 < 0.0) & self.pump_pf, np.inf, 1.0))
         qa_pf = np.where(q_zero, 1.0, qa)
-        hg_pf_raw = self.pump_n * re * np.where(q_zero, pw0, qa_pf ** nm1)
+        hg_pf_raw = self.pump_n * re * np.where(q_zero, pw0, self._pow(qa_pf, nm1))
         sml_p = hg_pf_raw < self.rqtol
         hg_pf = np.where(sml_p, self.rqtol, hg_pf_raw)
         hl_pf = np.where(sml_p, h0e + hg_pf * q, h0e + hg_pf * q / np.maximum(self.pump_n, 1e-30))

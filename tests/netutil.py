@@ -41,3 +41,37 @@ def make_net(nodes, links, accuracy=1e-3, ke=None):
 
 def res_heads(net):
     return np.nan_to_num(net.reservoir_head_ft())
+
+
+def check_forward_against_oracle(net, m, out, d, rh, acc, tol_h_ft=1e-10 / 0.3048):
+    """GPU forward output vs the oracle on the same inputs: identical Newton
+    iteration counts and final statuses; heads and flows within 3x the
+    oracle's own sensitivity to a 1-ulp perturbation of its power laws (the
+    reference's cross <= 3x self-jitter acceptance) or 1e-10 m."""
+    from oracle import gga
+    ke = net.node_ke
+    ext = gga.solve(m, d, rh, ke=ke, accuracy=acc, status_machine=True, extended=True)
+    ref = gga.solve(m, d, rh, ke=ke, accuracy=acc, status_machine=True)
+    assert out["iters"].cpu().numpy().tolist() == ref["iters"].tolist()
+    assert out["iters"].cpu().numpy().tolist() == ext["iters"].tolist()
+    assert np.array_equal(out["status"].cpu().numpy(), ref["status"])
+    assert bool(out["converged"].all())
+    m.pow_jitter = np.random.default_rng(123)
+    try:
+        jit = gga.solve(m, d, rh, ke=ke, accuracy=acc, status_machine=True, extended=True)
+    finally:
+        m.pow_jitter = None
+    assert jit["iters"].tolist() == ext["iters"].tolist()
+    noise = np.abs(jit["head_ft"] - ext["head_ft"]).max()
+    Hg = out["head_ft"].cpu().numpy()
+    assert np.abs(Hg - ext["head_ft"]).max() <= max(3.0 * noise, tol_h_ft)
+    qn = np.abs(jit["flow_cfs"] - ext["flow_cfs"]).max()
+    assert np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max() <= max(3.0 * qn, 1e-12)
+    return ref
+
+
+# feature_network cases (headloss, seed, tank level m): together they cover
+# pumps OPEN and XHEAD, check valves OPEN and CLOSED, ACTIVE PRVs and tank
+# links OPEN and TEMPCLOSED (checked on the CPU in test_oracle.py)
+FEATURE_CASES = [("H-W", 0, 18.0), ("H-W", 2, 18.0), ("H-W", 1, 25.0),
+                 ("D-W", 0, 18.0), ("D-W", 3, 18.0), ("D-W", 2, 25.0)]

@@ -11,7 +11,7 @@ from wdsgpu_amd.solver import GGASolver
 
 from netutil import make_net
 
-# BASELINE tolerance on heads: 1e-10 m
+# tolerance on heads: 1e-10 m
 TOL_H_FT = 1e-10 / 0.3048
 
 pytestmark = pytest.mark.gpu

@@ -100,6 +100,106 @@ def test_prv_opens_when_upstream_below_setting():
     assert out["head_ft"][2] < 400.0
 
 
+def _one_link_net(link, H0=200.0, el=0.0, demand=0.5, headloss="H-W", pump=None):
+    """reservoir -> one link -> junction with a demand (a tree: the link flow
+    is the demand and the junction head follows from one energy equation)."""
+    nodes = [(RESERVOIR, H0, 0.0), (JUNCTION, el, demand)]
+    net = make_net(nodes, [dict(n1=0, n2=1, **link)])
+    net.meta["headloss"] = headloss
+    if pump is not None:
+        net.pump_link = np.array([0], dtype=np.int32)
+        net.pump_ptype = np.array([1], dtype=np.int8)
+        for k in ("h0", "r", "n", "q0", "qmax", "hmax"):
+            setattr(net, "pump_" + k, np.array([pump[k]]))
+        net.valve_setting_user[0] = 1.0
+    return net
+
+
+def test_pump_power_curve_fit_passes_through_its_points():
+    from wdsgpu_amd.network import pump_power_curve
+    h0, h1, q1, h2, q2 = 60.0, 45.0, 80.0, 0.0, 160.0
+    c = pump_power_curve(h0, h1, q1, h2, q2, flow_units="LPS", si=True)
+    # back in user units: h(q) = -h0_int*0.3048 - r_int*0.3048/28.317^n * q^n
+    for q, h in ((0.0, h0), (q1, h1), (q2, h2)):
+        hq = (-c["h0"] - c["r"] * (q / 28.317) ** c["n"]) * 0.3048
+        assert abs(hq - h) < 1e-9
+
+
+def test_pump_lift_closed_form():
+    from wdsgpu_amd.network import PUMP, pump_power_curve
+    c = pump_power_curve(50.0, 37.5, 60.0, 0.0, 120.0, flow_units="CFS", si=False)
+    net = _one_link_net(dict(type=PUMP, diam_ft=0.0), H0=100.0, demand=30.0, pump=c)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    q = 30.0
+    assert abs(out["flow_cfs"][0] - q) < 1e-9
+    lift = -c["h0"] - c["r"] * q ** c["n"]
+    assert abs(out["head_ft"][1] - (100.0 + lift)) < 1e-8
+
+
+def test_darcy_weisbach_turbulent_closed_form():
+    # Swamee-Jain friction factor computed independently of the oracle's
+    # constants: f = 0.25 / log10(e/(3.7 D) + 5.74 / Re^0.9)^2, h = f L/D v^2/2g
+    from wdsgpu_amd.network import dw_resistance
+    L_ft, D_ft, e_ft, Q = 1500.0, 0.5, 0.0005, 0.6
+    net = _one_link_net(dict(len_ft=L_ft, diam_ft=D_ft, C=100.0), headloss="D-W", demand=Q)
+    net.roughness[:] = e_ft
+    net.r_hw[:] = dw_resistance(L_ft, D_ft)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    nu = 1.1e-5
+    A = np.pi * D_ft ** 2 / 4.0
+    v = Q / A
+    Re = v * D_ft / nu
+    assert Re > 4000
+    f = 0.25 / np.log10(e_ft / (3.7 * D_ft) + 5.74 / Re ** 0.9) ** 2
+    hL = f * L_ft / D_ft * v * v / (2.0 * 32.2)
+    assert abs((200.0 - out["head_ft"][1]) / hL - 1.0) < 1e-6
+
+
+def test_darcy_weisbach_laminar_closed_form():
+    # Hagen-Poiseuille: h = 32 nu L v / (g D^2)
+    from wdsgpu_amd.network import dw_resistance
+    L_ft, D_ft, Q = 300.0, 0.3, 1e-4
+    net = _one_link_net(dict(len_ft=L_ft, diam_ft=D_ft, C=100.0), headloss="D-W", demand=Q)
+    net.roughness[:] = 0.0005
+    net.r_hw[:] = dw_resistance(L_ft, D_ft)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    v = Q / (np.pi * D_ft ** 2 / 4.0)
+    assert v * D_ft / 1.1e-5 < 2000
+    hL = 32.0 * 1.1e-5 * L_ft * v / (32.2 * D_ft ** 2)
+    assert abs((200.0 - out["head_ft"][1]) / hL - 1.0) < 1e-3  # pi = 3.141592654
+
+
+def test_tcv_closed_form():
+    # minor loss K v^2 / 2g; EPANET's 0.02517 is 8 / (pi^2 g) with g = 32.2
+    # ft/s^2 rounded to four digits
+    from wdsgpu_amd.network import TCV
+    K, D_ft, Q = 6.0, 0.5, 0.4
+    net = _one_link_net(dict(type=TCV, diam_ft=D_ft, setting=K), demand=Q)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    v = Q / (np.pi * D_ft ** 2 / 4.0)
+    hL = K * v * v / (2.0 * 32.2)
+    assert abs((200.0 - out["head_ft"][1]) / hL - 1.0) < 2e-4
+
+
+def test_check_valve_closes_against_reverse_head():
+    # two reservoirs joined through a junction; the CV pipe points from the
+    # low reservoir towards the junction, so it must close
+    from wdsgpu_amd.network import CVPIPE, ST_CLOSED
+    nodes = [(RESERVOIR, 100.0, 0.0), (RESERVOIR, 150.0, 0.0), (JUNCTION, 0.0, 0.3)]
+    links = [dict(n1=0, n2=2, type=CVPIPE, len_ft=500.0, diam_ft=0.5, C=120.0),
+             dict(n1=1, n2=2, len_ft=500.0, diam_ft=0.5, C=120.0)]
+    net = make_net(nodes, links)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    assert out["status"][0] == ST_CLOSED
+    assert abs(out["flow_cfs"][0]) < 1e-5
+    assert abs(out["flow_cfs"][1] - 0.3) < 1e-5
+
+
 def test_converged_planar_solution_satisfies_equations():
     net = planar_network(1)
     m = gga.GGAModel(net)
@@ -232,3 +332,26 @@ def test_cpu_adjoint_transposed_solve_matches_dense():
     x1 = splu(J).solve(v, trans="T")
     x2 = np.linalg.solve(J.toarray().T, v)
     assert np.allclose(x1, x2, rtol=1e-12, atol=1e-12)
+
+
+def test_feature_cases_cover_the_status_machine():
+    seen = set()
+    from wdsgpu_amd.generators import feature_network
+    from wdsgpu_amd.network import CVPIPE, PUMP, ST_XHEAD
+    from netutil import FEATURE_CASES
+    for headloss, seed, level in FEATURE_CASES:
+        net = feature_network(seed, headloss=headloss, n_prv=2, tank_level_m=level)
+        m = gga.GGAModel(net)
+        d, rh = scenario_batch(net, 2, seed=11)
+        ref = gga.solve(m, d, np.nan_to_num(rh), ke=net.node_ke, accuracy=1e-3,
+                        status_machine=True)
+        lt = np.asarray(net.link_type)
+        st = ref["status"]
+        for kind, code in ((PUMP, ST_OPEN), (PUMP, ST_XHEAD), (CVPIPE, ST_OPEN),
+                           (CVPIPE, ST_CLOSED), (PRV, ST_ACTIVE)):
+            if (st[:, lt == kind] == code).any():
+                seen.add((kind, code))
+        if (st[:, m.tl_k] == 1).any():
+            seen.add(("tank", 1))
+    assert seen == {(PUMP, ST_OPEN), (PUMP, ST_XHEAD), (CVPIPE, ST_OPEN),
+                    (CVPIPE, ST_CLOSED), (PRV, ST_ACTIVE), ("tank", 1)}

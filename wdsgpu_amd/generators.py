@@ -13,14 +13,20 @@ Workloads:
   * ``planar_network``  -- planar random network: Delaunay graph of random
     points, a random breadth-first spanning tree grown from the reservoir
     mains, chords to loop density ~0.15, 3 PRVs on trunk mains, 10 leaks.
+  * ``feature_network`` -- grid network with every link and node kind of the
+    path: a pumped supply (3-point head curve, shutoff U[40,80] m, design
+    flow U[20,200] L/s), a second reservoir, a tank with level bounds, TCVs,
+    a check-valve pipe, PRVs and leak emitters; Hazen-Williams or
+    Darcy-Weisbach friction (roughness U[0.01,0.5] mm).
   * ``scenario_batch``  -- per-scenario i.i.d. lognormal demand multipliers.
 """
 from __future__ import annotations
 
 import numpy as np
 
-from .network import (Net, JUNCTION, RESERVOIR, PIPE, PRV, M_PER_FT,
-                      LPS_PER_CFS, hw_resistance, emitter_ke)
+from .network import (Net, JUNCTION, RESERVOIR, TANK, PIPE, CVPIPE, PUMP, PRV, TCV, M_PER_FT,
+                      LPS_PER_CFS, PUMP_POWER_FUNC, hw_resistance, dw_resistance, emitter_ke,
+                      pump_power_curve)
 
 DIAMS_MM = np.array([100.0, 150.0, 200.0, 250.0, 300.0, 400.0])
 
@@ -81,6 +87,137 @@ def grid_network(seed=0, nx=10, ny=10, n_chords=20):
     meta = dict(flow_units="LPS", headloss="H-W", accuracy=1e-3, trials=200,
                 seed=int(seed), workload="grid10x10")
     return _mk_net(ntype, elev, dem, links, meta)
+
+
+def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tcv=1,
+                    n_cv=1, tank=True, n_leaks=2, tank_level_m=18.0):
+    """Grid of nx*ny junctions (elevation U[0,30] m, demand U[0.2,1.5] L/s,
+    pipes 100-600 m) with: ``n_pumps`` pumps lifting from low reservoirs
+    (head 5 m) into corner junctions, each with a 3-point curve (0, h0),
+    (qd, 0.75 h0), (2 qd, 0) -- shutoff h0 U[40,80] m, design flow qd
+    U[20,200] L/s; a reservoir at 70 m on the opposite corner; a tank
+    (bottom 40 m, level ``tank_level_m`` in [2, 25] m) on an edge junction; ``n_tcv``
+    grid pipes turned into TCVs (loss coefficient U[2,10]), ``n_cv`` into
+    check-valve pipes; ``n_prv`` PRVs (setting U[25,35] m) each feeding a
+    two-junction branch of its own; ``n_leaks`` leak emitters.
+    headloss "H-W" (C U[90,140]) or "D-W" (roughness U[0.01,0.5] mm)."""
+    g = np.random.default_rng(seed)
+    nj = nx * ny
+    jid = lambda i, j: i * ny + j
+    elev = list(g.uniform(0.0, 30.0, nj))
+    dem = list(g.uniform(0.2, 1.5, nj))
+    ntype = [JUNCTION] * nj
+    links = []
+    pairs = []
+    for i in range(nx):
+        for j in range(ny):
+            if i + 1 < nx:
+                pairs.append((jid(i, j), jid(i + 1, j)))
+            if j + 1 < ny:
+                pairs.append((jid(i, j), jid(i, j + 1)))
+    for a, b in pairs:
+        links.append(dict(n1=a, n2=b, type=PIPE, len_m=g.uniform(100.0, 600.0),
+                          diam_mm=float(g.choice(DIAMS_MM[1:])), C=g.uniform(90.0, 140.0),
+                          rough_mm=g.uniform(0.01, 0.5)))
+
+    def add_node(t, el, d=0.0):
+        ntype.append(t)
+        elev.append(el)
+        dem.append(d)
+        return len(ntype) - 1
+
+    def main(n1, n2, diam=300.0, length=50.0):
+        links.append(dict(n1=n1, n2=n2, type=PIPE, len_m=length, diam_mm=diam, C=130.0,
+                          rough_mm=0.05))
+
+    pumps = []
+    corners = [jid(0, 0), jid(nx - 1, 0), jid(0, ny - 1)]
+    for k in range(n_pumps):
+        r = add_node(RESERVOIR, 5.0)
+        h0 = g.uniform(40.0, 80.0)
+        qd = g.uniform(20.0, 200.0)
+        pumps.append((len(links), pump_power_curve(h0, 0.75 * h0, qd, 0.0, 2.0 * qd)))
+        links.append(dict(n1=r, n2=corners[k % 3], type=PUMP, diam_mm=0.0))
+    r2 = add_node(RESERVOIR, 70.0)
+    main(r2, jid(nx - 1, ny - 1))
+    tank_node = None
+    if tank:
+        tank_node = add_node(TANK, 40.0)
+        main(tank_node, jid(nx // 2, ny - 1), diam=200.0, length=150.0)
+    # PRVs feed pressure zones of their own: grid junction -> PRV -> branch
+    # junction (5 m lower) -> pipe -> end junction (8 m lower)
+    taps = g.choice(nj, size=n_prv, replace=False) if n_prv else []
+    for a in taps:
+        a = int(a)
+        b = add_node(JUNCTION, elev[a] - 5.0, 2.0)
+        c = add_node(JUNCTION, elev[a] - 8.0, 1.0)
+        links.append(dict(n1=a, n2=b, type=PRV, diam_mm=150.0, setting_m=g.uniform(25.0, 35.0)))
+        links.append(dict(n1=b, n2=c, type=PIPE, len_m=g.uniform(100.0, 300.0), diam_mm=150.0,
+                          C=g.uniform(90.0, 140.0), rough_mm=g.uniform(0.01, 0.5)))
+    # TCVs and check-valve pipes replace grid pipes (keeping the diameter)
+    grid_links = list(range(len(pairs)))
+    g.shuffle(grid_links)
+    for k in grid_links[:n_tcv]:
+        links[k].update(type=TCV, setting=g.uniform(2.0, 10.0))
+    for k in grid_links[n_tcv:n_tcv + n_cv]:
+        links[k].update(type=CVPIPE)
+    N, L = len(ntype), len(links)
+    lt = np.array([lk["type"] for lk in links], dtype=np.int8)
+    ln_ft = np.array([lk.get("len_m", 0.0) for lk in links]) / M_PER_FT
+    d_ft = np.array([lk["diam_mm"] for lk in links]) / (1000.0 * M_PER_FT)
+    pipe = (lt == PIPE) | (lt == CVPIPE)
+    if headloss == "H-W":
+        rough = np.array([lk.get("C", 0.0) for lk in links])
+        r = np.zeros(L)
+        r[pipe] = hw_resistance(rough[pipe], ln_ft[pipe], d_ft[pipe])
+    elif headloss == "D-W":
+        # roughness in internal units: mm -> ft, stored as the pipe's Kc
+        rough = np.array([lk.get("rough_mm", 0.0) for lk in links]) / (1000.0 * M_PER_FT)
+        r = np.zeros(L)
+        r[pipe] = dw_resistance(ln_ft[pipe], d_ft[pipe])
+    else:
+        raise ValueError("headloss must be 'H-W' or 'D-W'")
+    rough = np.where(pipe, rough, 0.0)
+    st = np.array([2 if t in (PRV, TCV) else 1 for t in lt], dtype=np.int8)
+    setting = np.array([lk.get("setting_m", lk.get("setting", 0.0)) for lk in links])
+    for k, _ in pumps:
+        setting[k] = 1.0          # relative speed
+    nt = np.asarray(ntype, dtype=np.int8)
+    elev_ft = np.asarray(elev) / M_PER_FT
+    ke = np.zeros(N)
+    junc = np.where(nt == JUNCTION)[0]
+    leaks = g.choice(junc, size=n_leaks, replace=False) if n_leaks else np.zeros(0, int)
+    c_leak = np.zeros(N)
+    c_leak[leaks] = g.uniform(0.05, 0.5, n_leaks)
+    ke = emitter_ke(c_leak, "LPS", 2.0, M_PER_FT)
+    kw = {}
+    if pumps:
+        kw.update(pump_link=np.array([k for k, _ in pumps], dtype=np.int32),
+                  pump_ptype=np.full(len(pumps), PUMP_POWER_FUNC, dtype=np.int8),
+                  pump_h0=np.array([c["h0"] for _, c in pumps]),
+                  pump_r=np.array([c["r"] for _, c in pumps]),
+                  pump_n=np.array([c["n"] for _, c in pumps]),
+                  pump_q0=np.array([c["q0"] for _, c in pumps]),
+                  pump_qmax=np.array([c["qmax"] for _, c in pumps]),
+                  pump_hmax=np.array([c["hmax"] for _, c in pumps]))
+    if tank_node is not None:
+        kw.update(tank_node=np.array([tank_node], dtype=np.int32),
+                  tank_h0=np.array([(40.0 + tank_level_m) / M_PER_FT]),
+                  tank_hmin=np.array([(40.0 + 2.0) / M_PER_FT]),
+                  tank_hmax=np.array([(40.0 + 25.0) / M_PER_FT]),
+                  tank_area=np.array([np.pi * 10.0 ** 2 / M_PER_FT ** 2]),
+                  tank_overflow=np.zeros(1, dtype=np.int8))
+    meta = dict(flow_units="LPS", headloss=headloss, accuracy=1e-3, trials=200,
+                seed=int(seed), workload="feature%dx%d-%s" % (nx, ny, headloss),
+                emitter_c=c_leak, leak_nodes=leaks)
+    nid = [("R%d" % i if t != JUNCTION else "J%d" % i) for i, t in enumerate(ntype)]
+    return Net(node_id=nid, node_type=nt, elev_ft=elev_ft, node_ke=ke,
+               demand_cfs=np.asarray(dem) / LPS_PER_CFS,
+               link_id=["P%d" % k for k in range(L)], link_type=lt,
+               link_n1=np.array([lk["n1"] for lk in links], dtype=np.int32),
+               link_n2=np.array([lk["n2"] for lk in links], dtype=np.int32),
+               diam_ft=d_ft, len_ft=ln_ft, roughness=rough, km_int=np.zeros(L),
+               r_hw=r, init_status=st, valve_setting_user=setting, meta=meta, **kw)
 
 
 def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,

@@ -165,6 +165,26 @@ def dw_resistance(len_ft, diam_ft):
     return np.asarray(len_ft) / 2.0 / 32.2 / d / ((PI_EPANET * (d * d) / 4.0) ** 2)
 
 
+def pump_power_curve(h0, h1, q1, h2, q2, flow_units="LPS", si=True):
+    """Fit h = a - b q^c through (0, h0), (q1, h1), (q2, h2) in user units and
+    convert to internal units: EPANET's powercurve (input3.c:2101-2129) as
+    restated by dgga/parse.py:821-843.  Returns dict(h0, r, n, q0, qmax, hmax)
+    with h0 negative (the head gain enters hloss with a minus sign)."""
+    if h0 < TINY or h0 - h1 < TINY or h1 - h2 < TINY or q1 < TINY or q2 - q1 < TINY:
+        raise ValueError("pump curve fails the powercurve checks")
+    a = h0
+    h4 = h0 - h1
+    h5 = h0 - h2
+    c = np.log(h5 / h4) / np.log(q2 / q1)
+    if c <= 0.0 or c > 20.0:
+        raise ValueError("pump curve exponent %r out of range" % c)
+    b = -h4 / q1 ** c
+    qcf = FLOW_UCF[flow_units]
+    hcf = M_PER_FT if si else 1.0
+    return dict(h0=-a / hcf, r=-b * (qcf ** c / hcf), n=c, q0=q1 / qcf,
+                qmax=((-a / b) ** (1.0 / c)) / qcf, hmax=h0 / hcf)
+
+
 def emitter_ke(c_user, flow_units="LPS", qexp=2.0, ucf_pressure=M_PER_FT):
     """User emitter coefficient C -> internal Ke = (Ucf_Q^Qexp / Ucf_P) / C^Qexp
     for C>0, else 0 (dgga/parse.py:557-568)."""
