@@ -9,8 +9,10 @@ explicit sparse Jacobian, a full-Newton polish with ``scipy.sparse.linalg
 .splu`` and the adjoint solve J^T lambda = dL/dz followed by the theta
 contraction (demands, reservoir heads, emitter Ke, pipe resistance r).
 ACTIVE PRVs contribute the constraint row H[n2] - hset (prvcoeff fixed
-point).  Pumps and Darcy-Weisbach are not on this oracle (the GPU adjoint
-refuses them too).
+point).  Pumps enter through their exact head-loss derivatives
+(``_pump_coeffs_np``, dgga/autodiff.py:122-218, without the pump-parameter
+gradients); Darcy-Weisbach is not on this oracle (the reference's GPU
+adjoint refuses it).
 """
 from __future__ import annotations
 
@@ -26,8 +28,8 @@ from . import gga
 def link_coeffs(m, q, r_hw, status):
     """(hloss, hgrad, dhloss/dr, act_prv) per link (dgga/autodiff.py:239-362,
     221-236), H-W / TCV / valves / closed links."""
-    if m.headloss != "H-W" or m.n_pumps:
-        raise NotImplementedError("oracle adjoint: H-W without pumps")
+    if m.headloss != "H-W":
+        raise NotImplementedError("oracle adjoint: Hazen-Williams networks")
     qa = np.abs(q)
     sgn = np.where(q < 0, -1.0, 1.0)
     hg = m.hexp * r_hw * qa ** (m.hexp - 1.0)
@@ -67,11 +69,52 @@ def link_coeffs(m, q, r_hw, status):
         dr = np.where(vmask, 0.0, dr)
         hl = np.where(act_prv, 0.0, hl)
         hg = np.where(act_prv, 0.0, hg)
-    cl = np.asarray(status) <= ST_CLOSED
+    if m.n_pumps:
+        hl, hg, dr = pump_coeffs(m, q, hl, hg, dr)
+    cl = (np.asarray(status) <= ST_CLOSED) | m.sm_zero_speed
     hl = np.where(cl, CBIG * q, hl)
     hg = np.where(cl, CBIG, hg)
     dr = np.where(cl, 0.0, dr)
     return hl, hg, dr, act_prv
+
+
+def pump_coeffs(m, q, hl, hg, dr):
+    """Pump branches of the exact head-loss derivative (pumpcoeff semantics,
+    dgga/autodiff.py:122-218): NOCURVE CSMALL; CONST_HP r/Q with its clamp
+    branches -CBIG / -RQtol (the residual's own derivative, opposite in sign
+    to the assembled P); power function h0 + r |Q|^(n-1) Q with the RQtol
+    clamp; linear h0 + r Q.  Closed pumps are handled by the caller."""
+    hl, hg, dr = hl.copy(), hg.copy(), dr.copy()
+    for k in m.pump_links:
+        k = int(k)
+        qk = float(q[k])
+        qa = abs(qk)
+        dr[k] = 0.0
+        if m.pl_ptype[k] == 3:                       # NOCURVE
+            hl[k] = CSMALL * qk
+            hg[k] = CSMALL
+            continue
+        sp_ = float(m.pump_speed[k])
+        h0 = sp_ * sp_ * m.pl_h0[k]
+        n = float(m.pump_n[k])
+        r = m.pl_r[k] * sp_ ** (2.0 - n)
+        if m.pl_ptype[k] == 0:                       # CONST_HP
+            g = -r / qa / qa if qa > 0.0 else np.inf
+            if g > CBIG:
+                hl[k], hg[k] = -CBIG * qk, -CBIG
+            elif g < m.rqtol:
+                hl[k], hg[k] = -m.rqtol * qk, -m.rqtol
+            else:
+                hl[k], hg[k] = r / qk, g
+        elif n != 1.0:                               # power function
+            g = n * r * qa ** (n - 1.0)
+            if g < m.rqtol:
+                hl[k], hg[k] = h0 + m.rqtol * qk, m.rqtol
+            else:
+                hl[k], hg[k] = h0 + g * qk / n, g
+        else:                                        # linear curve
+            hl[k], hg[k] = h0 + r * qk, r
+    return hl, hg, dr
 
 
 def emitter_coeffs(m, e, ke):

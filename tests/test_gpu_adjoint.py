@@ -58,6 +58,34 @@ def test_adjoint_matches_cpu_reference_adjoint(planar):
     assert _rel(gr, cr.sum(0)) < 1e-10
 
 
+@pytest.mark.parametrize("seed", [0, 2])
+def test_adjoint_with_pumps_matches_cpu_reference_adjoint(cuda, seed):
+    # feature network (H-W): pump OPEN (seed 0) / XHEAD (seed 2), ACTIVE PRV
+    # branches, a TCV, a check valve, a tank and leak emitters
+    from wdsgpu_amd.generators import feature_network
+    net = feature_network(seed, headloss="H-W", n_prv=2)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    B = 3
+    d, rh = scenario_batch(net, B, seed=13)
+    rh = np.nan_to_num(rh)
+    ke = np.broadcast_to(net.node_ke, (B, net.N)).copy()
+    r = m.r_hw.copy()
+    g = np.random.default_rng(2)
+    W = g.normal(size=(B, net.N))
+    WQ = g.normal(size=(B, net.L)) * 0.1
+    H, Q, gd, grh, gke, gr, _ = _gpu_grads(s, d, rh, ke, r, W, WQ)
+    sol = adj.solve_polished(m, d, rh, ke, r, accuracy=1e-8)
+    pump = int(m.pump_links[0])
+    assert (sol["status"][:, pump] == (3 if seed == 0 else 0)).all()
+    assert np.abs(H - sol["head"]).max() < 1e-9
+    cd, crh, cke, cr = adj.cpu_adjoint(m, sol, W, WQ, np.full((B, net.N), 0.5))
+    assert _rel(gd, cd) < 1e-10
+    assert _rel(grh, crh) < 1e-10
+    assert _rel(gke, cke) < 1e-10
+    assert _rel(gr, cr.sum(0)) < 1e-10
+
+
 def test_adjoint_counters_reuse_the_forward_factor(planar):
     net, m, s = planar
     s.counters(reset=True)

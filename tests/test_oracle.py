@@ -318,6 +318,37 @@ def test_cpu_adjoint_matches_central_differences_grid():
     assert abs(grh[0].sum() - W[0].sum()) < 1e-9
 
 
+@pytest.mark.parametrize("seed", [0, 2])
+def test_cpu_adjoint_with_pumps_matches_central_differences(seed):
+    # feature network: a power-function pump (OPEN for seed 0, XHEAD for
+    # seed 2), an ACTIVE PRV branch, a TCV, a check valve, a tank and leaks
+    from wdsgpu_amd.generators import feature_network
+    net = feature_network(seed, headloss="H-W", n_prv=1)
+    m = gga.GGAModel(net)
+    d = net.demand_cfs[None].copy()
+    rh = res_heads(net)[None]
+    ke = net.node_ke[None]
+    r = m.r_hw.copy()
+    sol = adj.solve_polished(m, d, rh, ke, r, accuracy=1e-12)
+    pump = int(m.pump_links[0])
+    assert sol["status"][0, pump] == (3 if seed == 0 else 0)
+    W = np.zeros((1, m.N))
+    jn = m.junc_nodes
+    W[0, jn[[1, 7, 20, len(jn) - 1]]] = [1.0, -2.0, 0.5, 1.5]
+    gd, grh, gke, gr = adj.cpu_adjoint(m, sol, W, np.zeros((1, m.L)), np.zeros((1, m.N)))
+    for j in jn[[0, 9, 33, len(jn) - 2]]:
+        fd = _fd_grad(m, d, rh, ke, r, W, "d", j, 1e-3 * d[0, j])
+        assert abs(gd[0, j] - fd) <= 1e-6 * max(abs(fd), 1e-3)
+    for k in (3, 40):
+        fd = _fd_grad(m, d, rh, ke, r, W, "r", k, 1e-3 * r[k])
+        assert abs(gr[0, k] - fd) <= 1e-6 * max(abs(fd), 1e-6)
+    # fixed heads (the pump's suction reservoir, the other reservoir, the
+    # tank); no translation identity here: the ACTIVE PRV pins an absolute head
+    for f in m.fixed_nodes:
+        fd = _fd_grad(m, d, rh, ke, r, W, "rh", int(f), 1e-2)
+        assert abs(grh[0, f] - fd) <= 1e-6 * max(abs(fd), 1e-6)
+
+
 def test_cpu_adjoint_transposed_solve_matches_dense():
     # splu(J).solve(v, trans='T') against a dense solve of J^T
     net = _prv_net(120.0)

@@ -32,10 +32,10 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         s = solver
         if s.dtype != torch.float64:
             raise NotImplementedError("adjoint='gpu' is float64 only")
-        if s.headloss_form != "H-W" or s.n_pumps:
+        if s.headloss_form != "H-W":
             raise NotImplementedError(
-                "adjoint='gpu' covers Hazen-Williams networks without pumps "
-                "(pump theta and Darcy-Weisbach gradients are not on this path)")
+                "adjoint='gpu' covers Hazen-Williams networks (the Darcy-Weisbach "
+                "dhloss/dr contraction is not on this path, dgga/autodiff.py:795-799)")
         if s.n_pcv:
             j2 = np.asarray(s.prv_n2)
             j1 = np.asarray(s.prv_n1)
@@ -67,6 +67,12 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
                                                                    C.byref(a)),
                                 B, out["iters"]), "wds_solve")
         info, iters = s._host_status(out, B)
+        if (info == -300).any():
+            bad = np.nonzero(info == -300)[0].tolist()
+            raise NotImplementedError(
+                "adjoint='gpu': samples %s have a CONST_HP pump on its CBIG/RQtol clamp "
+                "branch at the converged state (the reduced system is not the "
+                "Jacobian's Schur complement there, dgga/autodiff.py:808-823)" % bad[:8])
         if (info != 0).any():
             bad = np.nonzero(info != 0)[0].tolist()
             raise RuntimeError("adjoint='gpu': samples %s failed in the solve or the polish "

@@ -325,9 +325,9 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
     set_error("wds_adjoint: bad argument");
     return WDS_EINVAL;
   }
-  if (P->has_pump || P->headloss != 0) {
-    set_error("the batched adjoint covers Hazen-Williams pipes, TCVs and PRVs "
-              "(pump and Darcy-Weisbach gradients are not on this path)");
+  if (P->headloss != 0) {
+    set_error("the batched adjoint covers Hazen-Williams networks (Darcy-Weisbach "
+              "dhloss/dr is not on this path, as in dgga/autodiff.py:795-799)");
     return WDS_EUNSUPPORTED;
   }
   AdjParams ap = make_adj_params(a);

@@ -345,6 +345,27 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
   return wave_max(rmax);
 }
 
+// A CONST_HP pump on its CBIG / RQtol clamp branch: the assembled P is
+// +CBIG / +RQtol while the residual's derivative is -CBIG / -RQtol, so the
+// reduced system is not the Jacobian's Schur complement there and the
+// reference refuses the adjoint (dgga/autodiff.py:808-823).
+constexpr int kInfoChpClamp = -300;
+
+__device__ inline bool chp_on_clamp(const DevPlan& pl, const Scn& c) {
+  if (!pl.has_pump) return false;
+  bool bad = false;
+  for (int k = lane_id(); k < pl.L; k += 64) {
+    const int f = pl.l_flags[k];
+    if (!(f & LF_PUMP_CHP) || c.S[k] <= ST_CLOSED || (f & LF_ZERO_SPEED)) continue;
+    const double qa = fmax(fabs(c.q[k]), 1e-30);
+    const double spc = fmax(pl.pu_speed[k], 1e-12);
+    const double re = pl.pu_r[k] * pow(spc, 2.0 - pl.pu_n[k]);
+    const double raw = -re / qa / qa;
+    bad = bad || raw > CBIG || raw < pl.rqtol;
+  }
+  return wave_any(bad);
+}
+
 // Polish steps + terminal factor (_adjgpu_polish, dgga/autodiff.py:1053-1104).
 // Returns 0, or a negative info code on failure.
 __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolveParams& sp, int b,
@@ -352,6 +373,7 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
   const int lane = lane_id();
   double* dH = c.v2;
   for (int step = 0; step < sp.polish_steps; ++step) {
+    if (chp_on_clamp(pl, c)) return kInfoChpClamp;
     polish_coeffs(pl, c);
     polish_assemble(pl, c);
     if (k3_factor(pl, c)) return -(step + 1);
@@ -397,6 +419,7 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
     lds_sync();
   }
   // terminal state: coefficients, factor kept for the adjoint, residual
+  if (chp_on_clamp(pl, c)) return kInfoChpClamp;
   polish_coeffs(pl, c);
   polish_assemble(pl, c);
   if (k3_factor(pl, c)) return -99;
