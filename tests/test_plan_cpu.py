@@ -35,9 +35,12 @@ def test_library_exports_every_declared_symbol():
 
 
 def run_program(s, p, V, X, mode):
-    """Emulate the wavefront executor: per-lane accumulators, stores on LAST,
-    and at the end of a wide level (log2 width in bits 23-25) a group sum whose
-    first lane finalises the target of the task its slice belongs to."""
+    """Emulate the wavefront executors on the exported (index-form) programs:
+    per-lane accumulators, stores on LAST, and at the end of a wide level
+    (log2 width in bits 23-25) a group sum that the group's first lane stores
+    into its task's target (its slice started from the target's value).
+    mode 0: LDL^T factor (w_ij -= w_ik w_jk invD_k, reciprocal pivots);
+    mode 1: unit-factor dot solve acc = X[t] - sum V[a] X[b]."""
     ins = s.plan_export(8 + 3 * p).reshape(-1, 64, 4)
     bad = False
     acc = np.zeros(64)
@@ -46,15 +49,13 @@ def run_program(s, p, V, X, mode):
     fin = [None] * 64
     for step in ins:
         if start:
-            fin = [(int(w) & TMASK, int(c), bool(w & DIAG)) if w & FIRST else None
-                   for c, w in zip(step[:, 2], step[:, 3])]
+            fin = [(int(w) & TMASK, bool(w & DIAG)) if w & FIRST else None for w in step[:, 3]]
             start = False
         for l in range(64):
             a, b, c, w = (int(v) for v in step[l])
             t = w & TMASK
-            init = 0.0 if (w & ZERO_INIT or mode == 2) else (V[t] if mode == 0 else X[t])
             if w & FIRST:
-                acc[l] = init
+                acc[l] = 0.0 if w & ZERO_INIT else (V[t] if mode == 0 else X[t])
             if w & NOP:
                 continue
             if mode == 0:
@@ -65,14 +66,10 @@ def run_program(s, p, V, X, mode):
                         V[t] = 1.0 / acc[l]
                     else:
                         V[t] = acc[l]
-            elif mode == 1:
+            else:
                 acc[l] -= V[a] * X[b]
                 if w & LAST:
-                    X[t] = acc[l] * V[c]
-            else:
-                acc[l] += V[a] * X[b]
-                if w & LAST:
-                    X[t] = X[t] - V[c] * acc[l]
+                    X[t] = acc[l]
         flags = step[:, 3] & (LEVEL_END | (7 << WIDTH_SHIFT))
         assert (flags == flags[0]).all()   # uniform across lanes
         if flags[0] & LEVEL_END:
@@ -84,16 +81,13 @@ def run_program(s, p, V, X, mode):
                 for g in range(0, 64, W):
                     if fin[g] is None:
                         continue
-                    t, c, diag = fin[g]
+                    t, diag = fin[g]
                     red = acc[g:g + W].sum()
                     if mode == 0:
-                        tot = V[t] + red
-                        bad |= diag and not tot > 0
-                        V[t] = 1.0 / tot if diag else tot
-                    elif mode == 1:
-                        X[t] = (X[t] + red) * V[c]
+                        bad |= diag and not red > 0
+                        V[t] = 1.0 / red if diag else red
                     else:
-                        X[t] = X[t] - V[c] * red
+                        X[t] = red
             start = True
     return bad, level_ends, wide_levels
 
@@ -124,10 +118,15 @@ def test_programs_factor_and_solve_like_dense(name):
     V[c2l[c2l >= 0]] = csr[0][c2l >= 0]
     bad, lev, wide_f = run_program(s, 0, V, None, 0)
     assert not bad and lev == st["nlevels"]
+    # unit factor: l_ij = w_ij / D_j below the diagonal
+    cp, ri = s.plan_export(1), s.plan_export(2)
+    for j in range(m.Nj):
+        V[cp[j] + 1:cp[j + 1]] *= V[cp[j]]
     X = np.zeros(m.Nj)
     X[slot] = F[0]
     _, _, wide_s = run_program(s, 1, V, X, 1)
-    run_program(s, 2, V, X, 2)
+    X *= V[cp[:-1]]                  # z = D^-1 y
+    run_program(s, 2, V, X, 1)
     if name == "planar":
         # the top of the elimination tree has few long dot products: those
         # levels must be split across lane groups

@@ -33,6 +33,16 @@ __device__ __forceinline__ void lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
+// Order LDS traffic of one wavefront without waiting: the hardware performs a
+// wave's LDS operations in issue order, so a wavefront-scope fence only has
+// to stop the compiler from reordering (it emits no s_waitcnt).  Used
+// between the levels of the sparse programs, where the next level's loads
+// only need to see this wave's own stores.
+__device__ __forceinline__ void lds_order() {
+  __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
 // Make this wave's global stores visible to its own later loads.
 __device__ __forceinline__ void mem_sync() {
   __threadfence_block();
@@ -57,9 +67,7 @@ __device__ __forceinline__ double wave_min(double v) {
 __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 
 // ---------------------------------------------------------------------------
-// Level-scheduled program executor.  MODE 0: LDL^T factorisation in place in
-// V (unscaled columns w, reciprocal pivots on the diagonal); MODE 1: forward
-// solve z = D^-1 L^-1 x in place in X; MODE 2: backward solve x = L^-T z.
+// Level-scheduled program executors.
 // kRing chunks (kRing * kUnroll steps) of instruction loads stay in flight in
 // a ring of kRing + 1 register slots: chunk c lives in slot c % (kRing + 1),
 // and the load of chunk c + kRing goes into the slot the previous chunk just
@@ -88,15 +96,18 @@ typedef __attribute__((address_space(3))) double lds_f64;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) i32x4 glb_i32x4;
 
-template <int MODE>
-__device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad) {
+// Numeric LDL^T factorisation in place in V: unscaled columns w (w_ij =
+// L_ij D_j) and reciprocal pivots on the diagonal; update
+// w_ij -= w_ik * w_jk * invD_k.
+__device__ inline void run_factor(const ProgDev& pg, double* Vg, int& bad,
+                                  double* prof = nullptr) {
   const int lane = lane_id();
   const int ns = pg.nsteps;
   if (ns == 0) return;
   lds_f64* const V = (lds_f64*)Vg;
-  lds_f64* const X = (lds_f64*)Xg;
-  // read once: stores through V/X could alias a generic ProgDev reference
-  lds_f64* const dummy = V + pg.dummy;
+  // read once: stores through V could alias a generic ProgDev reference;
+  // each lane has its own sink slot
+  lds_f64* const dummy = V + pg.dummy + lane;
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
   i32x4 ring[kSlots][kUnroll];
 #pragma unroll
@@ -105,8 +116,12 @@ __device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad)
     for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
   double acc = 0.0;
   // wide levels: the task whose target this lane's group finalises
-  int fin_t = 0, fin_c = 0;
+  int fin_t = 0;
   bool fin_diag = false, fin_act = false, lvl_start = true;
+#ifdef WDS_PROFILE_STAGES
+  long long prof_t0 = clock64();
+  int prof_lvl = 0;
+#endif
   for (int g0 = 0; g0 < ns; g0 += kSlots * kUnroll) {
 #pragma unroll
     for (int d = 0; d < kSlots; ++d) {
@@ -123,7 +138,6 @@ __device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad)
       // a wide level's slices all start at its first step, which is the
       // first step of a chunk: remember the task this lane helps finalise
       fin_t = cur[0].w & kTMask;
-      fin_c = cur[0].z;
       fin_diag = cur[0].w & kDiag;
       fin_act = cur[0].w & kFirst;
       lvl_start = false;
@@ -132,16 +146,10 @@ __device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad)
 #pragma unroll
     for (int u = 0; u < kUnroll; ++u) {
       const i32x4 in = cur[u];
-      const int t = in.w & kTMask;
       va[u] = V[in.x];
+      vb[u] = V[in.y];
       vc[u] = V[in.z];
-      if (MODE == 0) {
-        vb[u] = V[in.y];
-        vt[u] = V[t];
-      } else {
-        vb[u] = X[in.y];
-        vt[u] = X[t];
-      }
+      vt[u] = V[in.w & kTMask];
     }
     // branch-free: NOPs multiply by V[zero] == 0; every step stores, to its
     // target when the instruction is a task's last, else to V[dummy]
@@ -150,51 +158,120 @@ __device__ void run_program(const ProgDev& pg, double* Vg, double* Xg, int& bad)
       const int w = cur[u].w;
       const int t = w & kTMask;
       const bool first = w & kFirst, last = w & kLast;
-      // a wide slice's first instruction starts its partial sum at zero
-      const double init = (MODE == 2 || (w & kZeroInit)) ? 0.0 : vt[u];
-      double val;
-      lds_f64* dst;
-      if (MODE == 0) {
-        acc = fma(-va[u], vb[u] * vc[u], first ? init : acc);
-        const bool diag = w & kDiag;
-        const double r = recip(acc);
-        if (diag && last && !(acc > 0.0)) bad = 1;
-        val = diag ? r : acc;
-        dst = last ? V + t : dummy;
-      } else if (MODE == 1) {
-        acc = fma(-va[u], vb[u], first ? init : acc);
-        val = acc * vc[u];
-        dst = last ? X + t : dummy;
-      } else {
-        acc = fma(va[u], vb[u], first ? init : acc);
-        val = vt[u] - vc[u] * acc;
-        dst = last ? X + t : dummy;
-      }
-      *dst = val;
+      // a wide slice other than the group's first starts at zero
+      const double init = (w & kZeroInit) ? 0.0 : vt[u];
+      acc = fma(-va[u], vb[u] * vc[u], first ? init : acc);
+      const bool diag = w & kDiag;
+      const double r = recip(acc);
+      if (diag && last && !(acc > 0.0)) bad = 1;
+      *(last ? V + t : dummy) = diag ? r : acc;
     }
     // levels are padded to whole chunks: only a chunk's last step can end one
     const int wl = __builtin_amdgcn_readfirstlane(cur[kUnroll - 1].w);
     if (wl & kLevelEnd) {
       const int lw = (wl >> kWidthShift) & 7;
       if (lw) {
-        // wide level: butterfly-sum each aligned group of 2^lw lanes, then
-        // the group's first lane finalises its task's target
+        // wide level: butterfly-sum each aligned group of 2^lw lanes (the
+        // first lane's slice started from the target's value), then the
+        // group's first lane finalises the target
         double red = acc;
         for (int m = 1; m < (1 << lw); m <<= 1) red += __shfl_xor(red, m, 64);
         if (fin_act && (lane & ((1 << lw) - 1)) == 0) {
-          if (MODE == 0) {
-            const double tot = V[fin_t] + red;
-            if (fin_diag && !(tot > 0.0)) bad = 1;
-            V[fin_t] = fin_diag ? recip(tot) : tot;
-          } else if (MODE == 1) {
-            X[fin_t] = (X[fin_t] + red) * V[fin_c];
-          } else {
-            X[fin_t] = X[fin_t] - V[fin_c] * red;
-          }
+          if (fin_diag && !(red > 0.0)) bad = 1;
+          V[fin_t] = fin_diag ? recip(red) : red;
         }
       }
       lvl_start = true;
-      lds_sync();
+      lds_order();
+#ifdef WDS_PROFILE_STAGES
+      if (prof) {
+        const long long t1 = clock64();
+        if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
+        prof_t0 = t1;
+        ++prof_lvl;
+      }
+#endif
+    }
+    }
+  }
+}
+
+// LDS double at a byte address (the dynamic LDS of the kernels starts at 0)
+__device__ __forceinline__ lds_f64& lds_at(int byte_addr) {
+  return *(lds_f64*)(size_t)(unsigned)byte_addr;
+}
+
+// Triangular solve with the unit factor on the solve vector S (LDS):
+// acc = S[t] - sum V[a] S[b] per task, stored back into S[t].  Instruction
+// words are LDS byte addresses (a, b, init, store): init is S[t] on a task's
+// first step (the zero slot for a wide slice other than the group's first,
+// the lane's own sink slot on every other step) and store is S[t] on its last
+// step (the lane's sink otherwise).  The store word of a level's last step (always
+// a chunk's last) carries the level end and log2 group width in its high
+// bits, uniform across lanes.
+__device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
+  const int lane = lane_id();
+  const int ns = pg.nsteps;
+  if (ns == 0) return;
+  const int sent = pg.dummy + 8 * lane;  // byte address of this lane's sink slot
+  const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
+  i32x4 ring[kSlots][kUnroll];
+#pragma unroll
+  for (int d = 0; d < kRing; ++d)
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
+  double acc = 0.0;
+  int fin = sent;  // wide levels: byte address of the group's target
+  bool lvl_start = true;
+#ifdef WDS_PROFILE_STAGES
+  long long prof_t0 = clock64();
+  int prof_lvl = 0;
+#endif
+  for (int g0 = 0; g0 < ns; g0 += kSlots * kUnroll) {
+#pragma unroll
+    for (int d = 0; d < kSlots; ++d) {
+    const int g = g0 + d * kUnroll;
+    constexpr int kNextSlot[kSlots] = {4, 0, 1, 2, 3};
+    i32x4 cur[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      cur[u] = ring[d][u];
+      ring[kNextSlot[d]][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
+    }
+    if (lvl_start) {
+      fin = cur[0].z;  // the group's first lane starts from its target
+      lvl_start = false;
+    }
+    double va[kUnroll], vb[kUnroll], vi[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      va[u] = lds_at(cur[u].x);
+      vb[u] = lds_at(cur[u].y);
+      vi[u] = lds_at(cur[u].z);
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      acc = fma(-va[u], vb[u], cur[u].z != sent ? vi[u] : acc);
+      lds_at(u == kUnroll - 1 ? cur[u].w & kDotAddrMask : cur[u].w) = acc;
+    }
+    const int f = __builtin_amdgcn_readfirstlane(cur[kUnroll - 1].w);
+    if (f & kDotLevelEnd) {
+      const int lw = (f >> kDotWidthShift) & 7;
+      if (lw) {
+        double red = acc;
+        for (int m = 1; m < (1 << lw); m <<= 1) red += __shfl_xor(red, m, 64);
+        if ((lane & ((1 << lw) - 1)) == 0) lds_at(fin) = red;
+      }
+      lvl_start = true;
+      lds_order();
+#ifdef WDS_PROFILE_STAGES
+      if (prof) {
+        const long long t1 = clock64();
+        if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
+        prof_t0 = t1;
+        ++prof_lvl;
+      }
+#endif
     }
     }
   }

@@ -26,11 +26,37 @@ enum LinkFlag : int32_t {
   LF_J2 = 1 << 13,         // n2 is a junction
 };
 
+// LDS layout of a scenario (doubles): V [nnzL + 2] (factor, zero slot, spare),
+// heads [Nj + Nf], solve vector S [Nj], second vector [Nj], shared scratch
+// [2 kMaxPrv], then one sink slot per lane: the programs' stores that are not
+// a task's last go to the lane's own sink, so they never collide in a bank.
+#ifdef __HIP__
+#define WDS_HD __host__ __device__
+#else
+#define WDS_HD
+#endif
+
+WDS_HD inline int64_t lds_sink_index(int64_t nnzL, int64_t Nj, int64_t Nf) {
+  return nnzL + 2 + (Nj + Nf) + 2 * Nj + 2 * kMaxPrv;
+}
+// after the sinks: the Woodbury block (valve-end paths y, z, capacitance,
+// the small dense system, its right-hand side, probes, valve residuals)
+WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
+  return 2 * wp_tot + np * np + 4 * kMaxPrv * kMaxPrv + 2 * kMaxPrv + 2 * kMaxPrv + kMaxPrv;
+}
+
+// solve programs: the store word of a level's last step carries the level
+// end and log2 of the level's group width above the LDS byte address
+constexpr int32_t kDotAddrMask = (1 << 20) - 1;
+constexpr int32_t kDotLevelEnd = 1 << 30;
+constexpr int kDotWidthShift = 27;
+
 struct ProgDev {
   const int4* instr;        // step-major instructions, nsteps * 64
   int32_t nsteps;           // multiple of kSlots * kUnroll (NOP padded)
   int32_t nlevels;
-  int32_t dummy;            // V[dummy] (= nnzL + 1) absorbs non-final stores
+  int32_t dummy;            // first per-lane sink slot: factor, index from V;
+                            // solves, LDS byte address
 };
 
 struct DevPlan {
@@ -65,6 +91,10 @@ struct DevPlan {
   // (link order) then links with n2 == j; entry = link | (side << 31)
   const int32_t* jadj_ptr;    // [Nj+1]
   const int32_t* jadj;        // [..]
+  // fixed-head terms of each junction row: (link, fixed-node slot), links
+  // with a fixed n1 first, then links with a fixed n2
+  const int32_t* jg_ptr;      // [Nj+1]
+  const int2* jg;             // [..]
   // fixed-head node incidence (slot Nj+f): links with n1 == f then n2 == f
   const int32_t* fadj_ptr;    // [Nf+1]
   const int32_t* fadj;        // [..] link | (side << 31)
@@ -105,6 +135,7 @@ struct DevPlan {
   // reference CSR (full symmetric pattern, original junction order) -> L
   int32_t nnz_csr;
   const int32_t* csr_to_L;    // [nnz_csr] L position for lower-in-elimination-order entries, else -1
+  const int32_t* l_scale;     // [nnzL] L position of the column's pivot, -1 on diagonals
   const int32_t* diag_pos;    // [Nj] L position of slot j's diagonal (colptr)
   const int32_t* row_slot;    // [Nj] original junction row -> slot
   // programs

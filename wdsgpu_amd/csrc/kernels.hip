@@ -76,6 +76,37 @@ extern "C" int wds_plan_upload(wds_plan* P) {
     oi = bb.add(padded);
     os = ol = 0;
   };
+  // solve programs: index instructions -> LDS byte addresses (V at byte 0,
+  // the solve vector S after V, the heads and the fixed heads) and per-step
+  // uniform flags
+  const int32_t sink_i = (int32_t)lds_sink_index(S.nnzL, P->Nj, P->Nf);
+  const int32_t zero_b = 8 * (int32_t)S.nnzL, sink_b = 8 * sink_i;
+  const int32_t s_b = 8 * ((int32_t)S.nnzL + 2 + P->Nj + P->Nf);
+  auto dot_vectors = [&](const Program& pr, size_t& oi, size_t& os) {
+    const int32_t ns = exec_steps(pr);
+    std::vector<Instr> w((size_t)(ns + kRing * kUnroll) * kLanes);
+    for (size_t i = 0; i < w.size(); ++i) {
+      const int32_t own = sink_b + 8 * (int32_t)(i % kLanes);  // the lane's sink
+      w[i] = Instr{zero_b, s_b, own, own};
+    }
+    for (size_t i = 0; i < pr.instr.size(); ++i) {
+      const Instr& in = pr.instr[i];
+      const int32_t fl = in.t;
+      const int32_t own = sink_b + 8 * (int32_t)(i % kLanes);
+      if (!(fl & kNop)) {
+        const int32_t t = fl & kTMask;
+        const int32_t init = (fl & kFirst) ? ((fl & kZeroInit) ? zero_b : s_b + 8 * t) : own;
+        const int32_t store = (fl & kLast) ? s_b + 8 * t : own;
+        w[i] = Instr{8 * in.a, s_b + 8 * in.b, init, store};
+      }
+      // a level end (always the last step of a chunk) and the group width
+      // ride in the high bits of the store word of every lane
+      if (fl & kLevelEnd)
+        w[i].t |= kDotLevelEnd | (((fl >> kWidthShift) & 7) << kDotWidthShift);
+    }
+    oi = bb.add(w);
+    os = 0;
+  };
   size_t o_ns = bb.add(P->node_slot), o_sn = bb.add(P->slot_node), o_el = bb.add(P->el_slot);
   size_t o_s1 = bb.add(P->l_s1), o_s2 = bb.add(P->l_s2), o_fl = bb.add(P->l_flags),
          o_st0 = bb.add(P->l_st0);
@@ -91,13 +122,15 @@ extern "C" int wds_plan_upload(wds_plan* P) {
          o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
          o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
+  size_t o_jgp = bb.add(P->jg_ptr), o_jg = bb.add(P->jg);
   size_t o_wpp = bb.add(P->wp_ptr), o_wpn = bb.add(P->wp_node), o_wpd = bb.add(P->wp_diag),
          o_wtp = bb.add(P->wt_ptr), o_wt = bb.add(P->wt), o_wpr = bb.add(P->wpair);
   size_t o_c2l = bb.add(P->csr_to_L), o_dp = bb.add(P->diag_pos), o_rs = bb.add(P->row_slot);
-  size_t fi, fs, fl, wi, ws, wl, bi, bs, bl;
+  size_t fi, fs, fl, wi, ws, bi, bs;
   prog_vectors(S.factor, fi, fs, fl);
-  prog_vectors(S.fwd, wi, ws, wl);
-  prog_vectors(S.bwd, bi, bs, bl);
+  dot_vectors(S.fwd, wi, ws);
+  dot_vectors(S.bwd, bi, bs);
+  size_t o_lsc = bb.add(P->l_scale);
   void* d = nullptr;
   e = hipMalloc(&d, bb.bytes.size());
   if (e != hipSuccess) return hip_fail(e, "hipMalloc(plan)");
@@ -162,6 +195,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.prv_pos_off = at<int32_t>(d, o_po);
   D.prv_hset = at<double>(d, o_hs);
   D.prv_km = at<double>(d, o_pkm);
+  D.jg_ptr = at<int32_t>(d, o_jgp);
+  D.jg = at<int2>(d, o_jg);
   D.np = 2 * P->p;
   D.wp_tot = (int32_t)P->wp_node.size();
   D.wp_ptr = at<int32_t>(d, o_wpp);
@@ -180,17 +215,19 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.csr_to_L = at<int32_t>(d, o_c2l);
   D.diag_pos = at<int32_t>(d, o_dp);
   D.row_slot = at<int32_t>(d, o_rs);
-  auto set_prog = [&](ProgDev& pg, const Program& pr, size_t oi, size_t os, size_t ol) {
-    (void)os;
-    (void)ol;
+  auto set_prog = [&](ProgDev& pg, const Program& pr, size_t oi, size_t os, bool dot) {
     pg.instr = at<int4>(d, oi);
+    (void)os;
     pg.nsteps = exec_steps(pr);
-    pg.dummy = (int32_t)S.nnzL + 1;
+    pg.dummy = dot ? sink_b : sink_i;
     pg.nlevels = (int32_t)pr.level_end.size();
   };
-  set_prog(D.factor, S.factor, fi, fs, fl);
-  set_prog(D.fwd, S.fwd, wi, ws, wl);
-  set_prog(D.bwd, S.bwd, bi, bs, bl);
+  (void)fs;
+  (void)fl;
+  set_prog(D.factor, S.factor, fi, 0, false);
+  set_prog(D.fwd, S.fwd, wi, ws, true);
+  set_prog(D.bwd, S.bwd, bi, bs, true);
+  D.l_scale = at<int32_t>(d, o_lsc);
   P->d_blob = d;
   P->device = dev;
   return WDS_OK;
@@ -234,6 +271,23 @@ static int check_ready(wds_plan* P) {
     set_error("network needs " + std::to_string(P->stats.lds_bytes) +
               " bytes of LDS per scenario (> " + std::to_string(kMaxLds) + ")");
     return WDS_EUNSUPPORTED;
+  }
+  // the solve programs address LDS by absolute byte offsets from the start of
+  // the dynamic allocation: no kernel may carry static LDS
+  static bool lds_checked = false;
+  if (!lds_checked) {
+    const void* ks[] = {(const void*)k_solve, (const void*)k_newton_iter,
+                        (const void*)k_factor_solve, (const void*)k_adjoint};
+    for (const void* k : ks) {
+      hipFuncAttributes at{};
+      hipError_t e = hipFuncGetAttributes(&at, k);
+      if (e != hipSuccess) return hip_fail(e, "hipFuncGetAttributes");
+      if (at.sharedSizeBytes != 0) {
+        set_error("a solver kernel has static LDS; the solve programs assume none");
+        return WDS_EDEVICE;
+      }
+    }
+    lds_checked = true;
   }
   return WDS_OK;
 }

@@ -122,7 +122,7 @@ inline AdjParams make_adj_params(const wds_adjoint_args* a) {
 // Per-scenario context: LDS arrays and global workspace slices.
 struct Scn {
   // LDS
-  double* V;    // [nnzL + 1] factor (V[nnzL] == 0)
+  double* V;    // [nnzL + 2] factor (V[nnzL] == 0)
   double* Hn;   // [Nj + Nf] heads by slot
   double* R;    // [Nj] work vector
   double* X2;   // [Nj] second vector (solves of the polish / adjoint)
@@ -142,9 +142,13 @@ struct Scn {
   double* v3;   // [Nj]
   double* v4;   // [Nj]
   double* v5;   // [Nj]
-  double* Yp;   // [wp_tot] y_u = L^-1 e_u on the valve-end paths
-  double* Zp;   // [wp_tot] z_u = D^-1 y_u
-  double* Cm;   // [np*np] e_u^T A-hat^-1 e_v between valve ends
+  double* Yp;   // LDS [wp_tot] y_u = L^-1 e_u on the valve-end paths
+  double* Zp;   // LDS [wp_tot] z_u = D^-1 y_u
+  double* Cm;   // LDS [np*np] e_u^T A-hat^-1 e_v between valve ends
+  double* sM;   // LDS [(2 kMaxPrv)^2] small dense (capacitance) system
+  double* sw;   // LDS [2 kMaxPrv] its right-hand side / solution
+  double* sx0;  // LDS [2 kMaxPrv] probes e_u^T A-hat^-1 r
+  double* sprv; // LDS [kMaxPrv] valve residuals (polish) / valve cotangents (adjoint)
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
   // per-scenario inputs / state
@@ -159,7 +163,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   Scn c;
   const int Nj = pl.Nj, L = pl.L;
   c.V = lds;
-  c.Hn = lds + pl.nnzL + 2;  // V[nnzL] = 0 (zero slot), V[nnzL + 1] = store sink
+  c.Hn = lds + pl.nnzL + 2;  // V[nnzL] = 0 (zero slot), V[nnzL + 1] spare
   c.R = c.Hn + Nj + pl.Nf;
   c.X2 = c.R + Nj;
   c.sh = c.X2 + Nj;
@@ -192,12 +196,21 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   w += Nj;
   c.v5 = w;
   w += Nj;
-  c.Yp = w;
-  w += pl.wp_tot;
-  c.Zp = w;
-  w += pl.wp_tot;
-  c.Cm = w;
-  w += (int64_t)pl.np * pl.np;
+  w += 2 * (int64_t)pl.wp_tot + (int64_t)pl.np * pl.np;  // unused (kept for layout)
+  double* sm = lds + lds_sink_index(pl.nnzL, Nj, pl.Nf) + 64;
+  c.Yp = sm;
+  sm += pl.wp_tot;
+  c.Zp = sm;
+  sm += pl.wp_tot;
+  c.Cm = sm;
+  sm += pl.np * pl.np;
+  c.sM = sm;
+  sm += 4 * kMaxPrv * kMaxPrv;
+  c.sw = sm;
+  sm += 2 * kMaxPrv;
+  c.sx0 = sm;
+  sm += 2 * kMaxPrv;
+  c.sprv = sm;
   c.st0 = reinterpret_cast<int8_t*>(w);
   w += ((int64_t)L + 7) / 8;
   c.misc = w;
@@ -232,45 +245,49 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
 // Writes A values (global and LDS V), F (global) and Xflow (LDS R).
 // polish: pcv PRV links carry their prvcoeff P (adjoint/polish matrix,
 // dgga/autodiff.py:928-962) instead of being skipped.
+// K2 gathers: entries of a row (and off-diagonal positions) are fetched in
+// batches of kGather with every load issued before any is consumed, and the
+// accumulation is branch-free, so a batch costs one memory latency instead
+// of one per entry.  Sums keep the reference's order exactly.
+constexpr int kGather = 4;
+
 __device__ inline void k2_assemble(const DevPlan& pl, Scn& c, bool want_F) {
   const int lane = lane_id();
   for (int j = lane; j < pl.Nj; j += 64) {
     double dg = 0.0, F = 0.0, X = 0.0;
     const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
-    for (int e = e0; e < e1; ++e) {
-      const int ent = pl.jadj[e];
-      const int k = ent & 0x7fffffff;
-      const double P = c.P[k];
-      if (P == 0.0) continue;
-      dg += P;
-      if (want_F) {
-        const double Y = c.Y[k], q = c.q[k];
-        if (ent >= 0) {
-          F += Y;
-          X += -q;
-        } else {
-          F += -Y;
-          X += q;
+    for (int eb = e0; eb < e1; eb += kGather) {
+      int ent[kGather];
+      double Pv[kGather], Yv[kGather], qv[kGather];
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) ent[u] = (eb + u < e1) ? pl.jadj[eb + u] : 0;
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) {
+        const int k = ent[u] & 0x7fffffff;  // link 0 stands in for a missing entry
+        Pv[u] = c.P[k];
+        if (want_F) {
+          Yv[u] = c.Y[k];
+          qv[u] = c.q[k];
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < kGather; ++u) {
+        // links with P == 0 (pcv PRVs) take no part (dgga/solver.py:2939-2956)
+        const bool use = (eb + u < e1) && Pv[u] != 0.0;
+        dg = use ? dg + Pv[u] : dg;
+        if (want_F) {
+          const bool n1side = ent[u] >= 0;
+          F = use ? F + (n1side ? Yv[u] : -Yv[u]) : F;
+          X = use ? X + (n1side ? -qv[u] : qv[u]) : X;
         }
       }
     }
     if (want_F) {
-      // ground terms: links with a fixed-head n1 (g1), then fixed-head n2 (g2)
-      for (int e = e0; e < e1; ++e) {
-        const int ent = pl.jadj[e];
-        const int k = ent & 0x7fffffff;
-        if (ent < 0 && !(pl.l_flags[k] & LF_J1)) {
-          const double P = c.P[k];
-          if (P != 0.0) F += P * c.Hn[pl.l_s1[k]];
-        }
-      }
-      for (int e = e0; e < e1; ++e) {
-        const int ent = pl.jadj[e];
-        const int k = ent & 0x7fffffff;
-        if (ent >= 0 && !(pl.l_flags[k] & LF_J2)) {
-          const double P = c.P[k];
-          if (P != 0.0) F += P * c.Hn[pl.l_s2[k]];
-        }
+      // fixed-head terms: links with a fixed n1, then with a fixed n2
+      for (int g = pl.jg_ptr[j]; g < pl.jg_ptr[j + 1]; ++g) {
+        const int2 e = pl.jg[g];
+        const double P = c.P[e.x];
+        if (P != 0.0) F += P * c.Hn[e.y];
       }
     }
     const double ke = c.kej[j];
@@ -291,15 +308,36 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c, bool want_F) {
       c.R[j] = X;
     }
   }
-  for (int o = lane; o < pl.noff; o += 64) {
-    double v = 0.0;
-    for (int e = pl.off_ptr[o]; e < pl.off_ptr[o + 1]; ++e) {
-      const double P = c.P[pl.off_link[e]];
-      if (P != 0.0) v += -P;
+  for (int o0 = lane; o0 < pl.noff; o0 += 64 * kGather) {
+    int b[kGather], n[kGather], pos[kGather], k0[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int o = o0 + 64 * u;
+      const bool ok = o < pl.noff;
+      b[u] = ok ? pl.off_ptr[o] : 0;
+      n[u] = ok ? pl.off_ptr[o + 1] - b[u] : 0;
+      pos[u] = ok ? pl.off_pos[o] : 0;
     }
-    const int pos = pl.off_pos[o];
-    c.A[pos] = v;
-    c.V[pos] = v;
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) k0[u] = n[u] > 0 ? pl.off_link[b[u]] : 0;
+    double v[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const double P = c.P[k0[u]];
+      v[u] = (n[u] > 0 && P != 0.0) ? -P : 0.0;
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      // parallel links between one junction pair (rare): in link order
+      for (int e = b[u] + 1; e < b[u] + n[u]; ++e) {
+        const double P = c.P[pl.off_link[e]];
+        if (P != 0.0) v[u] += -P;
+      }
+      if (o0 + 64 * u < pl.noff) {
+        c.A[pos[u]] = v[u];
+        c.V[pos[u]] = v[u];
+      }
+    }
   }
   for (int f = lane; f < pl.nfill; f += 64) c.V[pl.fill_pos[f]] = 0.0;
   if (lane == 0) c.V[pl.nnzL] = 0.0;
@@ -351,27 +389,63 @@ __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
 
 // K3 + K4: factor V in place, then x = A^-1 b with n_refine refinement steps.
 // b is read from global (bsrc, slot order), x is left in X (LDS).
+// Solve programs on the solve vector S = c.R with the unit factor in V.
+// Forward y = L^-1 S then the pivot scaling z = D^-1 y (in place):
+__device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
+#ifdef WDS_PROFILE_STAGES
+  run_dot(pl.fwd, c.misc + 408 + 32);
+#else
+  run_dot(pl.fwd);
+#endif
+  for (int j = lane_id(); j < pl.Nj; j += 64) c.R[j] *= c.V[pl.diag_pos[j]];
+  lds_sync();
+}
+
+// backward x = L^-T z in place in S
+__device__ inline void solve_bwd(const DevPlan& pl, Scn& c) {
+#ifdef WDS_PROFILE_STAGES
+  run_dot(pl.bwd, c.misc + 408 + 64);
+#else
+  run_dot(pl.bwd);
+#endif
+}
+
+// X = A^-1 b (b global, slot order; X in LDS, not S) with n_refine steps of
+// refinement on the double-double residual of the assembled A.
 __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, double* X,
                                 int n_refine) {
   const int lane = lane_id();
-  int dummy = 0;
-  for (int j = lane; j < pl.Nj; j += 64) X[j] = bsrc[j];
+  double* const S = c.R;
+  for (int j = lane; j < pl.Nj; j += 64) S[j] = bsrc[j];
   lds_sync();
-  run_program<1>(pl.fwd, c.V, X, dummy);
-  run_program<2>(pl.bwd, c.V, X, dummy);
+  solve_fwd(pl, c);
+  solve_bwd(pl, c);
+  for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
+  lds_sync();
   for (int it = 0; it < n_refine; ++it) {
-    for (int i = lane; i < pl.Nj; i += 64) c.R[i] = residual_row_dd(pl, c.A, X, bsrc[i], i);
+    for (int i = lane; i < pl.Nj; i += 64) S[i] = residual_row_dd(pl, c.A, X, bsrc[i], i);
     lds_sync();
-    run_program<1>(pl.fwd, c.V, c.R, dummy);
-    run_program<2>(pl.bwd, c.V, c.R, dummy);
-    for (int j = lane; j < pl.Nj; j += 64) X[j] += c.R[j];
+    solve_fwd(pl, c);
+    solve_bwd(pl, c);
+    for (int j = lane; j < pl.Nj; j += 64) X[j] += S[j];
     lds_sync();
   }
 }
 
+// Numeric factorisation, then the unit-factor scaling l_ij = w_ij / D_j of
+// every off-diagonal entry (the solves and the Woodbury paths use L).
 __device__ inline bool k3_factor(const DevPlan& pl, Scn& c) {
   int bad = 0;
-  run_program<0>(pl.factor, c.V, nullptr, bad);
+#ifdef WDS_PROFILE_STAGES
+  run_factor(pl.factor, c.V, bad, c.misc + 408);
+#else
+  run_factor(pl.factor, c.V, bad);
+#endif
+  for (int p = lane_id(); p < pl.nnzL; p += 64) {
+    const int sd = pl.l_scale[p];
+    if (sd >= 0) c.V[p] *= c.V[sd];
+  }
+  lds_sync();
   return wave_any(bad != 0);
 }
 
@@ -654,7 +728,7 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
     c.Hn[j] = 0.0;
   }
 #ifdef WDS_PROFILE_STAGES
-  if (lane < 8) c.misc[400 + lane] = 0.0;
+  for (int i = lane; i < 112; i += 64) c.misc[400 + i] = 0.0;
 #endif
   mem_sync();
   lds_sync();

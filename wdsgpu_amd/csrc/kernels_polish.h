@@ -77,22 +77,13 @@ __device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
   lds_sync();
 }
 
-// x = A-hat^-1 b (b global, slot order) with n_ref refinements, into dst
-// (global).  The factor in V is reused.
-__device__ inline void solveA(const DevPlan& pl, Scn& c, const double* b, double* dst,
-                              int n_ref) {
-  k4_solve(pl, c, b, c.X2, n_ref);
-  for (int j = lane_id(); j < pl.Nj; j += 64) dst[j] = c.X2[j];
-  mem_sync();
-}
-
 // (A-hat x)_i for all rows into dst (global)
 __device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double* dst) {
   for (int i = lane_id(); i < pl.Nj; i += 64) dst[i] = spmv_row(pl, c.A, x, i);
   mem_sync();
 }
 
-// Solve the n x n system M y = w (row-major in misc) by Gaussian elimination
+// Solve the n x n system M y = w (row-major, LDS) by Gaussian elimination
 // with partial pivoting on lane 0; y overwrites w.  Returns false if singular.
 __device__ inline bool small_solve(double* M, double* w, int n) {
   bool ok = true;
@@ -133,7 +124,7 @@ __device__ inline bool small_solve(double* M, double* w, int n) {
         w[r] = s / M[r * n + r];
       }
   }
-  mem_sync();
+  lds_sync();
   return __shfl(ok ? 1 : 0, 0, 64) != 0;
 }
 
@@ -148,20 +139,20 @@ __device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
       double acc = (i == i0) ? 1.0 : 0.0;
       for (int t = pl.wt_ptr[i]; t < pl.wt_ptr[i + 1]; ++t) {
         const int2 e = pl.wt[t];
-        acc = fma(-(c.V[e.x] * c.V[pl.wp_diag[e.y]]), c.Yp[e.y], acc);
+        acc = fma(-c.V[e.x], c.Yp[e.y], acc);  // V holds the unit factor L
       }
       c.Yp[i] = acc;
       c.Zp[i] = acc * c.V[pl.wp_diag[i]];
     }
   }
-  mem_sync();
+  lds_sync();
   for (int pr = lane; pr < pl.np * pl.np; pr += 64) {
     const int4 w = pl.wpair[pr];
     double s = 0.0;
     for (int t = 0; t < w.z; ++t) s = fma(c.Yp[w.x + t], c.Zp[w.y + t], s);
     c.Cm[pr] = s;
   }
-  mem_sync();
+  lds_sync();
 }
 
 // e_u^T A-hat^-1 r for every valve end u, from z = D^-1 L^-1 r in X (LDS)
@@ -173,7 +164,7 @@ __device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, do
       s = fma(c.Yp[i], X[pl.wp_node[i]], s);
     x0[lane] = s;
   }
-  mem_sync();
+  lds_sync();
 }
 
 // z -= sum_c w[c] z_{u_c} for the first ncol paths (the forward image of
@@ -190,22 +181,21 @@ __device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const do
   }
 }
 
-// One solve of the row-replaced system into X (LDS) for right-hand side rr
-// (global).  adjoint = false: primal M_p (rows j2 -> dH[j2] = rr[j2], rows
-// j1 merged; 2p columns e_j2, e_j1); adjoint = true: node system M (rows j2
-// -> lam[j2] - lam[j1] = rr[j2]; p columns e_j2).
-__device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, double* X,
-                                  bool adjoint) {
+// One solve of the row-replaced system for right-hand side rr (global); the
+// result is left in the solve vector S.  adjoint = false: primal M_p (rows
+// j2 -> dH[j2] = rr[j2], rows j1 merged; 2p columns e_j2, e_j1); adjoint =
+// true: node system M (rows j2 -> lam[j2] - lam[j1] = rr[j2]; p columns e_j2).
+__device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, bool adjoint) {
   const int lane = lane_id();
   const int p = pl.p;
-  int dummy = 0;
-  double* M = c.misc;          // [n x n], n = 2p or p
-  double* w = c.misc + 272;    // [n]
-  double* x0 = c.misc + 288;   // [2p]
-  for (int j = lane; j < pl.Nj; j += 64) X[j] = rr[j];
+  double* M = c.sM;            // [n x n], n = 2p or p
+  double* w = c.sw;            // [n]
+  double* x0 = c.sx0;          // [2p]
+  double* const S = c.R;
+  for (int j = lane; j < pl.Nj; j += 64) S[j] = rr[j];
   lds_sync();
-  run_program<1>(pl.fwd, c.V, X, dummy);
-  wood_probe(pl, c, X, x0);
+  solve_fwd(pl, c);
+  wood_probe(pl, c, S, x0);
   const int n = adjoint ? p : 2 * p;
   if (lane == 0) {
     for (int a = 0; a < p; ++a) {
@@ -230,29 +220,28 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, d
       }
     }
   }
-  mem_sync();
-  if (!small_solve(M, w, n)) return false;
-  wood_apply(pl, c, X, w, n);
-  run_program<2>(pl.bwd, c.V, X, dummy);
   lds_sync();
+  if (!small_solve(M, w, n)) return false;
+  wood_apply(pl, c, S, w, n);
+  solve_bwd(pl, c);
   return true;
 }
 
 // Primal reduced Newton system M_p dH = r_p (_adjgpu_primal_solve,
-// dgga/autodiff.py:965-1029).  rhs_p in c.v1, r_prv in misc[256..]; result
-// in dst.  Uses v3 (rp), v4 (work), v5 (residual) and X2.
+// dgga/autodiff.py:965-1029).  rhs_p in c.v1, r_prv in sprv; result in dst
+// (LDS, not the solve vector).  Uses v3 (rp), v4 (work), v5 (residual).
 __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   const int lane = lane_id();
   const int p = pl.p;
   bool any_act = false;
   for (int i = 0; i < p; ++i) any_act |= c.sh[i] != 0.0;
   if (!any_act) {
-    solveA(pl, c, c.v1, dst, 2);
+    k4_solve(pl, c, c.v1, dst, 2);
     return true;
   }
   double* rp = c.v3;
   double* wk = c.v4;
-  const double* rprv = c.misc + 256;
+  const double* rprv = c.sprv;
   for (int j = lane; j < pl.Nj; j += 64) rp[j] = c.v1[j];
   mem_sync();
   if (lane == 0) {
@@ -262,10 +251,13 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
       rp[pl.prv_s2[i]] = (c.sh[i] != 0.0) ? -rprv[i] : c.v1[pl.prv_s2[i]];
   }
   mem_sync();
+  WDS_T0();
   wood_paths(pl, c);
-  if (!wood_solve(pl, c, rp, c.X2, false)) return false;
-  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.X2[j];
-  mem_sync();
+  WDS_TACC(c, 110);
+  if (!wood_solve(pl, c, rp, false)) return false;
+  WDS_TACC(c, 111);
+  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.R[j];
+  lds_sync();
   // two refinement steps on the residual of the true replaced system
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);  // base = A-hat dH
@@ -282,14 +274,14 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
       }
     }
     mem_sync();
-    if (!wood_solve(pl, c, res, c.X2, false)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.X2[j];
-    mem_sync();
+    if (!wood_solve(pl, c, res, false)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.R[j];
+    lds_sync();
   }
   return true;
 }
 
-// link residuals r_l (into Y), PRV residuals (misc[256..]), emitter residual
+// link residuals r_l (into Y), PRV residuals (sprv), emitter residual
 // r_e (into v4 when want_re) and node mass m -> rhs_p in v1
 // (_adjgpu_residuals + the right-hand side of _adjgpu_polish,
 // dgga/autodiff.py:1032-1050, 1066-1075).  Returns the inf-norm residual.
@@ -303,13 +295,14 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
     c.Y[k] = rl;
     if (P != 0.0) rmax = fmax(rmax, fabs(rl));
   }
-  double* rprv = c.misc + 256;
+  double* rprv = c.sprv;
   if (lane < pl.p) {
     const double rp = (c.Hn[pl.prv_s2[lane]] - pl.prv_hset[lane]) * c.sh[lane];
     rprv[lane] = rp;
     rmax = fmax(rmax, fabs(rp));
   }
   mem_sync();
+  lds_sync();
   for (int j = lane; j < pl.Nj; j += 64) {
     double m = 0.0, bt = 0.0;
     const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
@@ -371,17 +364,22 @@ __device__ inline bool chp_on_clamp(const DevPlan& pl, const Scn& c) {
 __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolveParams& sp, int b,
                                         double& resid) {
   const int lane = lane_id();
-  double* dH = c.v2;
+  double* dH = c.X2;  // LDS
   for (int step = 0; step < sp.polish_steps; ++step) {
+    WDS_T0();
     if (chp_on_clamp(pl, c)) return kInfoChpClamp;
     polish_coeffs(pl, c);
     polish_assemble(pl, c);
+    WDS_TACC(c, 104);
     if (k3_factor(pl, c)) return -(step + 1);
+    WDS_TACC(c, 105);
     polish_residuals(pl, c, true);
     // v4 holds r_e; keep it in F (free during the polish) before primal_solve
     for (int j = lane; j < pl.Nj; j += 64) c.F[j] = c.v4[j];
     mem_sync();
+    WDS_TACC(c, 106);
     if (!primal_solve(pl, c, dH)) return -100 - step;
+    WDS_TACC(c, 107);
     // dq = -P (r_l + dH[j2] - dH[j1]); ACTIVE PRVs from the downstream row
     for (int k = lane; k < pl.L; k += 64) {
       const int f = pl.l_flags[k];
@@ -417,8 +415,10 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
     }
     mem_sync();
     lds_sync();
+    WDS_TACC(c, 108);
   }
   // terminal state: coefficients, factor kept for the adjoint, residual
+  WDS_T0();
   if (chp_on_clamp(pl, c)) return kInfoChpClamp;
   polish_coeffs(pl, c);
   polish_assemble(pl, c);
@@ -426,23 +426,25 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
   double* fo = sp.factor + (int64_t)b * pl.nnzL;
   for (int t = lane; t < pl.nnzL; t += 64) fo[t] = c.V[t];
   resid = polish_residuals(pl, c, false);
+  WDS_TACC(c, 109);
   return 0;
 }
 
 // Node adjoint M lam = r (_adjgpu_node_solve, dgga/autodiff.py:1107-1154):
-// r in v1, gQ of the valves in misc[256..]; result in dst.
+// r in v1, gQ of the valves in sprv; result in dst (LDS, not the solve
+// vector).
 __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_ref) {
   const int lane = lane_id();
   const int p = pl.p;
   bool any_act = false;
   for (int i = 0; i < p; ++i) any_act |= c.sh[i] != 0.0;
   if (!any_act) {
-    solveA(pl, c, c.v1, dst, n_ref);
+    k4_solve(pl, c, c.v1, dst, n_ref);
     return true;
   }
   double* r2 = c.v3;
   double* wk = c.v4;
-  const double* gqp = c.misc + 256;
+  const double* gqp = c.sprv;
   for (int j = lane; j < pl.Nj; j += 64) r2[j] = c.v1[j];
   mem_sync();
   if (lane == 0)
@@ -450,9 +452,9 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
       if (c.sh[i] != 0.0) r2[pl.prv_s2[i]] = gqp[i];
   mem_sync();
   wood_paths(pl, c);
-  if (!wood_solve(pl, c, r2, c.X2, true)) return false;
-  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.X2[j];
-  mem_sync();
+  if (!wood_solve(pl, c, r2, true)) return false;
+  for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.R[j];
+  lds_sync();
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);
     double* res = c.v5;
@@ -465,9 +467,9 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
         res[j2] = r2[j2] - (dst[j2] - dst[j1]);
       }
     mem_sync();
-    if (!wood_solve(pl, c, res, c.X2, true)) return false;
-    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.X2[j];
-    mem_sync();
+    if (!wood_solve(pl, c, res, true)) return false;
+    for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.R[j];
+    lds_sync();
   }
   return true;
 }
@@ -527,9 +529,10 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
     r = r - GH(node);
     c.v1[j] = r;
   }
-  if (lane < pl.p) c.misc[256 + lane] = GQ(pl.prv_k[lane]);
+  if (lane < pl.p) c.sprv[lane] = GQ(pl.prv_k[lane]);
   mem_sync();
-  double* lam = c.F;  // F is free in the backward
+  lds_sync();
+  double* lam = c.X2;  // LDS
   if (!node_solve(pl, c, lam, ap.n_refine)) {
     if (lane == 0) ap.info[b] = -1;
     return;

@@ -127,6 +127,27 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   for (int s = 0; s < Nj; ++s) P->jadj_ptr[s + 1] = P->jadj_ptr[s] + (int32_t)ja[s].size();
   P->jadj.clear();
   for (int s = 0; s < Nj; ++s) P->jadj.insert(P->jadj.end(), ja[s].begin(), ja[s].end());
+  // fixed-head terms of each row's right-hand side, in the order the
+  // reference adds them: links whose n1 is fixed (n2 == j), then links whose
+  // n2 is fixed (n1 == j); entries (link, fixed-node slot)
+  P->jg_ptr.assign(Nj + 1, 0);
+  P->jg.clear();
+  for (int s = 0; s < Nj; ++s) {
+    for (int32_t ent : ja[s])
+      if (ent < 0) {
+        const int k = ent & 0x7fffffff;
+        if (junc_row[d->link_n1[k]] < 0) {
+          P->jg.push_back(k);
+          P->jg.push_back(P->node_slot[d->link_n1[k]]);
+        }
+      }
+    for (int32_t ent : ja[s])
+      if (ent >= 0 && junc_row[d->link_n2[ent]] < 0) {
+        P->jg.push_back(ent);
+        P->jg.push_back(P->node_slot[d->link_n2[ent]]);
+      }
+    P->jg_ptr[s + 1] = (int32_t)(P->jg.size() / 2);
+  }
   std::vector<std::vector<int32_t>> fa(Nf);
   for (int k = 0; k < L; ++k)
     if (fixed_idx[d->link_n1[k]] >= 0) fa[fixed_idx[d->link_n1[k]]].push_back(k);
@@ -296,6 +317,10 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   flat.erase(std::unique(flat.begin(), flat.end()), flat.end());
   P->csr_indptr.assign(Nj + 1, 0);
   P->csr_indices.resize(flat.size());
+  // pivot position of each factor entry's column (the unit-factor scaling)
+  P->l_scale.assign(S.nnzL, -1);
+  for (int j = 0; j < Nj; ++j)
+    for (int32_t q = S.colptr[j] + 1; q < S.colptr[j + 1]; ++q) P->l_scale[q] = S.colptr[j];
   P->csr_to_L.resize(flat.size());
   for (size_t e = 0; e < flat.size(); ++e) {
     const int x = (int)(flat[e] / Nj), y = (int)(flat[e] % Nj);
@@ -333,7 +358,8 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   st.factor_instr = (int64_t)S.factor.instr.size();
   st.fwd_instr = (int64_t)S.fwd.instr.size();
   st.bwd_instr = (int64_t)S.bwd.instr.size();
-  st.lds_bytes = 8 * ((int64_t)S.nnzL + 2 + (Nj + Nf) + 2 * (int64_t)Nj + 2 * kMaxPrv);
+  st.lds_bytes = 8 * (lds_sink_index(S.nnzL, Nj, Nf) + kLanes +
+                      lds_small_doubles((int64_t)P->wp_node.size(), 2 * (int64_t)P->p));
 }
 
 }  // namespace wds

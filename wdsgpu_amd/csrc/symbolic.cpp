@@ -139,7 +139,9 @@ static void emit_level(const std::vector<Task>& tasks, Program& prog, int32_t ze
         }
         for (size_t p = b; p < e; ++p) {
           Instr in = tk.ins[p];
-          in.t |= dflag | (p == b ? (kFirst | kZeroInit) : 0);
+          // the group's first lane starts from the target's value, the others
+          // from zero, so the group sum is the finished dot product
+          in.t |= dflag | (p == b ? (kFirst | (i == 0 ? 0 : kZeroInit)) : 0);
           ln.push_back(in);
         }
       }
@@ -256,25 +258,30 @@ void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
     }
     emit_level(tasks, s.factor, ZERO);
   }
-  // ---- forward solve: z_j = invD_j * (b_j - sum_k w_jk z_k) ----
+  // The solves use the unit factor: after the numeric factorisation the
+  // off-diagonal entries are scaled to l_ij = w_ij / D_j, so the forward
+  // solve is y_j = b_j - sum_k l_jk y_k, then z = D^-1 y (a diagonal pass),
+  // and the backward solve is x_j = z_j - sum_i l_ij x_i.  Both are the same
+  // dot-product form; rows without updates need no task.
+  // ---- forward solve ----
   for (int lev = 0; lev < s.nlevels; ++lev) {
     std::vector<Task> tasks;
     for (int32_t j : bylevel[lev]) {
       std::vector<Instr> v;
-      for (int32_t k : rowst[j])
-        v.push_back(Instr{(int32_t)s.pos(j, k), k, s.colptr[j], j});
-      if (v.empty()) v.push_back(Instr{ZERO, j, s.colptr[j], j});
+      for (int32_t k : rowst[j]) v.push_back(Instr{(int32_t)s.pos(j, k), k, 0, j});
+      if (v.empty()) continue;
       tasks.push_back(Task{std::move(v), false});
     }
+    if (tasks.empty()) continue;
     emit_level(tasks, s.fwd, ZERO);
   }
-  // ---- backward solve: x_j = z_j - invD_j * sum_i w_ij x_i (top level first) ----
+  // ---- backward solve (top level first) ----
   for (int lev = s.nlevels - 1; lev >= 0; --lev) {
     std::vector<Task> tasks;
     for (int32_t j : bylevel[lev]) {
       std::vector<Instr> v;
       for (int32_t p = s.colptr[j] + 1; p < s.colptr[j + 1]; ++p)
-        v.push_back(Instr{p, s.rowidx[p], s.colptr[j], j});
+        v.push_back(Instr{p, s.rowidx[p], 0, j});
       if (v.empty()) continue;
       tasks.push_back(Task{std::move(v), false});
     }
