@@ -282,9 +282,17 @@ __device__ __forceinline__ double spmv_row(const DevPlan& pl, const double* A, c
                                            int i) {
   double s = 0.0;
   const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int2 en = pl.spmv_ent[e];
-    s = fma(A[en.x], x[en.y], s);
+  for (int eb = e0; eb < e1; eb += 4) {
+    double av[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int2 en = pl.spmv_ent[min(eb + u, e1 - 1)];
+      av[u] = A[en.x];
+      xv[u] = x[en.y];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (eb + u < e1) s = fma(av[u], xv[u], s);
   }
   return s;
 }
@@ -303,15 +311,24 @@ __device__ __forceinline__ double residual_row_dd(const DevPlan& pl, const doubl
                                                   const double* x, double b, int i) {
   double hi = b, lo = 0.0;
   const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
-  for (int e = e0; e < e1; ++e) {
-    const int2 en = pl.spmv_ent[e];
-    const double a = A[en.x], xv = x[en.y];
-    const double p = a * xv;
-    const double pe = fma(a, xv, -p);
-    double s, err;
-    two_sum(hi, -p, s, err);
-    hi = s;
-    lo += err - pe;
+  for (int eb = e0; eb < e1; eb += 4) {
+    double av[4], xv[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int2 en = pl.spmv_ent[min(eb + u, e1 - 1)];
+      av[u] = A[en.x];
+      xv[u] = x[en.y];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (eb + u >= e1) break;
+      const double p = av[u] * xv[u];
+      const double pe = fma(av[u], xv[u], -p);
+      double s, err;
+      two_sum(hi, -p, s, err);
+      hi = s;
+      lo += err - pe;
+    }
   }
   return hi + lo;
 }
@@ -432,16 +449,16 @@ __device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double 
 }
 
 // P, Y of link k at flow q and status st (pcv PRVs -> 0, handled later)
-__device__ __forceinline__ LinkPY link_PY(const DevPlan& pl, int k, double q, double r,
-                                          int8_t st) {
-  const int f = pl.l_flags[k];
+// f = l_flags[k], km = l_km[k] preloaded by the caller
+__device__ __forceinline__ LinkPY link_PY_f(const DevPlan& pl, int k, int f, double km,
+                                            double q, double r, int8_t st) {
   double P, Y;
   if (f & LF_PIPE) {
     double hl, hg;
     if (pl.headloss == 0)
-      hw_pipe(pl, q, r, pl.l_km[k], hl, hg);
+      hw_pipe(pl, q, r, km, hl, hg);
     else
-      dw_pipe(pl, q, r, pl.l_km[k], pl.l_diam[k], pl.l_kc[k], hl, hg);
+      dw_pipe(pl, q, r, km, pl.l_diam[k], pl.l_kc[k], hl, hg);
     P = 1.0 / hg;
     Y = hl / hg;
   } else if (f & LF_TCV) {
@@ -462,6 +479,72 @@ __device__ __forceinline__ LinkPY link_PY(const DevPlan& pl, int k, double q, do
     Y = q;
   }
   return LinkPY{P, Y};
+}
+
+__device__ __forceinline__ LinkPY link_PY(const DevPlan& pl, int k, double q, double r,
+                                          int8_t st) {
+  return link_PY_f(pl, k, pl.l_flags[k], pl.l_km[k], q, r, st);
+}
+
+// Batched gathers: a loop over items issues all loads of kGather items before
+// it consumes any, so one memory latency is paid per batch instead of per
+// item; results are accumulated in the original item order.
+constexpr int kGather = 4;
+
+// Signed gather over one side of a junction row: for entries e in [e0, e1)
+// of jadj, acc_a += sgn * a[k] and acc_b += sgn * (b[k] * w[k]) in entry
+// order (b and w may be null: acc_b untouched); sgn = +1 adds the values,
+// -1 adds their negation, as the reference's scatter of -x does.
+__device__ inline void row_gather(const DevPlan& pl, int e0, int e1, double sgn, const double* a,
+                                  const double* b, const double* w, double& acc_a,
+                                  double& acc_b) {
+  for (int eb = e0; eb < e1; eb += kGather) {
+    double av[kGather], bv[kGather], wv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = pl.jadj[min(eb + u, e1 - 1)] & 0x7fffffff;
+      av[u] = a[k];
+      if (b) {
+        bv[u] = b[k];
+        wv[u] = w[k];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      if (eb + u >= e1) break;
+      acc_a += sgn > 0.0 ? av[u] : -av[u];
+      if (b) acc_b += sgn > 0.0 ? bv[u] * wv[u] : -(bv[u] * wv[u]);
+    }
+  }
+}
+
+// link coefficients P, Y of every link (K1 and the polish), batched
+__device__ inline void link_coeffs_all(const DevPlan& pl, const double* q, const double* r,
+                                       const int8_t* S, double* P, double* Y) {
+  const int lane = lane_id();
+  for (int k0 = lane; k0 < pl.L; k0 += 64 * kGather) {
+    double qv[kGather], rv[kGather], kmv[kGather];
+    int fv[kGather];
+    int8_t sv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = min(k0 + 64 * u, pl.L - 1);
+      qv[u] = q[k];
+      rv[u] = r[k];
+      sv[u] = S[k];
+      fv[u] = pl.l_flags[k];
+      kmv[u] = pl.l_km[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = k0 + 64 * u;
+      if (k < pl.L) {
+        const LinkPY py = link_PY_f(pl, k, fv[u], kmv[u], qv[u], rv[u], sv[u]);
+        P[k] = py.P;
+        Y[k] = py.Y;
+      }
+    }
+  }
 }
 
 // emitterheadloss (dgga/solver.py:2368-2377)

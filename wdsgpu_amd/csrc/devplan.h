@@ -91,6 +91,7 @@ struct DevPlan {
   // (link order) then links with n2 == j; entry = link | (side << 31)
   const int32_t* jadj_ptr;    // [Nj+1]
   const int32_t* jadj;        // [..]
+  const int32_t* jadj_mid;    // [Nj] first n2-side entry of each row
   // fixed-head terms of each junction row: (link, fixed-node slot), links
   // with a fixed n1 first, then links with a fixed n2
   const int32_t* jg_ptr;      // [Nj+1]

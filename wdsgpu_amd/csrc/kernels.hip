@@ -122,7 +122,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
          o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
          o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
-  size_t o_jgp = bb.add(P->jg_ptr), o_jg = bb.add(P->jg);
+  size_t o_jgp = bb.add(P->jg_ptr), o_jg = bb.add(P->jg), o_jm = bb.add(P->jadj_mid);
   size_t o_wpp = bb.add(P->wp_ptr), o_wpn = bb.add(P->wp_node), o_wpd = bb.add(P->wp_diag),
          o_wtp = bb.add(P->wt_ptr), o_wt = bb.add(P->wt), o_wpr = bb.add(P->wpair);
   size_t o_c2l = bb.add(P->csr_to_L), o_dp = bb.add(P->diag_pos), o_rs = bb.add(P->row_slot);
@@ -196,6 +196,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.prv_hset = at<double>(d, o_hs);
   D.prv_km = at<double>(d, o_pkm);
   D.jg_ptr = at<int32_t>(d, o_jgp);
+  D.jadj_mid = at<int32_t>(d, o_jm);
   D.jg = at<int2>(d, o_jg);
   D.np = 2 * P->p;
   D.wp_tot = (int32_t)P->wp_node.size();

@@ -224,11 +224,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
 // K1: link and emitter coefficients at the current state
 __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
-  for (int k = lane; k < pl.L; k += 64) {
-    const LinkPY py = link_PY(pl, k, c.q[k], c.r[k], c.S[k]);
-    c.P[k] = py.P;
-    c.Y[k] = py.Y;
-  }
+  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
   for (int j = lane; j < pl.Nj; j += 64) {
     const double ke = c.kej[j];
     if (ke > 0.0) {
@@ -247,9 +243,7 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
 // dgga/autodiff.py:928-962) instead of being skipped.
 // K2 gathers: entries of a row (and off-diagonal positions) are fetched in
 // batches of kGather with every load issued before any is consumed, and the
-// accumulation is branch-free, so a batch costs one memory latency instead
-// of one per entry.  Sums keep the reference's order exactly.
-constexpr int kGather = 4;
+// accumulation is branch-free.  Sums keep the reference's order exactly.
 
 __device__ inline void k2_assemble(const DevPlan& pl, Scn& c, bool want_F) {
   const int lane = lane_id();
@@ -453,15 +447,32 @@ __device__ inline bool k3_factor(const DevPlan& pl, Scn& c) {
 __device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
   const int lane = lane_id();
   double dqs = 0.0, qs = 0.0, dqe = 0.0, qse = 0.0;
-  for (int k = lane; k < pl.L; k += 64) {
-    const double dh = c.Hn[pl.l_s1[k]] - c.Hn[pl.l_s2[k]];
-    const double q = c.q[k];
-    double dq = c.Y[k] - c.P[k] * dh;
-    if ((pl.l_flags[k] & LF_PUMP_CHP) && dq > q) dq = q / 2.0;
-    const double qn = q - dq;
-    dqs += fabs(dq);
-    qs += fabs(qn);
-    c.q[k] = qn;
+  for (int k0 = lane; k0 < pl.L; k0 += 64 * kGather) {
+    int s1[kGather], s2[kGather], fl[kGather];
+    double qv[kGather], Yv[kGather], Pv[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = min(k0 + 64 * u, pl.L - 1);
+      s1[u] = pl.l_s1[k];
+      s2[u] = pl.l_s2[k];
+      fl[u] = pl.l_flags[k];
+      qv[u] = c.q[k];
+      Yv[u] = c.Y[k];
+      Pv[u] = c.P[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = k0 + 64 * u;
+      if (k >= pl.L) break;
+      const double dh = c.Hn[s1[u]] - c.Hn[s2[u]];
+      const double q = qv[u];
+      double dq = Yv[u] - Pv[u] * dh;
+      if ((fl[u] & LF_PUMP_CHP) && dq > q) dq = q / 2.0;
+      const double qn = q - dq;
+      dqs += fabs(dq);
+      qs += fabs(qn);
+      c.q[k] = qn;
+    }
   }
   for (int j = lane; j < pl.Nj; j += 64) {
     if (c.kej[j] > 0.0) {

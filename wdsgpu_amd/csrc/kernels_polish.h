@@ -26,12 +26,7 @@ namespace wds {
 // (global), hle/hge and the per-valve ACTIVE mask into sh[0..p).
 __device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
-  for (int k = lane; k < pl.L; k += 64) {
-    const double q = c.q[k];
-    LinkPY py = link_PY(pl, k, q, c.r[k], c.S[k]);
-    c.P[k] = py.P;
-    c.Y[k] = py.Y;
-  }
+  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
   for (int j = lane; j < pl.Nj; j += 64) {
     const double ke = c.kej[j];
     if (ke > 0.0) {
@@ -305,21 +300,10 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
   lds_sync();
   for (int j = lane; j < pl.Nj; j += 64) {
     double m = 0.0, bt = 0.0;
-    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
-    for (int e = e0; e < e1; ++e) {  // n2 == j first (reference scatter order)
-      const int ent = pl.jadj[e];
-      if (ent >= 0) continue;
-      const int k = ent & 0x7fffffff;
-      m += c.q[k];
-      bt += c.P[k] * c.Y[k];
-    }
-    for (int e = e0; e < e1; ++e) {
-      const int ent = pl.jadj[e];
-      if (ent < 0) continue;
-      const int k = ent;
-      m += -c.q[k];
-      bt += -(c.P[k] * c.Y[k]);
-    }
+    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1], em = pl.jadj_mid[j];
+    // n2 == j first (reference scatter order), then n1 == j negated
+    row_gather(pl, em, e1, 1.0, c.q, c.P, c.Y, m, bt);
+    row_gather(pl, e0, em, -1.0, c.q, c.P, c.Y, m, bt);
     double re = 0.0, emh = 0.0;
     if (c.kej[j] > 0.0) {
       re = c.hle[j] - (c.Hn[j] - pl.el_slot[j]);
@@ -510,18 +494,13 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
   lds_sync();
   // r = B^T P gQ - em gE / hge - gH (n2 side first, reference scatter order)
   for (int j = lane; j < pl.Nj; j += 64) {
-    double r = 0.0;
-    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
-    for (int e = e0; e < e1; ++e) {
-      const int ent = pl.jadj[e];
-      if (ent >= 0) continue;
-      const int k = ent & 0x7fffffff;
-      r += c.P[k] * GQ(k);
-    }
-    for (int e = e0; e < e1; ++e) {
-      const int ent = pl.jadj[e];
-      if (ent < 0) continue;
-      r += -(c.P[ent] * GQ(ent));
+    double r = 0.0, unused = 0.0;
+    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1], em = pl.jadj_mid[j];
+    if (gQ) {
+      row_gather(pl, em, e1, 1.0, c.P, c.P, gQ, unused, r);
+      row_gather(pl, e0, em, -1.0, c.P, c.P, gQ, unused, r);
+    } else {
+      // a zero cotangent still adds (+-0.0) terms: keep r exactly 0.0
     }
     const int node = pl.slot_node[j];
     const double emh = c.kej[j] > 0.0 ? 1.0 / c.hge[j] : 0.0;

@@ -127,6 +127,13 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   for (int s = 0; s < Nj; ++s) P->jadj_ptr[s + 1] = P->jadj_ptr[s] + (int32_t)ja[s].size();
   P->jadj.clear();
   for (int s = 0; s < Nj; ++s) P->jadj.insert(P->jadj.end(), ja[s].begin(), ja[s].end());
+  // first n2-side entry of each row (n1-side entries come first)
+  P->jadj_mid.assign(Nj, 0);
+  for (int s = 0; s < Nj; ++s) {
+    int32_t e = P->jadj_ptr[s];
+    while (e < P->jadj_ptr[s + 1] && P->jadj[e] >= 0) ++e;
+    P->jadj_mid[s] = e;
+  }
   // fixed-head terms of each row's right-hand side, in the order the
   // reference adds them: links whose n1 is fixed (n2 == j), then links whose
   // n2 is fixed (n1 == j); entries (link, fixed-node slot)

@@ -24,7 +24,7 @@ struct wds_plan {
   std::vector<int32_t> spmv_ent;  // pairs (pos, col)
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
-  std::vector<int32_t> l_scale;
+  std::vector<int32_t> l_scale, jadj_mid;
   std::vector<double> prv_hset, prv_km, tl_hmax, tl_hmin;
   // scalars
   int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
