@@ -410,7 +410,7 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
                                 int n_refine) {
   const int lane = lane_id();
   double* const S = c.R;
-  for (int j = lane; j < pl.Nj; j += 64) S[j] = bsrc[j];
+  vec_copy(S, bsrc, nullptr, pl.Nj);
   lds_sync();
   solve_fwd(pl, c);
   solve_bwd(pl, c);

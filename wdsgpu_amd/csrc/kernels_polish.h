@@ -79,48 +79,54 @@ __device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double
 }
 
 // Solve the n x n system M y = w (row-major, LDS) by Gaussian elimination
-// with partial pivoting on lane 0; y overwrites w.  Returns false if singular.
+// with partial pivoting (largest magnitude, lowest row on ties), rows
+// eliminated in parallel, one lane per row; y overwrites w.  Returns false
+// if singular.
 __device__ inline bool small_solve(double* M, double* w, int n) {
-  bool ok = true;
-  if (lane_id() == 0) {
-    for (int col = 0; col < n; ++col) {
-      int piv = col;
-      double best = fabs(M[col * n + col]);
-      for (int r = col + 1; r < n; ++r)
-        if (fabs(M[r * n + col]) > best) {
-          best = fabs(M[r * n + col]);
-          piv = r;
-        }
-      if (best == 0.0) {
-        ok = false;
-        break;
+  const int lane = lane_id();
+  for (int col = 0; col < n; ++col) {
+    double v = (lane >= col && lane < n) ? fabs(M[lane * n + col]) : -1.0;
+    int idx = lane;
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) {
+      const double ov = __shfl_xor(v, m, 64);
+      const int oi = __shfl_xor(idx, m, 64);
+      if (ov > v || (ov == v && oi < idx)) {
+        v = ov;
+        idx = oi;
       }
-      if (piv != col) {
-        for (int cc = 0; cc < n; ++cc) {
-          const double t = M[col * n + cc];
-          M[col * n + cc] = M[piv * n + cc];
-          M[piv * n + cc] = t;
-        }
+    }
+    if (!(v > 0.0)) return false;  // uniform: every lane holds the maximum
+    const int piv = idx;
+    if (piv != col) {
+      if (lane < n) {
+        const double t = M[col * n + lane];
+        M[col * n + lane] = M[piv * n + lane];
+        M[piv * n + lane] = t;
+      } else if (lane == n) {
         const double t = w[col];
         w[col] = w[piv];
         w[piv] = t;
       }
-      for (int r = col + 1; r < n; ++r) {
-        const double f = M[r * n + col] / M[col * n + col];
-        if (f == 0.0) continue;
-        for (int cc = col; cc < n; ++cc) M[r * n + cc] -= f * M[col * n + cc];
-        w[r] -= f * w[col];
+      lds_sync();
+    }
+    if (lane > col && lane < n) {
+      const double f = M[lane * n + col] / M[col * n + col];
+      if (f != 0.0) {
+        for (int cc = col; cc < n; ++cc) M[lane * n + cc] -= f * M[col * n + cc];
+        w[lane] -= f * w[col];
       }
     }
-    if (ok)
-      for (int r = n - 1; r >= 0; --r) {
-        double s = w[r];
-        for (int cc = r + 1; cc < n; ++cc) s -= M[r * n + cc] * w[cc];
-        w[r] = s / M[r * n + r];
-      }
+    lds_sync();
   }
+  if (lane == 0)
+    for (int r = n - 1; r >= 0; --r) {
+      double s = w[r];
+      for (int cc = r + 1; cc < n; ++cc) s -= M[r * n + cc] * w[cc];
+      w[r] = s / M[r * n + r];
+    }
   lds_sync();
-  return __shfl(ok ? 1 : 0, 0, 64) != 0;
+  return true;
 }
 
 // Paths of the valve ends for the factor in V: y_u, z_u = D^-1 y_u (one
@@ -187,32 +193,39 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   double* w = c.sw;            // [n]
   double* x0 = c.sx0;          // [2p]
   double* const S = c.R;
-  for (int j = lane; j < pl.Nj; j += 64) S[j] = rr[j];
+  vec_copy(S, rr, nullptr, pl.Nj);
   lds_sync();
   solve_fwd(pl, c);
   wood_probe(pl, c, S, x0);
   const int n = adjoint ? p : 2 * p;
-  if (lane == 0) {
-    for (int a = 0; a < p; ++a) {
-      const double act = c.sh[a];
-      const int j2 = pl.prv_s2[a];
-      for (int col = 0; col < n; ++col) {
-        const double s2 = c.Cm[a * pl.np + col];          // e_j2a^T A^-1 e_col
-        if (adjoint) {
-          const double s1 = c.Cm[(p + a) * pl.np + col];  // e_j1a^T A^-1 e_col
-          M[a * n + col] = (s2 - s1) * (act * c.sh[col]) + ((a == col) ? 1.0 - act : 0.0);
-        } else {
-          const double eye = (col == a || col == a + p) ? 1.0 : 0.0;
-          M[a * n + col] = s2 * act + ((a == col) ? 1.0 - act : 0.0);
-          M[(a + p) * n + col] = (eye - CBIG * s2) * act + ((a + p == col) ? 1.0 - act : 0.0);
-        }
-      }
-      if (adjoint) {
-        w[a] = (x0[a] - x0[p + a] - rr[j2]) * act;
-      } else {
-        w[a] = (x0[a] - rr[j2]) * act;
-        w[a + p] = (rr[j2] - CBIG * x0[a]) * act;
-      }
+  // capacitance rows, one entry per lane: row a < p is valve a's replaced
+  // downstream row; rows p + a (primal only) its merged upstream row
+  for (int e = lane; e < n * n; e += 64) {
+    const int row = e / n, col = e % n;
+    const int a = row < p ? row : row - p;
+    const double act = c.sh[a];
+    const double s2 = c.Cm[a * pl.np + col];          // e_j2a^T A^-1 e_col
+    double v;
+    if (adjoint) {
+      const double s1 = c.Cm[(p + a) * pl.np + col];  // e_j1a^T A^-1 e_col
+      v = (s2 - s1) * (act * c.sh[col]) + ((a == col) ? 1.0 - act : 0.0);
+    } else if (row < p) {
+      v = s2 * act + ((a == col) ? 1.0 - act : 0.0);
+    } else {
+      const double eye = (col == a || col == a + p) ? 1.0 : 0.0;
+      v = (eye - CBIG * s2) * act + ((a + p == col) ? 1.0 - act : 0.0);
+    }
+    M[row * n + col] = v;
+  }
+  if (lane < p) {
+    const int a = lane;
+    const double act = c.sh[a];
+    const double r2 = rr[pl.prv_s2[a]];
+    if (adjoint) {
+      w[a] = (x0[a] - x0[p + a] - r2) * act;
+    } else {
+      w[a] = (x0[a] - r2) * act;
+      w[a + p] = (r2 - CBIG * x0[a]) * act;
     }
   }
   lds_sync();
@@ -237,7 +250,7 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   double* rp = c.v3;
   double* wk = c.v4;
   const double* rprv = c.sprv;
-  for (int j = lane; j < pl.Nj; j += 64) rp[j] = c.v1[j];
+  vec_copy(rp, c.v1, nullptr, pl.Nj);
   mem_sync();
   if (lane == 0) {
     for (int i = 0; i < p; ++i)
@@ -257,7 +270,7 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);  // base = A-hat dH
     double* res = c.v5;
-    for (int j = lane; j < pl.Nj; j += 64) res[j] = rp[j] - wk[j];
+    vec_copy(res, rp, wk, pl.Nj);
     mem_sync();
     if (lane == 0) {
       for (int i = 0; i < p; ++i) {
@@ -283,12 +296,27 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
 __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_rhs) {
   const int lane = lane_id();
   double rmax = 0.0;
-  for (int k = lane; k < pl.L; k += 64) {
-    const double P = c.P[k];
-    const double hl = (P != 0.0) ? c.Y[k] / P : 0.0;
-    const double rl = hl - (c.Hn[pl.l_s1[k]] - c.Hn[pl.l_s2[k]]);
-    c.Y[k] = rl;
-    if (P != 0.0) rmax = fmax(rmax, fabs(rl));
+  for (int k0 = lane; k0 < pl.L; k0 += 64 * kGather) {
+    double Pv[kGather], Yv[kGather];
+    int s1[kGather], s2[kGather];
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = min(k0 + 64 * u, pl.L - 1);
+      Pv[u] = c.P[k];
+      Yv[u] = c.Y[k];
+      s1[u] = pl.l_s1[k];
+      s2[u] = pl.l_s2[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kGather; ++u) {
+      const int k = k0 + 64 * u;
+      if (k >= pl.L) break;
+      const double P = Pv[u];
+      const double hl = (P != 0.0) ? Yv[u] / P : 0.0;
+      const double rl = hl - (c.Hn[s1[u]] - c.Hn[s2[u]]);
+      c.Y[k] = rl;
+      if (P != 0.0) rmax = fmax(rmax, fabs(rl));
+    }
   }
   double* rprv = c.sprv;
   if (lane < pl.p) {
@@ -429,7 +457,7 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   double* r2 = c.v3;
   double* wk = c.v4;
   const double* gqp = c.sprv;
-  for (int j = lane; j < pl.Nj; j += 64) r2[j] = c.v1[j];
+  vec_copy(r2, c.v1, nullptr, pl.Nj);
   mem_sync();
   if (lane == 0)
     for (int i = 0; i < p; ++i)
@@ -442,7 +470,7 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);
     double* res = c.v5;
-    for (int j = lane; j < pl.Nj; j += 64) res[j] = r2[j] - wk[j];
+    vec_copy(res, r2, wk, pl.Nj);
     mem_sync();
     if (lane == 0)
       for (int i = 0; i < p; ++i) {
