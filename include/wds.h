@@ -145,6 +145,9 @@ typedef struct wds_solve_args {
   double* resid;               /* [B] polish residual inf-norm or NULL */
   double* factor;              /* [B][nnzL] terminal factor for the adjoint, or NULL */
   void* workspace;             /* wds_workspace_bytes(plan, B) */
+  const int8_t* status0;       /* [B][L] initial internal link statuses (a warm start
+                                  between extended-period frames) or NULL (the
+                                  network's initial statuses) */
 } wds_solve_args;
 
 /* Fused forward: the whole Newton loop of every scenario in one persistent

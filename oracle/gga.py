@@ -625,13 +625,16 @@ def chol_solve_refined(A, F, n_refine=2, extended=False):
 
 def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
           max_iter=None, accuracy=None, status_machine=True, assemble="dense",
-          n_refine=2, record=False, extended=False):
+          n_refine=2, record=False, extended=False, status0=None):
     """Batched dense GGA (dgga/solver.py:2703-3212, mode='dense',
     dense_status_machine=True).  demand/res_head/ke: [B,N] or [N];
     r_hw: [L] or [B,L].  Returns the reference's output dict (head_ft,
     flow_cfs, emitter_cfs, iters, relerr, converged, status, diag_ratio)
     plus, when ``record``, per-iteration traces.  ``extended`` evaluates the
-    refinement residuals in extended precision (see chol_solve_refined)."""
+    refinement residuals in extended precision (see chol_solve_refined).
+    ``status0`` ([L] or [B,L] internal codes) replaces the initial statuses
+    (the warm start of an extended-period frame, as run_gga's status0 in the
+    reference's EPANET-mode solve, dgga/solver.py:2855)."""
     m = model
     d = np.atleast_2d(np.asarray(demand, dtype=np.float64))
     rh = np.atleast_2d(np.asarray(res_head, dtype=np.float64))
@@ -667,7 +670,8 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
     relerr_out = np.zeros(B)
     sm = bool(status_machine)
     maxtrials = max_iter
-    S = np.broadcast_to(m.init_status_int, (B, m.L)).copy()
+    S = np.broadcast_to(m.init_status_int if status0 is None else np.asarray(status0, np.int8),
+                        (B, m.L)).copy()
     nextcheck = None
     diag_ratio = np.zeros(B) if m.n_pcv else None
     if sm:

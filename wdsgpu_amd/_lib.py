@@ -62,7 +62,7 @@ class SolveArgs(C.Structure):
                 ("head", C.c_void_p), ("flow", C.c_void_p), ("emitter", C.c_void_p),
                 ("iters", C.c_void_p), ("relerr", C.c_void_p), ("status", C.c_void_p),
                 ("diag_ratio", C.c_void_p), ("info", C.c_void_p), ("resid", C.c_void_p),
-                ("factor", C.c_void_p), ("workspace", C.c_void_p)]
+                ("factor", C.c_void_p), ("workspace", C.c_void_p), ("status0", C.c_void_p)]
 
 
 class AdjointArgs(C.Structure):

@@ -39,6 +39,7 @@ struct SolveParams {
   double* resid;
   double* factor;
   double* ws;
+  const int8_t* status0;
 };
 
 inline SolveParams make_params(const wds_solve_args* a) {
@@ -67,6 +68,7 @@ inline SolveParams make_params(const wds_solve_args* a) {
   s.resid = a->resid;
   s.factor = a->factor;
   s.ws = (double*)a->workspace;
+  s.status0 = a->status0;
   return s;
 }
 
@@ -731,7 +733,7 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
   const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
   for (int k = lane; k < pl.L; k += 64) {
     c.q[k] = sp.q0 ? sp.q0[(int64_t)b * pl.L + k] : pl.l_q0[k];
-    c.S[k] = pl.l_st0[k];
+    c.S[k] = sp.status0 ? sp.status0[(int64_t)b * pl.L + k] : pl.l_st0[k];
   }
   for (int j = lane; j < pl.Nj; j += 64) {
     const int node = pl.slot_node[j];
@@ -797,7 +799,7 @@ __global__ __launch_bounds__(64) void k_newton_iter(DevPlan pl, SolveParams sp, 
   if (it == 1) {
     for (int k = lane; k < pl.L; k += 64) {
       c.q[k] = sp.q0 ? sp.q0[(int64_t)b * pl.L + k] : pl.l_q0[k];
-      c.S[k] = pl.l_st0[k];
+      c.S[k] = sp.status0 ? sp.status0[(int64_t)b * pl.L + k] : pl.l_st0[k];
     }
     for (int j = lane; j < pl.Nj; j += 64) {
       const int node = pl.slot_node[j];

@@ -220,6 +220,44 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
                r_hw=r, init_status=st, valve_setting_user=setting, meta=meta, **kw)
 
 
+def eps_network(seed=0, nx=6, ny=6, hours=24, headloss="H-W"):
+    """Extended-period network: a ``feature_network`` grid whose supply pump
+    is switched by the tank's level (LOWLEVEL opens it at 12 m, HILEVEL
+    closes it at 20 m) and whose junction demands follow a seeded diurnal
+    pattern (24 hourly factors in [0.6, 1.4]); duration ``hours``, hydraulic
+    and pattern steps 1 h.  The tank (area 12 m^2) starts at 17 m; the
+    gravity reservoir is lowered to 52 m.""" 
+    from .network import ST_CLOSED, ST_OPEN
+    g = np.random.default_rng(seed + 1000)
+    net = feature_network(seed, nx=nx, ny=ny, headloss=headloss, n_pumps=1, n_prv=1,
+                          n_tcv=1, n_cv=1, tank=True, n_leaks=2, tank_level_m=17.0)
+    base = 1.0 + 0.4 * np.sin(2.0 * np.pi * (np.arange(24) - 6.0) / 24.0)
+    pat = np.clip(base + g.uniform(-0.05, 0.05, 24), 0.6, 1.4)
+    net.patterns = [pat]
+    nt = np.asarray(net.node_type)
+    net.dem_pat = np.where(nt == JUNCTION, 0, -1).astype(np.int32)
+    tank = int(net.tank_node[0])
+    net.tank_area = np.array([12.0 / M_PER_FT ** 2])
+    net.tank_hmin = np.array([(40.0 + 2.0) / M_PER_FT])
+    net.tank_hmax = np.array([(40.0 + 25.0) / M_PER_FT])
+    # the gravity reservoir sits below the full tank, so the tank drains at
+    # peak demand and the pump controls switch during the day
+    el = np.asarray(net.elev_ft).copy()
+    el[(nt == RESERVOIR) & (el > 60.0 / M_PER_FT)] = 52.0 / M_PER_FT
+    net.elev_ft = el
+    pump = int(net.pump_link[0])
+    net.ctl_link = np.array([pump, pump], dtype=np.int32)
+    net.ctl_node = np.array([tank, tank], dtype=np.int32)
+    net.ctl_type = np.array([0, 1], dtype=np.int8)
+    net.ctl_grade = np.array([(40.0 + 12.0) / M_PER_FT, (40.0 + 20.0) / M_PER_FT])
+    net.ctl_time = np.zeros(2, dtype=np.int64)
+    net.ctl_status = np.array([ST_OPEN, ST_CLOSED], dtype=np.int8)
+    net.meta.update(duration_sec=3600 * int(hours), hyd_step_sec=3600, pat_step_sec=3600,
+                    report_step_sec=3600, pat_start_sec=0, tstart_sec=0,
+                    workload="eps-" + net.meta["workload"])
+    return net
+
+
 def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,
                    n_leaks=10, demand_lps=(0.05, 0.5), prv_setting_m=(30.0, 50.0),
                    res_head_m=(60.0, 90.0)):

@@ -94,6 +94,15 @@ class Net:
     pump_q0: np.ndarray = None
     pump_qmax: np.ndarray = None
     pump_hmax: np.ndarray = None
+    # extended period (dgga/parse.py Net: patterns, controls, tank volumes)
+    patterns: list = None          # list of float64 multiplier arrays
+    dem_pat: np.ndarray = None     # int32[N] pattern of each node's demand, -1 none
+    ctl_link: np.ndarray = None    # simple controls (hydraul.c controls):
+    ctl_node: np.ndarray = None    #   controlling tank node or -1
+    ctl_type: np.ndarray = None    #   0 LOWLEVEL, 1 HILEVEL, 2 TIMER, 3 TIMEOFDAY
+    ctl_grade: np.ndarray = None   #   level controls: head (ft)
+    ctl_time: np.ndarray = None    #   time controls: seconds
+    ctl_status: np.ndarray = None  #   internal status to set (CLOSED 2 / OPEN 3)
 
     def __post_init__(self):
         empty = {"tank_node": np.int32, "tank_h0": np.float64,
@@ -103,9 +112,16 @@ class Net:
                  "pump_h0": np.float64, "pump_r": np.float64,
                  "pump_n": np.float64, "pump_q0": np.float64,
                  "pump_qmax": np.float64, "pump_hmax": np.float64}
+        empty.update({"ctl_link": np.int32, "ctl_node": np.int32, "ctl_type": np.int8,
+                      "ctl_grade": np.float64, "ctl_time": np.int64,
+                      "ctl_status": np.int8})
         for k, dt in empty.items():
             if getattr(self, k) is None:
                 setattr(self, k, np.zeros(0, dtype=dt))
+        if self.patterns is None:
+            self.patterns = []
+        if self.dem_pat is None:
+            self.dem_pat = np.full(len(self.node_id), -1, dtype=np.int32)
         defaults = dict(flow_units="LPS", headloss="H-W", trials=200,
                         accuracy=1e-3, demand_model="DDA", qexp=2.0,
                         emitter_exponent=0.5, extra_iter=-1, checkfreq=2,
@@ -126,6 +142,27 @@ class Net:
     @property
     def Nj(self):
         return int(np.count_nonzero(np.asarray(self.node_type) == JUNCTION))
+
+    # ---- extended period ----
+    def tank_volumes(self):
+        """(vmin, vmax, v0) of every tank: cylinders, V = A (h - El)
+        (tankvolume, hydraul.c:1038-1068, without volume curves)."""
+        el = np.asarray(self.elev_ft)[self.tank_node]
+        a = np.asarray(self.tank_area)
+        return (a * (self.tank_hmin - el), a * (self.tank_hmax - el),
+                a * (self.tank_h0 - el))
+
+    def demand_cfs_at(self, t_sec):
+        """Junction demands at time t: base demand times its pattern factor
+        for period (t + Pstart) // Pstep (demands, hydraul.c:465-504)."""
+        pstep = int(self.meta.get("pat_step_sec", 3600)) or 3600
+        p = (int(t_sec) + int(self.meta.get("pat_start_sec", 0))) // pstep
+        d = np.asarray(self.demand_cfs, dtype=np.float64).copy()
+        for i, pat in enumerate(self.patterns):
+            sel = np.asarray(self.dem_pat) == i
+            if sel.any():
+                d[sel] = d[sel] * float(pat[p % len(pat)])
+        return d * float(self.meta.get("demand_multiplier", 1.0))
 
     def reservoir_head_ft(self):
         """float64[N]: fixed heads at reservoirs/tanks, nan at junctions

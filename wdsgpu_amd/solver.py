@@ -88,6 +88,7 @@ class GGASolver:
         st = np.asarray(net.init_status)
         map_init = {0: ST_CLOSED, 1: ST_OPEN, 2: ST_ACTIVE}
         self.init_status_int = np.asarray([map_init[int(s)] for s in st], dtype=np.int8)
+        self.init_status_int_np = self.init_status_int
         setting = _f64(net.valve_setting_user).copy()
         ucf_p, ucf_q = net.ucf_pressure(), net.ucf_flow()
         init_setting = setting.copy()
@@ -128,6 +129,7 @@ class GGASolver:
         if self.n_pumps:
             q0 = np.where(lt == PUMP, init_setting * pq0, q0)
         q0 = np.where(closed, QZERO, q0)
+        self.init_flow_np = np.asarray(q0, dtype=np.float64).copy()  # initlinkflow
         self.n_tanks = int(len(net.tank_node))
         self._keep = dict(
             node_type=_i32(nt), elev=_f64(net.elev_ft), n1=_i32(net.link_n1),
@@ -283,7 +285,8 @@ class GGASolver:
         return buf
 
     def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
-              status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True):
+              status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True,
+              status0=None):
         d = self._prep(demand_cfs)
         rh = self._prep(res_head_ft)
         single = d.dim() == 1
@@ -327,8 +330,18 @@ class GGASolver:
             if want_factor else None)
         q0t = None if q0 is None else self._prep(q0).expand(B, -1).contiguous()
         e0t = None if e0 is None else self._prep(e0).expand(B, -1).contiguous()
+        s0t = None
+        if status0 is not None:
+            s0t = torch.as_tensor(status0, dtype=torch.int8, device=dev)
+            if s0t.dim() == 1:
+                s0t = s0t.unsqueeze(0)
+            if s0t.shape[-1] != self.L or s0t.shape[0] not in (1, B):
+                raise ValueError("link_status must be [L] or [B,L] internal status codes")
+            if bool(((s0t < 0) | (s0t > 7)).any()):
+                raise ValueError("link_status holds internal status codes 0..7")
+            s0t = s0t.expand(B, -1).contiguous()
         ws = self._workspace(B)
-        keep = (d.contiguous(), rh, ke, r, q0t, e0t, ws)
+        keep = (d.contiguous(), rh, ke, r, q0t, e0t, ws, s0t)
         a = _lib.SolveArgs(
             B=B, demand=_lib.ptr(keep[0]), res_head=_lib.ptr(rh), ke=_lib.ptr(ke),
             ke_stride=ke_stride, r_hw=_lib.ptr(r), r_stride=r_stride, q0=_lib.ptr(q0t),
@@ -339,7 +352,7 @@ class GGASolver:
             iters=_lib.ptr(out["iters"]), relerr=_lib.ptr(out["relerr"]),
             status=_lib.ptr(out["status"]), diag_ratio=_lib.ptr(out["diag_ratio"]),
             info=_lib.ptr(out["info"]), resid=_lib.ptr(out["resid"]),
-            factor=_lib.ptr(out["factor"]), workspace=_lib.ptr(ws))
+            factor=_lib.ptr(out["factor"]), workspace=_lib.ptr(ws), status0=_lib.ptr(s0t))
         return a, out, keep, single, B
 
     def _validate(self, assemble, linear_solver, status_machine, link_status, link_setting):
@@ -353,8 +366,11 @@ class GGASolver:
                                               link_setting is not None or status_machine):
             raise NotImplementedError(
                 "link_status/link_setting/status_machine need dense_status_machine=True")
-        if link_status is not None or link_setting is not None:
-            raise NotImplementedError("link_status/link_setting warm starts are not supported")
+        if link_setting is not None:
+            raise NotImplementedError("link_setting warm starts are not supported "
+                                      "(per-scenario pump speeds are not on this path)")
+        if link_status is not None and not status_machine:
+            raise NotImplementedError("link_status needs status_machine=True")
         if self.has_cv and not status_machine:
             raise NotImplementedError("networks with CV pipes need status_machine=True")
         if self.n_pcv and not status_machine:
@@ -380,7 +396,8 @@ class GGASolver:
         max_iter = self.max_iter_default if max_iter is None else int(max_iter)
         hacc = self.hacc_default if accuracy is None else float(accuracy)
         a, out, keep, single, B = self._args(demand_cfs, res_head_ft, ke_int, q0, e0,
-                                             max_iter, hacc, status_machine, r_hw=r_hw)
+                                             max_iter, hacc, status_machine, r_hw=r_hw,
+                                             status0=link_status)
         with torch.cuda.device(self.device):
             _lib.check(self._lib.wds_solve(self._plan, self._stream(), C.byref(a)), "wds_solve")
         return self._finish(out, single, max_iter, status_machine)
