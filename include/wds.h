@@ -53,7 +53,8 @@ typedef struct wds_network_desc {
   const double* kc;            /* [L] Darcy-Weisbach roughness (ft) */
   const double* q0;            /* [L] initial flow (cfs) */
   const double* setting;       /* [L] internal setting; pumps: speed */
-  const int32_t* pump_kind;    /* [L] -1 not a pump, 0 CONST_HP, 1 POWER_FUNC, 3 NOCURVE */
+  const int32_t* pump_kind;    /* [L] -1 not a pump, 0 CONST_HP, 1 POWER_FUNC, 2 CUSTOM,
+                                  3 NOCURVE */
   const double* pump_h0;       /* [L] */
   const double* pump_r;        /* [L] */
   const double* pump_n;        /* [L] (N snapped to 1 when |N-1| < 1e-6) */
@@ -66,6 +67,16 @@ typedef struct wds_network_desc {
   int32_t headloss;            /* 0 Hazen-Williams, 1 Darcy-Weisbach */
   double hexp, qexp, rqtol, viscos, htol, qtol;
   int32_t checkfreq, maxcheck, extra_iter;
+  /* CUSTOM pump curves (dgga/solver.py:1346-1370, 985-1011): the points of
+   * link k are [pump_curve_ptr[k], pump_curve_ptr[k+1]), flows strictly
+   * increasing, in the input's user units, converted on every evaluation
+   * with ucf_flow (Ucf[FLOW]) and ucf_head (Ucf[HEAD]) as curvecoeff does
+   * (hydcoeffs.c:794-831).  pump_curve_ptr may be null when no pump is
+   * CUSTOM. */
+  const int32_t* pump_curve_ptr;  /* [L+1] */
+  const double* pump_curve_x;
+  const double* pump_curve_y;
+  double ucf_flow, ucf_head;
 } wds_network_desc;
 
 typedef struct wds_plan_stats {

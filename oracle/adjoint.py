@@ -83,7 +83,9 @@ def pump_coeffs(m, q, hl, hg, dr):
     dgga/autodiff.py:122-218): NOCURVE CSMALL; CONST_HP r/Q with its clamp
     branches -CBIG / -RQtol (the residual's own derivative, opposite in sign
     to the assembled P); power function h0 + r |Q|^(n-1) Q with the RQtol
-    clamp; linear h0 + r Q.  Closed pumps are handled by the caller."""
+    clamp; linear h0 + r Q; CUSTOM H0 w^2 + R w Q on the segment curvecoeff
+    selects (the exact derivative inside a segment, autodiff.py:163-175).
+    Closed pumps are handled by the caller."""
     hl, hg, dr = hl.copy(), hg.copy(), dr.copy()
     for k in m.pump_links:
         k = int(k)
@@ -93,6 +95,10 @@ def pump_coeffs(m, q, hl, hg, dr):
         if m.pl_ptype[k] == 3:                       # NOCURVE
             hl[k] = CSMALL * qk
             hg[k] = CSMALL
+            continue
+        if m.pl_ptype[k] == 2:                       # CUSTOM: segment slope
+            hlc, hgc = m.custom_hl_hg(k, np.array([qk]))
+            hl[k], hg[k] = hlc[0], hgc[0]
             continue
         sp_ = float(m.pump_speed[k])
         h0 = sp_ * sp_ * m.pl_h0[k]

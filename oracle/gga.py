@@ -154,8 +154,16 @@ class GGAModel:
             self.pl_n[pl] = net.pump_n
             self.pl_q0[pl] = net.pump_q0
             self.pl_hmax[pl] = net.pump_hmax
-            if (self.pl_ptype[pl] == PUMP_CUSTOM).any():
-                raise NotImplementedError("CUSTOM pump curves")
+        # CUSTOM curves, points in user units (dgga/solver.py:1346-1370)
+        self.pump_curve = {}
+        cptr = np.asarray(net.pump_curve_ptr, dtype=np.int64)
+        for j, k in enumerate(pl):
+            if int(net.pump_ptype[j]) == PUMP_CUSTOM:
+                a, b = int(cptr[j]), int(cptr[j + 1])
+                if b - a < 2:
+                    raise RuntimeError("CUSTOM pump without curve points")
+                self.pump_curve[int(k)] = (np.asarray(net.pump_curve_q[a:b], dtype=np.float64),
+                                           np.asarray(net.pump_curve_h[a:b], dtype=np.float64))
         sp = init_setting
         self.pump_speed = sp.copy()
         n_np = self.pl_n.copy()
@@ -323,9 +331,48 @@ class GGAModel:
         Y = np.where(km > 0.0, hloss / hgrad, q)
         return P, Y
 
+    def curvecoeff(self, k, q):
+        """Local linear model of pump link k's CUSTOM curve at flow q (cfs,
+        already divided by the speed): dgga/solver.py:1452-1474 (EPANET
+        curvecoeff, hydcoeffs.c:794-831).  The right end of the segment is
+        the first point with x >= q; below the first point the first segment
+        and above the last point the last segment extend.  Returns (h0, r)
+        in internal units."""
+        q = q * self.ucf_flow
+        x, y = self.pump_curve[k]
+        npts = len(x)
+        k2 = 0
+        while k2 < npts and x[k2] < q:
+            k2 += 1
+        if k2 == 0:
+            k2 += 1
+        elif k2 == npts:
+            k2 -= 1
+        k1 = k2 - 1
+        r = (y[k2] - y[k1]) / (x[k2] - x[k1])
+        h0 = y[k1] - r * x[k1]
+        h0 = h0 / self.ucf_head
+        r = r * self.ucf_flow / self.ucf_head
+        return h0, r
+
+    def custom_hl_hg(self, k, q):
+        """hloss, hgrad of CUSTOM pump link k at flows q (any shape):
+        H0 = -h0, R = -r, N = 1, hgrad = R w, hloss = H0 w^2 + hgrad q
+        (dgga/solver.py:1548-1558, hydcoeffs.c:716-733)."""
+        sp = max(float(self.pump_speed[k]), 1e-12)
+        qf = np.asarray(q, dtype=np.float64)
+        hl = np.empty_like(qf)
+        hg = np.empty_like(qf)
+        for i, qk in np.ndenumerate(qf):
+            h0c, rc = self.curvecoeff(k, abs(float(qk)) / sp)
+            g = -rc * sp
+            hg[i] = g
+            hl[i] = -h0c * (sp * sp) + g * float(qk)
+        return hl, hg
+
     def pump_PY(self, q):
-        """Power-function / linear / constant-HP / no-curve pumps,
-        dgga/solver.py:2221-2326 (CUSTOM refused at construction)."""
+        """Power-function / linear / constant-HP / CUSTOM / no-curve pumps,
+        dgga/solver.py:2221-2326."""
         sgn = np.where(q < 0, -1.0, 1.0)
         qa = q * sgn
         spc = np.maximum(self.pump_speed, 1e-12)
@@ -350,6 +397,13 @@ class GGAModel:
         hl_pf = np.where(self.pump_lin, h0e + re * q, hl_pf)
         hg_p = np.where(self.pump_chp, hg_chp, hg_pf)
         hl_p = np.where(self.pump_chp, hl_chp, hl_pf)
+        if self.pump_curve:
+            hg_p = np.array(hg_p, dtype=np.float64)
+            hl_p = np.array(hl_p, dtype=np.float64)
+            for k in self.pump_curve:
+                hl_c, hg_c = self.custom_hl_hg(k, q[..., k])
+                hl_p[..., k] = hl_c
+                hg_p[..., k] = hg_c
         with np.errstate(divide="ignore", invalid="ignore"):
             P = 1.0 / hg_p
             Y = hl_p / hg_p

@@ -75,3 +75,35 @@ def check_forward_against_oracle(net, m, out, d, rh, acc, tol_h_ft=1e-10 / 0.304
 # links OPEN and TEMPCLOSED (checked on the CPU in test_oracle.py)
 FEATURE_CASES = [("H-W", 0, 18.0), ("H-W", 2, 18.0), ("H-W", 1, 25.0),
                  ("D-W", 0, 18.0), ("D-W", 3, 18.0), ("D-W", 2, 25.0)]
+
+
+def custom_pump_tree(points, demand_cfs, speed=1.0, H0=100.0, flow_units="CFS"):
+    """reservoir (head H0 ft) -> CUSTOM pump with the given curve points
+    (user units of flow_units) -> junction with a demand: the pump flow is
+    the demand and the junction head is H0 plus the pump's lift."""
+    from wdsgpu_amd.network import PUMP, PUMP_CUSTOM, SI_FLOW_UNITS, pump_custom_curve
+    c = pump_custom_curve(points, flow_units=flow_units, si=flow_units in SI_FLOW_UNITS)
+    net = make_net([(RESERVOIR, H0, 0.0), (JUNCTION, 0.0, demand_cfs)],
+                   [dict(n1=0, n2=1, type=PUMP, diam_ft=0.0)])
+    net.meta["flow_units"] = flow_units
+    net.pump_link = np.array([0], dtype=np.int32)
+    net.pump_ptype = np.array([PUMP_CUSTOM], dtype=np.int8)
+    for k in ("h0", "r", "n", "q0", "qmax", "hmax"):
+        setattr(net, "pump_" + k, np.array([c[k]]))
+    net.pump_curve_q = c["x"]
+    net.pump_curve_h = c["y"]
+    net.pump_curve_ptr = np.array([0, len(c["x"])], dtype=np.int32)
+    net.valve_setting_user[0] = speed
+    return net
+
+
+def curve_lift(points, q):
+    """Head gain of a piecewise-linear pump curve at flow q: linear
+    interpolation between the points, the end segments extended beyond."""
+    x = np.array([p[0] for p in points])
+    y = np.array([p[1] for p in points])
+    if q < x[0]:
+        return y[0] + (y[1] - y[0]) / (x[1] - x[0]) * (q - x[0])
+    if q > x[-1]:
+        return y[-1] + (y[-1] - y[-2]) / (x[-1] - x[-2]) * (q - x[-1])
+    return float(np.interp(q, x, y))

@@ -29,3 +29,36 @@ def test_feature_network_matches_oracle(cuda, headloss, seed, level):
                   accuracy=1e-3, status_machine=True)
     check_forward_against_oracle(net, m, out, d, rh, 1e-3)
 
+
+
+# CUSTOM multi-point pump curves: both pumps OPEN (H-W 0, D-W 5) or one OPEN
+# and one XHEAD (H-W 6, D-W 7)
+@pytest.mark.parametrize("headloss,seed", [("H-W", 0), ("H-W", 6), ("D-W", 5), ("D-W", 7)])
+def test_custom_pump_network_matches_oracle(cuda, headloss, seed):
+    net = feature_network(seed, headloss=headloss, n_prv=2, n_pumps=2, n_custom=2)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    d, rh = scenario_batch(net, 6, seed=11)
+    rh = np.nan_to_num(rh)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=1e-3, status_machine=True)
+    check_forward_against_oracle(net, m, out, d, rh, 1e-3)
+
+
+def test_custom_pump_segments_closed_form(cuda):
+    # one scenario per operating point: on the first point, inside every
+    # segment, on an inner point and beyond the last point
+    from netutil import curve_lift, custom_pump_tree
+    pts = [(2.0, 60.0), (6.0, 50.0), (10.0, 32.0), (14.0, 5.0)]
+    qs = np.array([2.0, 4.0, 6.0, 8.0, 12.0, 15.0])
+    net = custom_pump_tree(pts, 1.0)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    d = np.zeros((len(qs), 2))
+    d[:, 1] = qs
+    rh = np.broadcast_to(np.nan_to_num(net.reservoir_head_ft()), d.shape).copy()
+    out = s.solve(torch.tensor(d), torch.tensor(rh), accuracy=1e-10, status_machine=True)
+    H = out["head_ft"].cpu().numpy()
+    for b, q in enumerate(qs):
+        assert abs(H[b, 1] - (100.0 + curve_lift(pts, q))) < 1e-8
+    check_forward_against_oracle(net, m, out, d, rh, 1e-10)

@@ -137,6 +137,54 @@ def test_pump_lift_closed_form():
     assert abs(out["head_ft"][1] - (100.0 + lift)) < 1e-8
 
 
+CUSTOM_POINTS = [(2.0, 60.0), (6.0, 50.0), (10.0, 32.0), (14.0, 5.0)]
+
+
+@pytest.mark.parametrize("q", [1.0, 2.0, 4.0, 6.0, 8.0, 12.0, 15.0])
+def test_custom_pump_lift_closed_form(q):
+    # on the first point, inside each segment, on an inner point, beyond the
+    # last; below the first point the extended segment lifts above the
+    # shutoff head Hmax = first head, so pumpstatus closes it (XHEAD)
+    from netutil import curve_lift, custom_pump_tree
+    net = custom_pump_tree(CUSTOM_POINTS, q)
+    m = gga.GGAModel(net)
+    out = gga.solve(m, net.demand_cfs, res_heads(net), accuracy=1e-10, status_machine=True)
+    if q < CUSTOM_POINTS[0][0]:
+        assert out["status"][0] == 0
+        return
+    assert out["status"][0] == 3
+    assert abs(out["flow_cfs"][0] - q) < 1e-9
+    assert abs(out["head_ft"][1] - (100.0 + curve_lift(CUSTOM_POINTS, q))) < 1e-8
+
+
+def test_custom_pump_affinity_law_and_si_units():
+    # speed w: lift(q) = w^2 curve(q / w); SI points (L/s, m) are read in
+    # user units and converted with 28.317 L/s per cfs and 0.3048 m per ft
+    from netutil import curve_lift, custom_pump_tree
+    w, q = 0.8, 6.0
+    net = custom_pump_tree(CUSTOM_POINTS, q, speed=w)
+    out = gga.solve(gga.GGAModel(net), net.demand_cfs, res_heads(net), accuracy=1e-10,
+                    status_machine=True)
+    assert abs(out["head_ft"][1] - (100.0 + w * w * curve_lift(CUSTOM_POINTS, q / w))) < 1e-8
+    pts = [(40.0, 30.0), (120.0, 24.0), (200.0, 10.0)]
+    q_cfs = 100.0 / 28.317
+    net = custom_pump_tree(pts, q_cfs, flow_units="LPS")
+    out = gga.solve(gga.GGAModel(net), net.demand_cfs, res_heads(net), accuracy=1e-10,
+                    status_machine=True)
+    assert abs(out["head_ft"][1] - (100.0 + curve_lift(pts, 100.0) / 0.3048)) < 1e-8
+
+
+def test_custom_pump_curve_checks():
+    from wdsgpu_amd.network import pump_custom_curve
+    with pytest.raises(ValueError, match="227"):
+        pump_custom_curve([(1.0, 50.0), (2.0, 50.0)])
+    c = pump_custom_curve([(10.0, 50.0), (30.0, 40.0), (50.0, 20.0)], flow_units="LPS")
+    # EPANET input2.c:441-443: Q0 mid-range, Qmax last flow, Hmax first head
+    assert abs(c["q0"] * 28.317 - 30.0) < 1e-12
+    assert abs(c["qmax"] * 28.317 - 50.0) < 1e-12
+    assert abs(c["hmax"] * 0.3048 - 50.0) < 1e-12
+
+
 def test_darcy_weisbach_turbulent_closed_form():
     # Swamee-Jain friction factor computed independently of the oracle's
     # constants: f = 0.25 / log10(e/(3.7 D) + 5.74 / Re^0.9)^2, h = f L/D v^2/2g
@@ -318,12 +366,13 @@ def test_cpu_adjoint_matches_central_differences_grid():
     assert abs(grh[0].sum() - W[0].sum()) < 1e-9
 
 
-@pytest.mark.parametrize("seed", [0, 2])
-def test_cpu_adjoint_with_pumps_matches_central_differences(seed):
+@pytest.mark.parametrize("seed,n_custom", [(0, 0), (2, 0), (0, 1)])
+def test_cpu_adjoint_with_pumps_matches_central_differences(seed, n_custom):
     # feature network: a power-function pump (OPEN for seed 0, XHEAD for
-    # seed 2), an ACTIVE PRV branch, a TCV, a check valve, a tank and leaks
+    # seed 2) or a CUSTOM one (OPEN, inside a segment), an ACTIVE PRV
+    # branch, a TCV, a check valve, a tank and leaks
     from wdsgpu_amd.generators import feature_network
-    net = feature_network(seed, headloss="H-W", n_prv=1)
+    net = feature_network(seed, headloss="H-W", n_prv=1, n_custom=n_custom)
     m = gga.GGAModel(net)
     d = net.demand_cfs[None].copy()
     rh = res_heads(net)[None]

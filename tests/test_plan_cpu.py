@@ -157,6 +157,17 @@ def test_plan_errors_map_to_reference_exceptions():
                                         C=100.0)]), device="cpu")
 
 
+def test_custom_pump_curves_reach_the_plan():
+    from netutil import custom_pump_tree
+    net = custom_pump_tree([(2.0, 60.0), (6.0, 50.0), (10.0, 32.0)], 4.0)
+    GGASolver(net, device="cpu", dense_status_machine=True)
+    # the batched segment search needs strictly increasing flows
+    # (dgga/solver.py:989-996)
+    net.pump_curve_q = np.array([2.0, 2.0, 10.0])
+    with pytest.raises(NotImplementedError, match="strictly increasing"):
+        GGASolver(net, device="cpu", dense_status_machine=True)
+
+
 def test_solver_refuses_cpu_execution():
     net = grid_network(0)
     s = GGASolver(net, device="cpu", dense_status_machine=True)

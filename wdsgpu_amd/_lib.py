@@ -32,7 +32,9 @@ class NetworkDesc(C.Structure):
                 ("headloss", C.c_int32), ("hexp", C.c_double), ("qexp", C.c_double),
                 ("rqtol", C.c_double), ("viscos", C.c_double), ("htol", C.c_double),
                 ("qtol", C.c_double), ("checkfreq", C.c_int32),
-                ("maxcheck", C.c_int32), ("extra_iter", C.c_int32)]
+                ("maxcheck", C.c_int32), ("extra_iter", C.c_int32),
+                ("pump_curve_ptr", _ip), ("pump_curve_x", _dp), ("pump_curve_y", _dp),
+                ("ucf_flow", C.c_double), ("ucf_head", C.c_double)]
 
 
 class PlanStats(C.Structure):

@@ -409,11 +409,42 @@ __device__ __forceinline__ void valve_open(const DevPlan& pl, double q, double k
   }
 }
 
+// CUSTOM pump (dgga/solver.py:2288-2318, 1452-1474; hydcoeffs.c:716-733,
+// 794-831): the local line of the curve segment holding |q|/w in user
+// units; the segment's right end is the first point with x >= |q|/w, the
+// first and last segments extend beyond the curve.  N = 1: hgrad = R w,
+// hloss = H0 w^2 + hgrad q with H0 = -h0, R = -r of the segment.
+__device__ __forceinline__ void pump_custom(const DevPlan& pl, int k, double q, double& P,
+                                            double& Y) {
+  const double spc = fmax(pl.pu_speed[k], 1e-12);
+  const double qq = (fabs(q) / spc) * pl.ucf_flow;
+  const int a = pl.pu_cptr[k], npts = pl.pu_cptr[k + 1] - a;
+  const double* x = pl.pu_cx + a;
+  const double* y = pl.pu_cy + a;
+  int k2 = 0;
+  while (k2 < npts && x[k2] < qq) ++k2;
+  if (k2 == 0) k2 = 1;
+  else if (k2 == npts) k2 = npts - 1;
+  const int k1 = k2 - 1;
+  double r = (y[k2] - y[k1]) / (x[k2] - x[k1]);
+  double h0 = y[k1] - r * x[k1];
+  h0 = h0 / pl.ucf_head;
+  r = r * pl.ucf_flow / pl.ucf_head;
+  const double hg = -r * spc;
+  const double hl = -h0 * (spc * spc) + hg * q;
+  P = 1.0 / hg;
+  Y = hl / hg;
+}
+
 __device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double q, double& P,
                                         double& Y) {
   if (f & LF_PUMP_NOCURVE) {
     P = 1.0 / CSMALL;
     Y = q;
+    return;
+  }
+  if (f & LF_PUMP_CUSTOM) {
+    pump_custom(pl, k, q, P, Y);
     return;
   }
   const double sgn = q < 0.0 ? -1.0 : 1.0;

@@ -24,6 +24,7 @@ enum LinkFlag : int32_t {
   LF_PUMP_LIVE = 1 << 11,  // pump with speed > 0 (pumpstatus applies)
   LF_J1 = 1 << 12,         // n1 is a junction
   LF_J2 = 1 << 13,         // n2 is a junction
+  LF_PUMP_CUSTOM = 1 << 14,// multi-point pump curve
 };
 
 // LDS layout of a scenario (doubles): V [nnzL + 2] (factor, zero slot, spare),
@@ -87,6 +88,10 @@ struct DevPlan {
   const double* pu_n;         // [L]
   const double* pu_speed;     // [L]
   const double* pu_hmax;      // [L] pumpstatus cut-off head
+  const int32_t* pu_cptr;     // [L+1] CUSTOM curve points of link k: [pu_cptr[k], pu_cptr[k+1])
+  const double* pu_cx;        // curve flows (user units, strictly increasing)
+  const double* pu_cy;        // curve heads (user units)
+  double ucf_flow, ucf_head;
   // junction gather lists, reference assembly order: links with n1 == j
   // (link order) then links with n2 == j; entry = link | (side << 31)
   const int32_t* jadj_ptr;    // [Nj+1]

@@ -114,6 +114,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
          o_kv = bb.add(P->l_km_valve), o_kc = bb.add(P->l_kc), o_q0 = bb.add(P->l_q0);
   size_t o_ph = bb.add(P->pu_h0), o_pr = bb.add(P->pu_r), o_pn = bb.add(P->pu_n),
          o_ps = bb.add(P->pu_speed), o_pm = bb.add(P->pu_hmax);
+  size_t o_cp = bb.add(P->pu_cptr), o_cx = bb.add(P->pu_cx), o_cy = bb.add(P->pu_cy);
   size_t o_jp = bb.add(P->jadj_ptr), o_ja = bb.add(P->jadj);
   size_t o_fap = bb.add(P->fadj_ptr), o_fa = bb.add(P->fadj);
   size_t o_op = bb.add(P->off_pos), o_optr = bb.add(P->off_ptr), o_ol = bb.add(P->off_link);
@@ -177,6 +178,11 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.pu_n = at<double>(d, o_pn);
   D.pu_speed = at<double>(d, o_ps);
   D.pu_hmax = at<double>(d, o_pm);
+  D.pu_cptr = at<int32_t>(d, o_cp);
+  D.pu_cx = at<double>(d, o_cx);
+  D.pu_cy = at<double>(d, o_cy);
+  D.ucf_flow = P->ucf_flow;
+  D.ucf_head = P->ucf_head;
   D.jadj_ptr = at<int32_t>(d, o_jp);
   D.jadj = at<int32_t>(d, o_ja);
   D.fadj_ptr = at<int32_t>(d, o_fap);

@@ -81,6 +81,11 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   P->pu_n.assign(L, 1.0);
   P->pu_speed.assign(L, 0.0);
   P->pu_hmax.assign(L, 0.0);
+  P->pu_cptr.assign(L + 1, 0);
+  P->pu_cx.clear();
+  P->pu_cy.clear();
+  P->ucf_flow = d->ucf_flow;
+  P->ucf_head = d->ucf_head;
   for (int k = 0; k < L; ++k) {
     const int n1 = d->link_n1[k], n2 = d->link_n2[k], t = d->link_type[k];
     P->l_s1[k] = P->node_slot[n1];
@@ -101,8 +106,26 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       const double sp = d->setting[k];
       if (kind == 0) f |= LF_PUMP_CHP;
       else if (kind == 1) f |= LF_PUMP_PF;
+      else if (kind == 2) f |= LF_PUMP_CUSTOM;
       else if (kind == 3) f |= LF_PUMP_NOCURVE;
       else throw std::domain_error("pump curve kind " + std::to_string(kind) + " not supported");
+      if (kind == 2) {
+        // the segment search of the batched path equals curvecoeff's linear
+        // scan only for strictly increasing flows (dgga/solver.py:989-996)
+        if (!d->pump_curve_ptr || !d->pump_curve_x || !d->pump_curve_y)
+          throw std::invalid_argument("CUSTOM pump without curve points");
+        const int a = d->pump_curve_ptr[k], b = d->pump_curve_ptr[k + 1];
+        if (b - a < 2) throw std::invalid_argument("CUSTOM pump curve with fewer than 2 points");
+        if (!(d->ucf_flow > 0.0) || !(d->ucf_head > 0.0))
+          throw std::invalid_argument("CUSTOM pump curves need ucf_flow and ucf_head > 0");
+        for (int i = a + 1; i < b; ++i)
+          if (!(d->pump_curve_x[i] > d->pump_curve_x[i - 1]))
+            throw std::domain_error("CUSTOM pump curve flows are not strictly increasing");
+        for (int i = a; i < b; ++i) {
+          P->pu_cx.push_back(d->pump_curve_x[i]);
+          P->pu_cy.push_back(d->pump_curve_y[i]);
+        }
+      }
       if (kind == 1 && d->pump_n[k] == 1.0) f |= LF_PUMP_LIN;
       if (sp == 0.0) f |= LF_ZERO_SPEED;
       if (sp > 0.0) f |= LF_PUMP_LIVE;
@@ -112,6 +135,7 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       P->pu_speed[k] = sp;
       P->pu_hmax[k] = d->pump_hmax[k];
     }
+    P->pu_cptr[k + 1] = (int32_t)P->pu_cx.size();
     if (junc_row[n1] >= 0) f |= LF_J1;
     if (junc_row[n2] >= 0) f |= LF_J2;
     P->l_flags[k] = f;

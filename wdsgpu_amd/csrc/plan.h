@@ -17,7 +17,9 @@ struct wds_plan {
   std::vector<int32_t> node_slot, slot_node, l_s1, l_s2, l_flags;
   std::vector<int8_t> l_st0;
   std::vector<double> el_slot, l_diam, l_km, l_km_tcv, l_km_valve, l_kc, l_q0,
-      pu_h0, pu_r, pu_n, pu_speed, pu_hmax;
+      pu_h0, pu_r, pu_n, pu_speed, pu_hmax, pu_cx, pu_cy;
+  std::vector<int32_t> pu_cptr;
+  double ucf_flow = 1.0, ucf_head = 1.0;
   std::vector<int32_t> jadj_ptr, jadj, fadj_ptr, fadj, off_pos, off_ptr, off_link, fill_pos,
       spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, tl_k, tl_tank, tl_other,
       tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;

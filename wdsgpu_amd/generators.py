@@ -25,8 +25,8 @@ from __future__ import annotations
 import numpy as np
 
 from .network import (Net, JUNCTION, RESERVOIR, TANK, PIPE, CVPIPE, PUMP, PRV, TCV, M_PER_FT,
-                      LPS_PER_CFS, PUMP_POWER_FUNC, hw_resistance, dw_resistance, emitter_ke,
-                      pump_power_curve)
+                      LPS_PER_CFS, PUMP_POWER_FUNC, PUMP_CUSTOM, hw_resistance, dw_resistance,
+                      emitter_ke, pump_power_curve, pump_custom_curve)
 
 DIAMS_MM = np.array([100.0, 150.0, 200.0, 250.0, 300.0, 400.0])
 
@@ -90,7 +90,7 @@ def grid_network(seed=0, nx=10, ny=10, n_chords=20):
 
 
 def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tcv=1,
-                    n_cv=1, tank=True, n_leaks=2, tank_level_m=18.0):
+                    n_cv=1, tank=True, n_leaks=2, tank_level_m=18.0, n_custom=0):
     """Grid of nx*ny junctions (elevation U[0,30] m, demand U[0.2,1.5] L/s,
     pipes 100-600 m) with: ``n_pumps`` pumps lifting from low reservoirs
     (head 5 m) into corner junctions, each with a 3-point curve (0, h0),
@@ -99,7 +99,9 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
     (bottom 40 m, level ``tank_level_m`` in [2, 25] m) on an edge junction; ``n_tcv``
     grid pipes turned into TCVs (loss coefficient U[2,10]), ``n_cv`` into
     check-valve pipes; ``n_prv`` PRVs (setting U[25,35] m) each feeding a
-    two-junction branch of its own; ``n_leaks`` leak emitters.
+    two-junction branch of its own; ``n_leaks`` leak emitters.  The first
+    ``n_custom`` pumps get a 4-point CUSTOM curve instead, (0.25 qd, 0.97 h0),
+    (qd, 0.75 h0), (1.6 qd, 0.42 h0), (2.2 qd, 0.08 h0).
     headloss "H-W" (C U[90,140]) or "D-W" (roughness U[0.01,0.5] mm)."""
     g = np.random.default_rng(seed)
     nj = nx * ny
@@ -136,7 +138,12 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
         r = add_node(RESERVOIR, 5.0)
         h0 = g.uniform(40.0, 80.0)
         qd = g.uniform(20.0, 200.0)
-        pumps.append((len(links), pump_power_curve(h0, 0.75 * h0, qd, 0.0, 2.0 * qd)))
+        if k < n_custom:
+            cu = pump_custom_curve([(0.25 * qd, 0.97 * h0), (qd, 0.75 * h0),
+                                    (1.6 * qd, 0.42 * h0), (2.2 * qd, 0.08 * h0)])
+            pumps.append((len(links), cu))
+        else:
+            pumps.append((len(links), pump_power_curve(h0, 0.75 * h0, qd, 0.0, 2.0 * qd)))
         links.append(dict(n1=r, n2=corners[k % 3], type=PUMP, diam_mm=0.0))
     r2 = add_node(RESERVOIR, 70.0)
     main(r2, jid(nx - 1, ny - 1))
@@ -193,13 +200,18 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
     kw = {}
     if pumps:
         kw.update(pump_link=np.array([k for k, _ in pumps], dtype=np.int32),
-                  pump_ptype=np.full(len(pumps), PUMP_POWER_FUNC, dtype=np.int8),
+                  pump_ptype=np.array([PUMP_CUSTOM if "x" in c else PUMP_POWER_FUNC
+                                       for _, c in pumps], dtype=np.int8),
                   pump_h0=np.array([c["h0"] for _, c in pumps]),
                   pump_r=np.array([c["r"] for _, c in pumps]),
                   pump_n=np.array([c["n"] for _, c in pumps]),
                   pump_q0=np.array([c["q0"] for _, c in pumps]),
                   pump_qmax=np.array([c["qmax"] for _, c in pumps]),
-                  pump_hmax=np.array([c["hmax"] for _, c in pumps]))
+                  pump_hmax=np.array([c["hmax"] for _, c in pumps]),
+                  pump_curve_q=np.concatenate([c.get("x", np.zeros(0)) for _, c in pumps]),
+                  pump_curve_h=np.concatenate([c.get("y", np.zeros(0)) for _, c in pumps]),
+                  pump_curve_ptr=np.concatenate(
+                      [[0], np.cumsum([len(c.get("x", ())) for _, c in pumps])]).astype(np.int32))
     if tank_node is not None:
         kw.update(tank_node=np.array([tank_node], dtype=np.int32),
                   tank_h0=np.array([(40.0 + tank_level_m) / M_PER_FT]),

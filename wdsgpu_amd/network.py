@@ -94,6 +94,11 @@ class Net:
     pump_q0: np.ndarray = None
     pump_qmax: np.ndarray = None
     pump_hmax: np.ndarray = None
+    # CUSTOM pump curves: points of pump j in [ptr[j], ptr[j+1]), user units
+    # (flow, head) as written in the input file; empty for other pumps
+    pump_curve_q: np.ndarray = None
+    pump_curve_h: np.ndarray = None
+    pump_curve_ptr: np.ndarray = None
     # extended period (dgga/parse.py Net: patterns, controls, tank volumes)
     patterns: list = None          # list of float64 multiplier arrays
     dem_pat: np.ndarray = None     # int32[N] pattern of each node's demand, -1 none
@@ -118,6 +123,11 @@ class Net:
         for k, dt in empty.items():
             if getattr(self, k) is None:
                 setattr(self, k, np.zeros(0, dtype=dt))
+        if self.pump_curve_q is None:
+            self.pump_curve_q = np.zeros(0)
+            self.pump_curve_h = np.zeros(0)
+        if self.pump_curve_ptr is None:
+            self.pump_curve_ptr = np.zeros(len(self.pump_link) + 1, dtype=np.int32)
         if self.patterns is None:
             self.patterns = []
         if self.dem_pat is None:
@@ -220,6 +230,27 @@ def pump_power_curve(h0, h1, q1, h2, q2, flow_units="LPS", si=True):
     hcf = M_PER_FT if si else 1.0
     return dict(h0=-a / hcf, r=-b * (qcf ** c / hcf), n=c, q0=q1 / qcf,
                 qmax=((-a / b) ** (1.0 / c)) / qcf, hmax=h0 / hcf)
+
+
+def pump_custom_curve(points, flow_units="LPS", si=True):
+    """Multi-point (CUSTOM) pump curve, dgga/parse.py:797-820 (EPANET
+    input2.c:433-444 and input1.c:644-647): heads must fall strictly from
+    point to point (EPANET error 227).  The points stay in user units; the
+    solve converts them on every call (curvecoeff, hydcoeffs.c:794-831).
+    Returns dict(h0=0, r=0, n=1, q0, qmax, hmax, x, y) in internal units
+    except x, y."""
+    pts = [(float(q), float(h)) for q, h in points]
+    if len(pts) < 2:
+        raise ValueError("a CUSTOM pump curve needs at least two points")
+    for m in range(1, len(pts)):
+        if pts[m][1] >= pts[m - 1][1]:
+            raise ValueError("CUSTOM pump curve heads must decrease (EPANET error 227)")
+    qcf = FLOW_UCF[flow_units]
+    hcf = M_PER_FT if si else 1.0
+    qmax = pts[-1][0]
+    q0 = (pts[0][0] + qmax) / 2.0
+    return dict(h0=0.0, r=0.0, n=1.0, q0=q0 / qcf, qmax=qmax / qcf, hmax=pts[0][1] / hcf,
+                x=np.array([p[0] for p in pts]), y=np.array([p[1] for p in pts]))
 
 
 def emitter_ke(c_user, flow_units="LPS", qexp=2.0, ucf_pressure=M_PER_FT):

@@ -25,7 +25,8 @@ import torch
 from . import _lib
 from .network import (CSMALL, CBIG, BIG, TINY, MISSING, QZERO, HEXP_HW, PI_EPANET,
                       CVPIPE, PIPE, PUMP, PRV, PSV, PBV, FCV, TCV, JUNCTION,
-                      ST_CLOSED, ST_OPEN, ST_ACTIVE, PUMP_CONST_HP, PUMP_CUSTOM)
+                      ST_CLOSED, ST_OPEN, ST_ACTIVE, PUMP_CONST_HP, PUMP_CUSTOM, M_PER_FT,
+                      SI_FLOW_UNITS)
 
 
 def _i32(a):
@@ -113,8 +114,6 @@ class GGASolver:
         if self.n_pumps:
             pl = np.asarray(net.pump_link, dtype=np.int64)
             kinds = np.asarray(net.pump_ptype, dtype=np.int64)
-            if (kinds == PUMP_CUSTOM).any():
-                raise NotImplementedError("CUSTOM multi-point pump curves")
             pump_kind[pl] = kinds
             ph0[pl] = net.pump_h0
             pr[pl] = net.pump_r
@@ -124,6 +123,22 @@ class GGASolver:
             sp = init_setting[pl]
             phm[pl] = np.where(kinds == PUMP_CONST_HP, BIG, sp * sp * np.asarray(net.pump_hmax))
             pq0[pl] = net.pump_q0
+        # CUSTOM curves (dgga/solver.py:1346-1370, 985-1011): per-link point
+        # ranges in user units, converted at solve time (curvecoeff)
+        cptr = np.zeros(self.L + 1, dtype=np.int32)
+        cx, cy = [], []
+        if self.n_pumps:
+            pptr = np.asarray(net.pump_curve_ptr, dtype=np.int64)
+            cu = {int(pl[j]): j for j in range(self.n_pumps) if kinds[j] == PUMP_CUSTOM}
+            for kk in range(self.L):
+                j = cu.get(kk)
+                if j is not None:
+                    a, b = int(pptr[j]), int(pptr[j + 1])
+                    cx.extend(np.asarray(net.pump_curve_q[a:b], dtype=np.float64).tolist())
+                    cy.extend(np.asarray(net.pump_curve_h[a:b], dtype=np.float64).tolist())
+                cptr[kk + 1] = len(cx)
+        self.ucf_flow = ucf_q
+        self.ucf_head = M_PER_FT if str(net.meta["flow_units"]).upper() in SI_FLOW_UNITS else 1.0
         closed = st == 0
         q0 = PI_EPANET * diam ** 2 / 4.0
         if self.n_pumps:
@@ -138,7 +153,8 @@ class GGASolver:
             q0=_f64(q0), setting=_f64(init_setting), pump_kind=pump_kind, ph0=_f64(ph0),
             pr=_f64(pr), pn=_f64(pn), phm=_f64(phm),
             tank_node=_i32(net.tank_node), tank_hmin=_f64(net.tank_hmin),
-            tank_hmax=_f64(net.tank_hmax), tank_ovf=_i32(net.tank_overflow))
+            tank_hmax=_f64(net.tank_hmax), tank_ovf=_i32(net.tank_overflow),
+            cptr=cptr, cx=_f64(cx), cy=_f64(cy))
         k = self._keep
         P = lambda a, t: a.ctypes.data_as(C.POINTER(t))
         desc = _lib.NetworkDesc(
@@ -159,7 +175,9 @@ class GGASolver:
             qexp=self.qexp, rqtol=self.rqtol,
             viscos=float(net.meta.get("viscosity", 1.1e-5)), htol=self.htol,
             qtol=self.qtol, checkfreq=self.checkfreq, maxcheck=self.maxcheck,
-            extra_iter=self.extra_iter)
+            extra_iter=self.extra_iter, pump_curve_ptr=P(k["cptr"], C.c_int32),
+            pump_curve_x=P(k["cx"], C.c_double), pump_curve_y=P(k["cy"], C.c_double),
+            ucf_flow=self.ucf_flow, ucf_head=self.ucf_head)
         self._lib = _lib.load()
         h = C.c_void_p()
         order = {"mmd": 0, "natural": 1}[ordering]
