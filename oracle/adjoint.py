@@ -11,8 +11,8 @@ contraction (demands, reservoir heads, emitter Ke, pipe resistance r).
 ACTIVE PRVs contribute the constraint row H[n2] - hset (prvcoeff fixed
 point).  Pumps enter through their exact head-loss derivatives
 (``_pump_coeffs_np``, dgga/autodiff.py:122-218, without the pump-parameter
-gradients); Darcy-Weisbach is not on this oracle (the reference's GPU
-adjoint refuses it).
+gradients); Darcy-Weisbach pipes through the exact DWpipecoeff derivative
+and dhloss/dr = f |Q| Q, or 16 pi s Q when laminar (autodiff.py:249-296).
 """
 from __future__ import annotations
 
@@ -27,21 +27,28 @@ from . import gga
 
 def link_coeffs(m, q, r_hw, status):
     """(hloss, hgrad, dhloss/dr, act_prv) per link (dgga/autodiff.py:239-362,
-    221-236), H-W / TCV / valves / closed links."""
-    if m.headloss != "H-W":
-        raise NotImplementedError("oracle adjoint: Hazen-Williams networks")
+    221-236), H-W or D-W pipes / TCV / valves / closed links."""
     qa = np.abs(q)
     sgn = np.where(q < 0, -1.0, 1.0)
-    hg = m.hexp * r_hw * qa ** (m.hexp - 1.0)
-    lin = hg < m.rqtol
-    hg = np.where(lin, m.rqtol, hg)
-    hl = np.where(lin, hg * qa, hg * qa / m.hexp)
-    dr = np.where(lin, 0.0, qa ** m.hexp)
-    mlp = m.km_pipe > 0.0
-    hl = np.where(mlp, hl + m.km_pipe * qa * qa, hl)
-    hg = np.where(mlp, hg + 2.0 * m.km_pipe * qa, hg)
-    hl = hl * sgn
-    dr = dr * sgn
+    if m.headloss == "D-W":
+        # pipes: DWpipecoeff with its exact derivative; other links start on
+        # the RQtol line and are overwritten below (autodiff.py:255-258)
+        _, _, hl_p, hg_p, dr_p = m.dw_PY(q, r_hw)
+        pipe = m.lt <= 1
+        hl = np.where(pipe, hl_p, m.rqtol * q)
+        hg = np.where(pipe, hg_p, m.rqtol)
+        dr = np.where(pipe, dr_p, 0.0)
+    else:
+        hg = m.hexp * r_hw * qa ** (m.hexp - 1.0)
+        lin = hg < m.rqtol
+        hg = np.where(lin, m.rqtol, hg)
+        hl = np.where(lin, hg * qa, hg * qa / m.hexp)
+        dr = np.where(lin, 0.0, qa ** m.hexp)
+        mlp = m.km_pipe > 0.0
+        hl = np.where(mlp, hl + m.km_pipe * qa * qa, hl)
+        hg = np.where(mlp, hg + 2.0 * m.km_pipe * qa, hg)
+        hl = hl * sgn
+        dr = dr * sgn
     km = m.km_tcv
     hgv = 2.0 * km * qa
     linv = hgv < m.rqtol

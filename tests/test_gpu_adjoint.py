@@ -58,13 +58,16 @@ def test_adjoint_matches_cpu_reference_adjoint(planar):
     assert _rel(gr, cr.sum(0)) < 1e-10
 
 
-@pytest.mark.parametrize("seed,n_custom", [(0, 0), (2, 0), (0, 1)])
-def test_adjoint_with_pumps_matches_cpu_reference_adjoint(cuda, seed, n_custom):
-    # feature network (H-W): power-function pump OPEN (seed 0) / XHEAD (seed
-    # 2) or a CUSTOM pump OPEN inside a segment, ACTIVE PRV branches, a TCV,
-    # a check valve, a tank and leak emitters
+@pytest.mark.parametrize("headloss,seed,n_custom", [("H-W", 0, 0), ("H-W", 2, 0),
+                                                     ("H-W", 0, 1), ("D-W", 0, 0),
+                                                     ("D-W", 2, 0)])
+def test_adjoint_with_pumps_matches_cpu_reference_adjoint(cuda, headloss, seed, n_custom):
+    # feature network: power-function pump OPEN (seed 0) / XHEAD (seed 2) or
+    # a CUSTOM pump OPEN inside a segment, ACTIVE PRV branches, a TCV, a
+    # check valve, a tank and leak emitters; Darcy-Weisbach networks hold
+    # turbulent, transitional and laminar pipes
     from wdsgpu_amd.generators import feature_network
-    net = feature_network(seed, headloss="H-W", n_prv=2, n_custom=n_custom)
+    net = feature_network(seed, headloss=headloss, n_prv=2, n_custom=n_custom)
     m = gga.GGAModel(net)
     s = GGASolver(net, device=cuda, dense_status_machine=True)
     B = 3

@@ -32,10 +32,6 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         s = solver
         if s.dtype != torch.float64:
             raise NotImplementedError("adjoint='gpu' is float64 only")
-        if s.headloss_form != "H-W":
-            raise NotImplementedError(
-                "adjoint='gpu' covers Hazen-Williams networks (the Darcy-Weisbach "
-                "dhloss/dr contraction is not on this path, dgga/autodiff.py:795-799)")
         if s.n_pcv:
             j2 = np.asarray(s.prv_n2)
             j1 = np.asarray(s.prv_n1)

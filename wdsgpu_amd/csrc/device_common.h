@@ -356,19 +356,12 @@ __device__ __forceinline__ void hw_pipe(const DevPlan& pl, double q, double r, d
   hgrad = hg;
 }
 
-__device__ __forceinline__ void dw_pipe(const DevPlan& pl, double q, double r, double km,
-                                        double diam, double kc, double& hloss, double& hgrad) {
-  const double qa = fabs(q);
-  const double s = pl.viscos * diam;
-  const double e = kc / diam;
-  if (qa <= DW_A2 * s) {
-    const double r16 = 16.0 * PI_EPANET * s;
-    hloss = q * (r16 * r + km * qa);
-    hgrad = r16 * r + 2.0 * km * qa;
-    return;
-  }
+// Friction factor f and df/d|q| of a turbulent or transitional D-W pipe
+// (w = |q| / s >= A2): Swamee-Jain above A1, Dunlop's cubic below
+// (DWpipecoeff, hydcoeffs.c:640-668).
+__device__ __forceinline__ void dw_friction(double qa, double s, double e, double& f,
+                                            double& dfdq) {
   const double w = qa / s;
-  double f, dfdq;
   if (w >= DW_A1) {
     const double y1 = DW_A8 / pow(w, 0.9);
     const double y2 = e / 3.7 + y1;
@@ -388,9 +381,35 @@ __device__ __forceinline__ void dw_pipe(const DevPlan& pl, double q, double r, d
     f = x1 + r2 * (x2 + r2 * (x3 + r2 * x4));
     dfdq = (x2 + r2 * (2.0 * x3 + r2 * 3.0 * x4)) / s / DW_A2;
   }
+}
+
+__device__ __forceinline__ void dw_pipe(const DevPlan& pl, double q, double r, double km,
+                                        double diam, double kc, double& hloss, double& hgrad) {
+  const double qa = fabs(q);
+  const double s = pl.viscos * diam;
+  const double e = kc / diam;
+  if (qa <= DW_A2 * s) {
+    const double r16 = 16.0 * PI_EPANET * s;
+    hloss = q * (r16 * r + km * qa);
+    hgrad = r16 * r + 2.0 * km * qa;
+    return;
+  }
+  double f, dfdq;
+  dw_friction(qa, s, e, f, dfdq);
   const double r1 = f * r + km;
   hloss = r1 * qa * q;
   hgrad = (2.0 * r1 * qa) + (dfdq * r * qa * qa);
+}
+
+// d hloss / d r of an open D-W pipe (dgga/autodiff.py:249-296): laminar
+// 16 pi s Q; otherwise f |Q| Q (f depends on Q, not on r).
+__device__ __forceinline__ double dw_dr(const DevPlan& pl, double q, double diam, double kc) {
+  const double qa = fabs(q);
+  const double s = pl.viscos * diam;
+  if (qa <= DW_A2 * s) return 16.0 * PI_EPANET * s * q;
+  double f, dfdq;
+  dw_friction(qa, s, kc / diam, f, dfdq);
+  return f * qa * q;
 }
 
 __device__ __forceinline__ void valve_open(const DevPlan& pl, double q, double km, double& P,

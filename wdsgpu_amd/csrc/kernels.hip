@@ -386,11 +386,6 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
     set_error("wds_adjoint: bad argument");
     return WDS_EINVAL;
   }
-  if (P->headloss != 0) {
-    set_error("the batched adjoint covers Hazen-Williams networks (Darcy-Weisbach "
-              "dhloss/dr is not on this path, as in dgga/autodiff.py:795-799)");
-    return WDS_EUNSUPPORTED;
-  }
   AdjParams ap = make_adj_params(a);
   const int64_t lds = P->stats.lds_bytes;
   set_lds(k_adjoint, lds);

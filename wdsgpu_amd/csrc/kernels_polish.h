@@ -556,11 +556,13 @@ __global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
     const double l2 = (f & LF_J2) ? lam[pl.l_s2[k]] : 0.0;
     const double ll = c.P[k] * (GQ(k) - (l2 - l1));
     laml[k] = ll;
-    // dr = d hloss / d r (H-W pipe, open, unclamped)
+    // dr = d hloss / d r (open pipe; H-W unclamped)
     double dr = 0.0;
     const int8_t s = c.S[k];
     const bool closed_now = (s <= ST_CLOSED || (f & LF_ZERO_SPEED)) && !(f & LF_PCV);
-    if ((f & LF_PIPE) && !closed_now) {
+    if ((f & LF_PIPE) && !closed_now && pl.headloss != 0) {
+      dr = dw_dr(pl, c.q[k], pl.l_diam[k], pl.l_kc[k]);
+    } else if ((f & LF_PIPE) && !closed_now) {
       const double q = c.q[k];
       const double sgn = q < 0.0 ? -1.0 : 1.0;
       const double qa = q * sgn;
