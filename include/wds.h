@@ -159,6 +159,18 @@ typedef struct wds_solve_args {
   const int8_t* status0;       /* [B][L] initial internal link statuses (a warm start
                                   between extended-period frames) or NULL (the
                                   network's initial statuses) */
+  /* Status-machine schedule (record_schedule, dgga/solver.py:2740-2745,
+   * 3136-3160), all NULL when not recorded.  sched_flags [B][maxtrials]:
+   * bit0 the scenario ran iteration it (index it-1), bit1 linkstatus ran on
+   * the convergence branch, bit2 on the periodic branch.  sched_events
+   * [B][sched_cap]: one int32 per status change, (it << 20) | (kind << 19)
+   * | link with kind 0 prvstatus, 1 linkstatus; sched_count [B] counts the
+   * changes (entries past sched_cap are dropped, the count keeps growing).
+   * Needs status_machine and maxtrials < 2048. */
+  uint8_t* sched_flags;
+  int32_t* sched_events;
+  int32_t* sched_count;
+  int32_t sched_cap;
 } wds_solve_args;
 
 /* Fused forward: the whole Newton loop of every scenario in one persistent

@@ -679,7 +679,8 @@ def chol_solve_refined(A, F, n_refine=2, extended=False):
 
 def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
           max_iter=None, accuracy=None, status_machine=True, assemble="dense",
-          n_refine=2, record=False, extended=False, status0=None):
+          n_refine=2, record=False, extended=False, status0=None,
+          record_schedule=False):
     """Batched dense GGA (dgga/solver.py:2703-3212, mode='dense',
     dense_status_machine=True).  demand/res_head/ke: [B,N] or [N];
     r_hw: [L] or [B,L].  Returns the reference's output dict (head_ft,
@@ -688,7 +689,9 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
     refinement residuals in extended precision (see chol_solve_refined).
     ``status0`` ([L] or [B,L] internal codes) replaces the initial statuses
     (the warm start of an extended-period frame, as run_gga's status0 in the
-    reference's EPANET-mode solve, dgga/solver.py:2855)."""
+    reference's EPANET-mode solve, dgga/solver.py:2855).  ``record_schedule``
+    adds out["schedule"]: the status machine's events in the reference's
+    layout (dgga/solver.py:3136-3160)."""
     m = model
     d = np.atleast_2d(np.asarray(demand, dtype=np.float64))
     rh = np.atleast_2d(np.asarray(res_head, dtype=np.float64))
@@ -732,6 +735,7 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
         maxtrials = max_iter + (m.extra_iter if m.extra_iter > 0 else 0)
         nextcheck = np.full(B, m.checkfreq, dtype=np.int64)
     trace = dict(status=[], active=[], relerr=[]) if record else None
+    sched = [] if (record_schedule and status_machine) else None
     csr = assemble == "csr"
     for it in range(1, maxtrials + 1):
         P, Y = m.link_PY(q, S if sm else np.broadcast_to(
@@ -773,6 +777,9 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
             continue
         if m.n_pcv:
             S_v, vchg = m.prvstatus(S, H, q)
+            if sched is not None:
+                sched.append(dict(it=it, kind="valve", act=active.copy(),
+                                  chg=(S_v != S) & active[:, None]))
             S = np.where(active[:, None], S_v, S).astype(np.int8)
             valve_change = vchg & active
         else:
@@ -783,6 +790,9 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
         run_ls = br_conv | br_per
         if run_ls.any():
             S_new, chg = m.linkstatus(S, H, q)
+            if sched is not None:
+                sched.append(dict(it=it, conv=br_conv.copy(), per=br_per.copy(),
+                                  chg=(S_new != S) & run_ls[:, None]))
             S = np.where(run_ls[:, None], S_new, S).astype(np.int8)
             stat_change = chg & br_conv
         else:
@@ -811,4 +821,6 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
         out = {k: v[0] for k, v in out.items()}
     if record:
         out["trace"] = trace
+    if sched is not None:
+        out["schedule"] = sched
     return out

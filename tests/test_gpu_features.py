@@ -62,3 +62,48 @@ def test_custom_pump_segments_closed_form(cuda):
     for b, q in enumerate(qs):
         assert abs(H[b, 1] - (100.0 + curve_lift(pts, q))) < 1e-8
     check_forward_against_oracle(net, m, out, d, rh, 1e-10)
+
+
+def _same_schedule(got, ref):
+    assert len(got) == len(ref)
+    for g, r in zip(got, ref):
+        assert g.keys() == r.keys()
+        for k in r:
+            if isinstance(r[k], np.ndarray):
+                assert np.array_equal(g[k], r[k]), (r["it"], k)
+            else:
+                assert g[k] == r[k]
+
+
+@pytest.mark.parametrize("headloss,seed,cap", [("H-W", 0, None), ("H-W", 2, None),
+                                               ("D-W", 3, None), ("H-W", 1, 1)])
+def test_record_schedule_matches_oracle(cuda, headloss, seed, cap):
+    # prvstatus and linkstatus events of the batched status machine; cap 1
+    # overflows the first log and takes the re-run with a larger one
+    net = feature_network(seed, headloss=headloss, n_prv=2)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    s.sched_cap = cap
+    d, rh = scenario_batch(net, 6, seed=11)
+    rh = np.nan_to_num(rh)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=1e-3, status_machine=True, record_schedule=True)
+    ref = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-3, status_machine=True,
+                    record_schedule=True)
+    assert out["iters"].cpu().numpy().tolist() == ref["iters"].tolist()
+    assert any(e["chg"].any() for e in ref["schedule"])
+    _same_schedule(out["schedule"], ref["schedule"])
+
+
+def test_record_schedule_planar_prvs(cuda):
+    from wdsgpu_amd.generators import planar_network
+    net = planar_network(1)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    d, rh = scenario_batch(net, 4, seed=4)
+    rh = np.nan_to_num(rh)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=1e-3, status_machine=True, record_schedule=True)
+    ref = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-3, status_machine=True,
+                    record_schedule=True)
+    _same_schedule(out["schedule"], ref["schedule"])

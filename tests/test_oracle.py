@@ -444,3 +444,35 @@ def test_feature_cases_cover_the_status_machine():
             seen.add(("tank", 1))
     assert seen == {(PUMP, ST_OPEN), (PUMP, ST_XHEAD), (CVPIPE, ST_OPEN),
                     (CVPIPE, ST_CLOSED), (PRV, ST_ACTIVE), ("tank", 1)}
+
+
+@pytest.mark.parametrize("seed", [0, 2])
+def test_schedule_accounts_for_every_status_change(seed):
+    # record_schedule: at every executed iteration the links whose status
+    # moved are exactly those the prvstatus and linkstatus events mark, one
+    # prvstatus event runs per iteration, and every scenario ends on a
+    # convergence-branch linkstatus event at its final iteration
+    from wdsgpu_amd.generators import feature_network
+    net = feature_network(seed, headloss="H-W", n_prv=2)
+    m = gga.GGAModel(net)
+    d, rh = scenario_batch(net, 6, seed=11)
+    rh = np.nan_to_num(rh)
+    out = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-3, status_machine=True,
+                    record=True, record_schedule=True)
+    sched, tr = out["schedule"], out["trace"]["status"]
+    n_it = len(tr)
+    valve = [e for e in sched if e.get("kind") == "valve"]
+    assert [e["it"] for e in valve] == list(range(1, n_it + 1))
+    prev = np.broadcast_to(m.init_status_int, tr[0].shape)
+    for it in range(1, n_it + 1):
+        moved = tr[it - 1] != prev
+        marked = np.zeros_like(moved)
+        for e in sched:
+            if e["it"] == it:
+                marked |= e["chg"]
+        assert np.array_equal(moved, marked)
+        prev = tr[it - 1]
+    for b, last in enumerate(out["iters"]):
+        ls = [e for e in sched if "conv" in e and e["conv"][b]]
+        assert ls and ls[-1]["it"] == last
+        assert not any(e["it"] > last and e.get("act", e.get("conv"))[b] for e in sched)

@@ -64,7 +64,9 @@ class SolveArgs(C.Structure):
                 ("head", C.c_void_p), ("flow", C.c_void_p), ("emitter", C.c_void_p),
                 ("iters", C.c_void_p), ("relerr", C.c_void_p), ("status", C.c_void_p),
                 ("diag_ratio", C.c_void_p), ("info", C.c_void_p), ("resid", C.c_void_p),
-                ("factor", C.c_void_p), ("workspace", C.c_void_p), ("status0", C.c_void_p)]
+                ("factor", C.c_void_p), ("workspace", C.c_void_p), ("status0", C.c_void_p),
+                ("sched_flags", C.c_void_p), ("sched_events", C.c_void_p),
+                ("sched_count", C.c_void_p), ("sched_cap", C.c_int32)]
 
 
 class AdjointArgs(C.Structure):

@@ -331,6 +331,29 @@ extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const doub
   return launch_check("k_factor_solve");
 }
 
+// record_schedule buffers: all or none, status machine on, event words wide
+// enough for the iteration and link numbers
+static int check_sched(const wds_plan* P, const wds_solve_args* a) {
+  const bool any = a->sched_flags || a->sched_events || a->sched_count;
+  if (!any) return WDS_OK;
+  const int64_t maxtrials =
+      (int64_t)a->max_iter + (P->extra_iter > 0 ? P->extra_iter : 0);
+  if (!a->sched_flags || !a->sched_events || !a->sched_count || a->sched_cap < 0) {
+    set_error("schedule recording needs sched_flags, sched_events, sched_count and cap >= 0");
+    return WDS_EINVAL;
+  }
+  if (!a->status_machine) {
+    set_error("schedule recording covers the batched status machine only "
+              "(dgga/solver.py:2802-2806)");
+    return WDS_EUNSUPPORTED;
+  }
+  if (maxtrials >= 2048 || P->L >= (1 << 19)) {
+    set_error("schedule recording needs maxtrials < 2048 and L < 524288");
+    return WDS_EINVAL;
+  }
+  return WDS_OK;
+}
+
 extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
   int r = check_ready(P);
   if (r) return r;
@@ -348,6 +371,8 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
     set_error("networks with CV pipes need status_machine=1");
     return WDS_EUNSUPPORTED;
   }
+  r = check_sched(P, a);
+  if (r) return r;
   SolveParams sp = make_params(a);
   const int64_t lds = P->stats.lds_bytes;
   set_lds(k_solve, lds);
@@ -368,6 +393,8 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
     set_error("networks with PRVs or CV pipes need status_machine=1");
     return WDS_EUNSUPPORTED;
   }
+  r = check_sched(P, a);
+  if (r) return r;
   SolveParams sp = make_params(a);
   const int64_t lds = P->stats.lds_bytes;
   set_lds(k_newton_iter, lds);

@@ -40,6 +40,10 @@ struct SolveParams {
   double* factor;
   double* ws;
   const int8_t* status0;
+  uint8_t* sched_flags;
+  int32_t* sched_events;
+  int32_t* sched_count;
+  int32_t sched_cap;
 };
 
 inline SolveParams make_params(const wds_solve_args* a) {
@@ -69,6 +73,10 @@ inline SolveParams make_params(const wds_solve_args* a) {
   s.factor = a->factor;
   s.ws = (double*)a->workspace;
   s.status0 = a->status0;
+  s.sched_flags = a->sched_flags;
+  s.sched_events = a->sched_events;
+  s.sched_count = a->sched_count;
+  s.sched_cap = a->sched_cap;
   return s;
 }
 
@@ -492,8 +500,24 @@ __device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
   return qsum > hacc ? dqsum / qsum : dqsum;
 }
 
+// Status-change log of record_schedule: every lane whose link changed
+// status appends (it << 20) | (kind << 19) | link to its scenario's list;
+// slots come from a ballot prefix count, nev (uniform) counts all changes.
+struct SchedLog {
+  int32_t* ev;  // this scenario's events, or null when not recording
+  int cap, it, nev;
+};
+
+__device__ inline void sched_log(SchedLog* lg, int kind, bool chg, int link) {
+  if (!lg || !lg->ev) return;
+  const uint64_t m = __ballot(chg);
+  const int slot = lg->nev + (int)__popcll(m & __lanemask_lt());
+  if (chg && slot < lg->cap) lg->ev[slot] = (lg->it << 20) | (kind << 19) | link;
+  lg->nev += (int)__popcll(m);
+}
+
 // K6a: prvstatus for every PRV with a setting (dgga/solver.py:914-947)
-__device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c) {
+__device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog* lg = nullptr) {
   const int i = lane_id();
   bool chg = false;
   if (i < pl.p) {
@@ -519,6 +543,7 @@ __device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c) {
     chg = sn != sk;
     c.S[k] = sn;
   }
+  if (lg) sched_log(lg, 0, chg, i < pl.p ? pl.prv_k[i] : 0);
   const bool any = wave_any(chg);
   mem_sync();
   return any;
@@ -537,7 +562,7 @@ __device__ inline bool cv_closed(double htol, double qtol, bool s_is_open, doubl
 }
 
 // K6b: linkstatus (dgga/solver.py:697-739); returns "changed"
-__device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c) {
+__device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog* lg = nullptr) {
   const int lane = lane_id();
   bool chg = false;
   if (!pl.has_cv && !pl.has_pump && pl.ntl == 0) return false;
@@ -568,7 +593,16 @@ __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c) {
     if (live && ((full && cut_f) || (empty && cut_e))) c.S[k] = ST_TEMPCLOSED;
   }
   mem_sync();
-  for (int k = lane; k < pl.L; k += 64) chg |= c.S[k] != c.st0[k];
+  if (lg && lg->ev) {
+    for (int k0 = 0; k0 < pl.L; k0 += 64) {
+      const int k = k0 + lane;
+      const bool ck = k < pl.L && c.S[k] != c.st0[k];
+      sched_log(lg, 1, ck, k);
+      chg |= ck;
+    }
+  } else {
+    for (int k = lane; k < pl.L; k += 64) chg |= c.S[k] != c.st0[k];
+  }
   return wave_any(chg);
 }
 
@@ -638,8 +672,13 @@ __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const Solve
   return r;
 }
 
+__device__ inline int maxtrials_of(const DevPlan& pl, const SolveParams& sp) {
+  return sp.max_iter + ((sp.sm && pl.extra_iter > 0) ? pl.extra_iter : 0);
+}
+
 struct Ctrl {
   int active, iters, nextcheck, info;
+  int nev;  // status changes logged so far (record_schedule)
 };
 
 // status-machine bookkeeping after an iteration (dgga/solver.py:3113-3179);
@@ -647,15 +686,25 @@ struct Ctrl {
 __device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams& sp, int it,
                                     double relerr, Ctrl& st) {
   if (!sp.sm) return !(relerr > sp.hacc);
+  const int b = blockIdx.x;
+  SchedLog log{sp.sched_events ? sp.sched_events + (int64_t)b * sp.sched_cap : nullptr,
+               sp.sched_cap, it, st.nev};
+  SchedLog* lg = sp.sched_events ? &log : nullptr;
   bool vchg = false;
-  if (pl.p > 0) vchg = k6_prvstatus(pl, c);
+  if (pl.p > 0) vchg = k6_prvstatus(pl, c, lg);
   const bool conv = relerr <= sp.hacc;
   const bool br_conv = conv && it <= sp.max_iter;
   const bool br_per = !conv && it <= pl.maxcheck && st.nextcheck == it;
   bool stat_change = false;
   if (br_conv || br_per) {
-    const bool chg = k6_linkstatus(pl, c);
+    const bool chg = k6_linkstatus(pl, c, lg);
     stat_change = chg && br_conv;
+  }
+  if (lg) {
+    st.nev = log.nev;
+    if (lane_id() == 0)
+      sp.sched_flags[(int64_t)b * maxtrials_of(pl, sp) + (it - 1)] =
+          (uint8_t)(1 | (br_conv ? 2 : 0) | (br_per ? 4 : 0));
   }
   if (pl.p > 0) stat_change = stat_change || (vchg && br_conv);
   if (br_conv || br_per) {
@@ -663,10 +712,6 @@ __device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams
     if (br_per) st.nextcheck = st.nextcheck + pl.checkfreq;
   }
   return conv && ((it > sp.max_iter) || !stat_change);
-}
-
-__device__ inline int maxtrials_of(const DevPlan& pl, const SolveParams& sp) {
-  return sp.max_iter + ((sp.sm && pl.extra_iter > 0) ? pl.extra_iter : 0);
 }
 
 __device__ inline void write_outputs(const DevPlan& pl, Scn& c, const SolveParams& sp, int b) {
@@ -746,7 +791,7 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
   mem_sync();
   lds_sync();
   const int maxtrials = maxtrials_of(pl, sp);
-  Ctrl st{1, 0, pl.checkfreq, 0};
+  Ctrl st{1, 0, pl.checkfreq, 0, 0};
   double relerr_out = 0.0, dratio = 0.0;
   for (int it = 1; it <= maxtrials; ++it) {
     bool bad = false;
@@ -780,6 +825,7 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
     sp.info[b] = st.info;
     if (sp.diag_ratio) sp.diag_ratio[b] = dratio;
     if (sp.resid) sp.resid[b] = resid;
+    if (sp.sched_count) sp.sched_count[b] = st.nev;
   }
 }
 
@@ -821,7 +867,8 @@ __global__ __launch_bounds__(64) void k_newton_iter(DevPlan pl, SolveParams sp, 
   }
   mem_sync();
   lds_sync();
-  Ctrl st{cb[0], cb[1], cb[2], cb[3]};
+  Ctrl st{cb[0], cb[1], cb[2], cb[3], 0};
+  if (sp.sched_count) st.nev = it == 1 ? 0 : sp.sched_count[b];
   if (!st.active || st.info) return;
   const int maxtrials = maxtrials_of(pl, sp);
   bool bad = false;
@@ -844,6 +891,7 @@ __global__ __launch_bounds__(64) void k_newton_iter(DevPlan pl, SolveParams sp, 
     sp.iters[b] = st.iters;
     sp.info[b] = st.info;
     if (sp.diag_ratio && pl.p > 0) sp.diag_ratio[b] = dratio;
+    if (sp.sched_count) sp.sched_count[b] = st.nev;
   }
 }
 

@@ -40,6 +40,10 @@ def _f64(a):
 class GGASolver:
     """Batched GGA forward solver (dgga/solver.py:140-495 interface)."""
 
+    # record_schedule: status changes logged per scenario before a re-run
+    # with room for all of them (None: 4 L + 64)
+    sched_cap = None
+
     def __init__(self, net, device="cuda", dtype=torch.float64, mode="dense",
                  inp_path=None, dense_tank_bound_check=True,
                  dense_status_machine=False, ordering="mmd"):
@@ -408,17 +412,69 @@ class GGASolver:
         reference's dict: head_ft [B,N], flow_cfs [B,L], emitter_cfs [B,N],
         iters [B], relerr [B], converged [B], status [B,L] (status machine),
         diag_ratio [B] (networks with PRVs).  1-D inputs give 1-D outputs."""
-        if record_schedule:
-            raise NotImplementedError("record_schedule is not recorded on the fused path")
         self._validate(assemble, linear_solver, status_machine, link_status, link_setting)
+        if record_schedule and not status_machine:
+            raise NotImplementedError(
+                "record_schedule=True covers the batched status machine only "
+                "(status_machine=True; dgga/solver.py:2802-2806)")
         max_iter = self.max_iter_default if max_iter is None else int(max_iter)
         hacc = self.hacc_default if accuracy is None else float(accuracy)
         a, out, keep, single, B = self._args(demand_cfs, res_head_ft, ke_int, q0, e0,
                                              max_iter, hacc, status_machine, r_hw=r_hw,
                                              status0=link_status)
-        with torch.cuda.device(self.device):
-            _lib.check(self._lib.wds_solve(self._plan, self._stream(), C.byref(a)), "wds_solve")
-        return self._finish(out, single, max_iter, status_machine)
+        cap = self.sched_cap if self.sched_cap else 4 * self.L + 64
+        for _ in range(2):
+            sched = self._sched_buffers(a, B, self.maxtrials(max_iter, True), cap) \
+                if record_schedule else None
+            with torch.cuda.device(self.device):
+                _lib.check(self._lib.wds_solve(self._plan, self._stream(), C.byref(a)),
+                           "wds_solve")
+            if sched is None:
+                break
+            need = int(sched[2].max().item())
+            if need <= cap:
+                break
+            cap = need   # deterministic re-run with room for every change
+        res = self._finish(out, single, max_iter, status_machine)
+        if sched is not None:
+            res["schedule"] = self._schedule(sched, B)
+        return res
+
+    def _sched_buffers(self, a, B, maxtrials, cap):
+        dev = self.device
+        flags = torch.zeros(B, maxtrials, dtype=torch.uint8, device=dev)
+        events = torch.empty(B, max(cap, 1), dtype=torch.int32, device=dev)
+        count = torch.zeros(B, dtype=torch.int32, device=dev)
+        a.sched_flags, a.sched_events = _lib.ptr(flags), _lib.ptr(events)
+        a.sched_count, a.sched_cap = _lib.ptr(count), int(cap)
+        return flags, events, count
+
+    def _schedule(self, sched, B):
+        """record_schedule output in the reference's layout (dgga/solver.py:
+        3136-3160): per executed iteration, a prvstatus event {it, kind:
+        "valve", act[B], chg[B,L]} when the network has PRVs, and a
+        linkstatus event {it, conv[B], per[B], chg[B,L]} when any scenario
+        ran linkstatus; chg marks the links whose status that step changed."""
+        flags, events, count = (t.cpu().numpy() for t in sched)
+        ran = (flags & 1) != 0
+        n_it = int(np.nonzero(ran.any(0))[0].max()) + 1 if ran.any() else 0
+        chg = {}
+        for b in range(B):
+            ev = events[b, :int(count[b])].astype(np.int64)
+            for it, kind, k in zip(ev >> 20, (ev >> 19) & 1, ev & ((1 << 19) - 1)):
+                m = chg.setdefault((int(it), int(kind)), np.zeros((B, self.L), dtype=bool))
+                m[b, int(k)] = True
+        none = np.zeros((B, self.L), dtype=bool)
+        out = []
+        for it in range(1, n_it + 1):
+            f = flags[:, it - 1]
+            if self.n_pcv:
+                out.append(dict(it=it, kind="valve", act=(f & 1) != 0,
+                                chg=chg.get((it, 0), none).copy()))
+            conv, per = (f & 2) != 0, (f & 4) != 0
+            if (conv | per).any():
+                out.append(dict(it=it, conv=conv, per=per, chg=chg.get((it, 1), none).copy()))
+        return out
 
     def _finish(self, out, single, max_iter, status_machine):
         info, _ = self._host_status(out, int(out["info"].shape[0]))
