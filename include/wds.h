@@ -127,6 +127,18 @@ int wds_factor_solve(wds_plan* plan, void* stream, int32_t B, const double* csr,
                      const double* F, double* H, int32_t n_refine,
                      double* factor_out, int32_t* info, void* workspace);
 
+/* The two phases of wds_factor_solve on their own (the reference's cuDSS
+ * factorize and solve phases, dgga/solver.py:2630-2651).  wds_factor writes
+ * factor_out [B][nnzL] (the plan's internal LDL^T layout) and info [B] (0,
+ * or 1 on a non-positive pivot).  wds_trisolve solves A H = F with that
+ * factor plus n_refine refinement steps whose residuals use the CSR values
+ * csr [B][nnz_csr]; F and H [B][Nj] in original junction order. */
+int wds_factor(wds_plan* plan, void* stream, int32_t B, const double* csr,
+               double* factor_out, int32_t* info, void* workspace);
+int wds_trisolve(wds_plan* plan, void* stream, int32_t B, const double* factor,
+                 const double* csr, const double* F, double* H, int32_t n_refine,
+                 void* workspace);
+
 /* Per-scenario inputs and outputs of the forward solve. */
 typedef struct wds_solve_args {
   int32_t B;

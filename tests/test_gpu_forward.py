@@ -50,9 +50,20 @@ def test_factor_solve_matches_dense_cholesky(solvers):
         P, Y = m.link_PY(q, S, m.r_hw[None])
         csr, F, *_ = m.assemble(P, Y, q, H, e, net.node_ke[m.junc_nodes][None].repeat(B, 0),
                                 d[:, m.junc_nodes], S, csr=True)
-        Hg = s.factor_solve(torch.tensor(csr), torch.tensor(F)).cpu().numpy()
+        Hg_t, fac = s.factor_solve(torch.tensor(csr), torch.tensor(F), want_factor=True)
+        Hg = Hg_t.cpu().numpy()
         Hx = gga.chol_solve_refined(m.csr_to_dense(csr), F, extended=True)
         assert np.abs(Hg - Hx).max() <= 1e-9 * np.abs(Hx).max() * 1e-3
+        # the two phases on their own give the same bits
+        fac2 = s.factor(torch.tensor(csr))
+        assert torch.equal(fac, fac2)
+        H2 = s.trisolve(fac2, torch.tensor(csr), torch.tensor(F))
+        assert torch.equal(H2, Hg_t)
+        # one factor serves several right-hand sides
+        F3 = 0.5 * F + 0.1
+        H3 = s.trisolve(fac2, torch.tensor(csr), torch.tensor(F3)).cpu().numpy()
+        Hx3 = gga.chol_solve_refined(m.csr_to_dense(csr), F3, extended=True)
+        assert np.abs(H3 - Hx3).max() <= 1e-9 * np.abs(Hx3).max() * 1e-3
 
 
 @pytest.mark.parametrize("name,B", [("grid", 1), ("planar", 8)])
@@ -183,3 +194,30 @@ def test_singular_matrix_raises_naming_samples(cuda):
     rh = np.broadcast_to(np.nan_to_num(net.reservoir_head_ft()), (2, 3)).copy()
     with pytest.raises(RuntimeError, match=r"samples \[0, 1\]"):
         s.solve(torch.tensor(d), torch.tensor(rh), status_machine=True)
+
+
+def test_cache_api_of_the_reference(cuda):
+    # cudss_plan / cudss_cache_info / cudss_cache_max / cudss_free /
+    # cudss_session / cudss_counters (dgga/solver.py:2514-2603)
+    net = grid_network(0)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    rh = np.nan_to_num(net.reservoir_head_ft())
+    assert s.cudss_plan(3) >= 0.0
+    info = s.cudss_cache_info()
+    assert [e[0] for e in info] == [3] and info[0][5] > 0.0
+    a = s.solve(torch.tensor(np.tile(net.demand_cfs, (3, 1))), torch.tensor(np.tile(rh, (3, 1))),
+                status_machine=True)
+    assert len(s.cudss_cache_info()) == 1
+    s.cudss_cache_max = 1
+    s.solve(torch.tensor(np.tile(net.demand_cfs, (2, 1))), torch.tensor(np.tile(rh, (2, 1))),
+            status_machine=True)
+    assert [e[0] for e in s.cudss_cache_info()] == [2]
+    with s.cudss_session():
+        b = s.solve(torch.tensor(np.tile(net.demand_cfs, (3, 1))),
+                    torch.tensor(np.tile(rh, (3, 1))), status_machine=True)
+    assert s.cudss_cache_info() == []
+    assert torch.equal(a["head_ft"], b["head_ft"])
+    assert s.cudss_counters() == s.counters()
+    s.cudss_cache_max = 0
+    with pytest.raises(ValueError):
+        s.cudss_plan(4)

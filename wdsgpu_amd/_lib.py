@@ -92,6 +92,10 @@ SIGNATURES = {
     "wds_factor_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                    C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p,
                                    C.c_void_p, C.c_void_p]),
+    "wds_factor": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                             C.c_void_p, C.c_void_p]),
+    "wds_trisolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                               C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "wds_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
     "wds_newton_iter": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs),
                                   C.c_int32, C.c_void_p]),

@@ -128,7 +128,7 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             flow=_lib.ptr(out["flow"]), emitter=_lib.ptr(out["emitter"]),
             status=_lib.ptr(out["status"]), factor=_lib.ptr(out["factor"]),
             gH=_lib.ptr(gHt), gQ=_lib.ptr(gQt), gE=_lib.ptr(gEt),
-            n_refine=int(s.dense_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
+            n_refine=int(s.cudss_grad_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
             g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr), info=_lib.ptr(info), workspace=_lib.ptr(ws))
         with torch.cuda.device(dev):
             _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),

@@ -331,6 +331,41 @@ extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const doub
   return launch_check("k_factor_solve");
 }
 
+extern "C" int wds_factor(wds_plan* P, void* stream, int32_t B, const double* csr,
+                          double* factor_out, int32_t* info, void* workspace) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (B <= 0 || !csr || !factor_out || !info || !workspace) {
+    set_error("wds_factor: bad argument");
+    return WDS_EINVAL;
+  }
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_factor, lds);
+  hipLaunchKernelGGL(k_factor, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
+                     factor_out, info, (double*)workspace);
+  P->cnt.factorize += 1;
+  P->cnt.launches += 1;
+  return launch_check("k_factor");
+}
+
+extern "C" int wds_trisolve(wds_plan* P, void* stream, int32_t B, const double* factor,
+                            const double* csr, const double* F, double* H, int32_t n_refine,
+                            void* workspace) {
+  int r = check_ready(P);
+  if (r) return r;
+  if (B <= 0 || !factor || !csr || !F || !H || !workspace || n_refine < 0) {
+    set_error("wds_trisolve: bad argument");
+    return WDS_EINVAL;
+  }
+  const int64_t lds = P->stats.lds_bytes;
+  set_lds(k_trisolve, lds);
+  hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, factor,
+                     csr, F, H, n_refine, (double*)workspace);
+  P->cnt.solve += 1 + n_refine;
+  P->cnt.launches += 1;
+  return launch_check("k_trisolve");
+}
+
 // record_schedule buffers: all or none, status machine on, event words wide
 // enough for the iteration and link numbers
 static int check_sched(const wds_plan* P, const wds_solve_args* a) {

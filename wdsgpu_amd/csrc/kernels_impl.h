@@ -769,6 +769,62 @@ __global__ __launch_bounds__(64) void k_factor_solve(DevPlan pl, const double* c
   if (lane == 0) info[b] = 0;
 }
 
+// CSR values of scenario b into the factor slots (V) and the matrix copy
+// used by the refinement residual (A); fill-in positions start at zero
+__device__ inline void load_csr(const DevPlan& pl, Scn& c, const double* csr, int b,
+                                bool to_v) {
+  const int lane = lane_id();
+  const double* a = csr + (int64_t)b * pl.nnz_csr;
+  for (int f = lane; f < pl.nfill; f += 64) {
+    if (to_v) c.V[pl.fill_pos[f]] = 0.0;
+    c.A[pl.fill_pos[f]] = 0.0;
+  }
+  for (int e = lane; e < pl.nnz_csr; e += 64) {
+    const int t = pl.csr_to_L[e];
+    if (t >= 0) {
+      if (to_v) c.V[t] = a[e];
+      c.A[t] = a[e];
+    }
+  }
+}
+
+// Numeric factorisation only (the reference's cuDSS factorize phase,
+// dgga/solver.py:2630-2640): factor_out [B][nnzL], info [B].
+__global__ __launch_bounds__(64) void k_factor(DevPlan pl, const double* csr,
+                                               double* factor_out, int32_t* info, double* ws) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
+  load_csr(pl, c, csr, b, true);
+  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  mem_sync();
+  lds_sync();
+  const bool bad = k3_factor(pl, c);
+  for (int t = lane; t < pl.nnzL; t += 64) factor_out[(int64_t)b * pl.nnzL + t] = c.V[t];
+  if (lane == 0) info[b] = bad ? 1 : 0;
+}
+
+// Triangular solves with refinement against a stored factor (the cuDSS
+// solve phase and the refinement loop, dgga/solver.py:2641-2651,
+// 3052-3064): H [B][Nj] from F [B][Nj], both in original junction order.
+__global__ __launch_bounds__(64) void k_trisolve(DevPlan pl, const double* factor,
+                                                 const double* csr, const double* F, double* H,
+                                                 int n_refine, double* ws) {
+  extern __shared__ double lds[];
+  const int b = blockIdx.x, lane = lane_id();
+  Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
+  load_csr(pl, c, csr, b, false);
+  for (int t = lane; t < pl.nnzL; t += 64) c.V[t] = factor[(int64_t)b * pl.nnzL + t];
+  const double* Fb = F + (int64_t)b * pl.Nj;
+  for (int r = lane; r < pl.Nj; r += 64) c.F[pl.row_slot[r]] = Fb[r];
+  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  mem_sync();
+  lds_sync();
+  k4_solve(pl, c, c.F, c.Hn, n_refine);
+  double* Hb = H + (int64_t)b * pl.Nj;
+  for (int r = lane; r < pl.Nj; r += 64) Hb[r] = c.Hn[pl.row_slot[r]];
+}
+
 // Fused forward: the whole Newton loop of scenario blockIdx.x.
 __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
   extern __shared__ double lds[];

@@ -17,7 +17,10 @@ Differences from the reference, all refusals rather than approximations:
 """
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
+import time
+from collections import OrderedDict
 
 import numpy as np
 import torch
@@ -59,6 +62,15 @@ class GGASolver:
         self.dtype = dtype
         self.dense_status_machine = bool(dense_status_machine)
         self.dense_refine = 2
+        # the reference's linear-solver cache knobs (dgga/solver.py:378-423):
+        # workspaces are cached per batch size in an LRU of cudss_cache_max;
+        # the adjoint's refinement steps are cudss_grad_refine; factors live
+        # in each autograd context, so every forward keeps its own slot and
+        # cudss_grad_slots is accepted without a limit to enforce
+        self.cudss_cache_max = 8
+        self.cudss_grad_refine = 2
+        self.cudss_grad_slots = 1
+        self._ws_cache = OrderedDict()
         lt = np.asarray(net.link_type, dtype=np.int64)
         nt = np.asarray(net.node_type, dtype=np.int64)
         self.N, self.L = net.N, net.L
@@ -233,16 +245,88 @@ class GGASolver:
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
 
     def _workspace(self, B):
-        nbytes = int(_lib.check(self._lib.wds_workspace_bytes(self._plan, int(B)),
-                                "wds_workspace_bytes"))
-        return torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
+        key = (int(B), self.dtype, self.device, 0)
+        ent = self._ws_cache.get(key)
+        if ent is None:
+            t0 = time.perf_counter()
+            nbytes = int(_lib.check(self._lib.wds_workspace_bytes(self._plan, int(B)),
+                                    "wds_workspace_bytes"))
+            ws = torch.empty(nbytes // 8, dtype=torch.float64, device=self.device)
+            ent = dict(ws=ws, plan_ms=1e3 * (time.perf_counter() - t0),
+                       dev_mib=nbytes / 2.0 ** 20)
+            self._ws_cache[key] = ent
+            self._trim_cache()
+        else:
+            self._ws_cache.move_to_end(key)
+        return ent["ws"]
+
+    def _trim_cache(self):
+        cap = self.cudss_cache_max
+        if cap is not None and (not isinstance(cap, int) or cap < 1):
+            raise ValueError("cudss_cache_max must be >= 1 or None (no limit), got %r" % (cap,))
+        n = 0
+        while cap is not None and len(self._ws_cache) > cap:
+            self._ws_cache.popitem(last=False)
+            n += 1
+        return n
+
+    @property
+    def cudss_refine(self):
+        """Refinement steps of every forward linear solve (the reference's
+        cudss_refine, dgga/solver.py:384)."""
+        return self.dense_refine
+
+    @cudss_refine.setter
+    def cudss_refine(self, v):
+        self.dense_refine = int(v)
+
+    def cudss_free(self, empty_cache=False):
+        """Drop every cached workspace; returns how many
+        (dgga/solver.py:2529-2562).  Memory still held by tensors a caller
+        keeps (outputs, autograd factors) is theirs."""
+        n = len(self._ws_cache)
+        self._ws_cache.clear()
+        if empty_cache and torch.cuda.is_available():
+            torch.cuda.empty_cache()
+        return n
+
+    @contextlib.contextmanager
+    def cudss_session(self, empty_cache=False):
+        """Context manager that frees the cache on exit, exceptions included
+        (dgga/solver.py:2564-2572)."""
+        try:
+            yield self
+        finally:
+            self.cudss_free(empty_cache=empty_cache)
+
+    def cudss_cache_info(self):
+        """Cached workspaces, least recently used first: (B, dtype, device,
+        matrix type, allocation ms, MiB, slot) (dgga/solver.py:2574-2580)."""
+        return [(k[0], k[1], k[2], "SPD", v["plan_ms"], v["dev_mib"], k[3])
+                for k, v in self._ws_cache.items()]
+
+    def cudss_plan(self, B, dtype=None, device=None):
+        """Warm the plan upload and the workspace of batch size B; returns the
+        milliseconds spent (dgga/solver.py:2595-2603)."""
+        if dtype not in (None, self.dtype) or device not in (None, self.device):
+            raise NotImplementedError("the plan serves the solver's own dtype and device")
+        t0 = time.perf_counter()
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_plan_upload(self._plan), "wds_plan_upload")
+            self._workspace(int(B))
+        return 1e3 * (time.perf_counter() - t0)
 
     def counters(self, reset=False):
-        """cudss_counters equivalent (dgga/solver.py:2580-2593)."""
+        """Linear-algebra call counters (dgga/solver.py:2580-2593): factorize,
+        solve, bwd_solve, bwd_reuse, bwd_refactorize, launches."""
         c = _lib.Counters()
         _lib.check(self._lib.wds_counters_get(self._plan, C.byref(c), int(bool(reset))),
                    "wds_counters_get")
         return {f: int(getattr(c, f)) for f, _ in _lib.Counters._fields_}
+
+    def cudss_counters(self, reset=False):
+        """The reference's name for ``counters``."""
+        return self.counters(reset)
 
     def plan_export(self, what):
         n = int(self._lib.wds_plan_export(self._plan, int(what), None, 0))
@@ -285,6 +369,46 @@ class GGASolver:
         if bad:
             raise RuntimeError("batched LDL^T failed: samples %s are not positive definite" % bad)
         return (H, fac) if want_factor else H
+
+    def factor(self, csr_data):
+        """Numeric LDL^T of every scenario's CSR matrix [B,nnz] (the cuDSS
+        factorize phase, dgga/solver.py:2630-2640); returns the factor
+        [B,nnzL] in the plan's internal layout, for ``trisolve``."""
+        csr = self._prep(csr_data).contiguous()
+        B = int(csr.shape[0])
+        if csr.dim() != 2 or csr.shape[1] != int(self.stats["nnz_csr"]):
+            raise ValueError("factor: csr must be [B,%d]" % int(self.stats["nnz_csr"]))
+        fac = torch.empty(B, int(self.stats["nnzL"]), dtype=self.dtype, device=self.device)
+        info = torch.zeros(B, dtype=torch.int32, device=self.device)
+        ws = self._workspace(B)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_factor(self._plan, self._stream(), B, _lib.ptr(csr),
+                                            _lib.ptr(fac), _lib.ptr(info), _lib.ptr(ws)),
+                       "wds_factor")
+        bad = torch.nonzero(info != 0).flatten().tolist()
+        if bad:
+            raise RuntimeError("batched LDL^T failed: samples %s are not positive definite" % bad)
+        return fac
+
+    def trisolve(self, fac, csr_data, F, n_refine=2):
+        """H [B,Nj] with A H = F from a ``factor`` result plus n_refine
+        refinement steps on the CSR values (dgga/solver.py:2641-2651,
+        3052-3064)."""
+        csr = self._prep(csr_data).contiguous()
+        Fv = self._prep(F).contiguous()
+        B = int(Fv.shape[0])
+        if csr.shape != (B, int(self.stats["nnz_csr"])) or Fv.shape != (B, self.Nj) or \
+                tuple(fac.shape) != (B, int(self.stats["nnzL"])) or fac.device != self.device:
+            raise ValueError("trisolve: shapes %s %s %s" % (tuple(fac.shape), tuple(csr.shape),
+                                                           tuple(Fv.shape)))
+        fac = fac.to(self.dtype).contiguous()
+        H = torch.empty_like(Fv)
+        ws = self._workspace(B)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_trisolve(self._plan, self._stream(), B, _lib.ptr(fac),
+                                              _lib.ptr(csr), _lib.ptr(Fv), _lib.ptr(H),
+                                              int(n_refine), _lib.ptr(ws)), "wds_trisolve")
+        return H
 
     # ------------------------------------------------------------------
     def _host_status(self, out, B):
