@@ -96,6 +96,20 @@ typedef __attribute__((address_space(3))) double lds_f64;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) i32x4 glb_i32x4;
 
+// The ring's first kRing chunks, issued oldest first.  The scheduling
+// barriers keep that order: the wait counter is in-order, so if the
+// scheduler issued the first chunk's loads last, the loop header would have
+// to wait for all but the newest few loads and the prefetch would collapse
+// to a single chunk at every pass through the ring.
+__device__ __forceinline__ void fill_ring(i32x4 (&ring)[kSlots][kUnroll], const glb_i32x4* ip) {
+#pragma unroll
+  for (int d = 0; d < kRing; ++d) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
 // Numeric LDL^T factorisation in place in V: unscaled columns w (w_ij =
 // L_ij D_j) and reciprocal pivots on the diagonal; update
 // w_ij -= w_ik * w_jk * invD_k.
@@ -110,10 +124,7 @@ __device__ inline void run_factor(const ProgDev& pg, double* Vg, int& bad,
   lds_f64* const dummy = V + pg.dummy + lane;
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
   i32x4 ring[kSlots][kUnroll];
-#pragma unroll
-  for (int d = 0; d < kRing; ++d)
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
+  fill_ring(ring, ip);
   double acc = 0.0;
   // wide levels: the task whose target this lane's group finalises
   int fin_t = 0;
@@ -216,10 +227,7 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
   const int sent = pg.dummy + 8 * lane;  // byte address of this lane's sink slot
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
   i32x4 ring[kSlots][kUnroll];
-#pragma unroll
-  for (int d = 0; d < kRing; ++d)
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
+  fill_ring(ring, ip);
   double acc = 0.0;
   int fin = sent;  // wide levels: byte address of the group's target
   bool lvl_start = true;

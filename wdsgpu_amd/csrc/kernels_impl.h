@@ -508,16 +508,15 @@ struct SchedLog {
   int cap, it, nev;
 };
 
-__device__ inline void sched_log(SchedLog* lg, int kind, bool chg, int link) {
-  if (!lg || !lg->ev) return;
+__device__ inline void sched_log(SchedLog& lg, int kind, bool chg, int link) {
   const uint64_t m = __ballot(chg);
-  const int slot = lg->nev + (int)__popcll(m & __lanemask_lt());
-  if (chg && slot < lg->cap) lg->ev[slot] = (lg->it << 20) | (kind << 19) | link;
-  lg->nev += (int)__popcll(m);
+  const int slot = lg.nev + (int)__popcll(m & __lanemask_lt());
+  if (chg && slot < lg.cap) lg.ev[slot] = (lg.it << 20) | (kind << 19) | link;
+  lg.nev += (int)__popcll(m);
 }
 
 // K6a: prvstatus for every PRV with a setting (dgga/solver.py:914-947)
-__device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog* lg = nullptr) {
+__device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
   const int i = lane_id();
   bool chg = false;
   if (i < pl.p) {
@@ -543,7 +542,7 @@ __device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog* lg = nu
     chg = sn != sk;
     c.S[k] = sn;
   }
-  if (lg) sched_log(lg, 0, chg, i < pl.p ? pl.prv_k[i] : 0);
+  if (lg.ev) sched_log(lg, 0, chg, i < pl.p ? pl.prv_k[i] : 0);
   const bool any = wave_any(chg);
   mem_sync();
   return any;
@@ -562,7 +561,7 @@ __device__ inline bool cv_closed(double htol, double qtol, bool s_is_open, doubl
 }
 
 // K6b: linkstatus (dgga/solver.py:697-739); returns "changed"
-__device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog* lg = nullptr) {
+__device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
   const int lane = lane_id();
   bool chg = false;
   if (!pl.has_cv && !pl.has_pump && pl.ntl == 0) return false;
@@ -593,7 +592,7 @@ __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog* lg = n
     if (live && ((full && cut_f) || (empty && cut_e))) c.S[k] = ST_TEMPCLOSED;
   }
   mem_sync();
-  if (lg && lg->ev) {
+  if (lg.ev) {
     for (int k0 = 0; k0 < pl.L; k0 += 64) {
       const int k = k0 + lane;
       const bool ck = k < pl.L && c.S[k] != c.st0[k];
@@ -687,9 +686,8 @@ __device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams
                                     double relerr, Ctrl& st) {
   if (!sp.sm) return !(relerr > sp.hacc);
   const int b = blockIdx.x;
-  SchedLog log{sp.sched_events ? sp.sched_events + (int64_t)b * sp.sched_cap : nullptr,
-               sp.sched_cap, it, st.nev};
-  SchedLog* lg = sp.sched_events ? &log : nullptr;
+  SchedLog lg{sp.sched_events ? sp.sched_events + (int64_t)b * sp.sched_cap : nullptr,
+              sp.sched_cap, it, st.nev};
   bool vchg = false;
   if (pl.p > 0) vchg = k6_prvstatus(pl, c, lg);
   const bool conv = relerr <= sp.hacc;
@@ -700,8 +698,8 @@ __device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams
     const bool chg = k6_linkstatus(pl, c, lg);
     stat_change = chg && br_conv;
   }
-  if (lg) {
-    st.nev = log.nev;
+  if (lg.ev) {
+    st.nev = lg.nev;
     if (lane_id() == 0)
       sp.sched_flags[(int64_t)b * maxtrials_of(pl, sp) + (it - 1)] =
           (uint8_t)(1 | (br_conv ? 2 : 0) | (br_per ? 4 : 0));
