@@ -136,6 +136,49 @@ def test_programs_factor_and_solve_like_dense(name):
     assert np.abs(X[slot] - xr).max() <= 1e-10 * np.abs(xr).max()
 
 
+@pytest.mark.parametrize("name", ["grid", "planar"])
+def test_row_block_layout_multiplies_like_dense(name):
+    # the 64-row-block layout of the symmetric rows (rs_blk / rs_ell) that the
+    # batched residual and matrix-vector product walk: every row's entries in
+    # column order, pads (0, -1), and A x equal to the dense product
+    net = grid_network(0) if name == "grid" else planar_network(1)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device="cpu", dense_status_machine=True)
+    st = s.stats
+    d, rh = scenario_batch(net, 1, seed=5)
+    q = m.init_flow()[None]
+    e = np.where(net.node_ke[m.junc_nodes] > 0, 1.0, 0.0)[None]
+    H = m.full_heads(np.zeros((1, m.Nj)), rh[:, m.fixed_nodes])
+    S = m.init_status_int[None]
+    P, Y = m.link_PY(q, S, m.r_hw[None])
+    csr, F, *_ = m.assemble(P, Y, q, H, e, net.node_ke[m.junc_nodes][None], d[:, m.junc_nodes],
+                            S, csr=True)
+    c2l = s.plan_export(7)
+    AL = np.zeros(st["nnzL"] + 1)
+    AL[c2l[c2l >= 0]] = csr[0][c2l >= 0]
+    blk, ell = s.plan_export(17), s.plan_export(18).reshape(-1, 2)
+    nb = (m.Nj + 63) // 64
+    assert len(blk) == nb + 1
+    slot = np.argsort(s.plan_export(0))
+    x = np.random.default_rng(0).normal(size=m.Nj)
+    xs = np.zeros(m.Nj)
+    xs[slot] = x
+    ys = np.zeros(m.Nj)
+    for r in range(m.Nj):
+        b, lane = divmod(r, 64)
+        cols = []
+        for k in range(blk[b + 1] - blk[b]):
+            pos, col = ell[(blk[b] + k) * 64 + lane]
+            if col < 0:
+                assert pos == 0
+                continue
+            cols.append(col)
+            ys[r] += AL[pos] * xs[col]
+        assert cols == sorted(cols) and r in cols
+    A = m.csr_to_dense(csr)[0]
+    assert np.allclose(ys[slot], A @ x, rtol=1e-13, atol=1e-12 * np.abs(A).max())
+
+
 def test_minimum_degree_beats_natural_ordering():
     net = planar_network(1)
     a = GGASolver(net, device="cpu", dense_status_machine=True, ordering="mmd").stats

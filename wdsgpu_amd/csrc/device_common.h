@@ -22,6 +22,11 @@ constexpr double DW_AB = 3.28895476345399058690e-03;
 constexpr double DW_AC = -5.14214965799093883760e-03;
 constexpr double PI_EPANET = 3.141592654;
 
+// One wavefront per scenario whose LDS share admits a single wavefront per
+// SIMD: let the compiler schedule for latency with the whole register file
+// instead of squeezing registers for an occupancy the LDS rules out.
+#define WDS_SCENARIO_KERNEL __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+
 __device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
 
 // Order LDS traffic between program levels of one wavefront: wait for the
@@ -339,6 +344,151 @@ __device__ __forceinline__ double residual_row_dd(const DevPlan& pl, const doubl
     }
   }
   return hi + lo;
+}
+
+// R_j *= V[diag_pos[j]] (the reciprocal pivots) for every row, V and R in
+// LDS: the pivot positions of up to kRowG rows per lane load before use.
+constexpr int kRowG = 16;
+__device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, double* R) {
+  const int lane = lane_id();
+  const lds_f64* Vl = (const lds_f64*)V;
+  lds_f64* Rl = (lds_f64*)R;
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG) {
+    int dp[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) dp[u] = pl.diag_pos[min(j0 + 64 * u, pl.Nj - 1)];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) {
+      const int j = j0 + 64 * u;
+      if (j < pl.Nj) Rl[j] *= Vl[dp[u]];
+    }
+  }
+}
+
+// Every junction row at once over the row-block layout rs_blk / rs_ell:
+// two 64-row blocks per pass, kRowE entries of each loaded before any is
+// used (unconditionally, from clamped positions, so that no load waits for
+// another), then the matrix values: a pass costs two memory round trips
+// instead of three per row.  Entries past a row's end contribute exact
+// zeros (p = 0: two_sum(hi, -0) returns hi and error 0), and the entries
+// are taken in the order of residual_row_dd / spmv_row, so the sums are
+// bitwise the same.
+constexpr int kRowE = 8;
+
+struct RowBlocks {
+  int e0[2], K[2];
+};
+__device__ __forceinline__ RowBlocks row_blocks(const DevPlan& pl, int b0, int nb) {
+  RowBlocks r;
+#pragma unroll
+  for (int t = 0; t < 2; ++t) {
+    const int bk = min(b0 + t, nb - 1);
+    r.e0[t] = pl.rs_blk[bk];
+    r.K[t] = b0 + t < nb ? pl.rs_blk[bk + 1] - r.e0[t] : 0;
+  }
+  return r;
+}
+// entries e .. e + kRowE - 1 of both blocks and their matrix / vector values
+__device__ __forceinline__ void row_entries(const DevPlan& pl, const RowBlocks& rb, int e,
+                                            const double* A, const lds_f64* X,
+                                            double (&av)[2][kRowE], double (&xv)[2][kRowE]) {
+  const int lane = lane_id();
+  int2 en[2][kRowE];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < kRowE; ++u)
+      en[t][u] = pl.rs_ell[(rb.e0[t] + min(e + u, max(rb.K[t] - 1, 0))) * 64 + lane];
+  // all matrix-value loads, then all vector reads, then the selects: left
+  // alone, the scheduler sinks each load to its use and the round trips
+  // serialise
+  __builtin_amdgcn_sched_barrier(0);
+  double a[2][kRowE], x[2][kRowE];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < kRowE; ++u) a[t][u] = A[en[t][u].x];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < kRowE; ++u) x[t][u] = X[max(en[t][u].y, 0)];
+#pragma unroll
+  for (int t = 0; t < 2; ++t)
+#pragma unroll
+    for (int u = 0; u < kRowE; ++u) {
+      const bool ok = e + u < rb.K[t] && en[t][u].y >= 0;
+      av[t][u] = ok ? a[t][u] : 0.0;
+      xv[t][u] = ok ? x[t][u] : 0.0;
+    }
+  __builtin_amdgcn_sched_group_barrier(0x0020, 2 * kRowE, 0);
+  __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kRowE, 0);
+  __builtin_amdgcn_sched_barrier(0);
+}
+
+// dst_j = b_j - (A x)_j in double-double (residual_row_dd), x and dst in LDS
+__device__ inline void residual_all(const DevPlan& pl, const double* A, const double* x,
+                                    const double* b, double* dst) {
+  const int lane = lane_id();
+  const lds_f64* X = (const lds_f64*)x;
+  lds_f64* D = (lds_f64*)dst;
+  const int nb = (pl.Nj + 63) >> 6;
+  for (int b0 = 0; b0 < nb; b0 += 2) {
+    const RowBlocks rb = row_blocks(pl, b0, nb);
+    double hi[2], lo[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      hi[t] = b[min(min(b0 + t, nb - 1) * 64 + lane, pl.Nj - 1)];
+      lo[t] = 0.0;
+    }
+    const int Km = max(rb.K[0], rb.K[1]);
+    for (int e = 0; e < Km; e += kRowE) {
+      double av[2][kRowE], xv[2][kRowE];
+      row_entries(pl, rb, e, A, X, av, xv);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kRowE; ++u) {
+          const double p = av[t][u] * xv[t][u];
+          const double pe = fma(av[t][u], xv[t][u], -p);
+          double sm, err;
+          two_sum(hi[t], -p, sm, err);
+          hi[t] = sm;
+          lo[t] += err - pe;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int j = (b0 + t) * 64 + lane;
+      if (b0 + t < nb && j < pl.Nj) D[j] = hi[t] + lo[t];
+    }
+  }
+}
+
+// dst = A x (spmv_row for every row), x in LDS, dst in global memory
+__device__ inline void spmv_all(const DevPlan& pl, const double* A, const double* x,
+                                double* dst) {
+  const int lane = lane_id();
+  const lds_f64* X = (const lds_f64*)x;
+  __attribute__((address_space(1))) double* D = (__attribute__((address_space(1))) double*)dst;
+  const int nb = (pl.Nj + 63) >> 6;
+  for (int b0 = 0; b0 < nb; b0 += 2) {
+    const RowBlocks rb = row_blocks(pl, b0, nb);
+    double sum[2] = {0.0, 0.0};
+    const int Km = max(rb.K[0], rb.K[1]);
+    for (int e = 0; e < Km; e += kRowE) {
+      double av[2][kRowE], xv[2][kRowE];
+      row_entries(pl, rb, e, A, X, av, xv);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kRowE; ++u) sum[t] = fma(av[t][u], xv[t][u], sum[t]);
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int j = (b0 + t) * 64 + lane;
+      if (b0 + t < nb && j < pl.Nj) D[j] = sum[t];
+    }
+  }
 }
 
 // ---------------------------------------------------------------------------

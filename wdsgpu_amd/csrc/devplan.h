@@ -114,6 +114,10 @@ struct DevPlan {
   // symmetric SpMV rows over A (slot order): (L position, column slot)
   const int32_t* spmv_ptr;    // [Nj+1]
   const int2* spmv_ent;       // [..]
+  // the symmetric rows in 64-row blocks, entry-major (ELL): entry e of row
+  // 64 b + lane is rs_ell[(rs_blk[b] + e) * 64 + lane], pads (0, -1)
+  const int32_t* rs_blk;      // [ceil(Nj / 64) + 1]
+  const int2* rs_ell;
   // PRVs with a setting (file order)
   const int32_t* prv_k;       // [p]
   const int32_t* prv_s1;      // [p] slot of n1

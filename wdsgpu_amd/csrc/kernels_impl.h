@@ -401,7 +401,7 @@ __device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
 #else
   run_dot(pl.fwd);
 #endif
-  for (int j = lane_id(); j < pl.Nj; j += 64) c.R[j] *= c.V[pl.diag_pos[j]];
+  scale_by_pivots(pl, c.V, c.R);
   lds_sync();
 }
 
@@ -427,7 +427,7 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
   for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
   lds_sync();
   for (int it = 0; it < n_refine; ++it) {
-    for (int i = lane; i < pl.Nj; i += 64) S[i] = residual_row_dd(pl, c.A, X, bsrc[i], i);
+    residual_all(pl, c.A, X, bsrc, S);
     lds_sync();
     solve_fwd(pl, c);
     solve_bwd(pl, c);
@@ -730,7 +730,7 @@ __device__ inline void write_outputs(const DevPlan& pl, Scn& c, const SolveParam
 namespace wds {
 
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(64) void k_factor_solve(DevPlan pl, const double* csr,
+__global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr,
                                                      const double* F, double* H, int n_refine,
                                                      double* factor_out, int32_t* info,
                                                      double* ws) {
@@ -788,7 +788,7 @@ __device__ inline void load_csr(const DevPlan& pl, Scn& c, const double* csr, in
 
 // Numeric factorisation only (the reference's cuDSS factorize phase,
 // dgga/solver.py:2630-2640): factor_out [B][nnzL], info [B].
-__global__ __launch_bounds__(64) void k_factor(DevPlan pl, const double* csr,
+__global__ WDS_SCENARIO_KERNEL void k_factor(DevPlan pl, const double* csr,
                                                double* factor_out, int32_t* info, double* ws) {
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();
@@ -805,7 +805,7 @@ __global__ __launch_bounds__(64) void k_factor(DevPlan pl, const double* csr,
 // Triangular solves with refinement against a stored factor (the cuDSS
 // solve phase and the refinement loop, dgga/solver.py:2641-2651,
 // 3052-3064): H [B][Nj] from F [B][Nj], both in original junction order.
-__global__ __launch_bounds__(64) void k_trisolve(DevPlan pl, const double* factor,
+__global__ WDS_SCENARIO_KERNEL void k_trisolve(DevPlan pl, const double* factor,
                                                  const double* csr, const double* F, double* H,
                                                  int n_refine, double* ws) {
   extern __shared__ double lds[];
@@ -824,7 +824,7 @@ __global__ __launch_bounds__(64) void k_trisolve(DevPlan pl, const double* facto
 }
 
 // Fused forward: the whole Newton loop of scenario blockIdx.x.
-__global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
+__global__ WDS_SCENARIO_KERNEL void k_solve(DevPlan pl, SolveParams sp) {
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
@@ -886,7 +886,7 @@ __global__ __launch_bounds__(64) void k_solve(DevPlan pl, SolveParams sp) {
 // One Newton iteration per launch (the unfused path).  ctrl[b] = {active,
 // iters, nextcheck, info}; heads/flows/emitters/statuses persist in the
 // output arrays between launches (it == 1 initialises them).
-__global__ __launch_bounds__(64) void k_newton_iter(DevPlan pl, SolveParams sp, int it,
+__global__ WDS_SCENARIO_KERNEL void k_newton_iter(DevPlan pl, SolveParams sp, int it,
                                                     int32_t* ctrl) {
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();

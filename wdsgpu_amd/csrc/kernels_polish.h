@@ -73,8 +73,9 @@ __device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
 }
 
 // (A-hat x)_i for all rows into dst (global)
+// dst (global) = A x (x in LDS)
 __device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double* dst) {
-  for (int i = lane_id(); i < pl.Nj; i += 64) dst[i] = spmv_row(pl, c.A, x, i);
+  spmv_all(pl, c.A, x, dst);
   mem_sync();
 }
 
@@ -488,7 +489,7 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
 
 // Backward of the implicit solve (ImplicitGGASolveGPU.backward,
 // dgga/autodiff.py:1245-1312), one wavefront per scenario.
-__global__ __launch_bounds__(64) void k_adjoint(DevPlan pl, AdjParams ap) {
+__global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ap.ws, b, const_cast<double*>(ap.flow),

@@ -246,6 +246,26 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     }
     P->spmv_ptr[s + 1] = P->spmv_ptr[s] + (int32_t)r.size();
   }
+  {
+    const int nb = (Nj + kLanes - 1) / kLanes;
+    P->rs_blk.assign(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+      int32_t k = 0;
+      for (int s = b * kLanes; s < std::min(Nj, (b + 1) * kLanes); ++s)
+        k = std::max(k, P->spmv_ptr[s + 1] - P->spmv_ptr[s]);
+      P->rs_blk[b + 1] = P->rs_blk[b] + k;
+    }
+    P->rs_ell.assign((size_t)P->rs_blk[nb] * kLanes * 2, 0);
+    for (size_t i = 1; i < P->rs_ell.size(); i += 2) P->rs_ell[i] = -1;
+    for (int s = 0; s < Nj; ++s) {
+      const int b = s / kLanes, lane = s % kLanes;
+      for (int32_t e = 0; e < P->spmv_ptr[s + 1] - P->spmv_ptr[s]; ++e) {
+        const size_t x = ((size_t)(P->rs_blk[b] + e) * kLanes + lane) * 2;
+        P->rs_ell[x] = P->spmv_ent[2 * (size_t)(P->spmv_ptr[s] + e)];
+        P->rs_ell[x + 1] = P->spmv_ent[2 * (size_t)(P->spmv_ptr[s] + e) + 1];
+      }
+    }
+  }
   // ---- PRVs with a setting ----
   P->prv_k.clear();
   for (int k = 0; k < L; ++k) {
@@ -439,6 +459,9 @@ extern "C" int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* 
     case 5: return copy_out(plan->csr_indptr, dst, cap);
     case 6: return copy_out(plan->csr_indices, dst, cap);
     case 7: return copy_out(plan->csr_to_L, dst, cap);
+    case 17: return copy_out(plan->rs_blk, dst, cap);
+    case 18: return copy_out(plan->rs_ell, dst, cap);
+    case 19: return copy_out(plan->row_slot, dst, cap);
     default: break;
   }
   if (what >= 8 && what < 17) {

@@ -24,6 +24,9 @@ struct wds_plan {
       spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, tl_k, tl_tank, tl_other,
       tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;
   std::vector<int32_t> spmv_ent;  // pairs (pos, col)
+  // the same rows in 64-row blocks, entry-major within a block (ELL): entry e
+  // of row 64 b + lane is pair (rs_blk[b] + e) * 64 + lane; pads are (0, -1)
+  std::vector<int32_t> rs_blk, rs_ell;
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
   std::vector<int32_t> l_scale, jadj_mid;

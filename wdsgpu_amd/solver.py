@@ -398,10 +398,10 @@ class GGASolver:
         Fv = self._prep(F).contiguous()
         B = int(Fv.shape[0])
         if csr.shape != (B, int(self.stats["nnz_csr"])) or Fv.shape != (B, self.Nj) or \
-                tuple(fac.shape) != (B, int(self.stats["nnzL"])) or fac.device != self.device:
+                tuple(fac.shape) != (B, int(self.stats["nnzL"])):
             raise ValueError("trisolve: shapes %s %s %s" % (tuple(fac.shape), tuple(csr.shape),
                                                            tuple(Fv.shape)))
-        fac = fac.to(self.dtype).contiguous()
+        fac = fac.to(device=self.device, dtype=self.dtype).contiguous()
         H = torch.empty_like(Fv)
         ws = self._workspace(B)
         with torch.cuda.device(self.device):
