@@ -118,6 +118,12 @@ struct DevPlan {
   // 64 b + lane is rs_ell[(rs_blk[b] + e) * 64 + lane], pads (0, -1)
   const int32_t* rs_blk;      // [ceil(Nj / 64) + 1]
   const int2* rs_ell;
+  // assembly rows in the same blocks: (link | side bit, -1) for the row's
+  // links in jadj order, then (link, fixed slot) for its fixed-head terms in
+  // jg order; pads (0, -2)
+  const int32_t* as_blk;
+  const int2* as_ell;
+  const int4* off4;           // [noff] (L position, first link, first index into off_link, count)
   // PRVs with a setting (file order)
   const int32_t* prv_k;       // [p]
   const int32_t* prv_s1;      // [p] slot of n1

@@ -255,96 +255,142 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
 // batches of kGather with every load issued before any is consumed, and the
 // accumulation is branch-free.  Sums keep the reference's order exactly.
 
-__device__ inline void k2_assemble(const DevPlan& pl, Scn& c, bool want_F) {
+constexpr int kAsE = 8;   // assembly entries per row loaded per round
+constexpr int kOffG = 16; // off-diagonal entries per lane loaded per round
+
+template <bool kF>
+__device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
-  for (int j = lane; j < pl.Nj; j += 64) {
-    double dg = 0.0, F = 0.0, X = 0.0;
-    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1];
-    for (int eb = e0; eb < e1; eb += kGather) {
-      int ent[kGather];
-      double Pv[kGather], Yv[kGather], qv[kGather];
+  const int nb = (pl.Nj + 63) >> 6;
+  const lds_f64* Hn = (const lds_f64*)c.Hn;
+  lds_f64* Vl = (lds_f64*)c.V;
+  lds_f64* Rl = (lds_f64*)c.R;
+  for (int b0 = 0; b0 < nb; b0 += 2) {
+    int e0[2], K[2], jr[2], dp[2];
+    double kv[2], hg[2], hl[2], ev[2], dj[2], el[2];
 #pragma unroll
-      for (int u = 0; u < kGather; ++u) ent[u] = (eb + u < e1) ? pl.jadj[eb + u] : 0;
+    for (int t = 0; t < 2; ++t) {
+      const int bk = min(b0 + t, nb - 1);
+      e0[t] = pl.as_blk[bk];
+      K[t] = b0 + t < nb ? pl.as_blk[bk + 1] - e0[t] : 0;
+      jr[t] = bk * 64 + lane;
+      const int j = min(jr[t], pl.Nj - 1);
+      kv[t] = c.kej[j];
+      hg[t] = c.hge[j];
+      hl[t] = c.hle[j];
+      ev[t] = c.e[j];
+      dj[t] = c.dj[j];
+      el[t] = pl.el_slot[j];
+      dp[t] = pl.diag_pos[j];
+    }
+    double dg[2] = {0.0, 0.0}, F[2] = {0.0, 0.0}, X[2] = {0.0, 0.0};
+    const int Km = max(K[0], K[1]);
+    for (int e = 0; e < Km; e += kAsE) {
+      int2 en[2][kAsE];
 #pragma unroll
-      for (int u = 0; u < kGather; ++u) {
-        const int k = ent[u] & 0x7fffffff;  // link 0 stands in for a missing entry
-        Pv[u] = c.P[k];
-        if (want_F) {
-          Yv[u] = c.Y[k];
-          qv[u] = c.q[k];
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u)
+          en[t][u] = pl.as_ell[(e0[t] + min(e + u, max(K[t] - 1, 0))) * 64 + lane];
+      __builtin_amdgcn_sched_barrier(0);
+      double Pv[2][kAsE], Yv[2][kAsE], qv[2][kAsE], Hv[2][kAsE];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u) Pv[t][u] = c.P[en[t][u].x & 0x7fffffff];
+      if (kF) {
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int u = 0; u < kAsE; ++u) {
+            const int k = en[t][u].x & 0x7fffffff;
+            Yv[t][u] = c.Y[k];
+            qv[t][u] = c.q[k];
+          }
+#pragma unroll
+        for (int t = 0; t < 2; ++t)
+#pragma unroll
+          for (int u = 0; u < kAsE; ++u) Hv[t][u] = Hn[max(en[t][u].y, 0)];
+        __builtin_amdgcn_sched_group_barrier(0x0020, 6 * kAsE, 0);
+        __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kAsE, 0);
+      } else {
+        __builtin_amdgcn_sched_group_barrier(0x0020, 2 * kAsE, 0);
+      }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u) {
+          const bool in_row = e + u < K[t];
+          const int y = en[t][u].y;
+          const double P = Pv[t][u];
+          // links with P == 0 (pcv PRVs) take no part (dgga/solver.py:2939-2956)
+          const bool use = in_row && y == -1 && P != 0.0;
+          dg[t] = use ? dg[t] + P : dg[t];
+          if (kF) {
+            const bool n1side = en[t][u].x >= 0;
+            F[t] = use ? F[t] + (n1side ? Yv[t][u] : -Yv[t][u]) : F[t];
+            X[t] = use ? X[t] + (n1side ? -qv[t][u] : qv[t][u]) : X[t];
+            // fixed-head terms: links with a fixed n1, then with a fixed n2
+            const bool fix = in_row && y >= 0 && P != 0.0;
+            F[t] = fix ? F[t] + P * Hv[t][u] : F[t];
+          }
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      if (kv[t] > 0.0) {
+        dg[t] = dg[t] + 1.0 / hg[t];
+        if (kF) {
+          F[t] = F[t] + (hl[t] + el[t]) / hg[t];
+          X[t] = X[t] - ev[t];
         }
       }
-#pragma unroll
-      for (int u = 0; u < kGather; ++u) {
-        // links with P == 0 (pcv PRVs) take no part (dgga/solver.py:2939-2956)
-        const bool use = (eb + u < e1) && Pv[u] != 0.0;
-        dg = use ? dg + Pv[u] : dg;
-        if (want_F) {
-          const bool n1side = ent[u] >= 0;
-          F = use ? F + (n1side ? Yv[u] : -Yv[u]) : F;
-          X = use ? X + (n1side ? -qv[u] : qv[u]) : X;
+      if (b0 + t < nb && jr[t] < pl.Nj) {
+        c.A[dp[t]] = dg[t];
+        Vl[dp[t]] = dg[t];
+        if (kF) {
+          X[t] = X[t] - dj[t];
+          c.F[jr[t]] = F[t] + X[t];
+          Rl[jr[t]] = X[t];
         }
       }
-    }
-    if (want_F) {
-      // fixed-head terms: links with a fixed n1, then with a fixed n2
-      for (int g = pl.jg_ptr[j]; g < pl.jg_ptr[j + 1]; ++g) {
-        const int2 e = pl.jg[g];
-        const double P = c.P[e.x];
-        if (P != 0.0) F += P * c.Hn[e.y];
-      }
-    }
-    const double ke = c.kej[j];
-    if (ke > 0.0) {
-      const double hg = c.hge[j];
-      dg = dg + 1.0 / hg;
-      if (want_F) {
-        F = F + (c.hle[j] + pl.el_slot[j]) / hg;
-        X = X - c.e[j];
-      }
-    }
-    const int dp = pl.diag_pos[j];
-    c.A[dp] = dg;
-    c.V[dp] = dg;
-    if (want_F) {
-      X = X - c.dj[j];
-      c.F[j] = F + X;
-      c.R[j] = X;
     }
   }
-  for (int o0 = lane; o0 < pl.noff; o0 += 64 * kGather) {
-    int b[kGather], n[kGather], pos[kGather], k0[kGather];
+  for (int o0 = lane; o0 < pl.noff; o0 += 64 * kOffG) {
+    int4 w[kOffG];
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) {
-      const int o = o0 + 64 * u;
-      const bool ok = o < pl.noff;
-      b[u] = ok ? pl.off_ptr[o] : 0;
-      n[u] = ok ? pl.off_ptr[o + 1] - b[u] : 0;
-      pos[u] = ok ? pl.off_pos[o] : 0;
-    }
+    for (int u = 0; u < kOffG; ++u) w[u] = pl.off4[min(o0 + 64 * u, pl.noff - 1)];
+    __builtin_amdgcn_sched_barrier(0);
+    double v[kOffG];
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) k0[u] = n[u] > 0 ? pl.off_link[b[u]] : 0;
-    double v[kGather];
+    for (int u = 0; u < kOffG; ++u) v[u] = c.P[w[u].y];
+    __builtin_amdgcn_sched_group_barrier(0x0020, kOffG, 0);
+    __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) {
-      const double P = c.P[k0[u]];
-      v[u] = (n[u] > 0 && P != 0.0) ? -P : 0.0;
-    }
-#pragma unroll
-    for (int u = 0; u < kGather; ++u) {
+    for (int u = 0; u < kOffG; ++u) {
+      const double P = v[u];
+      v[u] = (w[u].w > 0 && P != 0.0) ? -P : 0.0;
       // parallel links between one junction pair (rare): in link order
-      for (int e = b[u] + 1; e < b[u] + n[u]; ++e) {
-        const double P = c.P[pl.off_link[e]];
-        if (P != 0.0) v[u] += -P;
+      for (int e = w[u].z + 1; e < w[u].z + w[u].w; ++e) {
+        const double Pe = c.P[pl.off_link[e]];
+        if (Pe != 0.0) v[u] += -Pe;
       }
       if (o0 + 64 * u < pl.noff) {
-        c.A[pos[u]] = v[u];
-        c.V[pos[u]] = v[u];
+        c.A[w[u].x] = v[u];
+        Vl[w[u].x] = v[u];
       }
     }
   }
-  for (int f = lane; f < pl.nfill; f += 64) c.V[pl.fill_pos[f]] = 0.0;
-  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  for (int f0 = lane; f0 < pl.nfill; f0 += 64 * kRowG) {
+    int fp[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) fp[u] = pl.fill_pos[min(f0 + 64 * u, pl.nfill - 1)];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u)
+      if (f0 + 64 * u < pl.nfill) Vl[fp[u]] = 0.0;
+  }
+  if (lane == 0) Vl[pl.nnzL] = 0.0;
   mem_sync();
   lds_sync();
 }
@@ -657,7 +703,7 @@ __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const Solve
   WDS_T0();
   k1_coeffs(pl, c);
   WDS_TACC(c, 0);
-  k2_assemble(pl, c, true);
+  k2_assemble<true>(pl, c);
   k2_prv(pl, c);
   if (pl.p > 0 && dratio) *dratio = diag_ratio_now(pl, c);
   WDS_TACC(c, 1);

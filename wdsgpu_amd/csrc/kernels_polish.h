@@ -57,7 +57,7 @@ __device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
 
 // Assemble A-hat = A(P) + diag(em/hge) + CBIG * ACTIVE at j2 into A and V
 __device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
-  k2_assemble(pl, c, false);
+  k2_assemble<false>(pl, c);
   if (lane_id() == 0) {
     for (int i = 0; i < pl.p; ++i) {
       if (c.sh[i] != 0.0) {

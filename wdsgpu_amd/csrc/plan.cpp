@@ -266,6 +266,42 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       }
     }
   }
+  {
+    const int nb = (Nj + kLanes - 1) / kLanes;
+    auto row_len = [&](int s) {
+      return (P->jadj_ptr[s + 1] - P->jadj_ptr[s]) + (P->jg_ptr[s + 1] - P->jg_ptr[s]);
+    };
+    P->as_blk.assign(nb + 1, 0);
+    for (int b = 0; b < nb; ++b) {
+      int32_t k = 0;
+      for (int s = b * kLanes; s < std::min(Nj, (b + 1) * kLanes); ++s) k = std::max(k, row_len(s));
+      P->as_blk[b + 1] = P->as_blk[b] + k;
+    }
+    P->as_ell.assign((size_t)P->as_blk[nb] * kLanes * 2, 0);
+    for (size_t i = 1; i < P->as_ell.size(); i += 2) P->as_ell[i] = -2;
+    for (int s = 0; s < Nj; ++s) {
+      const int b = s / kLanes, lane = s % kLanes;
+      int32_t e = 0;
+      auto put = [&](int32_t x, int32_t y) {
+        const size_t o = ((size_t)(P->as_blk[b] + e) * kLanes + lane) * 2;
+        P->as_ell[o] = x;
+        P->as_ell[o + 1] = y;
+        ++e;
+      };
+      for (int32_t i = P->jadj_ptr[s]; i < P->jadj_ptr[s + 1]; ++i) put(P->jadj[i], -1);
+      for (int32_t g = P->jg_ptr[s]; g < P->jg_ptr[s + 1]; ++g) put(P->jg[2 * g], P->jg[2 * g + 1]);
+    }
+    const size_t noff = P->off_pos.size();
+    P->off4.assign(noff * 4, 0);
+    for (size_t o = 0; o < noff; ++o) {
+      const int32_t b0 = P->off_ptr[o], n = P->off_ptr[o + 1] - b0;
+      P->off4[4 * o] = P->off_pos[o];
+      P->off4[4 * o + 1] = n > 0 ? P->off_link[b0] : 0;
+      P->off4[4 * o + 2] = b0;
+      P->off4[4 * o + 3] = n;
+    }
+  }
+
   // ---- PRVs with a setting ----
   P->prv_k.clear();
   for (int k = 0; k < L; ++k) {

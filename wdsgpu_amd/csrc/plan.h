@@ -27,6 +27,11 @@ struct wds_plan {
   // the same rows in 64-row blocks, entry-major within a block (ELL): entry e
   // of row 64 b + lane is pair (rs_blk[b] + e) * 64 + lane; pads are (0, -1)
   std::vector<int32_t> rs_blk, rs_ell;
+  // assembly rows in the same blocks: a row's links in jadj order (link |
+  // side bit, -1), then its fixed-head terms in jg order (link, fixed slot);
+  // pads (0, -2).  off4: per off-diagonal entry (L position, first link,
+  // first index into off_link, number of links)
+  std::vector<int32_t> as_blk, as_ell, off4;
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
   std::vector<int32_t> l_scale, jadj_mid;
