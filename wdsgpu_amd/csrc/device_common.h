@@ -71,6 +71,31 @@ __device__ __forceinline__ double wave_min(double v) {
 }
 __device__ __forceinline__ bool wave_any(bool p) { return __any(p); }
 
+// v of the lane the DPP control CTRL selects, for a double (both halves)
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over aligned groups of 2^lw lanes (1 <= lw <= 6), bitwise equal in
+// every lane to an xor butterfly over the same groups.  The steps within 16
+// lanes use DPP: quad permutes for xor 1 and xor 2, then row half-mirror and
+// row mirror, which pair a lane with one of the partner half; after each
+// step every lane of a group holds the same bits (the additions are
+// commutative), so the partner's value equals the butterfly's.  Wider steps
+// use bpermute.
+__device__ __forceinline__ double group_sum(double v, int lw) {
+  v += dpp_f64<0xB1>(v);                  // quad_perm [1, 0, 3, 2]
+  if (lw >= 2) v += dpp_f64<0x4E>(v);     // quad_perm [2, 3, 0, 1]
+  if (lw >= 3) v += dpp_f64<0x141>(v);    // row_half_mirror
+  if (lw >= 4) v += dpp_f64<0x140>(v);    // row_mirror
+  for (int m = 16; m < (1 << lw); m <<= 1) v += __shfl_xor(v, m, 64);
+  return v;
+}
+
 // ---------------------------------------------------------------------------
 // Level-scheduled program executors.
 // kRing chunks (kRing * kUnroll steps) of instruction loads stay in flight in
@@ -187,11 +212,10 @@ __device__ inline void run_factor(const ProgDev& pg, double* Vg, int& bad,
     if (wl & kLevelEnd) {
       const int lw = (wl >> kWidthShift) & 7;
       if (lw) {
-        // wide level: butterfly-sum each aligned group of 2^lw lanes (the
-        // first lane's slice started from the target's value), then the
-        // group's first lane finalises the target
-        double red = acc;
-        for (int m = 1; m < (1 << lw); m <<= 1) red += __shfl_xor(red, m, 64);
+        // wide level: sum each aligned group of 2^lw lanes (the first
+        // lane's slice started from the target's value), then the group's
+        // first lane finalises the target
+        const double red = group_sum(acc, lw);
         if (fin_act && (lane & ((1 << lw) - 1)) == 0) {
           if (fin_diag && !(red > 0.0)) bad = 1;
           V[fin_t] = fin_diag ? recip(red) : red;
@@ -271,8 +295,7 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
     if (f & kDotLevelEnd) {
       const int lw = (f >> kDotWidthShift) & 7;
       if (lw) {
-        double red = acc;
-        for (int m = 1; m < (1 << lw); m <<= 1) red += __shfl_xor(red, m, 64);
+        const double red = group_sum(acc, lw);
         if ((lane & ((1 << lw) - 1)) == 0) lds_at(fin) = red;
       }
       lvl_start = true;
