@@ -372,6 +372,7 @@ __device__ __forceinline__ double residual_row_dd(const DevPlan& pl, const doubl
 // R_j *= V[diag_pos[j]] (the reciprocal pivots) for every row, V and R in
 // LDS: the pivot positions of up to kRowG rows per lane load before use.
 constexpr int kRowG = 16;
+constexpr int kRowG8 = 8;   // rows per lane per round where each row loads several values
 __device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, double* R) {
   const int lane = lane_id();
   const lds_f64* Vl = (const lds_f64*)V;

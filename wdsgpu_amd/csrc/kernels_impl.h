@@ -235,13 +235,23 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
 __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
   link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const double ke = c.kej[j];
-    if (ke > 0.0) {
-      double hl, hg;
-      emitter_hl(pl, c.e[j], ke, hl, hg);
-      c.hle[j] = hl;
-      c.hge[j] = hg;
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    double kv[kRowG8], ev[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = min(j0 + 64 * u, pl.Nj - 1);
+      kv[u] = c.kej[j];
+      ev[u] = c.e[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j < pl.Nj && kv[u] > 0.0) {
+        double hl, hg;
+        emitter_hl(pl, ev[u], kv[u], hl, hg);
+        c.hle[j] = hl;
+        c.hge[j] = hg;
+      }
     }
   }
   mem_sync();
@@ -491,9 +501,14 @@ __device__ inline bool k3_factor(const DevPlan& pl, Scn& c) {
 #else
   run_factor(pl.factor, c.V, bad);
 #endif
-  for (int p = lane_id(); p < pl.nnzL; p += 64) {
-    const int sd = pl.l_scale[p];
-    if (sd >= 0) c.V[p] *= c.V[sd];
+  lds_f64* Vl = (lds_f64*)c.V;
+  for (int p0 = lane_id(); p0 < pl.nnzL; p0 += 64 * kRowG) {
+    int sd[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) sd[u] = pl.l_scale[min(p0 + 64 * u, pl.nnzL - 1)];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u)
+      if (p0 + 64 * u < pl.nnzL && sd[u] >= 0) Vl[p0 + 64 * u] *= Vl[sd[u]];
   }
   lds_sync();
   return wave_any(bad != 0);
@@ -530,14 +545,28 @@ __device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
       c.q[k] = qn;
     }
   }
-  for (int j = lane; j < pl.Nj; j += 64) {
-    if (c.kej[j] > 0.0) {
-      const double dh = c.Hn[j] - pl.el_slot[j];
-      const double dq = (c.hle[j] - dh) / c.hge[j];
-      const double en = c.e[j] - dq;
-      dqe += fabs(dq);
-      qse += fabs(en);
-      c.e[j] = en;
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    double kv[kRowG8], hlv[kRowG8], hgv[kRowG8], ev[kRowG8], elv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = min(j0 + 64 * u, pl.Nj - 1);
+      kv[u] = c.kej[j];
+      hlv[u] = c.hle[j];
+      hgv[u] = c.hge[j];
+      ev[u] = c.e[j];
+      elv[u] = pl.el_slot[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j < pl.Nj && kv[u] > 0.0) {
+        const double dh = c.Hn[j] - elv[u];
+        const double dq = (hlv[u] - dh) / hgv[u];
+        const double en = ev[u] - dq;
+        dqe += fabs(dq);
+        qse += fabs(en);
+        c.e[j] = en;
+      }
     }
   }
   const double dqsum = wave_sum(dqs) + wave_sum(dqe);
@@ -654,10 +683,18 @@ __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
 // diag ratio proxy max|diag| / min|diag| (dgga/solver.py:3026-3032)
 __device__ inline double diag_ratio_now(const DevPlan& pl, Scn& c) {
   double mx = 0.0, mn = __builtin_huge_val();
-  for (int j = lane_id(); j < pl.Nj; j += 64) {
-    const double v = fabs(c.V[pl.diag_pos[j]]);
-    mx = fmax(mx, v);
-    mn = fmin(mn, v);
+  const lds_f64* Vl = (const lds_f64*)c.V;
+  for (int j0 = lane_id(); j0 < pl.Nj; j0 += 64 * kRowG) {
+    int dp[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) dp[u] = pl.diag_pos[min(j0 + 64 * u, pl.Nj - 1)];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) {
+      if (j0 + 64 * u >= pl.Nj) break;
+      const double v = fabs(Vl[dp[u]]);
+      mx = fmax(mx, v);
+      mn = fmin(mn, v);
+    }
   }
   mx = wave_max(mx);
   mn = wave_min(mn);
