@@ -124,6 +124,11 @@ struct DevPlan {
   const int32_t* as_blk;
   const int2* as_ell;
   const int4* off4;           // [noff] (L position, first link, first index into off_link, count)
+  // the polish's mass rows: links with n2 == row, then n1 == row (jadj
+  // entries) per 64-row block; ps_len[row] entries per row
+  const int32_t* ps_blk;
+  const int32_t* ps_ell;
+  const int32_t* ps_len;
   // PRVs with a setting (file order)
   const int32_t* prv_k;       // [p]
   const int32_t* prv_s1;      // [p] slot of n1

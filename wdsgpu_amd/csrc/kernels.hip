@@ -121,6 +121,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   size_t o_fp = bb.add(P->fill_pos), o_sp = bb.add(P->spmv_ptr), o_se = bb.add(P->spmv_ent);
   size_t o_rb = bb.add(P->rs_blk), o_re = bb.add(P->rs_ell);
   size_t o_ab = bb.add(P->as_blk), o_ae = bb.add(P->as_ell), o_o4 = bb.add(P->off4);
+  size_t o_pb = bb.add(P->ps_blk), o_pe = bb.add(P->ps_ell), o_pln = bb.add(P->ps_len);
   size_t o_pk = bb.add(P->prv_k), o_p1 = bb.add(P->prv_s1), o_p2 = bb.add(P->prv_s2),
          o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
@@ -202,6 +203,9 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.as_blk = at<int32_t>(d, o_ab);
   D.as_ell = at<int2>(d, o_ae);
   D.off4 = at<int4>(d, o_o4);
+  D.ps_blk = at<int32_t>(d, o_pb);
+  D.ps_ell = at<int32_t>(d, o_pe);
+  D.ps_len = at<int32_t>(d, o_pln);
   D.prv_k = at<int32_t>(d, o_pk);
   D.prv_s1 = at<int32_t>(d, o_p1);
   D.prv_s2 = at<int32_t>(d, o_p2);

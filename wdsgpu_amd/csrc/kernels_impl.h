@@ -232,9 +232,9 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
 
 // ---------------------------------------------------------------------------
 // K1: link and emitter coefficients at the current state
-__device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
+// emitter hloss / hgrad of every junction with an emitter, rows batched
+__device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
-  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
   for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
     double kv[kRowG8], ev[kRowG8];
 #pragma unroll
@@ -254,6 +254,12 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
       }
     }
   }
+}
+
+__device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
+  const int lane = lane_id();
+  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
+  emitter_coeffs(pl, c);
   mem_sync();
 }
 

@@ -27,15 +27,7 @@ namespace wds {
 __device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
   link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const double ke = c.kej[j];
-    if (ke > 0.0) {
-      double hl, hg;
-      emitter_hl(pl, c.e[j], ke, hl, hg);
-      c.hle[j] = hl;
-      c.hge[j] = hg;
-    }
-  }
+  emitter_coeffs(pl, c);
   mem_sync();
   const int i = lane;
   if (i < pl.p) {
@@ -327,24 +319,79 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
   }
   mem_sync();
   lds_sync();
-  for (int j = lane; j < pl.Nj; j += 64) {
-    double m = 0.0, bt = 0.0;
-    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1], em = pl.jadj_mid[j];
-    // n2 == j first (reference scatter order), then n1 == j negated
-    row_gather(pl, em, e1, 1.0, c.q, c.P, c.Y, m, bt);
-    row_gather(pl, e0, em, -1.0, c.q, c.P, c.Y, m, bt);
-    double re = 0.0, emh = 0.0;
-    if (c.kej[j] > 0.0) {
-      re = c.hle[j] - (c.Hn[j] - pl.el_slot[j]);
-      emh = 1.0 / c.hge[j];
-      m = m - c.e[j];
-      rmax = fmax(rmax, fabs(re));
+  // mass rows over the polish row layout: a row's links with n2 == j first
+  // (reference scatter order), then those with n1 == j negated; two 64-row
+  // blocks per pass, every load of a pass issued before use
+  const int nb = (pl.Nj + 63) >> 6;
+  const lds_f64* Hl = (const lds_f64*)c.Hn;
+  for (int b0 = 0; b0 < nb; b0 += 2) {
+    int e0[2], K[2], len[2], jr[2];
+    double kv[2], hlv[2], hgv[2], ev[2], djv[2], elv[2], hn[2];
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      const int bk = min(b0 + t, nb - 1);
+      e0[t] = pl.ps_blk[bk];
+      K[t] = b0 + t < nb ? pl.ps_blk[bk + 1] - e0[t] : 0;
+      jr[t] = bk * 64 + lane;
+      const int j = min(jr[t], pl.Nj - 1);
+      len[t] = pl.ps_len[j];
+      kv[t] = c.kej[j];
+      hlv[t] = c.hle[j];
+      hgv[t] = c.hge[j];
+      ev[t] = c.e[j];
+      djv[t] = c.dj[j];
+      elv[t] = pl.el_slot[j];
+      hn[t] = Hl[j];
     }
-    m = m - c.dj[j];
-    rmax = fmax(rmax, fabs(m));
-    if (build_rhs) {
-      c.v1[j] = m - bt + emh * re;
-      c.v4[j] = re;
+    double m[2] = {0.0, 0.0}, bt[2] = {0.0, 0.0};
+    const int Km = max(K[0], K[1]);
+    for (int e = 0; e < Km; e += kAsE) {
+      int en[2][kAsE];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u)
+          en[t][u] = pl.ps_ell[(e0[t] + min(e + u, max(K[t] - 1, 0))) * 64 + lane];
+      __builtin_amdgcn_sched_barrier(0);
+      double qv[2][kAsE], Pv[2][kAsE], Yv[2][kAsE];
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u) {
+          const int k = en[t][u] & 0x7fffffff;
+          qv[t][u] = c.q[k];
+          Pv[t][u] = c.P[k];
+          Yv[t][u] = c.Y[k];
+        }
+      __builtin_amdgcn_sched_group_barrier(0x0020, 6 * kAsE, 0);
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int t = 0; t < 2; ++t)
+#pragma unroll
+        for (int u = 0; u < kAsE; ++u) {
+          if (e + u >= len[t]) continue;
+          const bool n2side = en[t][u] < 0;
+          const double pw = Pv[t][u] * Yv[t][u];
+          m[t] += n2side ? qv[t][u] : -qv[t][u];
+          bt[t] += n2side ? pw : -pw;
+        }
+    }
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+      if (!(b0 + t < nb && jr[t] < pl.Nj)) continue;
+      double mm = m[t], re = 0.0, emh = 0.0;
+      if (kv[t] > 0.0) {
+        re = hlv[t] - (hn[t] - elv[t]);
+        emh = 1.0 / hgv[t];
+        mm = mm - ev[t];
+        rmax = fmax(rmax, fabs(re));
+      }
+      mm = mm - djv[t];
+      rmax = fmax(rmax, fabs(mm));
+      if (build_rhs) {
+        c.v1[jr[t]] = mm - bt[t] + emh * re;
+        c.v4[jr[t]] = re;
+      }
     }
   }
   mem_sync();
@@ -388,17 +435,32 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
     WDS_TACC(c, 105);
     polish_residuals(pl, c, true);
     // v4 holds r_e; keep it in F (free during the polish) before primal_solve
-    for (int j = lane; j < pl.Nj; j += 64) c.F[j] = c.v4[j];
+    vec_copy(c.F, c.v4, nullptr, pl.Nj);
     mem_sync();
     WDS_TACC(c, 106);
     if (!primal_solve(pl, c, dH)) return -100 - step;
     WDS_TACC(c, 107);
     // dq = -P (r_l + dH[j2] - dH[j1]); ACTIVE PRVs from the downstream row
-    for (int k = lane; k < pl.L; k += 64) {
-      const int f = pl.l_flags[k];
-      const double b1 = (f & LF_J1) ? dH[pl.l_s1[k]] : 0.0;
-      const double b2 = (f & LF_J2) ? dH[pl.l_s2[k]] : 0.0;
-      c.P[k] = -c.P[k] * (c.Y[k] + (b2 - b1));  // P now holds dq
+    for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG8) {
+      int fv[kRowG8], s1[kRowG8], s2[kRowG8];
+      double Pv[kRowG8], Yv[kRowG8];
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) {
+        const int k = min(k0 + 64 * u, pl.L - 1);
+        fv[u] = pl.l_flags[k];
+        s1[u] = pl.l_s1[k];
+        s2[u] = pl.l_s2[k];
+        Pv[u] = c.P[k];
+        Yv[u] = c.Y[k];
+      }
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) {
+        const int k = k0 + 64 * u;
+        if (k >= pl.L) break;
+        const double b1 = (fv[u] & LF_J1) ? dH[s1[u]] : 0.0;
+        const double b2 = (fv[u] & LF_J2) ? dH[s2[u]] : 0.0;
+        c.P[k] = -Pv[u] * (Yv[u] + (b2 - b1));  // P now holds dq
+      }
     }
     mem_sync();
     bool any_act = false;
@@ -421,10 +483,35 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
         }
       mem_sync();
     }
-    for (int k = lane; k < pl.L; k += 64) c.q[k] = c.q[k] + c.P[k];
-    for (int j = lane; j < pl.Nj; j += 64) {
-      if (c.kej[j] > 0.0) c.e[j] = c.e[j] + (-c.F[j] + dH[j]) / c.hge[j];
-      c.Hn[j] = c.Hn[j] + dH[j];
+    for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG) {
+      double qv[kRowG], dv[kRowG];
+#pragma unroll
+      for (int u = 0; u < kRowG; ++u) {
+        const int k = min(k0 + 64 * u, pl.L - 1);
+        qv[u] = c.q[k];
+        dv[u] = c.P[k];
+      }
+#pragma unroll
+      for (int u = 0; u < kRowG; ++u)
+        if (k0 + 64 * u < pl.L) c.q[k0 + 64 * u] = qv[u] + dv[u];
+    }
+    for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+      double kv[kRowG8], ev[kRowG8], fv[kRowG8], hgv[kRowG8];
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) {
+        const int j = min(j0 + 64 * u, pl.Nj - 1);
+        kv[u] = c.kej[j];
+        ev[u] = c.e[j];
+        fv[u] = c.F[j];
+        hgv[u] = c.hge[j];
+      }
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) {
+        const int j = j0 + 64 * u;
+        if (j >= pl.Nj) break;
+        if (kv[u] > 0.0) c.e[j] = ev[u] + (-fv[u] + dH[j]) / hgv[u];
+        c.Hn[j] = c.Hn[j] + dH[j];
+      }
     }
     mem_sync();
     lds_sync();

@@ -291,6 +291,25 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       for (int32_t i = P->jadj_ptr[s]; i < P->jadj_ptr[s + 1]; ++i) put(P->jadj[i], -1);
       for (int32_t g = P->jg_ptr[s]; g < P->jg_ptr[s + 1]; ++g) put(P->jg[2 * g], P->jg[2 * g + 1]);
     }
+    P->ps_blk.assign(nb + 1, 0);
+    P->ps_len.assign(Nj, 0);
+    for (int s = 0; s < Nj; ++s) P->ps_len[s] = P->jadj_ptr[s + 1] - P->jadj_ptr[s];
+    for (int b = 0; b < nb; ++b) {
+      int32_t k = 0;
+      for (int s = b * kLanes; s < std::min(Nj, (b + 1) * kLanes); ++s) k = std::max(k, P->ps_len[s]);
+      P->ps_blk[b + 1] = P->ps_blk[b] + k;
+    }
+    P->ps_ell.assign((size_t)P->ps_blk[nb] * kLanes, 0);
+    for (int s = 0; s < Nj; ++s) {
+      const int b = s / kLanes, lane = s % kLanes;
+      int32_t e = 0;
+      auto put = [&](int32_t x) {
+        P->ps_ell[(size_t)(P->ps_blk[b] + e) * kLanes + lane] = x;
+        ++e;
+      };
+      for (int32_t i = P->jadj_mid[s]; i < P->jadj_ptr[s + 1]; ++i) put(P->jadj[i]);
+      for (int32_t i = P->jadj_ptr[s]; i < P->jadj_mid[s]; ++i) put(P->jadj[i]);
+    }
     const size_t noff = P->off_pos.size();
     P->off4.assign(noff * 4, 0);
     for (size_t o = 0; o < noff; ++o) {

@@ -32,6 +32,10 @@ struct wds_plan {
   // pads (0, -2).  off4: per off-diagonal entry (L position, first link,
   // first index into off_link, number of links)
   std::vector<int32_t> as_blk, as_ell, off4;
+  // the polish's mass rows in the same blocks: a row's links with n2 == row
+  // first, then those with n1 == row (jadj entries, side bit kept); pads 0
+  // with ps_len giving each row's entry count
+  std::vector<int32_t> ps_blk, ps_ell, ps_len;
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
   std::vector<int32_t> l_scale, jadj_mid;
