@@ -313,26 +313,6 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
   }
 }
 
-// y = A x over the symmetric rows (A values by L position, global memory)
-__device__ __forceinline__ double spmv_row(const DevPlan& pl, const double* A, const double* x,
-                                           int i) {
-  double s = 0.0;
-  const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
-  for (int eb = e0; eb < e1; eb += 4) {
-    double av[4], xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int2 en = pl.spmv_ent[min(eb + u, e1 - 1)];
-      av[u] = A[en.x];
-      xv[u] = x[en.y];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (eb + u < e1) s = fma(av[u], xv[u], s);
-  }
-  return s;
-}
-
 // b - (A x)_i evaluated in double-double: every product is split exactly
 // with an FMA (TwoProduct) and the running sum is carried as an unevaluated
 // pair with Knuth's TwoSum, so the residual is correct to about one rounding
@@ -342,31 +322,6 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
   s = a + b;
   const double bb = s - a;
   e = (a - (s - bb)) + (b - bb);
-}
-__device__ __forceinline__ double residual_row_dd(const DevPlan& pl, const double* A,
-                                                  const double* x, double b, int i) {
-  double hi = b, lo = 0.0;
-  const int e0 = pl.spmv_ptr[i], e1 = pl.spmv_ptr[i + 1];
-  for (int eb = e0; eb < e1; eb += 4) {
-    double av[4], xv[4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      const int2 en = pl.spmv_ent[min(eb + u, e1 - 1)];
-      av[u] = A[en.x];
-      xv[u] = x[en.y];
-    }
-#pragma unroll
-    for (int u = 0; u < 4; ++u) {
-      if (eb + u >= e1) break;
-      const double p = av[u] * xv[u];
-      const double pe = fma(av[u], xv[u], -p);
-      double s, err;
-      two_sum(hi, -p, s, err);
-      hi = s;
-      lo += err - pe;
-    }
-  }
-  return hi + lo;
 }
 
 // R_j *= V[diag_pos[j]] (the reciprocal pivots) for every row, V and R in
@@ -389,14 +344,13 @@ __device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, doubl
   }
 }
 
-// Every junction row at once over the row-block layout rs_blk / rs_ell:
-// two 64-row blocks per pass, kRowE entries of each loaded before any is
-// used (unconditionally, from clamped positions, so that no load waits for
-// another), then the matrix values: a pass costs two memory round trips
-// instead of three per row.  Entries past a row's end contribute exact
-// zeros (p = 0: two_sum(hi, -0) returns hi and error 0), and the entries
-// are taken in the order of residual_row_dd / spmv_row, so the sums are
-// bitwise the same.
+// Every junction row at once over the row-block layout rs_blk / rs_ell, in
+// whose slots A itself is stored: two 64-row blocks per pass, kRowE entries
+// of each (column words and matrix values together, unconditionally from
+// clamped slots, so that no load waits for another) loaded before any is
+// used: a pass costs one memory round trip.  Entries past a row's end
+// contribute exact zeros (p = 0: two_sum(hi, -0) returns hi and error 0),
+// and the entries are taken in each row's column order.
 constexpr int kRowE = 8;
 
 struct RowBlocks {
@@ -417,21 +371,20 @@ __device__ __forceinline__ void row_entries(const DevPlan& pl, const RowBlocks& 
                                             const double* A, const lds_f64* X,
                                             double (&av)[2][kRowE], double (&xv)[2][kRowE]) {
   const int lane = lane_id();
+  // A lives in the same slots as the entry words: both load in one round
+  // trip; then the vector reads, then the selects (left alone, the scheduler
+  // sinks each load to its use and the round trips serialise)
   int2 en[2][kRowE];
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < kRowE; ++u)
-      en[t][u] = pl.rs_ell[(rb.e0[t] + min(e + u, max(rb.K[t] - 1, 0))) * 64 + lane];
-  // all matrix-value loads, then all vector reads, then the selects: left
-  // alone, the scheduler sinks each load to its use and the round trips
-  // serialise
-  __builtin_amdgcn_sched_barrier(0);
   double a[2][kRowE], x[2][kRowE];
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
-    for (int u = 0; u < kRowE; ++u) a[t][u] = A[en[t][u].x];
+    for (int u = 0; u < kRowE; ++u) {
+      const int slot = (rb.e0[t] + min(e + u, max(rb.K[t] - 1, 0))) * 64 + lane;
+      en[t][u] = pl.rs_ell[slot];
+      a[t][u] = A[slot];
+    }
+  __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
   for (int t = 0; t < 2; ++t)
 #pragma unroll
@@ -444,12 +397,11 @@ __device__ __forceinline__ void row_entries(const DevPlan& pl, const RowBlocks& 
       av[t][u] = ok ? a[t][u] : 0.0;
       xv[t][u] = ok ? x[t][u] : 0.0;
     }
-  __builtin_amdgcn_sched_group_barrier(0x0020, 2 * kRowE, 0);
   __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kRowE, 0);
   __builtin_amdgcn_sched_barrier(0);
 }
 
-// dst_j = b_j - (A x)_j in double-double (residual_row_dd), x and dst in LDS
+// dst_j = b_j - (A x)_j in double-double, x and dst in LDS
 __device__ inline void residual_all(const DevPlan& pl, const double* A, const double* x,
                                     const double* b, double* dst) {
   const int lane = lane_id();
@@ -488,7 +440,7 @@ __device__ inline void residual_all(const DevPlan& pl, const double* A, const do
   }
 }
 
-// dst = A x (spmv_row for every row), x in LDS, dst in global memory
+// dst = A x, x in LDS, dst in global memory
 __device__ inline void spmv_all(const DevPlan& pl, const double* A, const double* x,
                                 double* dst) {
   const int lane = lane_id();

@@ -129,6 +129,15 @@ struct DevPlan {
   const int32_t* ps_blk;
   const int32_t* ps_ell;
   const int32_t* ps_len;
+  // A (global, per scenario) lives in the row-block layout of rs_ell: na
+  // doubles, off-diagonal values stored twice.  Slots of the diagonals, of
+  // each off-diagonal entry (pair), of each PRV's positions (off pair, d1,
+  // d2) and of each entry of the reference CSR pattern
+  int32_t na;
+  const int32_t* diag_ae;     // [Nj]
+  const int2* off_ae;         // [noff]
+  const int4* prv_ae;         // [p]
+  const int32_t* csr_to_ell;  // [nnz_csr]
   // PRVs with a setting (file order)
   const int32_t* prv_k;       // [p]
   const int32_t* prv_s1;      // [p] slot of n1

@@ -122,6 +122,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   size_t o_rb = bb.add(P->rs_blk), o_re = bb.add(P->rs_ell);
   size_t o_ab = bb.add(P->as_blk), o_ae = bb.add(P->as_ell), o_o4 = bb.add(P->off4);
   size_t o_pb = bb.add(P->ps_blk), o_pe = bb.add(P->ps_ell), o_pln = bb.add(P->ps_len);
+  size_t o_dae = bb.add(P->diag_ae), o_oae = bb.add(P->off_ae), o_pae = bb.add(P->prv_ae),
+         o_cae = bb.add(P->csr_to_ell);
   size_t o_pk = bb.add(P->prv_k), o_p1 = bb.add(P->prv_s1), o_p2 = bb.add(P->prv_s2),
          o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
@@ -206,6 +208,11 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.ps_blk = at<int32_t>(d, o_pb);
   D.ps_ell = at<int32_t>(d, o_pe);
   D.ps_len = at<int32_t>(d, o_pln);
+  D.na = (int32_t)std::max<size_t>(S.nnzL, P->rs_ell.size() / 2);
+  D.diag_ae = at<int32_t>(d, o_dae);
+  D.off_ae = at<int2>(d, o_oae);
+  D.prv_ae = at<int4>(d, o_pae);
+  D.csr_to_ell = at<int32_t>(d, o_cae);
   D.prv_k = at<int32_t>(d, o_pk);
   D.prv_s1 = at<int32_t>(d, o_p1);
   D.prv_s2 = at<int32_t>(d, o_p2);
@@ -259,7 +266,8 @@ extern "C" void wds_plan_destroy(wds_plan* P) {
 
 extern "C" int64_t wds_workspace_bytes(const wds_plan* P, int32_t B) {
   if (!P || B <= 0) return WDS_EINVAL;
-  return (int64_t)B * ws_doubles(P->L, P->Nj, (int32_t)P->sym.nnzL,
+  return (int64_t)B * ws_doubles(P->L, P->Nj,
+                                   (int32_t)std::max<size_t>(P->sym.nnzL, P->rs_ell.size() / 2),
                                    (int32_t)P->wp_node.size(), 2 * P->p) * 8;
 }
 

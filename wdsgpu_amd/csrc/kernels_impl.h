@@ -7,9 +7,9 @@ namespace wds {
 constexpr int64_t kMaxLds = 160 * 1024;
 
 // per-scenario global workspace, in doubles (wp_tot path nodes, np paths)
-__host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t nnzL,
+__host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t na,
                                               int32_t wp_tot, int32_t np) {
-  int64_t n = 2 * (int64_t)L + nnzL + 12 * (int64_t)Nj + 2 * (int64_t)wp_tot +
+  int64_t n = 2 * (int64_t)L + na + 12 * (int64_t)Nj + 2 * (int64_t)wp_tot +
               (int64_t)np * np + ((int64_t)L + 7) / 8 + 512;
   return (n + 31) / 32 * 32;
 }
@@ -140,7 +140,7 @@ struct Scn {
   // global workspace
   double* P;    // [L]
   double* Y;    // [L]
-  double* A;    // [nnzL] assembled values by L position
+  double* A;    // [na] assembled values in the row-block layout of rs_ell
   double* F;    // [Nj]
   double* e;    // [Nj] emitter flow by slot
   double* hle;  // [Nj]
@@ -177,13 +177,13 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.R = c.Hn + Nj + pl.Nf;
   c.X2 = c.R + Nj;
   c.sh = c.X2 + Nj;
-  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.nnzL, pl.wp_tot, pl.np);
+  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.na, pl.wp_tot, pl.np);
   c.P = w;
   w += L;
   c.Y = w;
   w += L;
   c.A = w;
-  w += pl.nnzL;
+  w += pl.na;
   c.F = w;
   w += Nj;
   c.e = w;
@@ -282,7 +282,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
   lds_f64* Vl = (lds_f64*)c.V;
   lds_f64* Rl = (lds_f64*)c.R;
   for (int b0 = 0; b0 < nb; b0 += 2) {
-    int e0[2], K[2], jr[2], dp[2];
+    int e0[2], K[2], jr[2], dp[2], da[2];
     double kv[2], hg[2], hl[2], ev[2], dj[2], el[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
@@ -298,6 +298,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
       dj[t] = c.dj[j];
       el[t] = pl.el_slot[j];
       dp[t] = pl.diag_pos[j];
+      da[t] = pl.diag_ae[j];
     }
     double dg[2] = {0.0, 0.0}, F[2] = {0.0, 0.0}, X[2] = {0.0, 0.0};
     const int Km = max(K[0], K[1]);
@@ -363,7 +364,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
         }
       }
       if (b0 + t < nb && jr[t] < pl.Nj) {
-        c.A[dp[t]] = dg[t];
+        c.A[da[t]] = dg[t];
         Vl[dp[t]] = dg[t];
         if (kF) {
           X[t] = X[t] - dj[t];
@@ -375,8 +376,12 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
   }
   for (int o0 = lane; o0 < pl.noff; o0 += 64 * kOffG) {
     int4 w[kOffG];
+    int2 wa[kOffG];
 #pragma unroll
-    for (int u = 0; u < kOffG; ++u) w[u] = pl.off4[min(o0 + 64 * u, pl.noff - 1)];
+    for (int u = 0; u < kOffG; ++u) {
+      w[u] = pl.off4[min(o0 + 64 * u, pl.noff - 1)];
+      wa[u] = pl.off_ae[min(o0 + 64 * u, pl.noff - 1)];
+    }
     __builtin_amdgcn_sched_barrier(0);
     double v[kOffG];
 #pragma unroll
@@ -393,7 +398,8 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
         if (Pe != 0.0) v[u] += -Pe;
       }
       if (o0 + 64 * u < pl.noff) {
-        c.A[w[u].x] = v[u];
+        c.A[wa[u].x] = v[u];
+        c.A[wa[u].y] = v[u];
         Vl[w[u].x] = v[u];
       }
     }
@@ -429,16 +435,19 @@ __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
       const double xf2 = c.R[j2];
       const double pna = pk_na * mna;
       const int po = pl.prv_pos_off[i], d1 = pl.diag_pos[j1], d2 = pl.diag_pos[j2];
+      const int4 ae = pl.prv_ae[i];
+      // V holds the same values as A here (k2_assemble writes both)
       double v;
-      v = c.A[po] + (-pna);
-      c.A[po] = v;
+      v = c.V[po] + (-pna);
+      c.A[ae.x] = v;
+      c.A[ae.y] = v;
       c.V[po] = v;
-      v = c.A[d1] + pna;
-      c.A[d1] = v;
+      v = c.V[d1] + pna;
+      c.A[ae.z] = v;
       c.V[d1] = v;
-      v = c.A[d2] + pna;
+      v = c.V[d2] + pna;
       v = v + CBIG * mact;
-      c.A[d2] = v;
+      c.A[ae.w] = v;
       c.V[d2] = v;
       const double dy = (yk_na - qk) * mna;
       const double f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
@@ -722,7 +731,6 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
     const double h = rh[pl.slot_node[pl.Nj + f]];
     c.Hn[pl.Nj + f] = (h != h) ? 0.0 : h;  // NaN reads as 0 (torch.nan_to_num)
   }
-  for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
 }
 
 // Stage cycle counters for a profiling build (-DWDS_PROFILE_STAGES): lane 0
@@ -819,6 +827,21 @@ __device__ inline void write_outputs(const DevPlan& pl, Scn& c, const SolveParam
 namespace wds {
 
 // ---------------------------------------------------------------------------
+// CSR values of scenario b into the factor slots (V) and the matrix copy
+// used by the refinement residual (A); fill-in positions start at zero
+__device__ inline void load_csr(const DevPlan& pl, Scn& c, const double* csr, int b,
+                                bool to_v) {
+  const int lane = lane_id();
+  const double* a = csr + (int64_t)b * pl.nnz_csr;
+  if (to_v)
+    for (int f = lane; f < pl.nfill; f += 64) c.V[pl.fill_pos[f]] = 0.0;
+  for (int e = lane; e < pl.nnz_csr; e += 64) {
+    const int t = pl.csr_to_L[e];
+    if (t >= 0 && to_v) c.V[t] = a[e];
+    c.A[pl.csr_to_ell[e]] = a[e];
+  }
+}
+
 __global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr,
                                                      const double* F, double* H, int n_refine,
                                                      double* factor_out, int32_t* info,
@@ -826,18 +849,7 @@ __global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
-  const double* a = csr + (int64_t)b * pl.nnz_csr;
-  for (int f = lane; f < pl.nfill; f += 64) {
-    c.V[pl.fill_pos[f]] = 0.0;
-    c.A[pl.fill_pos[f]] = 0.0;
-  }
-  for (int e = lane; e < pl.nnz_csr; e += 64) {
-    const int t = pl.csr_to_L[e];
-    if (t >= 0) {
-      c.V[t] = a[e];
-      c.A[t] = a[e];
-    }
-  }
+  load_csr(pl, c, csr, b, true);
   const double* Fb = F + (int64_t)b * pl.Nj;
   for (int r = lane; r < pl.Nj; r += 64) c.F[pl.row_slot[r]] = Fb[r];
   if (lane == 0) c.V[pl.nnzL] = 0.0;
@@ -854,25 +866,6 @@ __global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr
   double* Hb = H + (int64_t)b * pl.Nj;
   for (int r = lane; r < pl.Nj; r += 64) Hb[r] = c.Hn[pl.row_slot[r]];
   if (lane == 0) info[b] = 0;
-}
-
-// CSR values of scenario b into the factor slots (V) and the matrix copy
-// used by the refinement residual (A); fill-in positions start at zero
-__device__ inline void load_csr(const DevPlan& pl, Scn& c, const double* csr, int b,
-                                bool to_v) {
-  const int lane = lane_id();
-  const double* a = csr + (int64_t)b * pl.nnz_csr;
-  for (int f = lane; f < pl.nfill; f += 64) {
-    if (to_v) c.V[pl.fill_pos[f]] = 0.0;
-    c.A[pl.fill_pos[f]] = 0.0;
-  }
-  for (int e = lane; e < pl.nnz_csr; e += 64) {
-    const int t = pl.csr_to_L[e];
-    if (t >= 0) {
-      if (to_v) c.V[t] = a[e];
-      c.A[t] = a[e];
-    }
-  }
 }
 
 // Numeric factorisation only (the reference's cuDSS factorize phase,

@@ -54,8 +54,8 @@ __device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
     for (int i = 0; i < pl.p; ++i) {
       if (c.sh[i] != 0.0) {
         const int d2 = pl.diag_pos[pl.prv_s2[i]];
-        const double v = c.A[d2] + CBIG;
-        c.A[d2] = v;
+        const double v = c.V[d2] + CBIG;  // V holds A's values here
+        c.A[pl.prv_ae[i].w] = v;
         c.V[d2] = v;
       }
     }
@@ -597,7 +597,6 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
     c.kej[j] = ke[node];
     c.e[j] = E[node];
   }
-  for (int f = lane; f < pl.nfill; f += 64) c.A[pl.fill_pos[f]] = 0.0;
   mem_sync();
   lds_sync();
   // coefficients at the converged state; assemble A-hat (values for the

@@ -339,6 +339,33 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   P->p = (int32_t)P->prv_k.size();
   if (P->p > kMaxPrv)
     throw std::domain_error("more than " + std::to_string(kMaxPrv) + " PRVs with a setting");
+  // ---- slots of A in the row-block layout ----
+  {
+    const size_t nell = P->rs_ell.size() / 2;
+    P->ae_pos.assign(2 * (size_t)S.nnzL, -1);
+    for (size_t x = 0; x < nell; ++x) {
+      const int32_t pos = P->rs_ell[2 * x], col = P->rs_ell[2 * x + 1];
+      if (col < 0) continue;
+      int32_t* a = &P->ae_pos[2 * (size_t)pos];
+      if (a[0] < 0) a[0] = (int32_t)x;
+      else if (a[1] < 0) a[1] = (int32_t)x;
+      else throw std::logic_error("an entry of A in more than two rows");
+    }
+    P->diag_ae.resize(Nj);
+    for (int s = 0; s < Nj; ++s) P->diag_ae[s] = P->ae_pos[2 * (size_t)P->diag_pos[s]];
+    P->off_ae.resize(2 * P->off_pos.size());
+    for (size_t o = 0; o < P->off_pos.size(); ++o) {
+      P->off_ae[2 * o] = P->ae_pos[2 * (size_t)P->off_pos[o]];
+      P->off_ae[2 * o + 1] = P->ae_pos[2 * (size_t)P->off_pos[o] + 1];
+    }
+    P->prv_ae.assign(4 * (size_t)P->p, -1);
+    for (int i = 0; i < P->p; ++i) {
+      P->prv_ae[4 * i] = P->ae_pos[2 * (size_t)P->prv_pos_off[i]];
+      P->prv_ae[4 * i + 1] = P->ae_pos[2 * (size_t)P->prv_pos_off[i] + 1];
+      P->prv_ae[4 * i + 2] = P->diag_ae[P->prv_s1[i]];
+      P->prv_ae[4 * i + 3] = P->diag_ae[P->prv_s2[i]];
+    }
+  }
   // ---- elimination-tree paths of the valve ends ----
   // The Woodbury corrections of the polish and adjoint need A-hat^-1 e_u only
   // through y_u = L^-1 e_u, whose nonzeros lie on the etree path from u to its
@@ -436,6 +463,22 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     P->csr_to_L[e] = (x == y) ? S.colptr[sx] : (sx > sy ? (int32_t)S.pos(sx, sy) : -1);
   }
   for (int r = 0; r < Nj; ++r) P->csr_indptr[r + 1] += P->csr_indptr[r];
+  // slot of each CSR entry in the row-block layout of A
+  P->csr_to_ell.assign(P->csr_indices.size(), -1);
+  for (int r = 0; r < Nj; ++r) {
+    const int sr = P->row_slot[r], b = sr / kLanes, lane = sr % kLanes;
+    for (int32_t e = P->csr_indptr[r]; e < P->csr_indptr[r + 1]; ++e) {
+      const int sc = P->row_slot[P->csr_indices[e]];
+      for (int32_t k = P->rs_blk[b]; k < P->rs_blk[b + 1]; ++k) {
+        const size_t x = (size_t)k * kLanes + lane;
+        if (P->rs_ell[2 * x + 1] == sc) {
+          P->csr_to_ell[e] = (int32_t)x;
+          break;
+        }
+      }
+      if (P->csr_to_ell[e] < 0) throw std::logic_error("CSR entry outside the pattern of A");
+    }
+  }
   // ---- scalars ----
   P->headloss = d->headloss;
   P->hexp = d->hexp;

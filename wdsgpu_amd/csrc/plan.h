@@ -36,6 +36,12 @@ struct wds_plan {
   // first, then those with n1 == row (jadj entries, side bit kept); pads 0
   // with ps_len giving each row's entry count
   std::vector<int32_t> ps_blk, ps_ell, ps_len;
+  // the assembled matrix A is stored in the row-block layout of rs_ell (each
+  // off-diagonal value twice): ae_pos[t] the one or two slots of L position
+  // t (-1 none), diag_ae / off_ae / prv_ae those of the diagonals, the
+  // off-diagonal entries and the PRV positions (off pair, d1, d2), csr_to_ell
+  // the slot of each entry of the reference CSR pattern
+  std::vector<int32_t> ae_pos, diag_ae, off_ae, prv_ae, csr_to_ell;
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
   std::vector<int32_t> l_scale, jadj_mid;
