@@ -591,11 +591,28 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
   auto GH = [&](int n) { return gH ? gH[n] : 0.0; };
   auto GE = [&](int n) { return gE ? gE[n] : 0.0; };
   auto GQ = [&](int k) { return gQ ? gQ[k] : 0.0; };
-  for (int s = lane; s < pl.Nj + pl.Nf; s += 64) c.Hn[s] = H[pl.slot_node[s]];
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const int node = pl.slot_node[j];
-    c.kej[j] = ke[node];
-    c.e[j] = E[node];
+  const int ns = pl.Nj + pl.Nf;
+  for (int s0 = lane; s0 < ns; s0 += 64 * kRowG8) {
+    int nd[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) nd[u] = pl.slot_node[min(s0 + 64 * u, ns - 1)];
+    double hv[kRowG8], kv[kRowG8], ev[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      hv[u] = H[nd[u]];
+      kv[u] = ke[nd[u]];
+      ev[u] = E[nd[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int sl = s0 + 64 * u;
+      if (sl >= ns) break;
+      c.Hn[sl] = hv[u];
+      if (sl < pl.Nj) {
+        c.kej[sl] = kv[u];
+        c.e[sl] = ev[u];
+      }
+    }
   }
   mem_sync();
   lds_sync();
@@ -607,21 +624,41 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
   for (int t = lane; t < pl.nnzL; t += 64) c.V[t] = fac[t];
   if (lane == 0) c.V[pl.nnzL] = 0.0;
   lds_sync();
-  // r = B^T P gQ - em gE / hge - gH (n2 side first, reference scatter order)
-  for (int j = lane; j < pl.Nj; j += 64) {
-    double r = 0.0, unused = 0.0;
-    const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1], em = pl.jadj_mid[j];
-    if (gQ) {
-      row_gather(pl, em, e1, 1.0, c.P, c.P, gQ, unused, r);
-      row_gather(pl, e0, em, -1.0, c.P, c.P, gQ, unused, r);
-    } else {
-      // a zero cotangent still adds (+-0.0) terms: keep r exactly 0.0
+  // r = B^T P gQ - em gE / hge - gH (n2 side first, reference scatter order);
+  // rows in groups of kRowG8 per lane with their loads issued together
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    int nd[kRowG8];
+    double kv[kRowG8], hgv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = min(j0 + 64 * u, pl.Nj - 1);
+      nd[u] = pl.slot_node[j];
+      kv[u] = c.kej[j];
+      hgv[u] = c.hge[j];
     }
-    const int node = pl.slot_node[j];
-    const double emh = c.kej[j] > 0.0 ? 1.0 / c.hge[j] : 0.0;
-    r = r - emh * GE(node);
-    r = r - GH(node);
-    c.v1[j] = r;
+    double gev[kRowG8], ghv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      gev[u] = GE(nd[u]);
+      ghv[u] = GH(nd[u]);
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j >= pl.Nj) break;
+      double r = 0.0, unused = 0.0;
+      if (gQ) {
+        const int e0 = pl.jadj_ptr[j], e1 = pl.jadj_ptr[j + 1], em = pl.jadj_mid[j];
+        row_gather(pl, em, e1, 1.0, c.P, c.P, gQ, unused, r);
+        row_gather(pl, e0, em, -1.0, c.P, c.P, gQ, unused, r);
+      } else {
+        // a zero cotangent still adds (+-0.0) terms: keep r exactly 0.0
+      }
+      const double emh = kv[u] > 0.0 ? 1.0 / hgv[u] : 0.0;
+      r = r - emh * gev[u];
+      r = r - ghv[u];
+      c.v1[j] = r;
+    }
   }
   if (lane < pl.p) c.sprv[lane] = GQ(pl.prv_k[lane]);
   mem_sync();
@@ -637,42 +674,76 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
   double* grh = ap.grh + (int64_t)b * pl.N;
   double* gr = ap.gr + (int64_t)b * pl.L;
   double* laml = c.Y;  // lambda_l per link
-  for (int k = lane; k < pl.L; k += 64) {
-    const int f = pl.l_flags[k];
-    const double l1 = (f & LF_J1) ? lam[pl.l_s1[k]] : 0.0;
-    const double l2 = (f & LF_J2) ? lam[pl.l_s2[k]] : 0.0;
-    const double ll = c.P[k] * (GQ(k) - (l2 - l1));
-    laml[k] = ll;
-    // dr = d hloss / d r (open pipe; H-W unclamped)
-    double dr = 0.0;
-    const int8_t s = c.S[k];
-    const bool closed_now = (s <= ST_CLOSED || (f & LF_ZERO_SPEED)) && !(f & LF_PCV);
-    if ((f & LF_PIPE) && !closed_now && pl.headloss != 0) {
-      dr = dw_dr(pl, c.q[k], pl.l_diam[k], pl.l_kc[k]);
-    } else if ((f & LF_PIPE) && !closed_now) {
-      const double q = c.q[k];
-      const double sgn = q < 0.0 ? -1.0 : 1.0;
-      const double qa = q * sgn;
-      const double hgr = pl.hexp * c.r[k] * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
-      if (hgr >= pl.rqtol) dr = pow(qa, pl.hexp) * sgn;
+  for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG8) {
+    int fv[kRowG8], s1[kRowG8], s2[kRowG8];
+    int8_t sv[kRowG8];
+    double Pv[kRowG8], qv[kRowG8], rv[kRowG8], gq[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int k = min(k0 + 64 * u, pl.L - 1);
+      fv[u] = pl.l_flags[k];
+      s1[u] = pl.l_s1[k];
+      s2[u] = pl.l_s2[k];
+      sv[u] = c.S[k];
+      Pv[u] = c.P[k];
+      qv[u] = c.q[k];
+      rv[u] = c.r[k];
+      gq[u] = GQ(k);
     }
-    gr[k] = -(ll * dr);
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int k = k0 + 64 * u;
+      if (k >= pl.L) break;
+      const int f = fv[u];
+      const double l1 = (f & LF_J1) ? lam[s1[u]] : 0.0;
+      const double l2 = (f & LF_J2) ? lam[s2[u]] : 0.0;
+      const double ll = Pv[u] * (gq[u] - (l2 - l1));
+      laml[k] = ll;
+      // dr = d hloss / d r (open pipe; H-W unclamped)
+      double dr = 0.0;
+      const int8_t st = sv[u];
+      const bool closed_now = (st <= ST_CLOSED || (f & LF_ZERO_SPEED)) && !(f & LF_PCV);
+      if ((f & LF_PIPE) && !closed_now && pl.headloss != 0) {
+        dr = dw_dr(pl, qv[u], pl.l_diam[k], pl.l_kc[k]);
+      } else if ((f & LF_PIPE) && !closed_now) {
+        const double q = qv[u];
+        const double sgn = q < 0.0 ? -1.0 : 1.0;
+        const double qa = q * sgn;
+        const double hgr = pl.hexp * rv[u] * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+        if (hgr >= pl.rqtol) dr = pow(qa, pl.hexp) * sgn;
+      }
+      gr[k] = -(ll * dr);
+    }
   }
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const int node = pl.slot_node[j];
-    gd[node] = lam[j];
-    double g = 0.0;
-    const double kv = c.kej[j];
-    if (kv > 0.0) {
-      const double lame = (GE(node) + lam[j]) / c.hge[j];
-      const double ea = fabs(c.e[j]);
-      const double pw = pow(fmax(ea, 1e-30), pl.qexp - 1.0);
-      const double hraw = pl.qexp * fmax(kv, CSMALL) * pw;
-      const double dke = (hraw < pl.rqtol || kv < CSMALL) ? 0.0 : pw * c.e[j];
-      g = -(lame * dke);
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    int nd[kRowG8];
+    double kv[kRowG8], hgv[kRowG8], ev[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = min(j0 + 64 * u, pl.Nj - 1);
+      nd[u] = pl.slot_node[j];
+      kv[u] = c.kej[j];
+      hgv[u] = c.hge[j];
+      ev[u] = c.e[j];
     }
-    gke[node] = g;
-    grh[node] = 0.0;
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j >= pl.Nj) break;
+      const int node = nd[u];
+      gd[node] = lam[j];
+      double g = 0.0;
+      if (kv[u] > 0.0) {
+        const double lame = (GE(node) + lam[j]) / hgv[u];
+        const double ea = fabs(ev[u]);
+        const double pw = pow(fmax(ea, 1e-30), pl.qexp - 1.0);
+        const double hraw = pl.qexp * fmax(kv[u], CSMALL) * pw;
+        const double dke = (hraw < pl.rqtol || kv[u] < CSMALL) ? 0.0 : pw * ev[u];
+        g = -(lame * dke);
+      }
+      gke[node] = g;
+      grh[node] = 0.0;
+    }
   }
   mem_sync();
   for (int f = lane; f < pl.Nf; f += 64) {
