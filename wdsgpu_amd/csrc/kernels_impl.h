@@ -257,7 +257,6 @@ __device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c) {
 }
 
 __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
-  const int lane = lane_id();
   link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
   emitter_coeffs(pl, c);
   mem_sync();
@@ -722,10 +721,23 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
   const double* d = sp.d + (int64_t)b * pl.N;
   const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
   const double* rh = sp.rh + (int64_t)b * pl.N;
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const int node = pl.slot_node[j];
-    c.dj[j] = d[node];
-    c.kej[j] = ke[node];
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    int nd[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) nd[u] = pl.slot_node[min(j0 + 64 * u, pl.Nj - 1)];
+    double dv[kRowG8], kv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      dv[u] = d[nd[u]];
+      kv[u] = ke[nd[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j >= pl.Nj) break;
+      c.dj[j] = dv[u];
+      c.kej[j] = kv[u];
+    }
   }
   for (int f = lane; f < pl.Nf; f += 64) {
     const double h = rh[pl.slot_node[pl.Nj + f]];
@@ -813,10 +825,24 @@ __device__ inline void write_outputs(const DevPlan& pl, Scn& c, const SolveParam
   const int lane = lane_id();
   double* H = sp.head + (int64_t)b * pl.N;
   double* E = sp.emitter + (int64_t)b * pl.N;
-  for (int n = lane; n < pl.N; n += 64) {
-    const int s = pl.node_slot[n];
-    H[n] = c.Hn[s];
-    E[n] = (s < pl.Nj && c.kej[s] > 0.0) ? c.e[s] : 0.0;
+  for (int n0 = lane; n0 < pl.N; n0 += 64 * kRowG8) {
+    int sl[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) sl[u] = pl.node_slot[min(n0 + 64 * u, pl.N - 1)];
+    double kv[kRowG8], ev[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = min(sl[u], pl.Nj - 1);
+      kv[u] = c.kej[j];
+      ev[u] = c.e[j];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int n = n0 + 64 * u;
+      if (n >= pl.N) break;
+      H[n] = c.Hn[sl[u]];
+      E[n] = (sl[u] < pl.Nj && kv[u] > 0.0) ? ev[u] : 0.0;
+    }
   }
 }
 
@@ -905,22 +931,57 @@ __global__ WDS_SCENARIO_KERNEL void k_trisolve(DevPlan pl, const double* factor,
   for (int r = lane; r < pl.Nj; r += 64) Hb[r] = c.Hn[pl.row_slot[r]];
 }
 
+// initial flows, statuses, emitter flows and junction heads of scenario b
+// (warm starts when given), loads batched
+__device__ inline void init_state(const DevPlan& pl, Scn& c, const SolveParams& sp, int b) {
+  const int lane = lane_id();
+  const double* q0 = sp.q0 ? sp.q0 + (int64_t)b * pl.L : pl.l_q0;
+  const int8_t* s0 = sp.status0 ? sp.status0 + (int64_t)b * pl.L : pl.l_st0;
+  for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG) {
+    double qv[kRowG];
+    int8_t sv[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) {
+      const int k = min(k0 + 64 * u, pl.L - 1);
+      qv[u] = q0[k];
+      sv[u] = s0[k];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) {
+      const int k = k0 + 64 * u;
+      if (k >= pl.L) break;
+      c.q[k] = qv[u];
+      c.S[k] = sv[u];
+    }
+  }
+  const double* e0 = sp.e0 ? sp.e0 + (int64_t)b * pl.N : nullptr;
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    int nd[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) nd[u] = pl.slot_node[min(j0 + 64 * u, pl.Nj - 1)];
+    double ev[kRowG8], kv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      ev[u] = e0 ? e0[nd[u]] : 0.0;
+      kv[u] = c.kej[min(j0 + 64 * u, pl.Nj - 1)];
+    }
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) {
+      const int j = j0 + 64 * u;
+      if (j >= pl.Nj) break;
+      c.e[j] = e0 ? ev[u] : (kv[u] > 0.0 ? 1.0 : 0.0);
+      c.Hn[j] = 0.0;
+    }
+  }
+}
+
 // Fused forward: the whole Newton loop of scenario blockIdx.x.
 __global__ WDS_SCENARIO_KERNEL void k_solve(DevPlan pl, SolveParams sp) {
   extern __shared__ double lds[];
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
   load_inputs(pl, c, sp, b);
-  const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
-  for (int k = lane; k < pl.L; k += 64) {
-    c.q[k] = sp.q0 ? sp.q0[(int64_t)b * pl.L + k] : pl.l_q0[k];
-    c.S[k] = sp.status0 ? sp.status0[(int64_t)b * pl.L + k] : pl.l_st0[k];
-  }
-  for (int j = lane; j < pl.Nj; j += 64) {
-    const int node = pl.slot_node[j];
-    c.e[j] = sp.e0 ? sp.e0[(int64_t)b * pl.N + node] : (ke[node] > 0.0 ? 1.0 : 0.0);
-    c.Hn[j] = 0.0;
-  }
+  init_state(pl, c, sp, b);
 #ifdef WDS_PROFILE_STAGES
   for (int i = lane; i < 112; i += 64) c.misc[400 + i] = 0.0;
 #endif
