@@ -145,7 +145,8 @@ int wds_trisolve(wds_plan* plan, void* stream, int32_t B, const double* factor,
 typedef struct wds_solve_args {
   int32_t B;
   const double* demand;        /* [B][N] cfs */
-  const double* res_head;      /* [B][N] ft at fixed-head nodes (junction entries ignored) */
+  const double* res_head;      /* [B][N], or [N] with res_head_shared: ft at fixed-head
+                                  nodes (junction entries ignored) */
   const double* ke;            /* [B][N] or [N] emitter coefficients */
   int32_t ke_stride;           /* N (per scenario) or 0 (shared) */
   const double* r_hw;          /* [B][L] or [L] resistance */
@@ -185,6 +186,7 @@ typedef struct wds_solve_args {
   int32_t* sched_events;
   int32_t* sched_count;
   int32_t sched_cap;
+  int32_t res_head_shared;     /* 1: res_head is one [N] vector for every scenario */
 } wds_solve_args;
 
 /* Fused forward: the whole Newton loop of every scenario in one persistent

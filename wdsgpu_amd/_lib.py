@@ -66,7 +66,8 @@ class SolveArgs(C.Structure):
                 ("diag_ratio", C.c_void_p), ("info", C.c_void_p), ("resid", C.c_void_p),
                 ("factor", C.c_void_p), ("workspace", C.c_void_p), ("status0", C.c_void_p),
                 ("sched_flags", C.c_void_p), ("sched_events", C.c_void_p),
-                ("sched_count", C.c_void_p), ("sched_cap", C.c_int32)]
+                ("sched_count", C.c_void_p), ("sched_cap", C.c_int32),
+                ("res_head_shared", C.c_int32)]
 
 
 class AdjointArgs(C.Structure):

@@ -54,7 +54,6 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             rh = rh.unsqueeze(0)
         B = max(d.shape[0], rh.shape[0], ket.shape[0] if ket.dim() == 2 else 1)
         d = d.expand(B, -1)
-        rh = rh.expand(B, -1)
         a, out, keep, single, B = s._args(d, rh, ket, None, None, max_iter, accuracy, sm,
                                           r_hw=rt, polish_steps=int(polish_steps),
                                           want_factor=True)
@@ -136,18 +135,21 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
                        "wds_adjoint")
         binfo = s._pinned_info(B)
         binfo.copy_(info, non_blocking=True)
-        torch.cuda.current_stream(dev).synchronize()
-        if (binfo.numpy() != 0).any():
-            bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
-            raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
         dm, dv = ctx.dims, ctx.in_devs
 
         def _fit(g, dim, dev_):
             g = g.to(device=dev_)
             return g.sum(0) if dim == 1 else g
 
-        return (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
-                _fit(gr, dm[3], dv[3]), None, None, None, None, None, None, None)
+        # the batch reductions are queued behind the adjoint before the host
+        # waits for its status
+        grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
+                 _fit(gr, dm[3], dv[3]))
+        torch.cuda.current_stream(dev).synchronize()
+        if (binfo.numpy() != 0).any():
+            bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
+            raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
+        return grads + (None, None, None, None, None, None, None)
 
 
 def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,

@@ -427,7 +427,7 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
   }
   r = check_sched(P, a);
   if (r) return r;
-  SolveParams sp = make_params(a);
+  SolveParams sp = make_params(a, P->N);
   const int64_t lds = P->stats.lds_bytes;
   set_lds(k_solve, lds);
   hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
@@ -449,7 +449,7 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
   }
   r = check_sched(P, a);
   if (r) return r;
-  SolveParams sp = make_params(a);
+  SolveParams sp = make_params(a, P->N);
   const int64_t lds = P->stats.lds_bytes;
   set_lds(k_newton_iter, lds);
   hipLaunchKernelGGL(k_newton_iter, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp,

@@ -44,9 +44,10 @@ struct SolveParams {
   int32_t* sched_events;
   int32_t* sched_count;
   int32_t sched_cap;
+  int32_t rh_stride;  // N, or 0 for one vector shared by every scenario
 };
 
-inline SolveParams make_params(const wds_solve_args* a) {
+inline SolveParams make_params(const wds_solve_args* a, int32_t N) {
   SolveParams s;
   s.d = a->demand;
   s.rh = a->res_head;
@@ -77,6 +78,7 @@ inline SolveParams make_params(const wds_solve_args* a) {
   s.sched_events = a->sched_events;
   s.sched_count = a->sched_count;
   s.sched_cap = a->sched_cap;
+  s.rh_stride = a->res_head_shared ? 0 : N;
   return s;
 }
 
@@ -720,7 +722,7 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
   const int lane = lane_id();
   const double* d = sp.d + (int64_t)b * pl.N;
   const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
-  const double* rh = sp.rh + (int64_t)b * pl.N;
+  const double* rh = sp.rh + (int64_t)b * sp.rh_stride;
   for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
     int nd[kRowG8];
 #pragma unroll

@@ -412,14 +412,13 @@ class GGASolver:
 
     # ------------------------------------------------------------------
     def _host_status(self, out, B):
-        """info and iters of a solve on the host: two asynchronous copies into
-        a cached pinned buffer and one stream synchronisation."""
+        """info and iters of a solve on the host: one asynchronous copy into a
+        cached pinned buffer and one stream synchronisation."""
         buf = self._pinned.get(B)
         if buf is None:
             buf = torch.empty(2 * B, dtype=torch.int32, pin_memory=True)
             self._pinned = {B: buf}
-        buf[:B].copy_(out["info"], non_blocking=True)
-        buf[B:].copy_(out["iters"], non_blocking=True)
+        buf.copy_(out["st"], non_blocking=True)
         torch.cuda.current_stream(self.device).synchronize()
         return buf[:B].numpy(), buf[B:].numpy()
 
@@ -441,11 +440,12 @@ class GGASolver:
         if rh.dim() == 1:
             rh = rh.unsqueeze(0)
         B = int(d.shape[0])
-        if rh.shape[0] == 1 and B > 1:
-            rh = rh.expand(B, -1)
-        if d.shape[1] != self.N or rh.shape != (B, self.N):
+        if d.shape[1] != self.N or rh.shape[-1] != self.N or rh.shape[0] not in (1, B):
             raise ValueError("demand/res_head must be [N] or [B,N] with N=%d" % self.N)
-        rh = rh.contiguous()  # NaN heads (junction entries) read as 0 in the kernel
+        # one shared vector is passed as is (no [B,N] copy); NaN heads
+        # (junction entries) read as 0 in the kernel
+        rh_shared = rh.shape[0] == 1 and B > 1
+        rh = rh.contiguous()
         ke = self.node_ke_default if ke_int is None else self._prep(ke_int)
         if ke.dim() == 2 and ke.shape[0] == 1:
             ke = ke[0]
@@ -466,14 +466,15 @@ class GGASolver:
             head=torch.empty(B, self.N, dtype=self.dtype, device=dev),
             flow=torch.empty(B, self.L, dtype=self.dtype, device=dev),
             emitter=torch.empty(B, self.N, dtype=self.dtype, device=dev),
-            iters=scalar(B, dtype=torch.int32, device=dev),
+            st=scalar(2 * B, dtype=torch.int32, device=dev),
             relerr=scalar(B, dtype=self.dtype, device=dev),
             status=torch.empty(B, self.L, dtype=torch.int8, device=dev),
             diag_ratio=scalar(B, dtype=self.dtype, device=dev),
-            info=scalar(B, dtype=torch.int32, device=dev),
             resid=scalar(B, dtype=self.dtype, device=dev),
             factor=torch.empty(B, int(self.stats["nnzL"]), dtype=self.dtype, device=dev)
             if want_factor else None)
+        # info and iters share one buffer: one copy to the host reads both
+        out["info"], out["iters"] = out["st"][:B], out["st"][B:]
         q0t = None if q0 is None else self._prep(q0).expand(B, -1).contiguous()
         e0t = None if e0 is None else self._prep(e0).expand(B, -1).contiguous()
         s0t = None
@@ -490,6 +491,7 @@ class GGASolver:
         keep = (d.contiguous(), rh, ke, r, q0t, e0t, ws, s0t)
         a = _lib.SolveArgs(
             B=B, demand=_lib.ptr(keep[0]), res_head=_lib.ptr(rh), ke=_lib.ptr(ke),
+            res_head_shared=int(rh_shared),
             ke_stride=ke_stride, r_hw=_lib.ptr(r), r_stride=r_stride, q0=_lib.ptr(q0t),
             e0=_lib.ptr(e0t), max_iter=int(max_iter), accuracy=float(accuracy),
             status_machine=int(bool(status_machine)), n_refine=int(self.dense_refine),
