@@ -20,6 +20,18 @@ from . import _lib
 from .network import LPS_PER_CFS, M_PER_FT
 
 
+def _prv_topology(s):
+    """(several PRVs share a downstream junction, a PRV feeds another PRV's
+    upstream node) for the solver's network, computed once per solver."""
+    t = getattr(s, "_adj_prv_topology", None)
+    if t is None:
+        j2 = np.asarray(s.prv_n2)
+        j1 = np.asarray(s.prv_n1)
+        t = (bool(np.unique(j2).size < j2.size), bool(np.intersect1d(j2, j1).size))
+        s._adj_prv_topology = t
+    return t
+
+
 class ImplicitGGASolveGPU(torch.autograd.Function):
     """(demand, res_head, ke, r_hw, solver, accuracy, max_iter,
     status_machine, assemble, linear_solver, polish_steps) ->
@@ -33,12 +45,11 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         if s.dtype != torch.float64:
             raise NotImplementedError("adjoint='gpu' is float64 only")
         if s.n_pcv:
-            j2 = np.asarray(s.prv_n2)
-            j1 = np.asarray(s.prv_n1)
-            if np.unique(j2).size < j2.size:
+            shared, cascaded = _prv_topology(s)
+            if shared:
                 raise NotImplementedError(
                     "adjoint='gpu': several PRVs share a downstream junction")
-            if np.intersect1d(j2, j1).size:
+            if cascaded:
                 raise NotImplementedError("adjoint='gpu': cascaded PRVs")
         dims = (demand.dim(), res_head.dim(), ke.dim(), r_hw.dim())
         sm = bool(s.dense_status_machine) if status_machine is None else bool(status_machine)
@@ -198,7 +209,12 @@ class _HWResistance(torch.autograd.Function):
         dt = _hip_tensor(diam_ft, "hw_resistance")
         if Ct.shape[-1] != L or dt.numel() != L:
             raise ValueError("hw_resistance: C [..., L] with len/diam [L]")
-        mk = None if mask is None else mask.to(device=Ct.device, dtype=torch.int8).contiguous()
+        if mask is None:
+            mk = None
+        elif mask.dtype == torch.bool and mask.device == Ct.device:
+            mk = mask.contiguous().view(torch.int8)  # same bytes, 0/1: no conversion kernel
+        else:
+            mk = mask.to(device=Ct.device, dtype=torch.int8).contiguous()
         r = torch.empty_like(Ct)
         lib = _lib.load()
         with torch.cuda.device(Ct.device):

@@ -345,99 +345,101 @@ __device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, doubl
 }
 
 // Every junction row at once over the row-block layout rs_blk / rs_ell, in
-// whose slots A itself is stored: two 64-row blocks per pass, kRowE entries
-// of each (column words and matrix values together, unconditionally from
-// clamped slots, so that no load waits for another) loaded before any is
-// used: a pass costs one memory round trip.  Entries past a row's end
-// contribute exact zeros (p = 0: two_sum(hi, -0) returns hi and error 0),
-// and the entries are taken in each row's column order.
-constexpr int kRowE = 8;
+// whose slots A itself is stored.  A lane's slots form one flat sequence:
+// block b (rows 64 b + lane) owns slots rs_blk[b] .. rs_blk[b + 1] - 1, and
+// the block boundaries are the same in every lane.  A sweep loads kSweep
+// slots at a time (column words and matrix values, unconditionally from
+// clamped slots, so that no load waits for another), then walks them in
+// order, closing a row and opening the next at each block boundary with a
+// uniform branch; the next boundary and the next row's first value are read
+// one block ahead (every block holds at least one slot: each row has its
+// diagonal).  Entries past a row's end (column -1) contribute exact
+// zeros (p = 0: two_sum(hi, -0) returns hi and error 0), and the entries are
+// taken in each row's column order.
+constexpr int kSweep = 16;
 
-struct RowBlocks {
-  int e0[2], K[2];
-};
-__device__ __forceinline__ RowBlocks row_blocks(const DevPlan& pl, int b0, int nb) {
-  RowBlocks r;
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int bk = min(b0 + t, nb - 1);
-    r.e0[t] = pl.rs_blk[bk];
-    r.K[t] = b0 + t < nb ? pl.rs_blk[bk + 1] - r.e0[t] : 0;
-  }
-  return r;
-}
-// entries e .. e + kRowE - 1 of both blocks and their matrix / vector values
-__device__ __forceinline__ void row_entries(const DevPlan& pl, const RowBlocks& rb, int e,
-                                            const double* A, const lds_f64* X,
-                                            double (&av)[2][kRowE], double (&xv)[2][kRowE]) {
+// column words and matrix values of slots s0 .. s0 + kSweep - 1 (clamped to
+// the last slot E - 1)
+__device__ __forceinline__ void sweep_load(const DevPlan& pl, const double* A, int s0, int E,
+                                           int (&cl)[kSweep], double (&av)[kSweep]) {
   const int lane = lane_id();
-  // A lives in the same slots as the entry words: both load in one round
-  // trip; then the vector reads, then the selects (left alone, the scheduler
-  // sinks each load to its use and the round trips serialise)
-  int2 en[2][kRowE];
-  double a[2][kRowE], x[2][kRowE];
+  const int* cw = (const int*)pl.rs_ell;
 #pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < kRowE; ++u) {
-      const int slot = (rb.e0[t] + min(e + u, max(rb.K[t] - 1, 0))) * 64 + lane;
-      en[t][u] = pl.rs_ell[slot];
-      a[t][u] = A[slot];
-    }
-  __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < kRowE; ++u) x[t][u] = X[max(en[t][u].y, 0)];
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < kRowE; ++u) {
-      const bool ok = e + u < rb.K[t] && en[t][u].y >= 0;
-      av[t][u] = ok ? a[t][u] : 0.0;
-      xv[t][u] = ok ? x[t][u] : 0.0;
-    }
-  __builtin_amdgcn_sched_group_barrier(0x0100, 2 * kRowE, 0);
-  __builtin_amdgcn_sched_barrier(0);
+  for (int u = 0; u < kSweep; ++u) {
+    const int slot = min(s0 + u, E - 1) * 64 + lane;
+    cl[u] = cw[2 * slot + 1];
+    av[u] = A[slot];
+  }
 }
 
-// dst_j = b_j - (A x)_j in double-double, x and dst in LDS
+// vector entries of a loaded chunk; entries past a row's end read as zeros
+__device__ __forceinline__ void sweep_values(const lds_f64* X, const int (&cl)[kSweep],
+                                             double (&av)[kSweep], double (&xv)[kSweep]) {
+#pragma unroll
+  for (int u = 0; u < kSweep; ++u) xv[u] = X[max(cl[u], 0)];
+#pragma unroll
+  for (int u = 0; u < kSweep; ++u) {
+    const bool ok = cl[u] >= 0;
+    av[u] = ok ? av[u] : 0.0;
+    xv[u] = ok ? xv[u] : 0.0;
+  }
+}
+
+// dst_j = b_j - (A x)_j in double-double, x and dst in LDS, b global
 __device__ inline void residual_all(const DevPlan& pl, const double* A, const double* x,
                                     const double* b, double* dst) {
   const int lane = lane_id();
   const lds_f64* X = (const lds_f64*)x;
   lds_f64* D = (lds_f64*)dst;
   const int nb = (pl.Nj + 63) >> 6;
-  for (int b0 = 0; b0 < nb; b0 += 2) {
-    const RowBlocks rb = row_blocks(pl, b0, nb);
-    double hi[2], lo[2];
+  const int E = pl.rs_blk[nb];
+  const int jl = pl.Nj - 1;
+  int cl[kSweep];
+  double av[kSweep];
+  sweep_load(pl, A, 0, E, cl, av);
+  __builtin_amdgcn_sched_barrier(0);
+  // D = b (each lane copies and later reads only its own rows)
+  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG) {
+    double bv[kRowG];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      hi[t] = b[min(min(b0 + t, nb - 1) * 64 + lane, pl.Nj - 1)];
-      lo[t] = 0.0;
-    }
-    const int Km = max(rb.K[0], rb.K[1]);
-    for (int e = 0; e < Km; e += kRowE) {
-      double av[2][kRowE], xv[2][kRowE];
-      row_entries(pl, rb, e, A, X, av, xv);
+    for (int u = 0; u < kRowG; ++u) bv[u] = b[min(j0 + 64 * u, jl)];
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int u = 0; u < kRowE; ++u) {
-          const double p = av[t][u] * xv[t][u];
-          const double pe = fma(av[t][u], xv[t][u], -p);
-          double sm, err;
-          two_sum(hi[t], -p, sm, err);
-          hi[t] = sm;
-          lo[t] += err - pe;
-        }
-    }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int j = (b0 + t) * 64 + lane;
-      if (b0 + t < nb && j < pl.Nj) D[j] = hi[t] + lo[t];
-    }
+    for (int u = 0; u < kRowG; ++u)
+      if (j0 + 64 * u < pl.Nj) D[j0 + 64 * u] = bv[u];
   }
+  int bk = 0;
+  int be = pl.rs_blk[1];
+  int be2 = pl.rs_blk[min(2, nb)];
+  double hi = D[min(lane, jl)], lo = 0.0;
+  double hn = D[min(64 + lane, jl)];
+  for (int s0 = 0; s0 < E; s0 += kSweep) {
+    double xv[kSweep];
+    sweep_values(X, cl, av, xv);
+#pragma unroll
+    for (int u = 0; u < kSweep; ++u) {
+      const int s = s0 + u;
+      if (s >= E) continue;
+      if (s == be) {
+        const int j = bk * 64 + lane;
+        if (j < pl.Nj) D[j] = hi + lo;
+        ++bk;
+        be = be2;
+        be2 = pl.rs_blk[min(bk + 2, nb)];
+        hi = hn;
+        lo = 0.0;
+        hn = D[min((bk + 1) * 64 + lane, jl)];
+      }
+      const double p = av[u] * xv[u];
+      const double pe = fma(av[u], xv[u], -p);
+      double sm, err;
+      two_sum(hi, -p, sm, err);
+      hi = sm;
+      lo += err - pe;
+    }
+    if (s0 + kSweep < E) sweep_load(pl, A, s0 + kSweep, E, cl, av);
+  }
+  const int j = bk * 64 + lane;
+  if (j < pl.Nj) D[j] = hi + lo;
 }
 
 // dst = A x, x in LDS, dst in global memory
@@ -447,24 +449,35 @@ __device__ inline void spmv_all(const DevPlan& pl, const double* A, const double
   const lds_f64* X = (const lds_f64*)x;
   __attribute__((address_space(1))) double* D = (__attribute__((address_space(1))) double*)dst;
   const int nb = (pl.Nj + 63) >> 6;
-  for (int b0 = 0; b0 < nb; b0 += 2) {
-    const RowBlocks rb = row_blocks(pl, b0, nb);
-    double sum[2] = {0.0, 0.0};
-    const int Km = max(rb.K[0], rb.K[1]);
-    for (int e = 0; e < Km; e += kRowE) {
-      double av[2][kRowE], xv[2][kRowE];
-      row_entries(pl, rb, e, A, X, av, xv);
+  const int E = pl.rs_blk[nb];
+  int cl[kSweep];
+  double av[kSweep];
+  sweep_load(pl, A, 0, E, cl, av);
+  int bk = 0;
+  int be = pl.rs_blk[1];
+  int be2 = pl.rs_blk[min(2, nb)];
+  double sum = 0.0;
+  for (int s0 = 0; s0 < E; s0 += kSweep) {
+    double xv[kSweep];
+    sweep_values(X, cl, av, xv);
 #pragma unroll
-      for (int t = 0; t < 2; ++t)
-#pragma unroll
-        for (int u = 0; u < kRowE; ++u) sum[t] = fma(av[t][u], xv[t][u], sum[t]);
+    for (int u = 0; u < kSweep; ++u) {
+      const int s = s0 + u;
+      if (s >= E) continue;
+      if (s == be) {
+        const int j = bk * 64 + lane;
+        if (j < pl.Nj) D[j] = sum;
+        ++bk;
+        be = be2;
+        be2 = pl.rs_blk[min(bk + 2, nb)];
+        sum = 0.0;
+      }
+      sum = fma(av[u], xv[u], sum);
     }
-#pragma unroll
-    for (int t = 0; t < 2; ++t) {
-      const int j = (b0 + t) * 64 + lane;
-      if (b0 + t < nb && j < pl.Nj) D[j] = sum[t];
-    }
+    if (s0 + kSweep < E) sweep_load(pl, A, s0 + kSweep, E, cl, av);
   }
+  const int j = bk * 64 + lane;
+  if (j < pl.Nj) D[j] = sum;
 }
 
 // ---------------------------------------------------------------------------
