@@ -80,32 +80,84 @@ __device__ __forceinline__ double dpp_f64(double v) {
   return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
 
-// Sum over aligned groups of 2^lw lanes (1 <= lw <= 6), bitwise equal in
-// every lane to an xor butterfly over the same groups.  The steps within 16
-// lanes use DPP: quad permutes for xor 1 and xor 2, then row half-mirror and
-// row mirror, which pair a lane with one of the partner half; after each
-// step every lane of a group holds the same bits (the additions are
-// commutative), so the partner's value equals the butterfly's.  Wider steps
-// use bpermute.
-__device__ __forceinline__ double group_sum(double v, int lw) {
-  v += dpp_f64<0xB1>(v);                  // quad_perm [1, 0, 3, 2]
-  if (lw >= 2) v += dpp_f64<0x4E>(v);     // quad_perm [2, 3, 0, 1]
-  if (lw >= 3) v += dpp_f64<0x141>(v);    // row_half_mirror
-  if (lw >= 4) v += dpp_f64<0x140>(v);    // row_mirror
-  for (int m = 16; m < (1 << lw); m <<= 1) v += __shfl_xor(v, m, 64);
+// v of lane ^ 16 and of lane ^ 32: v_permlane16_swap / v_permlane32_swap
+// exchange rows between two registers; with both holding v, one output holds
+// the partner row for the even rows (halves), the other for the odd ones.
+__device__ __forceinline__ double xor_lanes16(double v) {
+  const long long b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+                                                   false, false);
+  const bool odd = lane_id() & 16;
+  const unsigned l = odd ? lo[0] : lo[1], h = odd ? hi[0] : hi[1];
+  return __longlong_as_double(((long long)h << 32) | l);
+}
+__device__ __forceinline__ double xor_lanes32(double v) {
+  const long long b = __double_as_longlong(v);
+  const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)b, (unsigned)b, false, false);
+  const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(b >> 32), (unsigned)(b >> 32),
+                                                   false, false);
+  const bool upper = lane_id() & 32;
+  const unsigned l = upper ? lo[0] : lo[1], h = upper ? hi[0] : hi[1];
+  return __longlong_as_double(((long long)h << 32) | l);
+}
+
+// Sum over aligned lane groups of per-lane width 2^lw (lw = 0: not in a
+// group, the value is left alone; lwm: the widest group's lw, the number of
+// stages), bitwise equal in every grouped lane to an xor butterfly over its
+// group.  The stages within 16 lanes use DPP: quad permutes for xor 1 and
+// xor 2, then row half-mirror and row mirror, which pair a lane with one of
+// the partner half; after each stage every lane of a block holds the same
+// bits (the additions are commutative), so the partner's value equals the
+// butterfly's.  A lane adds only in the stages below its own group width,
+// and a partner in those stages lies inside the same aligned group.  The
+// stages across rows use the permlane swaps.
+__device__ __forceinline__ double group_sum(double v, int lw, int lwm) {
+  // nested so that each further stage is a fall-through
+  double p = dpp_f64<0xB1>(v);                 // quad_perm [1, 0, 3, 2]
+  v = lw >= 1 ? v + p : v;
+  if (lwm >= 2) {
+    p = dpp_f64<0x4E>(v);                      // quad_perm [2, 3, 0, 1]
+    v = lw >= 2 ? v + p : v;
+    if (lwm >= 3) {
+      p = dpp_f64<0x141>(v);                   // row_half_mirror
+      v = lw >= 3 ? v + p : v;
+      if (lwm >= 4) {
+        p = dpp_f64<0x140>(v);                 // row_mirror
+        v = lw >= 4 ? v + p : v;
+        if (lwm >= 5) {
+          p = xor_lanes16(v);
+          v = lw >= 5 ? v + p : v;
+          if (lwm >= 6) {
+            p = xor_lanes32(v);
+            v = lw >= 6 ? v + p : v;
+          }
+        }
+      }
+    }
+  }
   return v;
 }
 
 // ---------------------------------------------------------------------------
-// Level-scheduled program executors.
-// kRing chunks (kRing * kUnroll steps) of instruction loads stay in flight in
-// a ring of kRing + 1 register slots: chunk c lives in slot c % (kRing + 1),
-// and the load of chunk c + kRing goes into the slot the previous chunk just
-// released, so no slot is copied and each wait covers only the chunk about to
-// run.  Instruction addresses depend on nothing loaded; the instruction array
-// carries kRing * kUnroll trailing NOP steps so the ring can over-fetch
-// without bounds checks.
-constexpr int kRing = 4;
+// Level-scheduled program executors, one step per pass.
+// Instruction words are prefetched kRing steps ahead into a ring of kRing + 1
+// register slots: step g lives in slot g % (kRing + 1), and the load of step
+// g + kRing goes into the slot step g - 1 just released, so no slot is
+// copied.  Instruction addresses depend on nothing loaded; the programs run
+// in whole turns of the ring (NOP padded) and the instruction array carries
+// kRing more NOP steps, so the ring can over-fetch without bounds checks.
+// (A guard per step instead of the padding costs more: the wait-count pass
+// cannot tell that a skipped step ends the loop and drains the whole
+// prefetch at every turn.)
+//
+// The operands of step g + 1 are read while step g finishes: inside a level
+// the tasks are independent, so when step g does not end its level, step
+// g + 1's LDS reads are issued before step g's store (the wavefront performs
+// its LDS operations in issue order, and no read of a level touches a store
+// of the same level); after a level end they are issued after the level's
+// last stores.
+constexpr int kRing = 8;
 constexpr int kSlots = kRing + 1;
 
 // 1/x for a positive pivot: v_rcp_f64 refined by two Newton steps
@@ -116,8 +168,6 @@ __device__ __forceinline__ double recip(double x) {
   return r;
 }
 
-static_assert(kStepAlign % kUnroll == 0, "levels must start on a chunk boundary");
-
 // Explicit address spaces for the executor: LDS operands compile to ds_*
 // (counted by lgkmcnt only) and instruction words to global loads (vmcnt
 // only), so waiting for an operand never drains the instruction prefetch, as
@@ -126,16 +176,14 @@ typedef __attribute__((address_space(3))) double lds_f64;
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) i32x4 glb_i32x4;
 
-// The ring's first kRing chunks, issued oldest first.  The scheduling
+// The ring's first kRing steps, issued oldest first.  The scheduling
 // barriers keep that order: the wait counter is in-order, so if the
-// scheduler issued the first chunk's loads last, the loop header would have
-// to wait for all but the newest few loads and the prefetch would collapse
-// to a single chunk at every pass through the ring.
-__device__ __forceinline__ void fill_ring(i32x4 (&ring)[kSlots][kUnroll], const glb_i32x4* ip) {
+// scheduler issued the first step's load last, the loop header would have
+// to wait for all but the newest few loads and the prefetch would collapse.
+__device__ __forceinline__ void fill_ring(i32x4 (&ring)[kSlots], const glb_i32x4* ip) {
 #pragma unroll
   for (int d = 0; d < kRing; ++d) {
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) ring[d][u] = ip[(int64_t)(d * kUnroll + u) * 64];
+    ring[d] = ip[(int64_t)d * 64];
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -153,85 +201,75 @@ __device__ inline void run_factor(const ProgDev& pg, double* Vg, int& bad,
   // each lane has its own sink slot
   lds_f64* const dummy = V + pg.dummy + lane;
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
-  i32x4 ring[kSlots][kUnroll];
+  i32x4 ring[kSlots];
   fill_ring(ring, ip);
   double acc = 0.0;
-  // wide levels: the task whose target this lane's group finalises
+  // grouped tasks: the target this lane's slice belongs to
   int fin_t = 0;
-  bool fin_diag = false, fin_act = false, lvl_start = true;
+  bool fin_diag = false;
 #ifdef WDS_PROFILE_STAGES
   long long prof_t0 = clock64();
   int prof_lvl = 0;
 #endif
-  for (int g0 = 0; g0 < ns; g0 += kSlots * kUnroll) {
+  double va = V[ring[0].x], vb = V[ring[0].y], vc = V[ring[0].z], vt = V[ring[0].w & kTMask];
+  for (int g0 = 0; g0 < ns; g0 += kSlots) {
 #pragma unroll
     for (int d = 0; d < kSlots; ++d) {
-    const int g = g0 + d * kUnroll;  // ns is a multiple of kSlots * kUnroll
-    constexpr int kNextSlot[kSlots] = {4, 0, 1, 2, 3};  // (d + kRing) % kSlots
-    static_assert(kSlots == 5, "kNextSlot is written for kRing == 4");
-    i32x4 cur[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      cur[u] = ring[d][u];
-      ring[kNextSlot[d]][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
-    }
-    if (lvl_start) {
-      // a wide level's slices all start at its first step, which is the
-      // first step of a chunk: remember the task this lane helps finalise
-      fin_t = cur[0].w & kTMask;
-      fin_diag = cur[0].w & kDiag;
-      fin_act = cur[0].w & kFirst;
-      lvl_start = false;
-    }
-    double va[kUnroll], vb[kUnroll], vc[kUnroll], vt[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const i32x4 in = cur[u];
-      va[u] = V[in.x];
-      vb[u] = V[in.y];
-      vc[u] = V[in.z];
-      vt[u] = V[in.w & kTMask];
-    }
-    // branch-free: NOPs multiply by V[zero] == 0; every step stores, to its
-    // target when the instruction is a task's last, else to V[dummy]
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const int w = cur[u].w;
-      const int t = w & kTMask;
-      const bool first = w & kFirst, last = w & kLast;
-      // a wide slice other than the group's first starts at zero
-      const double init = (w & kZeroInit) ? 0.0 : vt[u];
-      acc = fma(-va[u], vb[u] * vc[u], first ? init : acc);
-      const bool diag = w & kDiag;
-      const double r = recip(acc);
-      if (diag && last && !(acc > 0.0)) bad = 1;
-      *(last ? V + t : dummy) = diag ? r : acc;
-    }
-    // levels are padded to whole chunks: only a chunk's last step can end one
-    const int wl = __builtin_amdgcn_readfirstlane(cur[kUnroll - 1].w);
-    if (wl & kLevelEnd) {
-      const int lw = (wl >> kWidthShift) & 7;
-      if (lw) {
-        // wide level: sum each aligned group of 2^lw lanes (the first
-        // lane's slice started from the target's value), then the group's
-        // first lane finalises the target
-        const double red = group_sum(acc, lw);
-        if (fin_act && (lane & ((1 << lw) - 1)) == 0) {
-          if (fin_diag && !(red > 0.0)) bad = 1;
-          V[fin_t] = fin_diag ? recip(red) : red;
+      const int g = g0 + d;  // ns is a multiple of kSlots
+      {
+        const i32x4 cur = ring[d];
+        ring[(d + kRing) % kSlots] = ip[(int64_t)(g + kRing) * 64];
+        const i32x4 nxt = ring[(d + 1) % kSlots];
+        const int w = cur.w;
+        const int t = w & kTMask;
+        const bool first = w & kFirst, last = w & kLast, diag = w & kDiag;
+        if (first) {
+          fin_t = t;
+          fin_diag = diag;
+        }
+        // a slice other than its group's first starts at zero
+        const double init = (w & kZeroInit) ? 0.0 : vt;
+        acc = fma(-va, vb * vc, first ? init : acc);
+        const double r = recip(acc);
+        if (diag && last && !(acc > 0.0)) bad = 1;
+        // branch-free store: to the target when the instruction is a task's
+        // last, else to the lane's sink
+        lds_f64* const dst = last ? V + t : dummy;
+        const double val = diag ? r : acc;
+        const int f = __builtin_amdgcn_readfirstlane(w);
+        if (!(f & kLevelEnd)) {
+          va = V[nxt.x];
+          vb = V[nxt.y];
+          vc = V[nxt.z];
+          vt = V[nxt.w & kTMask];
+          *dst = val;
+        } else {
+          *dst = val;
+          // lane 0 is in the level's widest group
+          const int lwm = (f >> kWidthShift) & 7;
+          if (lwm) {
+            const int lw = (w >> kWidthShift) & 7;
+            const double red = group_sum(acc, lw, lwm);
+            if (lw && (lane & ((1 << lw) - 1)) == 0) {
+              if (fin_diag && !(red > 0.0)) bad = 1;
+              V[fin_t] = fin_diag ? recip(red) : red;
+            }
+          }
+          lds_order();
+#ifdef WDS_PROFILE_STAGES
+          if (prof) {
+            const long long t1 = clock64();
+            if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
+            prof_t0 = t1;
+            ++prof_lvl;
+          }
+#endif
+          va = V[nxt.x];
+          vb = V[nxt.y];
+          vc = V[nxt.z];
+          vt = V[nxt.w & kTMask];
         }
       }
-      lvl_start = true;
-      lds_order();
-#ifdef WDS_PROFILE_STAGES
-      if (prof) {
-        const long long t1 = clock64();
-        if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
-        prof_t0 = t1;
-        ++prof_lvl;
-      }
-#endif
-    }
     }
   }
 }
@@ -244,71 +282,65 @@ __device__ __forceinline__ lds_f64& lds_at(int byte_addr) {
 // Triangular solve with the unit factor on the solve vector S (LDS):
 // acc = S[t] - sum V[a] S[b] per task, stored back into S[t].  Instruction
 // words are LDS byte addresses (a, b, init, store): init is S[t] on a task's
-// first step (the zero slot for a wide slice other than the group's first,
-// the lane's own sink slot on every other step) and store is S[t] on its last
-// step (the lane's sink otherwise).  The store word of a level's last step (always
-// a chunk's last) carries the level end and log2 group width in its high
-// bits, uniform across lanes.
+// first step (the zero slot for a group slice other than the group's first,
+// the lane's own sink slot on every other step) and store is S[t] on its
+// last step (the lane's sink otherwise).  The store word of a level's last
+// step carries the level end and the lane's log2 group width in its high
+// bits.
 __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
   const int lane = lane_id();
   const int ns = pg.nsteps;
   if (ns == 0) return;
   const int sent = pg.dummy + 8 * lane;  // byte address of this lane's sink slot
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
-  i32x4 ring[kSlots][kUnroll];
+  i32x4 ring[kSlots];
   fill_ring(ring, ip);
   double acc = 0.0;
-  int fin = sent;  // wide levels: byte address of the group's target
-  bool lvl_start = true;
+  int fin = sent;  // grouped tasks: byte address of the slice's target
 #ifdef WDS_PROFILE_STAGES
   long long prof_t0 = clock64();
   int prof_lvl = 0;
 #endif
-  for (int g0 = 0; g0 < ns; g0 += kSlots * kUnroll) {
+  double va = lds_at(ring[0].x), vb = lds_at(ring[0].y), vi = lds_at(ring[0].z);
+  for (int g0 = 0; g0 < ns; g0 += kSlots) {
 #pragma unroll
     for (int d = 0; d < kSlots; ++d) {
-    const int g = g0 + d * kUnroll;
-    constexpr int kNextSlot[kSlots] = {4, 0, 1, 2, 3};
-    i32x4 cur[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      cur[u] = ring[d][u];
-      ring[kNextSlot[d]][u] = ip[(int64_t)(g + kRing * kUnroll + u) * 64];
-    }
-    if (lvl_start) {
-      fin = cur[0].z;  // the group's first lane starts from its target
-      lvl_start = false;
-    }
-    double va[kUnroll], vb[kUnroll], vi[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      va[u] = lds_at(cur[u].x);
-      vb[u] = lds_at(cur[u].y);
-      vi[u] = lds_at(cur[u].z);
-    }
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      acc = fma(-va[u], vb[u], cur[u].z != sent ? vi[u] : acc);
-      lds_at(u == kUnroll - 1 ? cur[u].w & kDotAddrMask : cur[u].w) = acc;
-    }
-    const int f = __builtin_amdgcn_readfirstlane(cur[kUnroll - 1].w);
-    if (f & kDotLevelEnd) {
-      const int lw = (f >> kDotWidthShift) & 7;
-      if (lw) {
-        const double red = group_sum(acc, lw);
-        if ((lane & ((1 << lw) - 1)) == 0) lds_at(fin) = red;
-      }
-      lvl_start = true;
-      lds_order();
+      const int g = g0 + d;  // ns is a multiple of kSlots
+      {
+        const i32x4 cur = ring[d];
+        ring[(d + kRing) % kSlots] = ip[(int64_t)(g + kRing) * 64];
+        const i32x4 nxt = ring[(d + 1) % kSlots];
+        const bool init = cur.z != sent;
+        if (init) fin = cur.z;
+        acc = fma(-va, vb, init ? vi : acc);
+        const int f = __builtin_amdgcn_readfirstlane(cur.w);
+        if (!(f & kDotLevelEnd)) {
+          va = lds_at(nxt.x);
+          vb = lds_at(nxt.y);
+          vi = lds_at(nxt.z);
+          lds_at(cur.w) = acc;
+        } else {
+          lds_at(cur.w & kDotAddrMask) = acc;
+          const int lwm = (f >> kDotWidthShift) & 7;
+          if (lwm) {
+            const int lw = (cur.w >> kDotWidthShift) & 7;
+            const double red = group_sum(acc, lw, lwm);
+            if (lw && (lane & ((1 << lw) - 1)) == 0) lds_at(fin) = red;
+          }
+          lds_order();
 #ifdef WDS_PROFILE_STAGES
-      if (prof) {
-        const long long t1 = clock64();
-        if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
-        prof_t0 = t1;
-        ++prof_lvl;
-      }
+          if (prof) {
+            const long long t1 = clock64();
+            if (lane == 0 && prof_lvl < 32) prof[prof_lvl] += (double)(t1 - prof_t0);
+            prof_t0 = t1;
+            ++prof_lvl;
+          }
 #endif
-    }
+          va = lds_at(nxt.x);
+          vb = lds_at(nxt.y);
+          vi = lds_at(nxt.z);
+        }
+      }
     }
   }
 }

@@ -5,7 +5,6 @@
 
 namespace wds {
 
-constexpr int kUnroll = 4;     // program steps processed per chunk
 constexpr int kMaxPrv = 8;     // pressure-reducing valves with a setting
 
 // per-link integer attributes (bit field in link_flags)
@@ -47,14 +46,16 @@ WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
 }
 
 // solve programs: the store word of a level's last step carries the level
-// end and log2 of the level's group width above the LDS byte address
+// end and log2 of the lane's group width above the LDS byte address
 constexpr int32_t kDotAddrMask = (1 << 20) - 1;
 constexpr int32_t kDotLevelEnd = 1 << 30;
 constexpr int kDotWidthShift = 27;
 
 struct ProgDev {
   const int4* instr;        // step-major instructions, nsteps * 64
-  int32_t nsteps;           // multiple of kSlots * kUnroll (NOP padded)
+  int32_t nsteps;           // steps the executor runs (NOP padded to whole
+                            // turns of its prefetch ring; the array holds
+                            // that ring's depth in further NOP steps)
   int32_t nlevels;
   int32_t dummy;            // first per-lane sink slot: factor, index from V;
                             // solves, LDS byte address

@@ -42,10 +42,9 @@ int hip_fail(hipError_t e, const char* what) {
   return WDS_EDEVICE;
 }
 
-// steps the executor runs: the program padded to whole groups of chunks
+// steps the executor runs: the program padded to whole turns of the ring
 int32_t exec_steps(const Program& pr) {
-  constexpr int32_t g = kSlots * kUnroll;
-  return (pr.nsteps() + g - 1) / g * g;
+  return (pr.nsteps() + kSlots - 1) / kSlots * kSlots;
 }
 
 template <class T>
@@ -68,10 +67,10 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   const Symbolic& S = P->sym;
   BlobBuilder bb;
   auto prog_vectors = [&](const Program& pr, size_t& oi, size_t& os, size_t& ol) {
-    // trailing NOP steps: the executor runs whole groups of kSlots chunks and
-    // its prefetch ring reads up to kRing * kUnroll steps past the end
+    // trailing NOP steps: the prefetch ring reads up to kRing steps past the
+    // end
     std::vector<Instr> padded(pr.instr);
-    padded.resize((size_t)(exec_steps(pr) + kRing * kUnroll) * kLanes,
+    padded.resize((size_t)(exec_steps(pr) + kRing) * kLanes,
                   nop_instr((int32_t)S.nnzL));
     oi = bb.add(padded);
     os = ol = 0;
@@ -84,7 +83,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   const int32_t s_b = 8 * ((int32_t)S.nnzL + 2 + P->Nj + P->Nf);
   auto dot_vectors = [&](const Program& pr, size_t& oi, size_t& os) {
     const int32_t ns = exec_steps(pr);
-    std::vector<Instr> w((size_t)(ns + kRing * kUnroll) * kLanes);
+    std::vector<Instr> w((size_t)(ns + kRing) * kLanes);
     for (size_t i = 0; i < w.size(); ++i) {
       const int32_t own = sink_b + 8 * (int32_t)(i % kLanes);  // the lane's sink
       w[i] = Instr{zero_b, s_b, own, own};
@@ -99,8 +98,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
         const int32_t store = (fl & kLast) ? s_b + 8 * t : own;
         w[i] = Instr{8 * in.a, s_b + 8 * in.b, init, store};
       }
-      // a level end (always the last step of a chunk) and the group width
-      // ride in the high bits of the store word of every lane
+      // a level end (every lane) and the lane's group width ride in the
+      // high bits of the store word
       if (fl & kLevelEnd)
         w[i].t |= kDotLevelEnd | (((fl >> kWidthShift) & 7) << kDotWidthShift);
     }

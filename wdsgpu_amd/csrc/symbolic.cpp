@@ -62,103 +62,149 @@ static void minimum_degree(int n, const std::vector<std::vector<int32_t>>& adj,
   }
 }
 
-static size_t align_steps(size_t s) { return (s + kStepAlign - 1) / kStepAlign * kStepAlign; }
-
-// Schedule the tasks of one level onto the 64 lanes and append the steps.
+// Schedule the tasks of one level onto the 64 lanes and append its steps.
 //
-// Narrow mode (group width 1): longest-processing-time assignment of whole
-// tasks to lanes; a task's first instruction loads its target (kFirst), its
-// last stores the result (kLast).
+// A level of S steps gives every task the smallest power-of-two group of g
+// lanes whose slices of its dot product fit in S steps (g = 1: the whole task
+// on one lane).  Groups take aligned lane blocks, largest first from lane 0,
+// so lane 0 always belongs to the level's widest group.  Single-lane tasks
+// fill the lanes' remaining steps, longest first onto the lane with the most
+// room; a grouped lane runs its single-lane tasks first and its slice last.
+// A single-lane task's first instruction loads its target (kFirst), its last
+// stores the result (kLast).  A slice starts from the target's value on the
+// group's first lane and from zero on the others (kFirst | kZeroInit; a lane
+// with no share contributes one explicit zero); at the level end the executor
+// sums each group with a butterfly of cross-lane exchanges, lanes adding only
+// within their own group, and the group's first lane finalises the target.
 //
-// Wide mode (group width W = 2..64, chosen when a level has few, long tasks
-// -- the top of the elimination tree): each task owns an aligned group of W
-// lanes and its dot product is cut into W slices that start from zero
-// (kFirst | kZeroInit); at the level end the executor sums each group with a
-// butterfly of cross-lane shuffles and the group's first lane finalises the
-// target.  The level's steps shrink from the longest dot product to
-// ceil(longest / W), at the price of one reduction.
+// Every S from the smallest that could fit to the single-lane schedule's
+// length is tried; the cost model counts a level's first step (its operands
+// wait for the previous level's stores) as kLevelCost, every further step
+// as kStepCost, and each butterfly stage as kStageCost.
 //
 // Layout: step-major with a fixed stride of 64 (idle lanes hold NOPs); the
-// level's last step carries kLevelEnd and log2(W) in every lane.
-static void emit_level(const std::vector<Task>& tasks, Program& prog, int32_t zero) {
-  if (tasks.empty()) return;
-  size_t mmax = 0;
-  for (const Task& t : tasks) mmax = std::max(mmax, t.ins.size());
-  // narrow-mode schedule
-  std::vector<int> idx(tasks.size());
-  for (size_t i = 0; i < idx.size(); ++i) idx[i] = (int)i;
-  std::stable_sort(idx.begin(), idx.end(), [&](int x, int y) {
+// level's last step carries kLevelEnd in every lane and each lane's log2
+// group width (0 outside a group).
+namespace {
+constexpr int kLevelCost = 13, kStepCost = 5, kStageCost = 2;
+
+struct LaneSched {
+  std::vector<std::vector<Instr>> lane;
+  int lw_max = 0;
+  size_t steps = 0;
+};
+
+int ceil_log2(size_t x) {
+  int l = 0;
+  while (((size_t)1 << l) < x) ++l;
+  return l;
+}
+
+// schedule of the level in S steps, or false if it does not fit
+bool pack_level(const std::vector<Task>& tasks, size_t S, int32_t zero, LaneSched& out) {
+  const size_t nt = tasks.size();
+  std::vector<int> lw(nt, 0);
+  for (size_t i = 0; i < nt; ++i) {
+    const size_t m = tasks[i].ins.size();
+    while ((m + ((size_t)1 << lw[i]) - 1) >> lw[i] > S) {
+      if (++lw[i] > 6) return false;
+    }
+  }
+  std::vector<int> grouped, single;
+  for (size_t i = 0; i < nt; ++i) (lw[i] ? grouped : single).push_back((int)i);
+  std::stable_sort(grouped.begin(), grouped.end(), [&](int x, int y) { return lw[x] > lw[y]; });
+  std::stable_sort(single.begin(), single.end(), [&](int x, int y) {
     return tasks[x].ins.size() > tasks[y].ins.size();
   });
-  using Item = std::pair<int64_t, int>;  // (load, lane)
-  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
-  for (int l = 0; l < kLanes; ++l) heap.push({0, l});
-  std::vector<std::vector<Instr>> lane(kLanes);
-  for (int ti : idx) {
-    Item it = heap.top();
-    heap.pop();
-    std::vector<Instr> v = tasks[ti].ins;
-    v.front().t |= kFirst;
-    v.back().t |= kLast | (tasks[ti].diag ? kDiag : 0);
-    for (const Instr& in : v) lane[it.second].push_back(in);
-    heap.push({it.first + (int64_t)v.size(), it.second});
-  }
-  size_t narrow_steps = 0;
-  for (auto& l : lane) narrow_steps = std::max(narrow_steps, l.size());
-  narrow_steps = align_steps(narrow_steps);
-  // best wide mode: one task per aligned group of W lanes
-  int best_lw = 0;
-  size_t best_cost = narrow_steps;
-  for (int lw = 1; lw <= 6; ++lw) {
-    const size_t W = (size_t)1 << lw;
-    if (tasks.size() * W > (size_t)kLanes) break;
-    const size_t cost = align_steps((mmax + W - 1) / W) + kWideOverheadSteps;
-    if (cost < best_cost) {
-      best_cost = cost;
-      best_lw = lw;
-    }
-  }
-  for (const Task& t : tasks) prog.useful += (int64_t)t.ins.size();
-  if (best_lw > 0) {
-    const int W = 1 << best_lw;
-    for (auto& l : lane) l.clear();
-    for (size_t g = 0; g < tasks.size(); ++g) {
-      const Task& tk = tasks[g];
-      const size_t m = tk.ins.size();
-      const size_t slice = (m + W - 1) / W;
-      const int32_t dflag = tk.diag ? kDiag : 0;
-      for (int i = 0; i < W; ++i) {
-        auto& ln = lane[g * W + i];
-        const size_t b = (size_t)i * slice, e = std::min(m, b + slice);
-        if (b >= e) {
-          // no share of the dot product: contribute an explicit zero
-          Instr in = nop_instr(zero);
-          in.t = (tk.ins.front().t & kTMask) | kFirst | kZeroInit | dflag;
-          ln.push_back(in);
-          continue;
-        }
-        for (size_t p = b; p < e; ++p) {
-          Instr in = tk.ins[p];
-          // the group's first lane starts from the target's value, the others
-          // from zero, so the group sum is the finished dot product
-          in.t |= dflag | (p == b ? (kFirst | (i == 0 ? 0 : kZeroInit)) : 0);
-          ln.push_back(in);
-        }
+  out.lane.assign(kLanes, {});
+  std::vector<std::vector<Instr>> slice(kLanes);
+  std::vector<int> lane_lw(kLanes, 0);
+  out.lw_max = 0;
+  int next = 0;
+  for (int ti : grouped) {
+    const int W = 1 << lw[ti];
+    if (next + W > kLanes) return false;
+    const Task& tk = tasks[ti];
+    const size_t m = tk.ins.size();
+    const size_t sl = (m + W - 1) / W;
+    const int32_t dflag = tk.diag ? kDiag : 0;
+    for (int i = 0; i < W; ++i) {
+      auto& ln = slice[next + i];
+      lane_lw[next + i] = lw[ti];
+      const size_t b = (size_t)i * sl, e = std::min(m, b + sl);
+      if (b >= e) {
+        // no share of the dot product: contribute an explicit zero
+        Instr in = nop_instr(zero);
+        in.t = (tk.ins.front().t & kTMask) | kFirst | kZeroInit | dflag;
+        ln.push_back(in);
+        continue;
+      }
+      for (size_t p = b; p < e; ++p) {
+        Instr in = tk.ins[p];
+        in.t |= dflag | (p == b ? (kFirst | (i == 0 ? 0 : kZeroInit)) : 0);
+        ln.push_back(in);
       }
     }
+    out.lw_max = std::max(out.lw_max, lw[ti]);
+    next += W;
   }
-  size_t steps = 0;
-  for (auto& l : lane) steps = std::max(steps, l.size());
-  steps = align_steps(steps);
-  for (size_t s = 0; s < steps; ++s) {
-    for (int l = 0; l < kLanes; ++l) {
-      Instr in = s < lane[l].size() ? lane[l][s] : nop_instr(zero);
-      if (s + 1 == steps) in.t |= kLevelEnd | (best_lw << kWidthShift);
-      prog.instr.push_back(in);
+  // single-lane tasks into the room each lane has left before its slice
+  using Item = std::pair<int64_t, int>;  // (-room, lane): most room first
+  std::priority_queue<Item, std::vector<Item>, std::greater<Item>> heap;
+  for (int l = 0; l < kLanes; ++l) heap.push({-(int64_t)(S - slice[l].size()), l});
+  for (int ti : single) {
+    Item it = heap.top();
+    heap.pop();
+    const int64_t room = -it.first;
+    std::vector<Instr> v = tasks[ti].ins;
+    if ((int64_t)v.size() > room) return false;
+    v.front().t |= kFirst;
+    v.back().t |= kLast | (tasks[ti].diag ? kDiag : 0);
+    for (const Instr& in : v) out.lane[it.second].push_back(in);
+    heap.push({-(room - (int64_t)v.size()), it.second});
+  }
+  out.steps = 0;
+  for (int l = 0; l < kLanes; ++l) {
+    for (const Instr& in : slice[l]) out.lane[l].push_back(in);
+    out.steps = std::max(out.steps, out.lane[l].size());
+  }
+  // the level end carries each lane's group width
+  for (int l = 0; l < kLanes; ++l) {
+    out.lane[l].resize(out.steps, nop_instr(zero));
+    out.lane[l].back().t |= kLevelEnd | (lane_lw[l] << kWidthShift);
+  }
+  return true;
+}
+}  // namespace
+
+static void emit_level(const std::vector<Task>& tasks, Program& prog, int32_t zero) {
+  if (tasks.empty()) return;
+  size_t total = 0, mmax = 0;
+  for (const Task& t : tasks) {
+    total += t.ins.size();
+    mmax = std::max(mmax, t.ins.size());
+  }
+  prog.useful += (int64_t)total;
+  LaneSched best, cur;
+  int64_t best_cost = -1;
+  // the single-lane schedule needs at most sum(len) steps; larger S only
+  // repeats it
+  const size_t lo = std::max<size_t>(1, (total + kLanes - 1) / kLanes);
+  for (size_t S = lo; S <= std::max(lo, total); ++S) {
+    if (!pack_level(tasks, S, zero, cur)) continue;
+    const int64_t cost = kLevelCost + kStepCost * (int64_t)(cur.steps - 1) +
+                         kStageCost * cur.lw_max;
+    if (best_cost < 0 || cost < best_cost) {
+      best_cost = cost;
+      best = cur;
     }
+    if (cur.lw_max == 0) break;  // no groups: a longer level only costs more
   }
+  if (best_cost < 0) throw std::runtime_error("symbolic: a level does not fit the wavefront");
+  for (size_t st = 0; st < best.steps; ++st)
+    for (int l = 0; l < kLanes; ++l) prog.instr.push_back(best.lane[l][st]);
   const int32_t prev = prog.level_end.empty() ? 0 : prog.level_end.back();
-  prog.level_end.push_back(prev + (int32_t)steps);
+  prog.level_end.push_back(prev + (int32_t)best.steps);
 }
 
 void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,

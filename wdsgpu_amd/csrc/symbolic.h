@@ -29,11 +29,10 @@ constexpr int32_t kLast = 1 << 29;      // store acc into the target after this 
 constexpr int32_t kDiag = 1 << 28;      // factor: target is a diagonal -> store 1/acc
 constexpr int32_t kNop = 1 << 27;       // padding, no load/store
 constexpr int32_t kLevelEnd = 1 << 26;  // last step of a level (set in every lane)
-constexpr int kWidthShift = 23;         // bits 23-25: log2 of the level's group width
-constexpr int32_t kZeroInit = 1 << 22;  // wide level: a slice's partial sum starts at 0
+constexpr int kWidthShift = 23;         // bits 23-25 at a level end: log2 of the lane's
+                                        // group width (0: not in a group)
+constexpr int32_t kZeroInit = 1 << 22;  // a group slice's partial sum starts at 0
 constexpr int32_t kTMask = (1 << 22) - 1;
-// estimated cost of a wide level's group reduction and finalisation, in steps
-constexpr size_t kWideOverheadSteps = 2;
 
 // one dot-product task of a level: its updates (flags are added by the
 // scheduler) and whether its target is a diagonal pivot
@@ -52,9 +51,6 @@ struct Program {
   int64_t useful = 0;                  // non-padding instructions
   int32_t nsteps() const { return level_end.empty() ? 0 : level_end.back(); }
 };
-// steps of every level are padded to a multiple of this so an unrolled
-// chunk never straddles a level boundary
-constexpr int kStepAlign = 4;
 
 struct Symbolic {
   int n = 0;                           // junction count
