@@ -750,16 +750,17 @@ __device__ inline void row_gather(const DevPlan& pl, int e0, int e1, double sgn,
 // dst[j] = a[j], or a[j] - b[j] when b is given, for j < n; loads batched
 __device__ inline void vec_copy(double* dst, const double* a, const double* b, int n) {
   const int lane = lane_id();
-  for (int j0 = lane; j0 < n; j0 += 64 * kGather) {
-    double av[kGather], bv[kGather];
+  // kRowG rows per lane per round: one memory round trip for up to 1024 rows
+  for (int j0 = lane; j0 < n; j0 += 64 * kRowG) {
+    double av[kRowG], bv[kRowG];
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) {
+    for (int u = 0; u < kRowG; ++u) {
       const int j = min(j0 + 64 * u, n - 1);
       av[u] = a[j];
       bv[u] = b ? b[j] : 0.0;
     }
 #pragma unroll
-    for (int u = 0; u < kGather; ++u) {
+    for (int u = 0; u < kRowG; ++u) {
       const int j = j0 + 64 * u;
       if (j < n) dst[j] = b ? av[u] - bv[u] : av[u];
     }

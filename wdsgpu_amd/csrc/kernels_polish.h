@@ -246,10 +246,37 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   vec_copy(rp, c.v1, nullptr, pl.Nj);
   mem_sync();
   if (lane == 0) {
-    for (int i = 0; i < p; ++i)
-      if (c.sh[i] != 0.0) rp[pl.prv_s1[i]] += c.v1[pl.prv_s2[i]];
-    for (int i = 0; i < p; ++i)
-      rp[pl.prv_s2[i]] = (c.sh[i] != 0.0) ? -rprv[i] : c.v1[pl.prv_s2[i]];
+    // rp[j1] += v1[j2] for the active valves in valve order, then
+    // rp[j2] = -r_prv (active) or v1[j2]: every value loads first (one round
+    // trip), the sums run in registers (an upstream node shared by two
+    // valves takes both terms in order; no upstream node is a downstream one)
+    int s1[kMaxPrv], s2[kMaxPrv];
+    double a1[kMaxPrv], a2[kMaxPrv], r1[kMaxPrv];
+#pragma unroll
+    for (int i = 0; i < kMaxPrv; ++i) {
+      const int k = min(i, p - 1);
+      s1[i] = pl.prv_s1[k];
+      s2[i] = pl.prv_s2[k];
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxPrv; ++i) {
+      a1[i] = c.v1[s1[i]];
+      a2[i] = c.v1[s2[i]];
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxPrv; ++i) {
+      double cur = a1[i];
+#pragma unroll
+      for (int k = 0; k < i; ++k)
+        if (c.sh[k] != 0.0 && s1[k] == s1[i]) cur = r1[k];
+      r1[i] = cur + a2[i];
+    }
+#pragma unroll
+    for (int i = 0; i < kMaxPrv; ++i)
+      if (i < p && c.sh[i] != 0.0) rp[s1[i]] = r1[i];
+#pragma unroll
+    for (int i = 0; i < kMaxPrv; ++i)
+      if (i < p) rp[s2[i]] = (c.sh[i] != 0.0) ? -rprv[i] : a2[i];
   }
   mem_sync();
   WDS_T0();
@@ -266,12 +293,30 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
     vec_copy(res, rp, wk, pl.Nj);
     mem_sync();
     if (lane == 0) {
-      for (int i = 0; i < p; ++i) {
-        if (c.sh[i] == 0.0) continue;
-        const int j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
-        const double row1 = wk[j1] + wk[j2] - CBIG * dst[j2];
-        res[j1] = rp[j1] - row1;
-        res[j2] = rp[j2] - dst[j2];
+      // the replaced rows of the active valves, stored in valve order (all
+      // loads first: none reads res)
+      int s1[kMaxPrv], s2[kMaxPrv];
+      double w1[kMaxPrv], w2[kMaxPrv], p1[kMaxPrv], p2[kMaxPrv];
+#pragma unroll
+      for (int i = 0; i < kMaxPrv; ++i) {
+        const int k = min(i, p - 1);
+        s1[i] = pl.prv_s1[k];
+        s2[i] = pl.prv_s2[k];
+      }
+#pragma unroll
+      for (int i = 0; i < kMaxPrv; ++i) {
+        w1[i] = wk[s1[i]];
+        w2[i] = wk[s2[i]];
+        p1[i] = rp[s1[i]];
+        p2[i] = rp[s2[i]];
+      }
+#pragma unroll
+      for (int i = 0; i < kMaxPrv; ++i) {
+        if (i >= p || c.sh[i] == 0.0) continue;
+        const double d2 = dst[s2[i]];
+        const double row1 = w1[i] + w2[i] - CBIG * d2;
+        res[s1[i]] = p1[i] - row1;
+        res[s2[i]] = p2[i] - d2;
       }
     }
     mem_sync();
@@ -547,9 +592,8 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   const double* gqp = c.sprv;
   vec_copy(r2, c.v1, nullptr, pl.Nj);
   mem_sync();
-  if (lane == 0)
-    for (int i = 0; i < p; ++i)
-      if (c.sh[i] != 0.0) r2[pl.prv_s2[i]] = gqp[i];
+  // one lane per valve: the downstream junctions are distinct
+  if (lane < p && c.sh[lane] != 0.0) r2[pl.prv_s2[lane]] = gqp[lane];
   mem_sync();
   wood_paths(pl, c);
   if (!wood_solve(pl, c, r2, true)) return false;
@@ -560,12 +604,10 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
     double* res = c.v5;
     vec_copy(res, r2, wk, pl.Nj);
     mem_sync();
-    if (lane == 0)
-      for (int i = 0; i < p; ++i) {
-        if (c.sh[i] == 0.0) continue;
-        const int j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
-        res[j2] = r2[j2] - (dst[j2] - dst[j1]);
-      }
+    if (lane < p && c.sh[lane] != 0.0) {
+      const int j1 = pl.prv_s1[lane], j2 = pl.prv_s2[lane];
+      res[j2] = r2[j2] - (dst[j2] - dst[j1]);
+    }
     mem_sync();
     if (!wood_solve(pl, c, res, true)) return false;
     for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.R[j];
