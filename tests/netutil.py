@@ -107,3 +107,28 @@ def curve_lift(points, q):
     if q > x[-1]:
         return y[-1] + (y[-1] - y[-2]) / (x[-1] - x[-2]) * (q - x[-1])
     return float(np.interp(q, x, y))
+
+
+def hub_network(n_ring=60, n_spokes=48, seed=0):
+    """A reservoir, a ring of junctions with a few chords, and one hub
+    junction piped to n_spokes ring junctions: the hub row of the factor
+    holds about n_spokes updates, so the top of the elimination tree has one
+    long dot product (a 64-lane group at one step per level)."""
+    rng = np.random.default_rng(seed)
+    nodes = [(RESERVOIR, 300.0, 0.0)]
+    nodes += [(JUNCTION, 100.0 + 5.0 * rng.random(), 0.05 + 0.1 * rng.random())
+              for _ in range(n_ring)]
+    hub = len(nodes)
+    nodes.append((JUNCTION, 102.0, 0.2))
+
+    def pipe(a, b, d=0.8, L=800.0):
+        return dict(n1=a, n2=b, len_ft=L * (0.5 + rng.random()), diam_ft=d, C=110.0)
+
+    links = [pipe(0, 1, 1.5, 300.0), pipe(0, hub, 1.5, 300.0)]
+    links += [pipe(i, i + 1) for i in range(1, n_ring)] + [pipe(n_ring, 1)]
+    for _ in range(6):
+        a, b = rng.choice(np.arange(1, n_ring + 1), 2, replace=False)
+        links.append(pipe(int(a), int(b), 0.5))
+    for j in rng.choice(np.arange(1, n_ring + 1), n_spokes, replace=False):
+        links.append(pipe(hub, int(j), 0.6))
+    return make_net(nodes, links, accuracy=1e-6)

@@ -221,3 +221,20 @@ def test_cache_api_of_the_reference(cuda):
     s.cudss_cache_max = 0
     with pytest.raises(ValueError):
         s.cudss_plan(4)
+
+
+def test_hub_network_matches_oracle(cuda):
+    # one junction piped to 48 others: the top level of the forward program
+    # is a single dot product spread over a 64-lane group (cross-row
+    # butterfly stages), the factor's top levels likewise
+    from netutil import check_forward_against_oracle, hub_network, res_heads
+    net = hub_network(seed=0)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    B = 4
+    rng = np.random.default_rng(7)
+    d = net.demand_cfs[None] * (1.0 + 0.2 * rng.standard_normal((B, net.N)))
+    rh = np.broadcast_to(res_heads(net), (B, net.N)).copy()
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=1e-6, status_machine=True)
+    check_forward_against_oracle(net, m, out, d, rh, 1e-6)

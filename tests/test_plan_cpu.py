@@ -16,7 +16,7 @@ from wdsgpu_amd.generators import grid_network, planar_network, scenario_batch
 from wdsgpu_amd.network import CVPIPE, GPV, JUNCTION, PIPE, RESERVOIR
 from wdsgpu_amd.solver import GGASolver
 
-from netutil import make_net
+from netutil import hub_network, make_net
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 FIRST, LAST, DIAG, NOP, LEVEL_END = 1 << 30, 1 << 29, 1 << 28, 1 << 27, 1 << 26
@@ -93,9 +93,10 @@ def run_program(s, p, V, X, mode):
     return bad, level_ends, grouped_levels
 
 
-@pytest.mark.parametrize("name", ["grid", "planar"])
+@pytest.mark.parametrize("name", ["grid", "planar", "hub"])
 def test_programs_factor_and_solve_like_dense(name):
-    net = grid_network(0) if name == "grid" else planar_network(1)
+    net = {"grid": lambda: grid_network(0), "planar": lambda: planar_network(1),
+           "hub": lambda: hub_network(seed=0)}[name]()
     m = gga.GGAModel(net)
     s = GGASolver(net, device="cpu", dense_status_machine=True)
     st = s.stats
@@ -128,6 +129,11 @@ def test_programs_factor_and_solve_like_dense(name):
     _, _, wide_s = run_program(s, 1, V, X, 1)
     X *= V[cp[:-1]]                  # z = D^-1 y
     run_program(s, 2, V, X, 1)
+    if name == "hub":
+        # the hub row's dot product spans a whole 64-lane group in one step
+        ins = s.plan_export(8 + 3 * 1).reshape(-1, 64, 4)
+        ends = ins[(ins[:, 0, 3] & LEVEL_END) != 0]
+        assert ((ends[:, 0, 3] >> WIDTH_SHIFT) & 7).max() == 6
     if name == "planar":
         # the top of the elimination tree has few long dot products: those
         # levels must be split across lane groups
