@@ -150,12 +150,22 @@ __device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
 }
 
 // e_u^T A-hat^-1 r for every valve end u, from z = D^-1 L^-1 r in X (LDS)
+// (one lane per path; the path's node slots load kPathG at a time before
+// any is used, the sum runs in path order)
+constexpr int kPathG = 16;
 __device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, double* x0) {
   const int lane = lane_id();
   if (lane < pl.np) {
+    const int i0 = pl.wp_ptr[lane], i1 = pl.wp_ptr[lane + 1];
     double s = 0.0;
-    for (int i = pl.wp_ptr[lane]; i < pl.wp_ptr[lane + 1]; ++i)
-      s = fma(c.Yp[i], X[pl.wp_node[i]], s);
+    for (int b = i0; b < i1; b += kPathG) {
+      int nd[kPathG];
+#pragma unroll
+      for (int u = 0; u < kPathG; ++u) nd[u] = pl.wp_node[min(b + u, i1 - 1)];
+#pragma unroll
+      for (int u = 0; u < kPathG; ++u)
+        if (b + u < i1) s = fma(c.Yp[b + u], X[nd[u]], s);
+    }
     x0[lane] = s;
   }
   lds_sync();
@@ -167,6 +177,25 @@ __device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, do
 __device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const double* w,
                                   int ncol) {
   const int lane = lane_id();
+  constexpr int kApG = 4;
+  if (pl.wp_tot <= 64 * kApG) {
+    // every path node's slot loads once, up front (entry i = lane + 64 u);
+    // the columns then run in order from registers and LDS
+    int nd[kApG];
+#pragma unroll
+    for (int u = 0; u < kApG; ++u) nd[u] = pl.wp_node[min(lane + 64 * u, pl.wp_tot - 1)];
+    for (int col = 0; col < ncol; ++col) {
+      const double wc = w[col];
+      const int c0 = pl.wp_ptr[col], c1 = pl.wp_ptr[col + 1];
+#pragma unroll
+      for (int u = 0; u < kApG; ++u) {
+        const int i = lane + 64 * u;
+        if (i >= c0 && i < c1) X[nd[u]] -= wc * c.Zp[i];
+      }
+      lds_sync();
+    }
+    return;
+  }
   for (int col = 0; col < ncol; ++col) {
     const double wc = w[col];
     for (int i = pl.wp_ptr[col] + lane; i < pl.wp_ptr[col + 1]; i += 64)
