@@ -360,11 +360,24 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 // LDS: the pivot positions of up to kRowG rows per lane load before use.
 constexpr int kRowG = 16;
 constexpr int kRowG8 = 8;   // rows per lane per round where each row loads several values
-__device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, double* R) {
+// The pivot positions of the first kRowG rows per lane, loaded ahead (before
+// the forward program runs) so that the scaling after it waits on no memory.
+__device__ __forceinline__ void pivots_prefetch(const DevPlan& pl, int (&dp)[kRowG]) {
+  const int lane = lane_id();
+#pragma unroll
+  for (int u = 0; u < kRowG; ++u) dp[u] = pl.diag_pos[min(lane + 64 * u, pl.Nj - 1)];
+}
+__device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, double* R,
+                                       const int (&dp0)[kRowG]) {
   const int lane = lane_id();
   const lds_f64* Vl = (const lds_f64*)V;
   lds_f64* Rl = (lds_f64*)R;
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG) {
+#pragma unroll
+  for (int u = 0; u < kRowG; ++u) {
+    const int j = lane + 64 * u;
+    if (j < pl.Nj) Rl[j] *= Vl[dp0[u]];
+  }
+  for (int j0 = lane + 64 * kRowG; j0 < pl.Nj; j0 += 64 * kRowG) {
     int dp[kRowG];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u) dp[u] = pl.diag_pos[min(j0 + 64 * u, pl.Nj - 1)];

@@ -468,12 +468,14 @@ __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
 // Solve programs on the solve vector S = c.R with the unit factor in V.
 // Forward y = L^-1 S then the pivot scaling z = D^-1 y (in place):
 __device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
+  int dp[kRowG];
+  pivots_prefetch(pl, dp);
 #ifdef WDS_PROFILE_STAGES
   run_dot(pl.fwd, c.misc + 408 + 32);
 #else
   run_dot(pl.fwd);
 #endif
-  scale_by_pivots(pl, c.V, c.R);
+  scale_by_pivots(pl, c.V, c.R, dp);
   lds_sync();
 }
 
