@@ -154,9 +154,10 @@ __device__ __forceinline__ double group_sum(double v, int lw, int lwm) {
 // The operands of step g + 1 are read while step g finishes: inside a level
 // the tasks are independent, so when step g does not end its level, step
 // g + 1's LDS reads are issued before step g's store (the wavefront performs
-// its LDS operations in issue order, and no read of a level touches a store
-// of the same level); after a level end they are issued after the level's
-// last stores.
+// its LDS operations in issue order, and no read of a level whose value is
+// used touches a store of the same level: only a lane's own sink slot is
+// both stored and read, and a sink read is discarded); after a level end
+// they are issued after the level's last stores.
 constexpr int kRing = 8;
 constexpr int kSlots = kRing + 1;
 
