@@ -238,3 +238,17 @@ def test_hub_network_matches_oracle(cuda):
     out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
                   accuracy=1e-6, status_machine=True)
     check_forward_against_oracle(net, m, out, d, rh, 1e-6)
+
+
+@pytest.mark.parametrize("seed", [2, 3, 7])
+def test_other_planar_topologies_match_oracle(cuda, seed):
+    # other seeded planar networks: other elimination trees (23 to 39 levels)
+    # and so other mixed-group schedules of the factor and solve programs
+    from netutil import check_forward_against_oracle
+    net = planar_network(seed)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    d, rh = scenario_batch(net, 3, seed=11)
+    out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
+                  accuracy=1e-6, status_machine=True)
+    check_forward_against_oracle(net, m, out, d, rh, 1e-6)
