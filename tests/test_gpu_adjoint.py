@@ -156,3 +156,26 @@ def test_adjoint_matches_central_differences(cuda):
         worst = max(worst, err)
         assert err < 1e-6, (kind, j, an, fd)
     assert worst < 1e-6
+
+
+@pytest.mark.parametrize("seed", [2, 7])
+def test_adjoint_on_other_planar_topologies(cuda, seed):
+    # other elimination trees, so other mixed-group factor and solve
+    # programs under the polish, the Woodbury paths and the node adjoint
+    net = planar_network(seed)
+    m = gga.GGAModel(net)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    B = 2
+    d, rh = scenario_batch(net, B, seed=17)
+    ke = np.broadcast_to(net.node_ke, (B, net.N)).copy()
+    r = m.r_hw.copy()
+    g = np.random.default_rng(3)
+    W = g.normal(size=(B, net.N))
+    WQ = g.normal(size=(B, net.L)) * 0.1
+    H, Q, gd, grh, gke, gr, _ = _gpu_grads(s, d, rh, ke, r, W, WQ)
+    sol = adj.solve_polished(m, d, rh, ke, r, accuracy=1e-8)
+    assert np.abs(H - sol["head"]).max() < 1e-9
+    cd, crh, cke, cr = adj.cpu_adjoint(m, sol, W, WQ, np.full((B, net.N), 0.5))
+    assert _rel(gd, cd) < 1e-10
+    assert _rel(gke, cke) < 1e-10
+    assert _rel(gr, cr.sum(0)) < 1e-10
