@@ -174,6 +174,22 @@ def test_batch_equals_single_bitwise(solvers):
         assert int(o["iters"]) == int(a["iters"][b])
 
 
+def test_batch_beyond_one_wavefront_per_simd_is_bitwise(solvers):
+    # 1100 scenarios: more workgroups than SIMDs, so some SIMDs run two
+    # scenarios; every scenario's result is still its own solve's bits
+    net, m, s = solvers["planar"]
+    _, d, rh, acc = _case("planar", 1100)
+    ke = torch.tensor(net.node_ke)
+    a = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=ke, accuracy=acc, status_machine=True)
+    pick = [0, 549, 1023, 1024, 1099]
+    o = s.solve(torch.tensor(d[pick]), torch.tensor(rh[pick]), ke_int=ke, accuracy=acc,
+                status_machine=True)
+    assert torch.equal(o["head_ft"], a["head_ft"][pick])
+    assert torch.equal(o["flow_cfs"], a["flow_cfs"][pick])
+    assert o["iters"].tolist() == a["iters"][pick].tolist()
+    assert bool(a["converged"].all())
+
+
 def test_unconverged_reports_maxtrials_plus_one(solvers):
     net, m, s = solvers["grid"]
     _, d, rh, _ = _case("grid", 2)
