@@ -158,7 +158,7 @@ __device__ __forceinline__ double group_sum(double v, int lw, int lwm) {
 // used touches a store of the same level: only a lane's own sink slot is
 // both stored and read, and a sink read is discarded); after a level end
 // they are issued after the level's last stores.
-constexpr int kRing = 8;
+constexpr int kRing = 4;
 constexpr int kSlots = kRing + 1;
 
 // 1/x for a positive pivot: v_rcp_f64 refined by two Newton steps
