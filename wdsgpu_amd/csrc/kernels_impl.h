@@ -275,39 +275,6 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
 constexpr int kAsE = 8;   // assembly entries per row loaded per round
 constexpr int kOffG = 16; // off-diagonal entries per lane loaded per round
 
-// One pass of the row loop: two 64-row blocks, their row data and their
-// first kAsE entries.  The loads of a pass depend on nothing computed, so
-// the next pass's are issued while this pass's link values are gathered.
-struct AsmPass {
-  int e0[2], K[2], dp[2], da[2];
-  double kv[2], hg[2], hl[2], ev[2], dj[2], el[2];
-  int2 en[2][kAsE];
-};
-__device__ __forceinline__ void asm_pass_load(const DevPlan& pl, const Scn& c, int b0, int nb,
-                                              AsmPass& a) {
-  const int lane = lane_id();
-#pragma unroll
-  for (int t = 0; t < 2; ++t) {
-    const int bk = min(b0 + t, nb - 1);
-    a.e0[t] = pl.as_blk[bk];
-    a.K[t] = b0 + t < nb ? pl.as_blk[bk + 1] - a.e0[t] : 0;
-    const int j = min(bk * 64 + lane, pl.Nj - 1);
-    a.kv[t] = c.kej[j];
-    a.hg[t] = c.hge[j];
-    a.hl[t] = c.hle[j];
-    a.ev[t] = c.e[j];
-    a.dj[t] = c.dj[j];
-    a.el[t] = pl.el_slot[j];
-    a.dp[t] = pl.diag_pos[j];
-    a.da[t] = pl.diag_ae[j];
-  }
-#pragma unroll
-  for (int t = 0; t < 2; ++t)
-#pragma unroll
-    for (int u = 0; u < kAsE; ++u)
-      a.en[t][u] = pl.as_ell[(a.e0[t] + min(u, max(a.K[t] - 1, 0))) * 64 + lane];
-}
-
 template <bool kF>
 __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
@@ -315,33 +282,34 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
   const lds_f64* Hn = (const lds_f64*)c.Hn;
   lds_f64* Vl = (lds_f64*)c.V;
   lds_f64* Rl = (lds_f64*)c.R;
-  AsmPass cur;
-  asm_pass_load(pl, c, 0, nb, cur);
   for (int b0 = 0; b0 < nb; b0 += 2) {
-    // the next pass (clamped past the end: harmless reloads of the last)
-    AsmPass nxt;
-    asm_pass_load(pl, c, min(b0 + 2, nb - 1), nb, nxt);
-    __builtin_amdgcn_sched_barrier(0);
-    int jr[2];
+    int e0[2], K[2], jr[2], dp[2], da[2];
+    double kv[2], hg[2], hl[2], ev[2], dj[2], el[2];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) jr[t] = min(b0 + t, nb - 1) * 64 + lane;
+    for (int t = 0; t < 2; ++t) {
+      const int bk = min(b0 + t, nb - 1);
+      e0[t] = pl.as_blk[bk];
+      K[t] = b0 + t < nb ? pl.as_blk[bk + 1] - e0[t] : 0;
+      jr[t] = bk * 64 + lane;
+      const int j = min(jr[t], pl.Nj - 1);
+      kv[t] = c.kej[j];
+      hg[t] = c.hge[j];
+      hl[t] = c.hle[j];
+      ev[t] = c.e[j];
+      dj[t] = c.dj[j];
+      el[t] = pl.el_slot[j];
+      dp[t] = pl.diag_pos[j];
+      da[t] = pl.diag_ae[j];
+    }
     double dg[2] = {0.0, 0.0}, F[2] = {0.0, 0.0}, X[2] = {0.0, 0.0};
-    const int Km = max(cur.K[0], cur.K[1]);
+    const int Km = max(K[0], K[1]);
     for (int e = 0; e < Km; e += kAsE) {
       int2 en[2][kAsE];
-      if (e == 0) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t)
+      for (int t = 0; t < 2; ++t)
 #pragma unroll
-          for (int u = 0; u < kAsE; ++u) en[t][u] = cur.en[t][u];
-      } else {
-        // rows longer than kAsE entries (rare): the rest loads here
-#pragma unroll
-        for (int t = 0; t < 2; ++t)
-#pragma unroll
-          for (int u = 0; u < kAsE; ++u)
-            en[t][u] = pl.as_ell[(cur.e0[t] + min(e + u, max(cur.K[t] - 1, 0))) * 64 + lane];
-      }
+        for (int u = 0; u < kAsE; ++u)
+          en[t][u] = pl.as_ell[(e0[t] + min(e + u, max(K[t] - 1, 0))) * 64 + lane];
       __builtin_amdgcn_sched_barrier(0);
       double Pv[2][kAsE], Yv[2][kAsE], qv[2][kAsE], Hv[2][kAsE];
 #pragma unroll
@@ -371,7 +339,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
       for (int t = 0; t < 2; ++t)
 #pragma unroll
         for (int u = 0; u < kAsE; ++u) {
-          const bool in_row = e + u < cur.K[t];
+          const bool in_row = e + u < K[t];
           const int y = en[t][u].y;
           const double P = Pv[t][u];
           // links with P == 0 (pcv PRVs) take no part (dgga/solver.py:2939-2956)
@@ -389,24 +357,23 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      if (cur.kv[t] > 0.0) {
-        dg[t] = dg[t] + 1.0 / cur.hg[t];
+      if (kv[t] > 0.0) {
+        dg[t] = dg[t] + 1.0 / hg[t];
         if (kF) {
-          F[t] = F[t] + (cur.hl[t] + cur.el[t]) / cur.hg[t];
-          X[t] = X[t] - cur.ev[t];
+          F[t] = F[t] + (hl[t] + el[t]) / hg[t];
+          X[t] = X[t] - ev[t];
         }
       }
       if (b0 + t < nb && jr[t] < pl.Nj) {
-        c.A[cur.da[t]] = dg[t];
-        Vl[cur.dp[t]] = dg[t];
+        c.A[da[t]] = dg[t];
+        Vl[dp[t]] = dg[t];
         if (kF) {
-          X[t] = X[t] - cur.dj[t];
+          X[t] = X[t] - dj[t];
           c.F[jr[t]] = F[t] + X[t];
           Rl[jr[t]] = X[t];
         }
       }
     }
-    cur = nxt;
   }
   for (int o0 = lane; o0 < pl.noff; o0 += 64 * kOffG) {
     int4 w[kOffG];
