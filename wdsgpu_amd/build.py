@@ -42,8 +42,13 @@ def build(verbose=False, arch="gfx950"):
             # round every product like the reference's CPU path does; the
             # factor/solve loops and the double-double residual use
             # explicit fma() where they want it
+            # amdgpu-sched-strategy=max-memory-clause: the machine scheduler
+            # keeps independent loads together (one wavefront per SIMD has
+            # nothing else to hide their latency with); measured 2% faster
+            # on the fused forward, bit-identical results
             cmd[1:1] = ["-x", "hip", "--offload-arch=" + arch, "-ffp-contract=off",
-                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+                        "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                        "-mllvm", "-amdgpu-sched-strategy=max-memory-clause"]
         if verbose:
             print(" ".join(cmd), flush=True)
         subprocess.run(cmd, check=True)
