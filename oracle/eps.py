@@ -65,6 +65,10 @@ def run_eps(net, demand_scale=None, accuracy=None, max_frames=100000, pow_jitter
     def tank_vol(j, h):
         return vmin[j] + (h - hmin[j]) * area[j]
 
+    # design flows of the constant-horsepower pumps (hydraul.c resetpumpflow)
+    pump_q0 = {int(k): float(net.pump_q0[j]) for j, k in enumerate(net.pump_link)
+               if int(net.pump_ptype[j]) == 0}
+
     while True:
         # demands
         d = net.demand_cfs_at(htime) * scale
@@ -88,6 +92,10 @@ def run_eps(net, demand_scale=None, accuracy=None, max_frames=100000, pow_jitter
             if reset:
                 s1 = ST_CLOSED if S[k] <= ST_CLOSED else 3
                 s2 = int(net.ctl_status[i])
+                # a constant-horsepower pump reopened by a control restarts
+                # from its design flow (resetpumpflow, dgga/eps.py:205-209)
+                if s1 == ST_CLOSED and s2 == 3 and k in pump_q0:
+                    q[k] = pump_q0[k]
                 if s1 != s2:
                     S[k] = s2
         # hydraulic solve, warm started

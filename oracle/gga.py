@@ -642,6 +642,27 @@ class GGAModel:
         return H
 
 
+def wave_tree_sum(x):
+    """Row sums of x [B, n] in one fixed order: 64 partial sums, partial l
+    adding x[:, l], x[:, l + 64], x[:, l + 128], ... in ascending order from
+    +0.0, then an xor butterfly p_l <- p_l + p_(l ^ m) for m = 32, 16, ..., 1
+    (every lane ends with the same value).  The reference sums with torch's
+    pairwise reduction (dgga/solver.py:3104-3110) and its scalar replica
+    with a left fold (2058-2062); this build fixes the wavefront tree above
+    for both the HIP kernel and this oracle, so their relerr, and hence
+    their iteration counts, agree bit for bit."""
+    x = np.atleast_2d(np.asarray(x, dtype=np.float64))
+    B, n = x.shape
+    p = np.zeros((B, 64))
+    for c0 in range(0, n, 64):
+        blk = x[:, c0:c0 + 64]
+        p[:, :blk.shape[1]] = p[:, :blk.shape[1]] + blk
+    lanes = np.arange(64)
+    for m in (32, 16, 8, 4, 2, 1):
+        p = p + p[:, lanes ^ m]
+    return p[:, 0]
+
+
 def chol_solve_refined(A, F, n_refine=2, extended=False):
     """Dense Cholesky + cholesky_solve + n_refine refinement steps with the
     residual A*H summed along the last axis (dgga/solver.py:3037-3064).
@@ -759,8 +780,8 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
         dh_e = H[:, m.junc_nodes] - m.el_junc
         dq_e = (hloss_e - dh_e) / hgrad_e
         e_new = e_j - em * dq_e
-        dqsum = np.sum(np.abs(dq), axis=1) + np.sum(em * np.abs(dq_e), axis=1)
-        qsum = np.sum(np.abs(q_new), axis=1) + np.sum(em * np.abs(e_new), axis=1)
+        dqsum = wave_tree_sum(np.abs(dq)) + wave_tree_sum(em * np.abs(dq_e))
+        qsum = wave_tree_sum(np.abs(q_new)) + wave_tree_sum(em * np.abs(e_new))
         relerr = np.where(qsum > hacc, dqsum / qsum, dqsum)
         am = active[:, None]
         q = np.where(am, q_new, q)

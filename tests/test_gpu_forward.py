@@ -268,3 +268,28 @@ def test_other_planar_topologies_match_oracle(cuda, seed):
     out = s.solve(torch.tensor(d), torch.tensor(rh), ke_int=torch.tensor(net.node_ke),
                   accuracy=1e-6, status_machine=True)
     check_forward_against_oracle(net, m, out, d, rh, 1e-6)
+
+
+def test_mismatched_input_shapes_raise_before_any_launch(solvers):
+    # every per-scenario array is checked against B, N and L on the host
+    net, m, s = solvers["planar"]
+    _, d, rh, acc = _case("planar", 8)
+    dt, rht = torch.tensor(d), torch.tensor(rh)
+    bad = [dict(r_hw=torch.tensor(np.tile(m.r_hw, (2, 1)))),       # [2, L] for B = 8
+           dict(r_hw=torch.tensor(m.r_hw[:-1])),                    # short r
+           dict(ke_int=torch.tensor(net.node_ke[:-1])),             # short ke
+           dict(ke_int=torch.tensor(np.tile(net.node_ke, (3, 1)))), # [3, N] for B = 8
+           dict(q0=torch.tensor(m.init_flow()[:-2])),               # short warm start
+           dict(e0=torch.tensor(np.zeros((2, net.N))))]
+    for kw in bad:
+        with pytest.raises(ValueError):
+            s.solve(dt, rht, accuracy=acc, status_machine=True, **kw)
+    with pytest.raises(ValueError):
+        s.solve(dt[:, :-1], rht, accuracy=acc, status_machine=True)
+    # a per-scenario r with 1-D demand fixes B from r in the implicit solve
+    from wdsgpu_amd.autodiff import implicit_solve
+    r2 = torch.tensor(np.tile(m.r_hw, (2, 1)))
+    with torch.no_grad():
+        H, Q, E = implicit_solve(s, torch.tensor(d[0]), torch.tensor(rh[0]), r_hw=r2,
+                                 accuracy=1e-8, adjoint="gpu", status_machine=True)
+    assert tuple(H.shape) == (2, net.N)

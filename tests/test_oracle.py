@@ -304,9 +304,10 @@ def test_assembled_matrix_is_bitwise_symmetric():
 
 def test_shipped_inp_fixture_converges_with_equal_iterations_across_batch():
     # networks/random_main/rand_0009.inp (the reference's smoke-test network),
-    # copied as tests/golden/rand_0009.inp
+    # copied as tests/golden/networks/random_main/rand_0009.inp
     import os
-    path = os.path.join(os.path.dirname(__file__), "golden", "rand_0009.inp")
+    path = os.path.join(os.path.dirname(__file__), "golden", "networks", "random_main",
+                        "rand_0009.inp")
     net = read_inp(path)
     m = gga.GGAModel(net)
     out = gga.solve(m, net.demand_cfs, res_heads(net), status_machine=True)

@@ -63,7 +63,10 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             d = d.unsqueeze(0)
         if rh.dim() == 1:
             rh = rh.unsqueeze(0)
-        B = max(d.shape[0], rh.shape[0], ket.shape[0] if ket.dim() == 2 else 1)
+        B = max(d.shape[0], rh.shape[0], ket.shape[0] if ket.dim() == 2 else 1,
+                rt.shape[0] if rt.dim() == 2 else 1)
+        if d.shape[0] not in (1, B):
+            raise ValueError("demand has %d scenarios, another input %d" % (d.shape[0], B))
         d = d.expand(B, -1)
         a, out, keep, single, B = s._args(d, rh, ket, None, None, max_iter, accuracy, sm,
                                           r_hw=rt, polish_steps=int(polish_steps),
@@ -101,7 +104,7 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         ctx.B = B
         s._last_resid = out["resid"]
         head, flow, emit = out["head"], out["flow"], out["emitter"]
-        if all(x == 1 for x in dims[:3]):
+        if all(x == 1 for x in dims):
             head, flow, emit = head[0], flow[0], emit[0]
         return head, flow, emit
 

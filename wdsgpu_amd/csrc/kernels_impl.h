@@ -563,24 +563,24 @@ __device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
       c.q[k] = qn;
     }
   }
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
-    double kv[kRowG8], hlv[kRowG8], hgv[kRowG8], ev[kRowG8], elv[kRowG8];
+  // emitter rows in original junction order (row r = lane + 64 i), so the
+  // reduction tree below is defined on file order: lane partials over links
+  // k = lane + 64 i and rows r = lane + 64 i in ascending order, then an xor
+  // butterfly (m = 32 .. 1)
+  for (int r0 = lane; r0 < pl.Nj; r0 += 64 * kRowG8) {
+    int jv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) jv[u] = pl.row_slot[min(r0 + 64 * u, pl.Nj - 1)];
+    double kv[kRowG8];
+#pragma unroll
+    for (int u = 0; u < kRowG8; ++u) kv[u] = c.kej[jv[u]];
 #pragma unroll
     for (int u = 0; u < kRowG8; ++u) {
-      const int j = min(j0 + 64 * u, pl.Nj - 1);
-      kv[u] = c.kej[j];
-      hlv[u] = c.hle[j];
-      hgv[u] = c.hge[j];
-      ev[u] = c.e[j];
-      elv[u] = pl.el_slot[j];
-    }
-#pragma unroll
-    for (int u = 0; u < kRowG8; ++u) {
-      const int j = j0 + 64 * u;
-      if (j < pl.Nj && kv[u] > 0.0) {
-        const double dh = c.Hn[j] - elv[u];
-        const double dq = (hlv[u] - dh) / hgv[u];
-        const double en = ev[u] - dq;
+      const int j = jv[u];
+      if (r0 + 64 * u < pl.Nj && kv[u] > 0.0) {
+        const double dh = c.Hn[j] - pl.el_slot[j];
+        const double dq = (c.hle[j] - dh) / c.hge[j];
+        const double en = c.e[j] - dq;
         dqe += fabs(dq);
         qse += fabs(en);
         c.e[j] = en;

@@ -63,6 +63,9 @@ class EpsDriver:
                                   for n in np.asarray(net.ctl_node)], dtype=np.int64)
         if np.any((np.asarray(net.ctl_node) >= 0) & (self.ctl_tank < 0)):
             raise NotImplementedError("level controls on a reservoir are not supported")
+        # design flows of the constant-horsepower pumps (resetpumpflow)
+        self.chp_q0 = {int(k): float(net.pump_q0[j]) for j, k in enumerate(net.pump_link)
+                       if int(net.pump_ptype[j]) == 0}
 
     def _tank_vol(self, j, h):
         return self.vmin[j] + (h - self.hmin[j]) * self.area[j]
@@ -100,7 +103,7 @@ class EpsDriver:
             idx = np.nonzero(active)[0]
             n = idx.size
             d = np.stack([net.demand_cfs_at(int(htime[b])) for b in idx]) * scale[idx]
-            self._controls(idx, htime, H, V, fixed_dem, S)
+            self._controls(idx, htime, H, V, fixed_dem, S, q)
             out = s.solve(torch.as_tensor(d, device=dev), torch.as_tensor(H[idx], device=dev),
                           ke_int=ke, q0=torch.as_tensor(q[idx], device=dev),
                           e0=None if e is None else torch.as_tensor(e[idx], device=dev),
@@ -145,7 +148,7 @@ class EpsDriver:
                     active[b] = False
         return [{k: np.asarray(v) for k, v in f.items()} for f in frames]
 
-    def _controls(self, idx, htime, H, V, fixed_dem, S):
+    def _controls(self, idx, htime, H, V, fixed_dem, S, q):
         net = self.net
         for i in range(len(net.ctl_link)):
             k, n, ctype = int(net.ctl_link[i]), int(net.ctl_node[i]), int(net.ctl_type[i])
@@ -165,6 +168,11 @@ class EpsDriver:
                 reset |= (htime[idx] + self.tstart) % SEC_PER_DAY == int(net.ctl_time[i])
             s1 = np.where(S[idx, k] <= ST_CLOSED, ST_CLOSED, ST_OPEN)
             s2 = int(net.ctl_status[i])
+            # a constant-horsepower pump reopened by a control restarts from
+            # its design flow (resetpumpflow, dgga/eps.py:205-209)
+            if k in self.chp_q0 and s2 == ST_OPEN:
+                reopen = reset & (s1 == ST_CLOSED)
+                q[idx[reopen], k] = self.chp_q0[k]
             chg = reset & (s1 != s2)
             S[idx[chg], k] = s2
 

@@ -291,58 +291,157 @@ def _sections(path):
     return out
 
 
+def _hours(tok, unit=""):
+    """Hours of an INP time field: h[:mm[:ss]] or a number with an optional
+    unit word SEC/MIN/HOUR/DAY or AM/PM (EPANET hour(), input3.c)."""
+    u = unit.upper()
+    if ":" in tok:
+        parts = [float(x) for x in tok.split(":")]
+        while len(parts) < 3:
+            parts.append(0.0)
+        h = parts[0] + parts[1] / 60.0 + parts[2] / 3600.0
+    else:
+        h = float(tok)
+        if u.startswith("SEC"):
+            return h / 3600.0
+        if u.startswith("MIN"):
+            return h / 60.0
+        if u.startswith("DAY"):
+            return h * 24.0
+    if u == "AM":
+        h = 0.0 if h >= 12.0 and h < 13.0 else h
+    elif u == "PM":
+        h = h if h >= 12.0 and h < 13.0 else h + 12.0
+    return h
+
+
 def read_inp(path):
-    """Read the subset of EPANET INP text used by the GGA hot path:
-    [JUNCTIONS] [RESERVOIRS] [TANKS] [PIPES] [VALVES] [DEMANDS] [EMITTERS]
-    [STATUS] [OPTIONS].  Pumps and patterns are refused (NotImplementedError)
-    rather than silently dropped.  Unit conversion follows dgga/parse.py
-    (lengths and diameters to ft, flows to cfs, emitter C to Ke, H-W r)."""
+    """Read the EPANET INP sections the GGA path and its extended-period
+    driver use: [OPTIONS] [TIMES] [PATTERNS] [CURVES] [JUNCTIONS]
+    [RESERVOIRS] [TANKS] [PIPES] [PUMPS] [VALVES] [DEMANDS] [EMITTERS]
+    [STATUS] [CONTROLS] (simple controls).  Unit conversion follows
+    dgga/parse.py:424-1042: lengths and diameters to ft, flows to cfs,
+    emitter C to Ke, Hazen-Williams r, pump curves fitted in user units
+    (EPANET powercurve for 1- and 3-point curves, CUSTOM otherwise, CONST_HP
+    for POWER).  A junction's [DEMANDS] rows replace its [JUNCTIONS] demand
+    (input3.c demanddata); junctions without rows keep theirs.  What the
+    path cannot represent is refused with NotImplementedError rather than
+    dropped: reservoir head patterns, pump speed patterns, tank volume
+    curves, several demand categories with different patterns at one node,
+    control settings other than OPEN/CLOSED."""
     s = _sections(path)
     opts = {}
     for tk in s.get("[OPTIONS]", []):
         key = tk[0].upper()
-        if key in ("UNITS", "HEADLOSS") and len(tk) > 1:
+        if key in ("UNITS", "HEADLOSS", "PRESSURE") and len(tk) > 1:
             opts[key] = tk[1].upper()
-        elif key in ("TRIALS", "ACCURACY", "CHECKFREQ", "MAXCHECK") and len(tk) > 1:
+        elif key in ("TRIALS", "ACCURACY", "CHECKFREQ", "MAXCHECK", "VISCOSITY") \
+                and len(tk) > 1:
             opts[key] = float(tk[1])
         elif key == "EMITTER" and len(tk) > 2:
             opts["EMITTER"] = float(tk[2])
+        elif key == "PATTERN" and len(tk) > 1:
+            opts["PATTERN"] = tk[1]
+        elif key == "DEMAND" and len(tk) > 2 and tk[1].upper().startswith("MULT"):
+            opts["DMULT"] = float(tk[2])
+        elif key == "SPECIFIC" and len(tk) > 2:
+            opts["SPGRAV"] = float(tk[2])
+        elif key == "UNBALANCED" and len(tk) > 1:
+            if tk[1].upper().startswith("CONTINUE"):
+                opts["EXTRA"] = int(tk[2]) if len(tk) > 2 else 0
+            else:
+                opts["EXTRA"] = -1
     units = opts.get("UNITS", "GPM")
     si = units in SI_FLOW_UNITS
     qcf = FLOW_UCF[units]
     hcf = M_PER_FT if si else 1.0
     dcf = 1000.0 * M_PER_FT if si else 12.0
+    spg = opts.get("SPGRAV", 1.0)
+    press = opts.get("PRESSURE", "METERS" if si else "PSI")
+    if press == "KPA":
+        pcf = 6.895 * PSI_PER_FT * spg
+    elif press in ("METERS", "METER"):
+        pcf = M_PER_FT * spg
+    else:
+        pcf = PSI_PER_FT * spg
     headloss = opts.get("HEADLOSS", "H-W")
-    if s.get("[PUMPS]"):
-        raise NotImplementedError(f"{path}: [PUMPS] are not read by read_inp")
+    # ---- [TIMES] ----
+    times = {}
+    for tk in s.get("[TIMES]", []):
+        key = " ".join(w.upper() for w in tk[:2])
+        for name, pre in (("duration_sec", "DURATION"), ("hyd_step_sec", "HYDRAULIC TIMESTEP"),
+                          ("pat_step_sec", "PATTERN TIMESTEP"), ("pat_start_sec", "PATTERN START"),
+                          ("report_step_sec", "REPORT TIMESTEP"),
+                          ("tstart_sec", "START CLOCKTIME")):
+            nw = len(pre.split())
+            if key.startswith(pre) or (nw == 1 and tk[0].upper() == pre):
+                vals = tk[nw:]
+                if vals:
+                    times[name] = int(round(3600.0 * _hours(vals[0], vals[1] if len(vals) > 1
+                                                            else "")))
+    # ---- [PATTERNS] / [CURVES] ----
+    pat_index, patterns = {}, []
+    for tk in s.get("[PATTERNS]", []):
+        if tk[0] not in pat_index:
+            pat_index[tk[0]] = len(patterns)
+            patterns.append([])
+        patterns[pat_index[tk[0]]].extend(float(x) for x in tk[1:])
+    curves = {}
+    for tk in s.get("[CURVES]", []):
+        curves.setdefault(tk[0], []).append((float(tk[1]), float(tk[2])))
+    default_pat = opts.get("PATTERN", "1")
+
+    def pat_of(name):
+        if name is None:
+            return pat_index.get(default_pat, -1)
+        if name not in pat_index:
+            raise ValueError(f"{path}: unknown pattern {name!r}")
+        return pat_index[name]
+
+    # ---- nodes ----
     node_id, node_type, elev = [], [], []
+    junc_dem = {}
     for tk in s.get("[JUNCTIONS]", []):
         node_id.append(tk[0]); node_type.append(JUNCTION); elev.append(float(tk[1]) / hcf)
+        if len(tk) > 2:
+            junc_dem[tk[0]] = (float(tk[2]), tk[3] if len(tk) > 3 else None)
     for tk in s.get("[RESERVOIRS]", []):
+        if len(tk) > 2:
+            raise NotImplementedError(f"{path}: reservoir head patterns are not supported")
         node_id.append(tk[0]); node_type.append(RESERVOIR); elev.append(float(tk[1]) / hcf)
     tanks = []
     for tk in s.get("[TANKS]", []):
+        if len(tk) > 7 and tk[7] not in ("*",):
+            raise NotImplementedError(f"{path}: tank volume curves are not supported")
         node_id.append(tk[0]); node_type.append(TANK); elev.append(float(tk[1]) / hcf)
         tanks.append(tk)
     idx = {n: i for i, n in enumerate(node_id)}
     N = len(node_id)
     demand = np.zeros(N)
-    dem_rows = s.get("[DEMANDS]", [])
-    if dem_rows:
-        for tk in dem_rows:
-            demand[idx[tk[0]]] += float(tk[1]) / qcf
-    else:
-        for tk in s.get("[JUNCTIONS]", []):
-            if len(tk) > 2:
-                demand[idx[tk[0]]] += float(tk[2]) / qcf
+    dem_pat = np.full(N, -1, dtype=np.int32)
+    rows = {}
+    for tk in s.get("[DEMANDS]", []):
+        rows.setdefault(tk[0], []).append((float(tk[1]), tk[2] if len(tk) > 2 else None))
+    for nid, (d0, p0) in junc_dem.items():
+        if nid not in rows:
+            rows[nid] = [(d0, p0)]
+    for nid, lst in rows.items():
+        i = idx[nid]
+        pats = {pat_of(p) for dv, p in lst if dv != 0.0} or {pat_of(lst[0][1])}
+        if len(pats) > 1:
+            raise NotImplementedError(f"{path}: node {nid} has demand categories with "
+                                      "different patterns")
+        for dv, _ in lst:
+            demand[i] += dv / qcf
+        dem_pat[i] = pats.pop()
     qexp = 1.0 / opts.get("EMITTER", 0.5)
     ke = np.zeros(N)
-    pcf = M_PER_FT if si else PSI_PER_FT
     for tk in s.get("[EMITTERS]", []):
         i = idx[tk[0]]
         c = float(tk[1])
         if node_type[i] == JUNCTION and c > 0:
             ke[i] = (qcf ** qexp / pcf) / c ** qexp
+    # ---- links ----
     link_id, lt, n1, n2, dia, ln, rough, km, st, setting = ([] for _ in range(10))
     for tk in s.get("[PIPES]", []):
         link_id.append(tk[0]); n1.append(idx[tk[1]]); n2.append(idx[tk[2]])
@@ -354,6 +453,17 @@ def read_inp(path):
         lt.append(CVPIPE if status == "CV" else PIPE)
         st.append(0 if status == "CLOSED" else 1)
         setting.append(0.0)
+    pumps = []
+    for tk in s.get("[PUMPS]", []):
+        k = len(link_id)
+        link_id.append(tk[0]); n1.append(idx[tk[1]]); n2.append(idx[tk[2]])
+        dia.append(0.0); ln.append(0.0); rough.append(0.0); km.append(0.0)
+        lt.append(PUMP); st.append(1)
+        kw = {tk[m].upper()[:4]: tk[m + 1] for m in range(3, len(tk) - 1, 2)}
+        if "PATT" in kw:
+            raise NotImplementedError(f"{path}: pump speed patterns are not supported")
+        setting.append(float(kw.get("SPEE", 1.0)))
+        pumps.append((k, kw))
     vt = {"PRV": PRV, "PSV": PSV, "PBV": PBV, "FCV": FCV, "TCV": TCV, "GPV": GPV}
     for tk in s.get("[VALVES]", []):
         link_id.append(tk[0]); n1.append(idx[tk[1]]); n2.append(idx[tk[2]])
@@ -370,6 +480,9 @@ def read_inp(path):
             st[k] = 0
         elif word == "OPEN":
             st[k] = 1
+        elif lt[k] == PUMP:
+            setting[k] = float(tk[1])
+            st[k] = 0 if setting[k] == 0.0 else 1
     ln_a, dia_a = np.asarray(ln), np.asarray(dia)
     rough_a = np.asarray(rough)
     lt_a = np.asarray(lt, dtype=np.int8)
@@ -381,14 +494,20 @@ def read_inp(path):
         r[pipe] = dw_resistance(ln_a[pipe], dia_a[pipe])
     else:
         raise NotImplementedError(f"headloss {headloss}")
-    meta = dict(flow_units=units, headloss=headloss,
+    meta = dict(flow_units=units, headloss=headloss, press_units=press,
                 trials=int(opts.get("TRIALS", 200)),
                 accuracy=float(opts.get("ACCURACY", 1e-3)), qexp=qexp,
-                emitter_exponent=1.0 / qexp)
+                emitter_exponent=1.0 / qexp, spgrav=spg,
+                demand_multiplier=float(opts.get("DMULT", 1.0)))
+    if "VISCOSITY" in opts:
+        meta["viscosity"] = VISCOS_DEFAULT * opts["VISCOSITY"]
+    if "EXTRA" in opts:
+        meta["extra_iter"] = int(opts["EXTRA"])
     if "CHECKFREQ" in opts:
         meta["checkfreq"] = int(opts["CHECKFREQ"])
     if "MAXCHECK" in opts:
         meta["maxcheck"] = int(opts["MAXCHECK"])
+    meta.update(times)
     net = Net(node_id=node_id, node_type=np.asarray(node_type, dtype=np.int8),
               elev_ft=np.asarray(elev), node_ke=ke, demand_cfs=demand,
               link_id=link_id, link_type=lt_a,
@@ -396,7 +515,11 @@ def read_inp(path):
               link_n2=np.asarray(n2, dtype=np.int32), diam_ft=dia_a,
               len_ft=ln_a, roughness=rough_a, km_int=np.asarray(km), r_hw=r,
               init_status=np.asarray(st, dtype=np.int8),
-              valve_setting_user=np.asarray(setting), meta=meta)
+              valve_setting_user=np.asarray(setting, dtype=np.float64), meta=meta,
+              patterns=[np.asarray(p, dtype=np.float64) for p in patterns],
+              dem_pat=dem_pat)
+    if pumps:
+        _read_pumps(net, pumps, curves, path, units, si)
     if tanks:
         tn = np.asarray([idx[t[0]] for t in tanks], dtype=np.int32)
         el = net.elev_ft[tn]
@@ -406,5 +529,105 @@ def read_inp(path):
         net.tank_hmax = el + np.asarray([float(t[4]) for t in tanks]) / hcf
         dd = np.asarray([float(t[5]) for t in tanks]) / hcf
         net.tank_area = PI_EPANET * dd * dd / 4.0
-        net.tank_overflow = np.zeros(len(tanks), dtype=np.int8)
+        net.tank_overflow = np.asarray([1 if len(t) > 8 and t[8].upper().startswith("Y")
+                                        else 0 for t in tanks], dtype=np.int8)
+    _read_controls(net, s.get("[CONTROLS]", []), idx, lidx, hcf, pcf, path)
     return net
+
+
+def _read_pumps(net, pumps, curves, path, units, si):
+    """Pump curve types and coefficients (dgga/parse.py:740-860; EPANET
+    updatepumpparams, input2.c:372-462): POWER -> CONST_HP with
+    r = -8.814 P (P in hp; kW / 0.7457 for SI units); a 1-point HEAD curve
+    -> power function through (0, 1.33334 h1), (q1, h1), (2 q1, 0); a
+    3-point curve starting at zero flow -> power function; any other curve
+    -> CUSTOM (points kept in user units)."""
+    kinds, h0s, rs, ns, q0s, qmaxs, hmaxs = ([] for _ in range(7))
+    cq, ch, cptr = [], [], [0]
+    for k, kw in pumps:
+        if "POWE" in kw:
+            p = float(kw["POWE"])
+            if p <= 0.0:
+                raise ValueError(f"{path}: pump {net.link_id[k]} power must be positive")
+            r = -8.814 * p / (0.7457 if si else 1.0)
+            kinds.append(PUMP_CONST_HP); h0s.append(0.0); rs.append(r); ns.append(-1.0)
+            q0s.append(1.0); qmaxs.append(BIG); hmaxs.append(BIG)
+        elif "HEAD" in kw:
+            pts = curves.get(kw["HEAD"])
+            if not pts:
+                raise ValueError(f"{path}: pump {net.link_id[k]} names unknown curve "
+                                 f"{kw['HEAD']!r}")
+            if len(pts) == 1 or (len(pts) == 3 and pts[0][0] == 0.0):
+                if len(pts) == 1:
+                    q1, h1 = pts[0]
+                    h0, h2, q2 = 1.33334 * h1, 0.0, 2.0 * q1
+                else:
+                    h0 = pts[0][1]
+                    (q1, h1), (q2, h2) = pts[1], pts[2]
+                c = pump_power_curve(h0, h1, q1, h2, q2, flow_units=units, si=si)
+                kinds.append(PUMP_POWER_FUNC)
+            else:
+                c = pump_custom_curve(pts, flow_units=units, si=si)
+                kinds.append(PUMP_CUSTOM)
+                cq.extend(c["x"].tolist()); ch.extend(c["y"].tolist())
+            h0s.append(c["h0"]); rs.append(c["r"]); ns.append(c["n"])
+            q0s.append(c["q0"]); qmaxs.append(c["qmax"]); hmaxs.append(c["hmax"])
+        else:
+            raise ValueError(f"{path}: pump {net.link_id[k]} has neither a curve nor a power")
+        cptr.append(len(cq))
+    net.pump_link = np.asarray([k for k, _ in pumps], dtype=np.int32)
+    net.pump_ptype = np.asarray(kinds, dtype=np.int8)
+    net.pump_h0, net.pump_r, net.pump_n = np.asarray(h0s), np.asarray(rs), np.asarray(ns)
+    net.pump_q0, net.pump_qmax, net.pump_hmax = np.asarray(q0s), np.asarray(qmaxs), \
+        np.asarray(hmaxs)
+    net.pump_curve_q, net.pump_curve_h = np.asarray(cq), np.asarray(ch)
+    net.pump_curve_ptr = np.asarray(cptr, dtype=np.int32)
+
+
+def _read_controls(net, rows, idx, lidx, hcf, pcf, path):
+    """Simple controls (controldata, input3.c; dgga/parse.py:862-930):
+    LINK id OPEN|CLOSED IF NODE id ABOVE|BELOW value, or AT TIME / AT
+    CLOCKTIME.  Level thresholds become heads (tank: El + v / Ucf[ELEV];
+    junction: El + v / Ucf[PRESSURE])."""
+    cl, cn, ct, cg, ctm, cs = ([] for _ in range(6))
+    for tk in rows:
+        if len(tk) < 6 or tk[0].upper() != "LINK":
+            raise NotImplementedError(f"{path}: control {' '.join(tk)!r} is not a simple "
+                                      "LINK control")
+        k = lidx[tk[1]]
+        if net.link_type[k] == CVPIPE:
+            raise ValueError(f"{path}: a control cannot act on a check valve")
+        word = tk[2].upper()
+        if word.startswith("OPEN"):
+            status = ST_OPEN
+        elif word.startswith("CLOSED"):
+            status = ST_CLOSED
+        else:
+            raise NotImplementedError(f"{path}: control settings other than OPEN/CLOSED")
+        u4 = tk[4].upper()
+        node, grade, tsec = -1, 0.0, 0
+        if u4.startswith("TIME") or u4.startswith("CLOCKTIME"):
+            ctype = 2 if u4.startswith("TIME") else 3
+            tsec = int(3600.0 * _hours(tk[5], tk[6] if len(tk) > 6 else ""))
+            if ctype == 3:
+                tsec %= 86400
+        else:
+            node = idx[tk[5]]
+            u6 = tk[6].upper()
+            if u6.startswith("BELOW"):
+                ctype = 0
+            elif u6.startswith("ABOVE"):
+                ctype = 1
+            else:
+                raise ValueError(f"{path}: control condition {tk[6]!r}")
+            v = float(tk[7])
+            el = float(net.elev_ft[node])
+            grade = el + v / (hcf if net.node_type[node] != JUNCTION else pcf)
+        cl.append(k); cn.append(node); ct.append(ctype); cg.append(grade)
+        ctm.append(tsec); cs.append(status)
+    net.ctl_link = np.asarray(cl, dtype=np.int32)
+    net.ctl_node = np.asarray(cn, dtype=np.int32)
+    net.ctl_type = np.asarray(ct, dtype=np.int8)
+    net.ctl_grade = np.asarray(cg, dtype=np.float64)
+    net.ctl_time = np.asarray(ctm, dtype=np.int64)
+    net.ctl_status = np.asarray(cs, dtype=np.int8)

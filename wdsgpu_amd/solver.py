@@ -429,6 +429,15 @@ class GGASolver:
             self._pinned_adj = {B: buf}
         return buf
 
+    @staticmethod
+    def _check_shape(t, n, B, what):
+        """t must be [n] or [1|B, n] (ValueError otherwise)."""
+        ok = (t.dim() == 1 and t.shape[0] == n) or \
+            (t.dim() == 2 and t.shape[1] == n and t.shape[0] in (1, B))
+        if not ok:
+            raise ValueError("%s must be [%d] or [1|%d, %d], got %s"
+                             % (what, n, B, n, tuple(t.shape)))
+
     def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
               status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True,
               status0=None):
@@ -440,20 +449,23 @@ class GGASolver:
         if rh.dim() == 1:
             rh = rh.unsqueeze(0)
         B = int(d.shape[0])
-        if d.shape[1] != self.N or rh.shape[-1] != self.N or rh.shape[0] not in (1, B):
+        if d.dim() != 2 or rh.dim() != 2 or d.shape[1] != self.N or \
+                rh.shape[-1] != self.N or rh.shape[0] not in (1, B) or B < 1:
             raise ValueError("demand/res_head must be [N] or [B,N] with N=%d" % self.N)
         # one shared vector is passed as is (no [B,N] copy); NaN heads
         # (junction entries) read as 0 in the kernel
         rh_shared = rh.shape[0] == 1 and B > 1
         rh = rh.contiguous()
+        # every per-scenario array the kernels read is checked against B, N
+        # and L here: the kernels index them without bounds checks
         ke = self.node_ke_default if ke_int is None else self._prep(ke_int)
+        self._check_shape(ke, self.N, B, "ke")
         if ke.dim() == 2 and ke.shape[0] == 1:
             ke = ke[0]
         ke = ke.contiguous()
         ke_stride = self.N if ke.dim() == 2 else 0
-        if ke.dim() == 2 and ke.shape[0] != B:
-            raise ValueError("ke must be [N] or [B,N]")
         r = self.r_hw if r_hw is None else self._prep(r_hw)
+        self._check_shape(r, self.L, B, "r_hw")
         if r.dim() == 2 and r.shape[0] == 1:
             r = r[0]
         r = r.contiguous()
@@ -475,8 +487,15 @@ class GGASolver:
             if want_factor else None)
         # info and iters share one buffer: one copy to the host reads both
         out["info"], out["iters"] = out["st"][:B], out["st"][B:]
-        q0t = None if q0 is None else self._prep(q0).expand(B, -1).contiguous()
-        e0t = None if e0 is None else self._prep(e0).expand(B, -1).contiguous()
+        q0t = e0t = None
+        if q0 is not None:
+            q0t = self._prep(q0)
+            self._check_shape(q0t, self.L, B, "q0")
+            q0t = q0t.expand(B, -1).contiguous()
+        if e0 is not None:
+            e0t = self._prep(e0)
+            self._check_shape(e0t, self.N, B, "e0")
+            e0t = e0t.expand(B, -1).contiguous()
         s0t = None
         if status0 is not None:
             s0t = torch.as_tensor(status0, dtype=torch.int8, device=dev)
@@ -532,9 +551,13 @@ class GGASolver:
 
     def solve(self, demand_cfs, res_head_ft, ke_int=None, q0=None, e0=None,
               max_iter=None, accuracy=None, link_status=None, link_setting=None,
-              status_machine=False, assemble="csr", linear_solver="hip",
+              status_machine=False, assemble="dense", linear_solver="dense",
               record_schedule=False, r_hw=None):
-        """Batched steady-state solve (dgga/solver.py:2703-3212).  Returns the
+        """Batched steady-state solve (dgga/solver.py:2703-3212), with the
+        reference's keyword defaults; every assemble / linear_solver value
+        this path accepts ("dense", "csr"; "dense", "hip") runs the same HIP
+        sparse LDL^T solve, whose results match the reference's dense
+        Cholesky path (the keyword only names the route).  Returns the
         reference's dict: head_ft [B,N], flow_cfs [B,L], emitter_cfs [B,N],
         iters [B], relerr [B], converged [B], status [B,L] (status machine),
         diag_ratio [B] (networks with PRVs).  1-D inputs give 1-D outputs."""
