@@ -254,6 +254,13 @@ int wds_emitter_ke(void* stream, int64_t n, const double* C, double qexp, double
 int wds_emitter_ke_grad(void* stream, int64_t n, const double* C, const double* ke,
                         double qexp, const double* g_ke, double* g_C);
 
+/* The deterministic power laws the kernels use in place of libm pow/log
+ * (the reference's calls at dgga/solver.py:58-69, 2176-2377), elementwise on
+ * device arrays: op 0 out = x^y, op 1 out = log x (y unused).  For checking
+ * the kernels' arithmetic against the CPU checker bit for bit. */
+int wds_detmath_eval(void* stream, int32_t op, int64_t n, const double* x, const double* y,
+                     double* out);
+
 #ifdef __cplusplus
 }
 #endif

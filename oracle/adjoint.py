@@ -23,6 +23,7 @@ from scipy.sparse.linalg import splu
 from wdsgpu_amd.network import CSMALL, CBIG, MISSING, ST_CLOSED, ST_ACTIVE, PRV, TCV
 
 from . import gga
+from .detmath import det_pow
 
 
 def link_coeffs(m, q, r_hw, status):
@@ -39,11 +40,11 @@ def link_coeffs(m, q, r_hw, status):
         hg = np.where(pipe, hg_p, m.rqtol)
         dr = np.where(pipe, dr_p, 0.0)
     else:
-        hg = m.hexp * r_hw * qa ** (m.hexp - 1.0)
+        hg = m.hexp * r_hw * det_pow(qa, m.hexp - 1.0)
         lin = hg < m.rqtol
         hg = np.where(lin, m.rqtol, hg)
         hl = np.where(lin, hg * qa, hg * qa / m.hexp)
-        dr = np.where(lin, 0.0, qa ** m.hexp)
+        dr = np.where(lin, 0.0, det_pow(qa, m.hexp))
         mlp = m.km_pipe > 0.0
         hl = np.where(mlp, hl + m.km_pipe * qa * qa, hl)
         hg = np.where(mlp, hg + 2.0 * m.km_pipe * qa, hg)
@@ -110,7 +111,7 @@ def pump_coeffs(m, q, hl, hg, dr):
         sp_ = float(m.pump_speed[k])
         h0 = sp_ * sp_ * m.pl_h0[k]
         n = float(m.pump_n[k])
-        r = m.pl_r[k] * sp_ ** (2.0 - n)
+        r = m.pl_r[k] * float(det_pow(sp_, 2.0 - n))
         if m.pl_ptype[k] == 0:                       # CONST_HP
             g = -r / qa / qa if qa > 0.0 else np.inf
             if g > CBIG:
@@ -120,7 +121,7 @@ def pump_coeffs(m, q, hl, hg, dr):
             else:
                 hl[k], hg[k] = r / qk, g
         elif n != 1.0:                               # power function
-            g = n * r * qa ** (n - 1.0)
+            g = n * r * float(det_pow(qa, n - 1.0))
             if g < m.rqtol:
                 hl[k], hg[k] = h0 + m.rqtol * qk, m.rqtol
             else:
@@ -134,11 +135,11 @@ def emitter_coeffs(m, e, ke):
     """dgga/autodiff.py:365-377."""
     ke_adj = np.maximum(CSMALL, ke)
     ea = np.abs(e)
-    hg = m.qexp * ke_adj * ea ** (m.qexp - 1.0)
+    hg = m.qexp * ke_adj * det_pow(ea, m.qexp - 1.0)
     lin = hg < m.rqtol
     hg = np.where(lin, m.rqtol, hg)
     hl = np.where(lin, hg * e, hg * e / m.qexp)
-    dke = np.where(lin | (ke < CSMALL), 0.0, ea ** (m.qexp - 1.0) * e)
+    dke = np.where(lin | (ke < CSMALL), 0.0, det_pow(ea, m.qexp - 1.0) * e)
     return hl, hg, dke
 
 

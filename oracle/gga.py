@@ -20,6 +20,8 @@ from __future__ import annotations
 import numpy as np
 import scipy.linalg as sla
 
+from .detmath import det_log, det_pow
+
 from wdsgpu_amd.network import (
     CSMALL, CBIG, BIG, TINY, MISSING, QZERO, HEXP_HW, PI_EPANET,
     DW_A1, DW_A2, DW_A8, DW_A9, DW_AB, DW_AC,
@@ -238,6 +240,10 @@ class GGAModel:
             q = np.where(self.is_pump, self.init_setting * self.pl_q0, q)
         return np.where(self.closed, QZERO, q)
 
+    # Power laws and logarithms use the deterministic pow / log that the HIP
+    # kernels share (oracle/detmath.py): the reference's libm calls
+    # (dgga/solver.py:58-69) agree with it to one ulp, and the two paths of
+    # this build then compute identical coefficients.
     # When set to a numpy Generator, every power-law and logarithm evaluation
     # (pipe, Darcy-Weisbach friction, pump, emitter) is perturbed by -1, 0 or
     # +1 ulp at random: the oracle's own sensitivity to one
@@ -246,14 +252,14 @@ class GGAModel:
     pow_jitter = None
 
     def _pow(self, x, y):
-        p = x ** y
+        p = det_pow(x, y)
         if self.pow_jitter is not None:
             u = self.pow_jitter.integers(-1, 2, size=np.shape(p))
             p = p * (1.0 + u * 2.0 ** -52)
         return p
 
     def _log(self, x):
-        p = np.log(x)
+        p = det_log(x)
         if self.pow_jitter is not None:
             u = self.pow_jitter.integers(-1, 2, size=np.shape(p))
             p = p * (1.0 + u * 2.0 ** -52)
@@ -663,6 +669,82 @@ def wave_tree_sum(x):
     return p[:, 0]
 
 
+def _two_prod_v(a, b):
+    """Exact product a*b = p + e (Veltkamp splitting, elementwise)."""
+    p = a * b
+    t = 134217729.0 * a
+    ah = t - (t - a)
+    al = a - ah
+    t = 134217729.0 * b
+    bh = t - (t - b)
+    bl = b - bh
+    return p, ((ah * bh - p) + ah * bl + al * bh) + al * bl
+
+
+def csr_residual_dd(m, vals, x, F):
+    """F - A x for CSR values [B, nnz] (the model's CSR pattern), every
+    product split exactly and every row summed in double-double, rounded
+    once: the residual to about one rounding of its own size."""
+    if not hasattr(m, "_ell"):
+        rows = m.csr_row
+        start = np.searchsorted(rows, np.arange(m.Nj))
+        cnt = np.bincount(rows, minlength=m.Nj)
+        ell = np.full((m.Nj, int(cnt.max())), -1, dtype=np.int64)
+        for w in range(ell.shape[1]):
+            has = cnt > w
+            ell[has, w] = start[has] + w
+        m._ell = ell
+    ell = m._ell
+    hi = F.copy()
+    lo = np.zeros_like(F)
+    xc = x[:, m.csr_col]
+    p, e = _two_prod_v(vals, xc)
+    for w in range(ell.shape[1]):
+        idx = ell[:, w]
+        ok = idx >= 0
+        pw = np.where(ok, p[:, np.maximum(idx, 0)], 0.0)
+        ew = np.where(ok, e[:, np.maximum(idx, 0)], 0.0)
+        s = hi - pw
+        bb = s - hi
+        err = (hi - (s - bb)) + (-pw - bb)
+        hi = s
+        lo = lo + (err - ew)
+    return hi + lo
+
+
+def sparse_solve_refined(m, vals, F, active=None, n_refine=4):
+    """A H = F per scenario with SuperLU on the CSR values [B, nnz], then
+    n_refine steps of refinement with the double-double residual: the
+    correctly rounded solution while kappa * eps < 1 (the same target as
+    chol_solve_refined(..., extended=True)).  Raises RuntimeError naming
+    singular samples.  Rows of inactive scenarios are returned as zeros."""
+    import scipy.sparse as sps
+    from scipy.sparse.linalg import splu
+    B, Nj = F.shape
+    H = np.zeros_like(F)
+    todo = np.arange(B) if active is None else np.nonzero(active)[0]
+    lus = {}
+    bad = []
+    for b in todo:
+        A = sps.csc_matrix((vals[b], (m.csr_row, m.csr_col)), shape=(Nj, Nj))
+        try:
+            lus[b] = splu(A, permc_spec="MMD_AT_PLUS_A")
+        except RuntimeError:
+            bad.append(int(b))
+            continue
+        H[b] = lus[b].solve(F[b])
+    if bad:
+        raise RuntimeError("sparse factorisation failed: samples %s are singular" % bad)
+    idx = np.asarray(sorted(lus))
+    if idx.size == 0:
+        return H
+    for _ in range(n_refine):
+        r = csr_residual_dd(m, vals[idx], H[idx], F[idx])
+        for i, b in enumerate(idx):
+            H[b] = H[b] + lus[b].solve(r[i])
+    return H
+
+
 def chol_solve_refined(A, F, n_refine=2, extended=False):
     """Dense Cholesky + cholesky_solve + n_refine refinement steps with the
     residual A*H summed along the last axis (dgga/solver.py:3037-3064).
@@ -701,18 +783,24 @@ def chol_solve_refined(A, F, n_refine=2, extended=False):
 def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
           max_iter=None, accuracy=None, status_machine=True, assemble="dense",
           n_refine=2, record=False, extended=False, status0=None,
-          record_schedule=False):
+          record_schedule=False, linsolve="dense"):
     """Batched dense GGA (dgga/solver.py:2703-3212, mode='dense',
     dense_status_machine=True).  demand/res_head/ke: [B,N] or [N];
     r_hw: [L] or [B,L].  Returns the reference's output dict (head_ft,
     flow_cfs, emitter_cfs, iters, relerr, converged, status, diag_ratio)
-    plus, when ``record``, per-iteration traces.  ``extended`` evaluates the
-    refinement residuals in extended precision (see chol_solve_refined).
+    plus, when ``record``, per-iteration traces.  ``extended`` replaces each
+    linear solve by its correctly rounded solution (sparse_solve_refined),
+    which separates the reference's own rounding noise from a real
+    discrepancy in a comparison.
     ``status0`` ([L] or [B,L] internal codes) replaces the initial statuses
     (the warm start of an extended-period frame, as run_gga's status0 in the
     reference's EPANET-mode solve, dgga/solver.py:2855).  ``record_schedule``
     adds out["schedule"]: the status machine's events in the reference's
-    layout (dgga/solver.py:3136-3160)."""
+    layout (dgga/solver.py:3136-3160).  ``linsolve="sparse"`` solves each
+    scenario's CSR system with SuperLU refined to the correctly rounded
+    solution (sparse_solve_refined), the result the dense path gives with
+    ``extended``, at a fraction of its cost on large networks; scenarios
+    already converged are skipped (their heads are frozen anyway)."""
     m = model
     d = np.atleast_2d(np.asarray(demand, dtype=np.float64))
     rh = np.atleast_2d(np.asarray(res_head, dtype=np.float64))
@@ -757,19 +845,30 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
         nextcheck = np.full(B, m.checkfreq, dtype=np.int64)
     trace = dict(status=[], active=[], relerr=[]) if record else None
     sched = [] if (record_schedule and status_machine) else None
-    csr = assemble == "csr"
+    if extended:
+        # the correctly rounded solve: double-double residuals (an x87
+        # extended residual is one ulp off on ~1 % of the heads here)
+        linsolve = "sparse"
+    csr = assemble == "csr" or linsolve == "sparse"
     for it in range(1, maxtrials + 1):
         P, Y = m.link_PY(q, S if sm else np.broadcast_to(
             np.where(m.closed_dense, ST_CLOSED, ST_OPEN), (B, m.L)), r)
         A, F, Xflow, P, Y, em, hloss_e, hgrad_e = m.assemble(
             P, Y, q, H, e_j, ke_j, d_j, S, csr=csr)
-        if csr:
-            A = m.csr_to_dense(A)
-        if m.n_pcv:
-            dn = np.abs(np.diagonal(A, axis1=1, axis2=2))
-            dr_now = dn.max(1) / np.maximum(dn.min(1), 1e-300)
-            diag_ratio = np.where(active, dr_now, diag_ratio)
-        Hj = chol_solve_refined(A, F, n_refine, extended)
+        if linsolve == "sparse":
+            if m.n_pcv:
+                dn = np.abs(A[:, m.csr_diag])
+                dr_now = dn.max(1) / np.maximum(dn.min(1), 1e-300)
+                diag_ratio = np.where(active, dr_now, diag_ratio)
+            Hj = sparse_solve_refined(m, A, F, active)
+        else:
+            if csr:
+                A = m.csr_to_dense(A)
+            if m.n_pcv:
+                dn = np.abs(np.diagonal(A, axis1=1, axis2=2))
+                dr_now = dn.max(1) / np.maximum(dn.min(1), 1e-300)
+                diag_ratio = np.where(active, dr_now, diag_ratio)
+            Hj = chol_solve_refined(A, F, n_refine, extended)
         H_junc = np.where(active[:, None], Hj, H_junc)
         H = m.full_heads(H_junc, rh_fix)
         dh = H[:, m.n1] - H[:, m.n2]

@@ -43,11 +43,16 @@ def res_heads(net):
     return np.nan_to_num(net.reservoir_head_ft())
 
 
-def check_forward_against_oracle(net, m, out, d, rh, acc, tol_h_ft=1e-10 / 0.3048):
+def check_forward_against_oracle(net, m, out, d, rh, acc, tol_h_ft=1e-10 / 0.3048,
+                                 tol_q=1e-9):
     """GPU forward output vs the oracle on the same inputs: identical Newton
-    iteration counts and final statuses; heads and flows within 3x the
-    oracle's own sensitivity to a 1-ulp perturbation of its power laws (the
-    reference's cross <= 3x self-jitter acceptance) or 1e-10 m."""
+    iteration counts and final statuses, heads within 1e-10 m of the
+    oracle's refined iterate and flows within tol_q cfs.  Both paths evaluate
+    the power laws with the same deterministic pow/log, so their iterates
+    differ only by the roundings of the linear solves (the oracle refines in
+    extended precision, the kernels in double-double).  The measured
+    differences and, for scale, the oracle's own change under a one-ulp
+    perturbation of its power laws are printed."""
     from oracle import gga
     ke = net.node_ke
     ext = gga.solve(m, d, rh, ke=ke, accuracy=acc, status_machine=True, extended=True)
@@ -61,12 +66,14 @@ def check_forward_against_oracle(net, m, out, d, rh, acc, tol_h_ft=1e-10 / 0.304
         jit = gga.solve(m, d, rh, ke=ke, accuracy=acc, status_machine=True, extended=True)
     finally:
         m.pow_jitter = None
-    assert jit["iters"].tolist() == ext["iters"].tolist()
     noise = np.abs(jit["head_ft"] - ext["head_ft"]).max()
     Hg = out["head_ft"].cpu().numpy()
-    assert np.abs(Hg - ext["head_ft"]).max() <= max(3.0 * noise, tol_h_ft)
-    qn = np.abs(jit["flow_cfs"] - ext["flow_cfs"]).max()
-    assert np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max() <= max(3.0 * qn, 1e-12)
+    dh = np.abs(Hg - ext["head_ft"]).max()
+    dq = np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max()
+    print("max|dH| %.3e ft (1-ulp pow jitter of the oracle %.3e ft), max|dQ| %.3e cfs"
+          % (dh, noise, dq), flush=True)
+    assert dh <= tol_h_ft
+    assert dq <= tol_q
     return ref
 
 

@@ -81,24 +81,14 @@ def test_forward_matches_oracle(solvers, name, B):
     assert out["iters"].cpu().numpy().tolist() == ext["iters"].tolist()
     assert np.array_equal(out["status"].cpu().numpy(), ref["status"])
     assert bool(out["converged"].all())
-    # The last Newton iterate of a loosely converged solve (relerr ~1e-4 for
-    # accuracy 1e-3) carries every earlier rounding through the flow update:
-    # the oracle itself moves by ~1e-9 ft when its power laws are perturbed
-    # by one ulp.  Accept within 3x that self-jitter (the reference's
-    # acceptance practice, data/sparse_gpu_plan.md:404-445) or 1e-10 m.
-    m.pow_jitter = np.random.default_rng(123)
-    try:
-        jit = gga.solve(m, d, rh, ke=net.node_ke, accuracy=acc, status_machine=True,
-                        extended=True)
-    finally:
-        m.pow_jitter = None
-    assert jit["iters"].tolist() == ext["iters"].tolist()
-    noise = np.abs(jit["head_ft"] - ext["head_ft"]).max()
-    assert np.abs(Hg - ext["head_ft"]).max() <= max(3.0 * noise, TOL_H_FT)
-    assert np.abs(Hg - ref["head_ft"]).max() <= max(3.0 * noise, TOL_H_FT) + \
-        np.abs(ref["head_ft"] - ext["head_ft"]).max()
-    qn = np.abs(jit["flow_cfs"] - ext["flow_cfs"]).max()
-    assert np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max() <= max(3.0 * qn, 1e-12)
+    # Both paths evaluate the power laws with the same deterministic pow/log,
+    # so the iterates differ only by the linear solves' roundings: heads
+    # within 1e-10 m of the oracle's extended-precision-refined iterate
+    dh = np.abs(Hg - ext["head_ft"]).max()
+    dq = np.abs(out["flow_cfs"].cpu().numpy() - ext["flow_cfs"]).max()
+    print("%s B=%d: max|dH| %.3e ft, max|dQ| %.3e cfs" % (name, B, dh, dq), flush=True)
+    assert dh <= TOL_H_FT
+    assert dq <= 1e-9
     if m.n_pcv:
         dr = out["diag_ratio"].cpu().numpy()
         assert np.allclose(dr, ref["diag_ratio"], rtol=1e-9)

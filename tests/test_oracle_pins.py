@@ -310,3 +310,36 @@ def test_relerr_tree_is_the_lane_butterfly():
                 lanes = [lanes[l] + lanes[l ^ mm] for l in range(64)]
             assert got[b] == lanes[0]
             assert abs(got[b] - math.fsum(x[b])) <= 2 * n * 2.0 ** -53 * math.fsum(x[b])
+
+
+# ---------------------------------------------------------------- linear solve
+def test_sparse_refined_solve_is_correctly_rounded():
+    # the oracle's reference solve (extended=True): every entry within half
+    # an ulp of the exact solution, which an exact rational residual and
+    # one more correction reveal
+    from fractions import Fraction
+    import scipy.sparse as sps
+    from scipy.sparse.linalg import splu
+    from wdsgpu_amd.generators import grid_network, scenario_batch
+    net = grid_network(0)
+    m = gga.GGAModel(net)
+    B = 2
+    d, rh = scenario_batch(net, B, seed=9)
+    rh = np.nan_to_num(rh)
+    q = np.broadcast_to(m.init_flow(), (B, m.L)).copy()
+    H = m.full_heads(np.zeros((B, m.Nj)), rh[:, m.fixed_nodes])
+    S = np.broadcast_to(m.init_status_int, (B, m.L)).copy()
+    P, Y = m.link_PY(q, S, m.r_hw[None])
+    csr, F, *_ = m.assemble(P, Y, q, H, np.zeros((B, m.Nj)), np.zeros((B, m.Nj)),
+                            d[:, m.junc_nodes], S, csr=True)
+    x = gga.sparse_solve_refined(m, csr, F)
+    for b in range(B):
+        A = sps.csc_matrix((csr[b], (m.csr_row, m.csr_col)), shape=(m.Nj, m.Nj))
+        r = [Fraction(v) for v in F[b]]
+        for k in range(csr.shape[1]):
+            r[m.csr_row[k]] -= Fraction(csr[b, k]) * Fraction(x[b, m.csr_col[k]])
+        delta = splu(A).solve(np.array([float(v) for v in r]))
+        assert np.all(np.abs(delta) <= 0.5000001 * np.spacing(np.abs(x[b])))
+    # the dense Cholesky path with plain refinement lands within a few ulps
+    xd = gga.chol_solve_refined(m.csr_to_dense(csr), F)
+    assert np.all(np.abs(xd - x) <= 8 * np.spacing(np.abs(x)))

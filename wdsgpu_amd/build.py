@@ -16,7 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwds.so")
 SOURCES = ["kernels.hip", "plan.cpp", "symbolic.cpp"]
-HEADERS = ["devplan.h", "device_common.h", "kernels_impl.h", "kernels_polish.h", "plan.h",
+HEADERS = ["detmath.h", "devplan.h", "device_common.h", "kernels_impl.h", "kernels_polish.h", "plan.h",
            "symbolic.h"]
 
 

@@ -3,6 +3,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "detmath.h"
 #include "devplan.h"
 #include "symbolic.h"
 
@@ -537,7 +538,7 @@ __device__ __forceinline__ void hw_pipe(const DevPlan& pl, double q, double r, d
                                         double& hloss, double& hgrad) {
   const double sgn = q < 0.0 ? -1.0 : 1.0;
   const double qa = q * sgn;
-  double hg = pl.hexp * r * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+  double hg = pl.hexp * r * det_pow(fmax(qa, 1e-30), pl.hexp - 1.0);
   const bool lin = hg < pl.rqtol;
   if (lin) hg = pl.rqtol;
   double hl = lin ? hg * qa : hg * qa / pl.hexp;
@@ -556,14 +557,14 @@ __device__ __forceinline__ void dw_friction(double qa, double s, double e, doubl
                                             double& dfdq) {
   const double w = qa / s;
   if (w >= DW_A1) {
-    const double y1 = DW_A8 / pow(w, 0.9);
+    const double y1 = DW_A8 / det_pow(w, 0.9);
     const double y2 = e / 3.7 + y1;
-    const double y3 = DW_A9 * log(y2);
+    const double y3 = DW_A9 * det_log(y2);
     f = 1.0 / (y3 * y3);
     dfdq = 1.8 * f * y1 * DW_A9 / y2 / y3 / qa;
   } else {
     const double y2 = e / 3.7 + DW_AB;
-    const double y3 = DW_A9 * log(y2);
+    const double y3 = DW_A9 * det_log(y2);
     const double fa = 1.0 / (y3 * y3);
     const double fb = (2.0 + DW_AC / (y2 * y3)) * fa;
     const double r2 = w / DW_A2;
@@ -664,7 +665,7 @@ __device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double 
   const double spc = fmax(pl.pu_speed[k], 1e-12);
   const double n = pl.pu_n[k];
   const double h0e = spc * spc * pl.pu_h0[k];
-  const double re = pl.pu_r[k] * pow(spc, 2.0 - n);
+  const double re = pl.pu_r[k] * det_pow(spc, 2.0 - n);
   double hg, hl;
   if (f & LF_PUMP_CHP) {
     const double qa_c = fmax(qa, 1e-30);
@@ -681,7 +682,7 @@ __device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double 
     if (qa == 0.0)
       pw = nm1 > 0.0 ? 0.0 : (nm1 < 0.0 ? __builtin_huge_val() : 1.0);
     else
-      pw = pow(qa, nm1);
+      pw = det_pow(qa, nm1);
     const double raw = n * re * pw;
     const bool sml = raw < pl.rqtol;
     hg = sml ? pl.rqtol : raw;
@@ -814,7 +815,7 @@ __device__ inline void link_coeffs_all(const DevPlan& pl, const double* q, const
 __device__ __forceinline__ void emitter_hl(const DevPlan& pl, double e, double ke, double& hloss,
                                            double& hgrad) {
   const double ke_adj = fmax(ke, CSMALL);
-  double hg = pl.qexp * ke_adj * pow(fmax(fabs(e), 1e-30), pl.qexp - 1.0);
+  double hg = pl.qexp * ke_adj * det_pow(fmax(fabs(e), 1e-30), pl.qexp - 1.0);
   const bool lin = hg < pl.rqtol;
   if (lin) hg = pl.rqtol;
   hloss = lin ? hg * e : hg * e / pl.qexp;

@@ -517,6 +517,14 @@ __global__ void k_emitter_ke_grad(int64_t n, const double* C, const double* ke, 
   }
 }
 
+// deterministic power laws on arrays (op 0: x^y, op 1: log x)
+__global__ void k_detmath(int32_t op, int64_t n, const double* x, const double* y,
+                          double* out) {
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x)
+    out[i] = op == 0 ? det_pow(x[i], y[i]) : det_log(x[i]);
+}
+
 dim3 ew_grid(int64_t n) {
   int64_t g = (n + 255) / 256;
   return dim3((unsigned)std::min<int64_t>(std::max<int64_t>(g, 1), 4096));
@@ -571,4 +579,16 @@ extern "C" int wds_emitter_ke_grad(void* stream, int64_t n, const double* C, con
   hipLaunchKernelGGL(k_emitter_ke_grad, ew_grid(n), dim3(256), 0, (hipStream_t)stream, n, C,
                      ke, qexp, g_ke, g_C);
   return launch_check("k_emitter_ke_grad");
+}
+
+extern "C" int wds_detmath_eval(void* stream, int32_t op, int64_t n, const double* x,
+                                const double* y, double* out) {
+  if (n < 0 || (op != 0 && op != 1) || (n > 0 && (!x || !out || (op == 0 && !y)))) {
+    set_error("wds_detmath_eval: bad argument");
+    return WDS_EINVAL;
+  }
+  if (n == 0) return WDS_OK;
+  hipLaunchKernelGGL(k_detmath, ew_grid(n), dim3(256), 0, (hipStream_t)stream, op, n, x, y,
+                     out);
+  return launch_check("k_detmath");
 }

@@ -486,7 +486,7 @@ __device__ inline bool chp_on_clamp(const DevPlan& pl, const Scn& c) {
     if (!(f & LF_PUMP_CHP) || c.S[k] <= ST_CLOSED || (f & LF_ZERO_SPEED)) continue;
     const double qa = fmax(fabs(c.q[k]), 1e-30);
     const double spc = fmax(pl.pu_speed[k], 1e-12);
-    const double re = pl.pu_r[k] * pow(spc, 2.0 - pl.pu_n[k]);
+    const double re = pl.pu_r[k] * det_pow(spc, 2.0 - pl.pu_n[k]);
     const double raw = -re / qa / qa;
     bad = bad || raw > CBIG || raw < pl.rqtol;
   }
@@ -780,8 +780,8 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
         const double q = qv[u];
         const double sgn = q < 0.0 ? -1.0 : 1.0;
         const double qa = q * sgn;
-        const double hgr = pl.hexp * rv[u] * pow(fmax(qa, 1e-30), pl.hexp - 1.0);
-        if (hgr >= pl.rqtol) dr = pow(qa, pl.hexp) * sgn;
+        const double hgr = pl.hexp * rv[u] * det_pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+        if (hgr >= pl.rqtol) dr = det_pow(qa, pl.hexp) * sgn;
       }
       gr[k] = -(ll * dr);
     }
@@ -807,7 +807,7 @@ __global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
       if (kv[u] > 0.0) {
         const double lame = (GE(node) + lam[j]) / hgv[u];
         const double ea = fabs(ev[u]);
-        const double pw = pow(fmax(ea, 1e-30), pl.qexp - 1.0);
+        const double pw = det_pow(fmax(ea, 1e-30), pl.qexp - 1.0);
         const double hraw = pl.qexp * fmax(kv[u], CSMALL) * pw;
         const double dke = (hraw < pl.rqtol || kv[u] < CSMALL) ? 0.0 : pw * ev[u];
         g = -(lame * dke);
