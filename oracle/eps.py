@@ -27,11 +27,13 @@ def _round_c(x):
     return int(x + 0.5) if x >= 0.0 else int(x - 0.5)
 
 
-def run_eps(net, demand_scale=None, accuracy=None, max_frames=100000, pow_jitter=None):
+def run_eps(net, demand_scale=None, accuracy=None, max_frames=100000, pow_jitter=None,
+            extended=False):
     """One scenario; returns frames dict of arrays (t_sec, head_ft, flow_cfs,
     status, iterations, relerr, tank_head_ft).  ``pow_jitter`` (a numpy
     Generator) perturbs the power laws by one ulp, which measures the
-    trajectory's own sensitivity to rounding."""
+    trajectory's own sensitivity to rounding.  ``extended`` solves every
+    frame with the correctly rounded linear solves (gga.solve)."""
     m = gga.GGAModel(net)
     m.pow_jitter = pow_jitter
     meta = net.meta
@@ -100,7 +102,7 @@ def run_eps(net, demand_scale=None, accuracy=None, max_frames=100000, pow_jitter
                     S[k] = s2
         # hydraulic solve, warm started
         out = gga.solve(m, d, H, ke=net.node_ke, q0=q, e0=e, accuracy=hacc,
-                        status_machine=True, status0=S)
+                        status_machine=True, status0=S, extended=extended)
         q, S = out["flow_cfs"], out["status"]
         e = out["emitter_cfs"]
         H = out["head_ft"].copy()
