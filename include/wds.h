@@ -196,6 +196,22 @@ typedef struct wds_solve_args {
  * dgga/autodiff.py:1171-1243). */
 int wds_solve(wds_plan* plan, void* stream, const wds_solve_args* args);
 
+/* wds_solve without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * allocation).  args->workspace must be NULL: the workspace is a file-scope
+ * device array of WDS_NOALLOC_SCRATCH_BYTES (32 MiB) that every GPU the
+ * library runs on holds until the process ends; calls needing more
+ * (wds_workspace_bytes(plan, B) > WDS_NOALLOC_SCRATCH_BYTES: about 230
+ * scenarios of a 782-junction network) fail with WDS_EINVAL.  The plan must
+ * already be uploaded to the current device (wds_plan_upload allocates).
+ * All calls on one GPU share that GPU's array, and its contents are those of
+ * the last call: calls that overlap on one GPU (different streams, threads
+ * or plans) race and must be serialised by the caller.  Otherwise identical
+ * to wds_solve, bit for bit. */
+#define WDS_NOALLOC_SCRATCH_BYTES (32ll << 20)
+int wds_solve_noalloc(wds_plan* plan, void* stream, const wds_solve_args* args);
+
 /* One Newton iteration of every active scenario with the state kept in
  * global memory (the unfused path: one launch per iteration, host loop).
  * state: q [B][L], e [B][N] (by node), head [B][N], status [B][L],

@@ -98,6 +98,7 @@ SIGNATURES = {
     "wds_trisolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "wds_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
+    "wds_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
     "wds_newton_iter": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs),
                                   C.c_int32, C.c_void_p]),
     "wds_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdjointArgs)]),

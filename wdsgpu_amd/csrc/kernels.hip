@@ -14,6 +14,7 @@
 #include <cstring>
 #include <stdexcept>
 #include <string>
+#include <mutex>
 #include <vector>
 
 #include "plan.h"
@@ -432,6 +433,78 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
   hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
   P->cnt.launches += 1;
   return launch_check("k_solve");
+}
+
+// ---------------------------------------------------------------------------
+// wds_solve_noalloc: the fused forward with its workspace in a file-scope
+// device array instead of caller memory, for callers that cannot allocate.
+// Every GPU the library runs on holds its own copy of the array until the
+// process ends; host code addresses it only through the pointer
+// hipGetSymbolAddress returns for the current device (the array's name is a
+// host-side stand-in, not a device address).
+static __device__ __align__(256) double scratch[WDS_NOALLOC_SCRATCH_BYTES / 8];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch[kMaxDevices] = {};
+
+// device address of this device's scratch array, looked up once per device
+// with that device current
+int scratch_of(int dev, void** out) {
+  if (dev < 0 || dev >= kMaxDevices) {
+    set_error("wds_solve_noalloc: device index out of range");
+    return WDS_EDEVICE;
+  }
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch[dev]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch[dev] = p;
+  }
+  *out = g_scratch[dev];
+  return WDS_OK;
+}
+}  // namespace
+
+extern "C" int wds_solve_noalloc(wds_plan* P, void* stream, const wds_solve_args* a) {
+  if (!P || !a) {
+    set_error("wds_solve_noalloc: null argument");
+    return WDS_EINVAL;
+  }
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  // the plan upload allocates: it must have happened already
+  if (!P->d_blob || P->device != dev) {
+    set_error("wds_solve_noalloc: the plan is not uploaded to the current device "
+              "(call wds_plan_upload first)");
+    return WDS_EINVAL;
+  }
+  if (a->workspace) {
+    set_error("wds_solve_noalloc: workspace must be NULL (the library's scratch is used)");
+    return WDS_EINVAL;
+  }
+  if (a->B <= 0) {
+    set_error("wds_solve_noalloc: bad argument");
+    return WDS_EINVAL;
+  }
+  const int64_t need = wds_workspace_bytes(P, a->B);
+  if (need > (int64_t)WDS_NOALLOC_SCRATCH_BYTES) {
+    set_error("wds_solve_noalloc: " + std::to_string(a->B) + " scenarios need " +
+              std::to_string(need) + " bytes of scratch, more than the " +
+              std::to_string((int64_t)WDS_NOALLOC_SCRATCH_BYTES) + " available (at most " +
+              std::to_string((int64_t)WDS_NOALLOC_SCRATCH_BYTES / (need / a->B)) +
+              " scenarios of this network per call)");
+    return WDS_EINVAL;
+  }
+  void* ptr = nullptr;
+  int r = scratch_of(dev, &ptr);
+  if (r) return r;
+  wds_solve_args b = *a;
+  b.workspace = ptr;
+  return wds_solve(P, stream, &b);
 }
 
 extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* a, int32_t it,
