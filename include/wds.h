@@ -87,6 +87,9 @@ typedef struct wds_plan_stats {
   int32_t factor_steps, fwd_steps, bwd_steps;
   int64_t factor_instr, fwd_instr, bwd_instr;
   int64_t lds_bytes;           /* LDS per scenario of the fused kernels */
+  int64_t arena_bytes;         /* arena mode: global memory per scenario holding that
+                                  layout (networks whose lds_bytes exceed a CU's
+                                  160 KB); 0 in LDS mode */
 } wds_plan_stats;
 
 /* Counters (GGASolver.cudss_counters, dgga/solver.py:2580-2593): batched
@@ -99,7 +102,9 @@ typedef struct wds_counters {
 /* Symbolic analysis, once per topology (replaces cuDSS plan(),
  * dgga/solver.py:2474-2480, and models genmmd/etree/col_counts,
  * dgga/smatrix.py:352-547, dgga/symbolic.py:101-230).  Host only: no GPU
- * call.  ordering: 0 minimum degree, 1 natural. */
+ * call.  ordering: 0 minimum degree, 1 natural; | 0x100 runs the kernels in
+ * arena mode (the per-scenario layout in global memory) even when it would
+ * fit in LDS, which a network larger than a CU's LDS gets anyway. */
 int wds_plan_create(const wds_network_desc* net, int32_t ordering, wds_plan** out);
 void wds_plan_destroy(wds_plan* plan);
 int wds_plan_stats_get(const wds_plan* plan, wds_plan_stats* out);

@@ -208,7 +208,8 @@ def polish(m, d, rh, ke, r_hw, q, e_j, Hj, status, steps=3):
     return q, e_j, Hj, float(np.abs(F).max())
 
 
-def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_steps=3):
+def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_steps=3,
+                   linsolve="dense"):
     """Dense GGA with the status machine, then per-scenario polish with the
     converged status frozen (dgga/autodiff.py:539-603)."""
     d = np.atleast_2d(d)
@@ -216,7 +217,7 @@ def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_step
     rh = np.broadcast_to(np.atleast_2d(rh), (B, m.N))
     ke = np.broadcast_to(np.atleast_2d(ke), (B, m.N))
     out = gga.solve(m, d, rh, ke=ke, r_hw=r_hw, accuracy=accuracy, max_iter=max_iter,
-                    status_machine=True)
+                    status_machine=True, linsolve=linsolve)
     q_all = out["flow_cfs"].copy()
     e_all = out["emitter_cfs"][:, m.junc_nodes].copy()
     H_all = out["head_ft"].copy()

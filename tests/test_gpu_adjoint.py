@@ -179,3 +179,19 @@ def test_adjoint_on_other_planar_topologies(cuda, seed):
     assert _rel(gd, cd) < 1e-10
     assert _rel(gke, cke) < 1e-10
     assert _rel(gr, cr.sum(0)) < 1e-10
+
+
+def test_unconverged_forward_raises_now_or_at_backward(planar):
+    # without a backward to come the forward checks at once; with one, the
+    # check waits for the backward's synchronisation and raises there
+    net, m, s = planar
+    d, rh = scenario_batch(net, 2, seed=12)
+    args = (torch.tensor(d, device=s.device), torch.tensor(rh, device=s.device))
+    with torch.no_grad(), pytest.raises(RuntimeError, match="did not converge"):
+        implicit_solve(s, *args, accuracy=1e-8, max_iter=1, adjoint="gpu",
+                       status_machine=True)
+    dt = args[0].clone().requires_grad_(True)
+    H, Q, E = implicit_solve(s, dt, args[1], accuracy=1e-8, max_iter=1, adjoint="gpu",
+                             status_machine=True)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        H.sum().backward()

@@ -44,7 +44,8 @@ class PlanStats(C.Structure):
                 ("factor_updates", C.c_int64), ("factor_steps", C.c_int32),
                 ("fwd_steps", C.c_int32), ("bwd_steps", C.c_int32),
                 ("factor_instr", C.c_int64), ("fwd_instr", C.c_int64),
-                ("bwd_instr", C.c_int64), ("lds_bytes", C.c_int64)]
+                ("bwd_instr", C.c_int64), ("lds_bytes", C.c_int64),
+                ("arena_bytes", C.c_int64)]
 
 
 class Counters(C.Structure):

@@ -32,6 +32,27 @@ def _prv_topology(s):
     return t
 
 
+def _check_forward(info, iters, maxtrials, out, accuracy):
+    """The forward's errors (dgga/autodiff.py:1206-1213): a CONST_HP pump on
+    its clamp branch, a failed solve or polish, an unconverged scenario."""
+    if (info == -300).any():
+        bad = np.nonzero(info == -300)[0].tolist()
+        raise NotImplementedError(
+            "adjoint='gpu': samples %s have a CONST_HP pump on its CBIG/RQtol clamp "
+            "branch at the converged state (the reduced system is not the "
+            "Jacobian's Schur complement there, dgga/autodiff.py:808-823)" % bad[:8])
+    if (info != 0).any():
+        bad = np.nonzero(info != 0)[0].tolist()
+        raise RuntimeError("adjoint='gpu': samples %s failed in the solve or the polish "
+                           "(info %s)" % (bad[:8], info[bad[:8]].tolist()))
+    if (iters > maxtrials).any():
+        bad = np.nonzero(iters > maxtrials)[0].tolist()
+        raise RuntimeError(
+            "adjoint='gpu': scenarios %s did not converge (relerr %s > accuracy %g); "
+            "the adjoint is undefined there" % (bad[:8], out["relerr"][bad[:4]].tolist(),
+                                               float(accuracy)))
+
+
 class ImplicitGGASolveGPU(torch.autograd.Function):
     """(demand, res_head, ke, r_hw, solver, accuracy, max_iter,
     status_machine, assemble, linear_solver, polish_steps) ->
@@ -40,7 +61,7 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
     @staticmethod
     def forward(ctx, demand, res_head, ke, r_hw, solver, accuracy=1e-12, max_iter=200,
                 status_machine=None, assemble="dense", linear_solver="dense",
-                polish_steps=2):
+                polish_steps=2, defer_check=False):
         s = solver
         if s.dtype != torch.float64:
             raise NotImplementedError("adjoint='gpu' is float64 only")
@@ -75,24 +96,19 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             _lib.check(s._timed("k_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
                                                                    C.byref(a)),
                                 B, out["iters"]), "wds_solve")
-        info, iters = s._host_status(out, B)
-        if (info == -300).any():
-            bad = np.nonzero(info == -300)[0].tolist()
-            raise NotImplementedError(
-                "adjoint='gpu': samples %s have a CONST_HP pump on its CBIG/RQtol clamp "
-                "branch at the converged state (the reduced system is not the "
-                "Jacobian's Schur complement there, dgga/autodiff.py:808-823)" % bad[:8])
-        if (info != 0).any():
-            bad = np.nonzero(info != 0)[0].tolist()
-            raise RuntimeError("adjoint='gpu': samples %s failed in the solve or the polish "
-                               "(info %s)" % (bad[:8], info[bad[:8]].tolist()))
         maxtrials = s.maxtrials(max_iter, sm)
-        if (iters > maxtrials).any():
-            bad = np.nonzero(iters > maxtrials)[0].tolist()
-            raise RuntimeError(
-                "adjoint='gpu': scenarios %s did not converge (relerr %s > accuracy %g); "
-                "the adjoint is undefined there" % (bad[:8], out["relerr"][bad[:4]].tolist(),
-                                                   float(accuracy)))
+        ctx.fwd_check = None
+        if defer_check and any(ctx.needs_input_grad[:4]):
+            # a backward will follow (implicit_solve under grad mode): the
+            # solve's status goes to pinned host memory asynchronously and is
+            # checked at the backward's one synchronisation, so the loss and
+            # the backward queue behind the forward without a host round trip
+            buf = torch.empty(2 * B, dtype=torch.int32, pin_memory=True)
+            buf.copy_(out["st"], non_blocking=True)
+            ctx.fwd_check = (buf, maxtrials, float(accuracy))
+        else:
+            info, iters = s._host_status(out, B)
+            _check_forward(info, iters, maxtrials, out, accuracy)
         # outputs that do not reach the loss arrive in backward as None and
         # are passed to the kernel as null pointers (zero cotangents)
         ctx.set_materialize_grads(False)
@@ -160,10 +176,14 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
                  _fit(gr, dm[3], dv[3]))
         torch.cuda.current_stream(dev).synchronize()
+        if ctx.fwd_check is not None:
+            buf, maxtrials, acc = ctx.fwd_check
+            ctx.fwd_check = None
+            _check_forward(buf[:B].numpy(), buf[B:].numpy(), maxtrials, out, acc)
         if (binfo.numpy() != 0).any():
             bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
             raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
-        return grads + (None, None, None, None, None, None, None)
+        return grads + (None, None, None, None, None, None, None, None)
 
 
 def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
@@ -188,9 +208,12 @@ def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
         r_hw = solver.r_hw
     t = lambda x: x.to(dtype=dt, device=dev) if isinstance(x, torch.Tensor) \
         else torch.as_tensor(np.asarray(x), dtype=dt, device=dev)
-    return ImplicitGGASolveGPU.apply(t(demand), t(res_head), t(ke), t(r_hw), solver,
-                                     accuracy, max_iter, status_machine, assemble,
-                                     linear_solver, int(polish_steps))
+    args = (t(demand), t(res_head), t(ke), t(r_hw))
+    # with a backward to come, the forward's status check waits for the
+    # backward's synchronisation (ImplicitGGASolveGPU.forward)
+    defer = torch.is_grad_enabled() and any(x.requires_grad for x in args)
+    return ImplicitGGASolveGPU.apply(*args, solver, accuracy, max_iter, status_machine,
+                                     assemble, linear_solver, int(polish_steps), defer)
 
 
 def _stream_of(t):

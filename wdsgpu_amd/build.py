@@ -15,8 +15,8 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libwds.so")
-SOURCES = ["kernels.hip", "plan.cpp", "symbolic.cpp"]
-HEADERS = ["detmath.h", "devplan.h", "device_common.h", "kernels_impl.h", "kernels_polish.h", "plan.h",
+SOURCES = ["kernels.hip", "kernels_arena.hip", "plan.cpp", "symbolic.cpp"]
+HEADERS = ["arena.h", "detmath.h", "devplan.h", "device_common.h", "kernels_impl.h", "kernels_polish.h", "plan.h",
            "symbolic.h"]
 
 

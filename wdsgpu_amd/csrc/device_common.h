@@ -26,10 +26,27 @@ constexpr double PI_EPANET = 3.141592654;
 // One wavefront per scenario whose LDS share admits a single wavefront per
 // SIMD: let the compiler schedule for latency with the whole register file
 // instead of squeezing registers for an occupancy the LDS rules out.
+// (Arena mode keeps the same bound: at two wavefronts per SIMD the fused
+// forward would spill to scratch.)
 #define WDS_SCENARIO_KERNEL __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
 
 __device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
 
+// Make this wave's global stores visible to its own later loads.
+__device__ __forceinline__ void mem_sync() {
+  __threadfence_block();
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Two builds of the same kernels (the WDS_K names differ):
+//  * LDS mode: a scenario's factor, head and solve vectors, sinks and
+//    Woodbury block live in the workgroup's dynamic LDS;
+//  * arena mode (WDS_GLOBAL_ARENA), for networks whose scenario exceeds the
+//    CU's LDS: the same layout, at the same byte offsets, in a per-scenario
+//    arena at the end of its global workspace slice.  The program words
+//    address it by byte offset from the arena base.
+#ifndef WDS_GLOBAL_ARENA
+#define WDS_K(name) name
 // Order LDS traffic between program levels of one wavefront: wait for the
 // wave's outstanding LDS operations only (lgkmcnt(0); vector-memory loads of
 // prefetched instructions stay in flight) and stop the compiler from moving
@@ -48,12 +65,13 @@ __device__ __forceinline__ void lds_order() {
   __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
   __builtin_amdgcn_wave_barrier();
 }
-
-// Make this wave's global stores visible to its own later loads.
-__device__ __forceinline__ void mem_sync() {
-  __threadfence_block();
-  __builtin_amdgcn_wave_barrier();
-}
+#else
+#define WDS_K(name) name##_arena
+// arena mode: the scenario's vectors are global memory; a wave's stores are
+// complete and visible to its own later loads after vmcnt(0)
+__device__ __forceinline__ void lds_sync() { mem_sync(); }
+__device__ __forceinline__ void lds_order() { mem_sync(); }
+#endif
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -174,7 +192,11 @@ __device__ __forceinline__ double recip(double x) {
 // (counted by lgkmcnt only) and instruction words to global loads (vmcnt
 // only), so waiting for an operand never drains the instruction prefetch, as
 // it would with flat accesses through generic pointers.
+#ifndef WDS_GLOBAL_ARENA
 typedef __attribute__((address_space(3))) double lds_f64;
+#else
+typedef __attribute__((address_space(1))) double lds_f64;
+#endif
 typedef int i32x4 __attribute__((ext_vector_type(4)));
 typedef const __attribute__((address_space(1))) i32x4 glb_i32x4;
 
@@ -276,10 +298,18 @@ __device__ inline void run_factor(const ProgDev& pg, double* Vg, int& bad,
   }
 }
 
-// LDS double at a byte address (the dynamic LDS of the kernels starts at 0)
-__device__ __forceinline__ lds_f64& lds_at(int byte_addr) {
+// The scenario's double at a byte address: LDS mode, from the start of the
+// dynamic LDS (address 0; base unused); arena mode, from the arena base.
+#ifndef WDS_GLOBAL_ARENA
+__device__ __forceinline__ lds_f64& lds_at(const char* base, int byte_addr) {
+  (void)base;
   return *(lds_f64*)(size_t)(unsigned)byte_addr;
 }
+#else
+__device__ __forceinline__ lds_f64& lds_at(const char* base, int byte_addr) {
+  return *(lds_f64*)(base + (unsigned)byte_addr);
+}
+#endif
 
 // Triangular solve with the unit factor on the solve vector S (LDS):
 // acc = S[t] - sum V[a] S[b] per task, stored back into S[t].  Instruction
@@ -289,7 +319,9 @@ __device__ __forceinline__ lds_f64& lds_at(int byte_addr) {
 // last step (the lane's sink otherwise).  The store word of a level's last
 // step carries the level end and the lane's log2 group width in its high
 // bits.
-__device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
+__device__ inline void run_dot(const ProgDev& pg, const double* arena,
+                               double* prof = nullptr) {
+  const char* ab = (const char*)arena;
   const int lane = lane_id();
   const int ns = pg.nsteps;
   if (ns == 0) return;
@@ -303,7 +335,7 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
   long long prof_t0 = clock64();
   int prof_lvl = 0;
 #endif
-  double va = lds_at(ring[0].x), vb = lds_at(ring[0].y), vi = lds_at(ring[0].z);
+  double va = lds_at(ab, ring[0].x), vb = lds_at(ab, ring[0].y), vi = lds_at(ab, ring[0].z);
   for (int g0 = 0; g0 < ns; g0 += kSlots) {
 #pragma unroll
     for (int d = 0; d < kSlots; ++d) {
@@ -317,17 +349,17 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
         acc = fma(-va, vb, init ? vi : acc);
         const int f = __builtin_amdgcn_readfirstlane(cur.w);
         if (!(f & kDotLevelEnd)) {
-          va = lds_at(nxt.x);
-          vb = lds_at(nxt.y);
-          vi = lds_at(nxt.z);
-          lds_at(cur.w) = acc;
+          va = lds_at(ab, nxt.x);
+          vb = lds_at(ab, nxt.y);
+          vi = lds_at(ab, nxt.z);
+          lds_at(ab, cur.w) = acc;
         } else {
-          lds_at(cur.w & kDotAddrMask) = acc;
+          lds_at(ab, cur.w & kDotAddrMask) = acc;
           const int lwm = (f >> kDotWidthShift) & 7;
           if (lwm) {
             const int lw = (cur.w >> kDotWidthShift) & 7;
             const double red = group_sum(acc, lw, lwm);
-            if (lw && (lane & ((1 << lw) - 1)) == 0) lds_at(fin) = red;
+            if (lw && (lane & ((1 << lw) - 1)) == 0) lds_at(ab, fin) = red;
           }
           lds_order();
 #ifdef WDS_PROFILE_STAGES
@@ -338,9 +370,9 @@ __device__ inline void run_dot(const ProgDev& pg, double* prof = nullptr) {
             ++prof_lvl;
           }
 #endif
-          va = lds_at(nxt.x);
-          vb = lds_at(nxt.y);
-          vi = lds_at(nxt.z);
+          va = lds_at(ab, nxt.x);
+          vb = lds_at(ab, nxt.y);
+          vi = lds_at(ab, nxt.z);
         }
       }
     }

@@ -6,6 +6,8 @@
 namespace wds {
 
 constexpr int kMaxPrv = 8;     // pressure-reducing valves with a setting
+constexpr int64_t kMaxLdsBytes = 160 * 1024;  // LDS of one CU: a scenario's layout
+                                              // above this runs in arena mode
 
 // per-link integer attributes (bit field in link_flags)
 enum LinkFlag : int32_t {
@@ -47,7 +49,7 @@ WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
 
 // solve programs: the store word of a level's last step carries the level
 // end and log2 of the lane's group width above the LDS byte address
-constexpr int32_t kDotAddrMask = (1 << 20) - 1;
+constexpr int32_t kDotAddrMask = (1 << 27) - 1;   // byte offsets below 128 MiB
 constexpr int32_t kDotLevelEnd = 1 << 30;
 constexpr int kDotWidthShift = 27;
 
@@ -64,6 +66,10 @@ struct ProgDev {
 struct DevPlan {
   // sizes
   int32_t N, L, Nj, Nf, nnzL, p, ntl;
+  // arena mode: doubles of the per-scenario arena that holds the LDS layout
+  // in global memory (at the end of each scenario's workspace slice); 0 in
+  // LDS mode
+  int64_t arena;
   int32_t headloss;         // 0 Hazen-Williams, 1 Darcy-Weisbach
   double hexp, qexp, rqtol, viscos, htol, qtol;
   int32_t checkfreq, maxcheck, extra_iter;

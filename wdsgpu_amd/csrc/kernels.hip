@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "plan.h"
+#include "arena.h"
 #include "device_common.h"
 #include "kernels_impl.h"
 
@@ -146,6 +147,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
     return hip_fail(e, "hipMemcpy(plan)");
   }
   DevPlan& D = P->dev;
+  D.arena = P->arena;
   D.N = P->N;
   D.L = P->L;
   D.Nj = P->Nj;
@@ -266,9 +268,11 @@ extern "C" void wds_plan_destroy(wds_plan* P) {
 
 extern "C" int64_t wds_workspace_bytes(const wds_plan* P, int32_t B) {
   if (!P || B <= 0) return WDS_EINVAL;
-  return (int64_t)B * ws_doubles(P->L, P->Nj,
-                                   (int32_t)std::max<size_t>(P->sym.nnzL, P->rs_ell.size() / 2),
-                                   (int32_t)P->wp_node.size(), 2 * P->p) * 8;
+  return (int64_t)B *
+         (ws_doubles(P->L, P->Nj, (int32_t)std::max<size_t>(P->sym.nnzL, P->rs_ell.size() / 2),
+                     (int32_t)P->wp_node.size(), 2 * P->p) +
+          P->arena) *
+         8;
 }
 
 extern "C" int wds_counters_get(wds_plan* P, wds_counters* out, int32_t reset) {
@@ -293,7 +297,7 @@ static int check_ready(wds_plan* P) {
     set_error("plan was uploaded to another device");
     return WDS_EINVAL;
   }
-  if (P->stats.lds_bytes > kMaxLds) {
+  if (P->arena == 0 && P->stats.lds_bytes > kMaxLds) {
     set_error("network needs " + std::to_string(P->stats.lds_bytes) +
               " bytes of LDS per scenario (> " + std::to_string(kMaxLds) + ")");
     return WDS_EUNSUPPORTED;
@@ -341,9 +345,14 @@ extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const doub
     return WDS_EINVAL;
   }
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_factor_solve, lds);
-  hipLaunchKernelGGL(k_factor_solve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr, F,
-                     H, n_refine, factor_out, info, (double*)workspace);
+  if (P->arena) {
+    arena_factor_solve(P->dev, csr, F, H, n_refine, factor_out, info, (double*)workspace, B,
+                       (hipStream_t)stream);
+  } else {
+    set_lds(k_factor_solve, lds);
+    hipLaunchKernelGGL(k_factor_solve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
+                       F, H, n_refine, factor_out, info, (double*)workspace);
+  }
   P->cnt.factorize += 1;
   P->cnt.solve += 1 + n_refine;
   P->cnt.launches += 1;
@@ -359,9 +368,13 @@ extern "C" int wds_factor(wds_plan* P, void* stream, int32_t B, const double* cs
     return WDS_EINVAL;
   }
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_factor, lds);
-  hipLaunchKernelGGL(k_factor, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
-                     factor_out, info, (double*)workspace);
+  if (P->arena) {
+    arena_factor(P->dev, csr, factor_out, info, (double*)workspace, B, (hipStream_t)stream);
+  } else {
+    set_lds(k_factor, lds);
+    hipLaunchKernelGGL(k_factor, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
+                       factor_out, info, (double*)workspace);
+  }
   P->cnt.factorize += 1;
   P->cnt.launches += 1;
   return launch_check("k_factor");
@@ -377,9 +390,14 @@ extern "C" int wds_trisolve(wds_plan* P, void* stream, int32_t B, const double* 
     return WDS_EINVAL;
   }
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_trisolve, lds);
-  hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, factor,
-                     csr, F, H, n_refine, (double*)workspace);
+  if (P->arena) {
+    arena_trisolve(P->dev, factor, csr, F, H, n_refine, (double*)workspace, B,
+                   (hipStream_t)stream);
+  } else {
+    set_lds(k_trisolve, lds);
+    hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, factor,
+                       csr, F, H, n_refine, (double*)workspace);
+  }
   P->cnt.solve += 1 + n_refine;
   P->cnt.launches += 1;
   return launch_check("k_trisolve");
@@ -429,8 +447,12 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
   if (r) return r;
   SolveParams sp = make_params(a, P->N);
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_solve, lds);
-  hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
+  if (P->arena) {
+    arena_solve(P->dev, sp, a->B, (hipStream_t)stream);
+  } else {
+    set_lds(k_solve, lds);
+    hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
+  }
   P->cnt.launches += 1;
   return launch_check("k_solve");
 }
@@ -523,9 +545,13 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
   if (r) return r;
   SolveParams sp = make_params(a, P->N);
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_newton_iter, lds);
-  hipLaunchKernelGGL(k_newton_iter, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp,
-                     it, ctrl);
+  if (P->arena) {
+    arena_newton_iter(P->dev, sp, it, ctrl, a->B, (hipStream_t)stream);
+  } else {
+    set_lds(k_newton_iter, lds);
+    hipLaunchKernelGGL(k_newton_iter, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev,
+                       sp, it, ctrl);
+  }
   P->cnt.launches += 1;
   return launch_check("k_newton_iter");
 }
@@ -541,8 +567,12 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
   }
   AdjParams ap = make_adj_params(a);
   const int64_t lds = P->stats.lds_bytes;
-  set_lds(k_adjoint, lds);
-  hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, ap);
+  if (P->arena) {
+    arena_adjoint(P->dev, ap, a->B, (hipStream_t)stream);
+  } else {
+    set_lds(k_adjoint, lds);
+    hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, ap);
+  }
   P->cnt.bwd_solve += 1;
   P->cnt.bwd_reuse += 1;
   P->cnt.launches += 1;

@@ -4,7 +4,7 @@
 
 namespace wds {
 
-constexpr int64_t kMaxLds = 160 * 1024;
+constexpr int64_t kMaxLds = kMaxLdsBytes;
 
 // per-scenario global workspace, in doubles (wp_tot path nodes, np paths)
 __host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t na,
@@ -169,6 +169,22 @@ struct Scn {
   const double* r;  // [L] resistance
 };
 
+// doubles of one scenario's workspace slice: the workspace proper, then (arena
+// mode) the arena holding the LDS layout
+__host__ __device__ inline int64_t ws_stride(const DevPlan& pl) {
+  return ws_doubles(pl.L, pl.Nj, pl.na, pl.wp_tot, pl.np) + pl.arena;
+}
+
+// the scenario's LDS layout: the workgroup's dynamic LDS, or (arena mode) the
+// arena at the end of the scenario's workspace slice
+#ifndef WDS_GLOBAL_ARENA
+#define WDS_SCENARIO_MEM(ws) extern __shared__ double lds[]
+#else
+#define WDS_SCENARIO_MEM(ws)                                                  \
+  double* const lds = (ws) + (int64_t)blockIdx.x * ws_stride(pl) +           \
+                      ws_doubles(pl.L, pl.Nj, pl.na, pl.wp_tot, pl.np)
+#endif
+
 __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, int b,
                                double* flow_all, int8_t* status_all, const double* r_all,
                                int r_stride) {
@@ -179,7 +195,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.R = c.Hn + Nj + pl.Nf;
   c.X2 = c.R + Nj;
   c.sh = c.X2 + Nj;
-  double* w = ws_all + (int64_t)b * ws_doubles(L, Nj, pl.na, pl.wp_tot, pl.np);
+  double* w = ws_all + (int64_t)b * ws_stride(pl);
   c.P = w;
   w += L;
   c.Y = w;
@@ -471,9 +487,9 @@ __device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
   int dp[kRowG];
   pivots_prefetch(pl, dp);
 #ifdef WDS_PROFILE_STAGES
-  run_dot(pl.fwd, c.misc + 408 + 32);
+  run_dot(pl.fwd, c.V, c.misc + 408 + 32);
 #else
-  run_dot(pl.fwd);
+  run_dot(pl.fwd, c.V);
 #endif
   scale_by_pivots(pl, c.V, c.R, dp);
   lds_sync();
@@ -482,9 +498,9 @@ __device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
 // backward x = L^-T z in place in S
 __device__ inline void solve_bwd(const DevPlan& pl, Scn& c) {
 #ifdef WDS_PROFILE_STAGES
-  run_dot(pl.bwd, c.misc + 408 + 64);
+  run_dot(pl.bwd, c.V, c.misc + 408 + 64);
 #else
-  run_dot(pl.bwd);
+  run_dot(pl.bwd, c.V);
 #endif
 }
 
@@ -872,11 +888,11 @@ __device__ inline void load_csr(const DevPlan& pl, Scn& c, const double* csr, in
   }
 }
 
-__global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr,
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor_solve)(DevPlan pl, const double* csr,
                                                      const double* F, double* H, int n_refine,
                                                      double* factor_out, int32_t* info,
                                                      double* ws) {
-  extern __shared__ double lds[];
+  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, true);
@@ -900,9 +916,9 @@ __global__ WDS_SCENARIO_KERNEL void k_factor_solve(DevPlan pl, const double* csr
 
 // Numeric factorisation only (the reference's cuDSS factorize phase,
 // dgga/solver.py:2630-2640): factor_out [B][nnzL], info [B].
-__global__ WDS_SCENARIO_KERNEL void k_factor(DevPlan pl, const double* csr,
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor)(DevPlan pl, const double* csr,
                                                double* factor_out, int32_t* info, double* ws) {
-  extern __shared__ double lds[];
+  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, true);
@@ -917,10 +933,10 @@ __global__ WDS_SCENARIO_KERNEL void k_factor(DevPlan pl, const double* csr,
 // Triangular solves with refinement against a stored factor (the cuDSS
 // solve phase and the refinement loop, dgga/solver.py:2641-2651,
 // 3052-3064): H [B][Nj] from F [B][Nj], both in original junction order.
-__global__ WDS_SCENARIO_KERNEL void k_trisolve(DevPlan pl, const double* factor,
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_trisolve)(DevPlan pl, const double* factor,
                                                  const double* csr, const double* F, double* H,
                                                  int n_refine, double* ws) {
-  extern __shared__ double lds[];
+  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, false);
@@ -980,8 +996,8 @@ __device__ inline void init_state(const DevPlan& pl, Scn& c, const SolveParams& 
 }
 
 // Fused forward: the whole Newton loop of scenario blockIdx.x.
-__global__ WDS_SCENARIO_KERNEL void k_solve(DevPlan pl, SolveParams sp) {
-  extern __shared__ double lds[];
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
+  WDS_SCENARIO_MEM(sp.ws);
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
   load_inputs(pl, c, sp, b);
@@ -1033,9 +1049,9 @@ __global__ WDS_SCENARIO_KERNEL void k_solve(DevPlan pl, SolveParams sp) {
 // One Newton iteration per launch (the unfused path).  ctrl[b] = {active,
 // iters, nextcheck, info}; heads/flows/emitters/statuses persist in the
 // output arrays between launches (it == 1 initialises them).
-__global__ WDS_SCENARIO_KERNEL void k_newton_iter(DevPlan pl, SolveParams sp, int it,
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams sp, int it,
                                                     int32_t* ctrl) {
-  extern __shared__ double lds[];
+  WDS_SCENARIO_MEM(sp.ws);
   const int b = blockIdx.x, lane = lane_id();
   int32_t* cb = ctrl + 4 * b;
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);

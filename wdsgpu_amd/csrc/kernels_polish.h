@@ -647,8 +647,8 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
 
 // Backward of the implicit solve (ImplicitGGASolveGPU.backward,
 // dgga/autodiff.py:1245-1312), one wavefront per scenario.
-__global__ WDS_SCENARIO_KERNEL void k_adjoint(DevPlan pl, AdjParams ap) {
-  extern __shared__ double lds[];
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
+  WDS_SCENARIO_MEM(ap.ws);
   const int b = blockIdx.x, lane = lane_id();
   Scn c = make_scn(pl, lds, ap.ws, b, const_cast<double*>(ap.flow),
                    const_cast<int8_t*>(ap.status), ap.r, ap.r_stride);

@@ -52,7 +52,7 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     const int a = junc_row[d->link_n1[k]], b = junc_row[d->link_n2[k]];
     if (a >= 0 && b >= 0) adj[a].push_back(b);
   }
-  analyse(Nj, adj, ordering, P->sym);
+  analyse(Nj, adj, ordering & 0xff, P->sym);
   const Symbolic& S = P->sym;
   auto slot_of_row = [&](int r) { return S.iperm[r]; };
   P->node_slot.assign(N, -1);
@@ -509,6 +509,15 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   st.bwd_instr = (int64_t)S.bwd.instr.size();
   st.lds_bytes = 8 * (lds_sink_index(S.nnzL, Nj, Nf) + kLanes +
                       lds_small_doubles((int64_t)P->wp_node.size(), 2 * (int64_t)P->p));
+  // the program words address the layout by byte offset below 2^27
+  if (st.lds_bytes >= ((int64_t)1 << 27))
+    throw std::domain_error("network too large: " + std::to_string(st.lds_bytes) +
+                            " bytes per scenario exceed the 128 MiB the programs address");
+  // arena mode when the layout exceeds a CU's LDS (or when asked for)
+  P->arena = ((ordering & 0x100) || st.lds_bytes > kMaxLdsBytes)
+                 ? (st.lds_bytes / 8 + 31) / 32 * 32
+                 : 0;
+  st.arena_bytes = 8 * P->arena;
 }
 
 }  // namespace wds

@@ -50,6 +50,9 @@ struct wds_plan {
   int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
   int32_t has_cv = 0, has_pump = 0;
   double hexp = 1.852, qexp = 2.0, rqtol = 1e-7, viscos = 1.1e-5, htol = 5e-4, qtol = 1e-4;
+  // arena mode (doubles per scenario of the global-memory copy of the LDS
+  // layout; 0 in LDS mode)
+  int64_t arena = 0;
   // device state
   wds::DevPlan dev{};
   void* d_blob = nullptr;

@@ -41,7 +41,12 @@ def _f64(a):
 
 
 class GGASolver:
-    """Batched GGA forward solver (dgga/solver.py:140-495 interface)."""
+    """Batched GGA forward solver (dgga/solver.py:140-495 interface).
+
+    Beyond the reference's arguments: ``ordering`` ("mmd" minimum degree or
+    "natural") and ``arena`` (run the scenario kernels in arena mode, with
+    the per-scenario factor and vectors in global memory, even for a network
+    that fits a CU's LDS; used to test that mode)."""
 
     # record_schedule: status changes logged per scenario before a re-run
     # with room for all of them (None: 4 L + 64)
@@ -49,7 +54,7 @@ class GGASolver:
 
     def __init__(self, net, device="cuda", dtype=torch.float64, mode="dense",
                  inp_path=None, dense_tank_bound_check=True,
-                 dense_status_machine=False, ordering="mmd"):
+                 dense_status_machine=False, ordering="mmd", arena=False):
         if mode != "dense":
             raise NotImplementedError(
                 "mode=%r: the MI355X path is the batched dense-mode solver "
@@ -196,7 +201,10 @@ class GGASolver:
             ucf_flow=self.ucf_flow, ucf_head=self.ucf_head)
         self._lib = _lib.load()
         h = C.c_void_p()
-        order = {"mmd": 0, "natural": 1}[ordering]
+        # arena=True runs the kernels with each scenario's factor and vectors in
+        # a global-memory arena even when they would fit in LDS (networks
+        # larger than a CU's LDS run that way regardless)
+        order = {"mmd": 0, "natural": 1}[ordering] | (0x100 if arena else 0)
         _lib.check(self._lib.wds_plan_create(C.byref(desc), order, C.byref(h)),
                    "wds_plan_create")
         self._plan = h
