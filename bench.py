@@ -38,6 +38,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain-baseline", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=256)
+    ap.add_argument("--stages", action="store_true",
+                    help="also time the standalone factor (k_factor) and triangular-solve "
+                         "(k_trisolve) kernels at the same B against their algorithmic bytes")
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run one plain 512 MiB read + 512 MiB write elementwise "
                          "kernel, the reference point that scales rocprofv3 FETCH_SIZE / "
@@ -61,19 +64,23 @@ def main():
     dev = torch.device("cuda:0")
     B = args.B
     log("building workload B=%d" % B)
-    wl = Calibration(B=B, device=dev)
+    # status checks run one step behind (and once more after the timed
+    # steps), so consecutive steps queue without a host round trip
+    wl = Calibration(B=B, device=dev, lazy_checks=True)
     s = wl.solver
     st = s.stats
     ne = int(np.count_nonzero(wl.net.node_ke > 0))
     for _ in range(args.warmup):
         wl.step()
     torch.cuda.synchronize()
+    wl.check()
     s.event_log = []
     log("timing %d steps" % args.steps)
     t0 = time.perf_counter()
     for _ in range(args.steps):
         loss, gd, grh, gC, gCe = wl.step()
     torch.cuda.synchronize()
+    wl.check()
     t1 = time.perf_counter()
     ms_per_step = (t1 - t0) * 1e3 / args.steps
     value = B * args.steps / (t1 - t0)
@@ -116,6 +123,8 @@ def main():
         torch.mul(src, 1.0, out=dst)
         torch.cuda.synchronize()
         del src, dst
+    if args.stages:
+        out["stages"] = stage_rooflines(s, B)
     # ---- plain per-scenario GPU GGA with finite-difference gradients ----
     if not args.no_plain_baseline:
         out["plain_fd_baseline"] = plain_fd(wl)
@@ -143,6 +152,57 @@ def measured_traffic():
         return None
     return {"bytes_per_launch": int(float(m.group(1))),
             "source": os.path.relpath(TRAFFIC_FILE, ROOT)}
+
+
+def stage_rooflines(s, B, reps=20):
+    """The factor and solve phases as standalone kernels (wds_factor,
+    wds_trisolve with 2 refinements) on B symmetric positive definite
+    matrices in the network's CSR pattern (random link weights, diagonally
+    dominant): their programs do not depend on the values.  HIP events on
+    the launching stream around each launch; algorithmic bytes per scenario
+    K3 = 8 (nnzA_lo + nnzL) and K4 + K4b = 8 (2 nnzL + 2 Nj) (1 + r) +
+    8 (nnzA_lo + 3 Nj) r (wdsgpu_amd/roofline.py)."""
+    st = s.stats
+    indptr = s.plan_export(5)
+    indices = s.plan_export(6)
+    Nj, nnz = st["Nj"], st["nnz_csr"]
+    rows = np.repeat(np.arange(Nj), np.diff(indptr))
+    g = np.random.default_rng(3)
+    w = g.uniform(0.5, 2.0, (B, nnz))
+    upper = indices > rows
+    # symmetric weights: the value of (i, j) and (j, i) from the upper entry
+    key = np.where(upper, rows * Nj + indices, indices * Nj + rows)
+    _, inv = np.unique(key, return_inverse=True)
+    wu = np.zeros((B, inv.max() + 1))
+    wu[:, inv[upper]] = w[:, upper]
+    vals = -wu[:, inv]
+    diag = rows == indices
+    off = np.where(diag, 0.0, vals)
+    rowsum = np.zeros((B, Nj))
+    np.add.at(rowsum, (slice(None), rows), -off)
+    vals = np.where(diag, rowsum[:, rows] + 1.0, off)
+    dev = s.device
+    csr = torch.tensor(vals, device=dev)
+    F = torch.tensor(g.normal(size=(B, Nj)), device=dev)
+    fac = s.factor(csr)
+    s.trisolve(fac, csr, F)
+    torch.cuda.synchronize()
+    out = {}
+    for name, fn, nbytes in (
+            ("k_factor", lambda: s.factor(csr),
+             8 * (st["nnz_lower"] + st["nnzL"])),
+            ("k_trisolve", lambda: s.trisolve(fac, csr, F),
+             8 * (2 * st["nnzL"] + 2 * Nj) * 3 + 8 * (st["nnz_lower"] + 3 * Nj) * 2)):
+        s.event_log = []
+        for _ in range(reps):
+            s._timed(name, fn, B)
+        torch.cuda.synchronize()
+        ms = float(np.mean([e0.elapsed_time(e1) for (_, e0, e1, _, _) in s.event_log]))
+        s.event_log = None
+        gbs = nbytes * B / (ms * 1e-3) / 1e9
+        out[name] = {"kernel_ms": round(ms, 4), "algorithmic_bytes": int(nbytes * B),
+                     "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
+    return out
 
 
 def plain_fd(wl, n_coords=80):

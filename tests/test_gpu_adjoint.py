@@ -195,3 +195,29 @@ def test_unconverged_forward_raises_now_or_at_backward(planar):
                              status_machine=True)
     with pytest.raises(RuntimeError, match="did not converge"):
         H.sum().backward()
+
+
+def test_lazy_checks_raise_at_the_next_call_or_check(cuda):
+    # lazy_checks: a step's status is checked behind an event; an
+    # unconverged forward surfaces at check() (or the next call), a good
+    # step leaves nothing behind
+    net = planar_network(1)
+    s = GGASolver(net, device=cuda, dense_status_machine=True)
+    s.lazy_checks = True
+    d, rh = scenario_batch(net, 2, seed=12)
+    rht = torch.tensor(rh, device=cuda)
+
+    def step(max_iter):
+        dt = torch.tensor(d, device=cuda, requires_grad=True)
+        H, Q, E = implicit_solve(s, dt, rht, accuracy=1e-8, max_iter=max_iter,
+                                 adjoint="gpu", status_machine=True)
+        H.sum().backward()
+        return dt.grad
+
+    g = step(200)
+    s.check()
+    assert torch.isfinite(g).all()
+    step(1)
+    with pytest.raises(RuntimeError, match="did not converge"):
+        s.check()
+    s.check()          # nothing left queued

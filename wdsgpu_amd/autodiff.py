@@ -53,6 +53,16 @@ def _check_forward(info, iters, maxtrials, out, accuracy):
                                                float(accuracy)))
 
 
+def _check_step(binfo, fwd, out, B):
+    """The forward's deferred check (if any), then the node adjoint's."""
+    if fwd is not None:
+        buf, maxtrials, acc = fwd
+        _check_forward(buf[:B].numpy(), buf[B:].numpy(), maxtrials, out, acc)
+    if (binfo.numpy() != 0).any():
+        bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
+        raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
+
+
 class ImplicitGGASolveGPU(torch.autograd.Function):
     """(demand, res_head, ke, r_hw, solver, accuracy, max_iter,
     status_machine, assemble, linear_solver, polish_steps) ->
@@ -89,6 +99,8 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         if d.shape[0] not in (1, B):
             raise ValueError("demand has %d scenarios, another input %d" % (d.shape[0], B))
         d = d.expand(B, -1)
+        if s._pending:
+            s.poll()
         a, out, keep, single, B = s._args(d, rh, ket, None, None, max_iter, accuracy, sm,
                                           r_hw=rt, polish_steps=int(polish_steps),
                                           want_factor=True)
@@ -163,7 +175,8 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
             _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
                                                                        C.byref(a)), B),
                        "wds_adjoint")
-        binfo = s._pinned_info(B)
+        binfo = torch.empty(B, dtype=torch.int32, pin_memory=True) if s.lazy_checks \
+            else s._pinned_info(B)
         binfo.copy_(info, non_blocking=True)
         dm, dv = ctx.dims, ctx.in_devs
 
@@ -175,14 +188,14 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
         # waits for its status
         grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
                  _fit(gr, dm[3], dv[3]))
-        torch.cuda.current_stream(dev).synchronize()
-        if ctx.fwd_check is not None:
-            buf, maxtrials, acc = ctx.fwd_check
-            ctx.fwd_check = None
-            _check_forward(buf[:B].numpy(), buf[B:].numpy(), maxtrials, out, acc)
-        if (binfo.numpy() != 0).any():
-            bad = np.nonzero(binfo.numpy() != 0)[0].tolist()
-            raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
+        fwd = ctx.fwd_check
+        ctx.fwd_check = None
+        if s.lazy_checks:
+            # checked at the next call into the solver or at s.check()
+            s._defer(_check_step, binfo, fwd, out, B)
+        else:
+            torch.cuda.current_stream(dev).synchronize()
+            _check_step(binfo, fwd, out, B)
         return grads + (None, None, None, None, None, None, None, None)
 
 

@@ -229,6 +229,34 @@ class GGASolver:
         self.event_log = None
         self._pinned = {}
         self._pinned_adj = {}
+        # lazy_checks: the implicit solve's status words (forward and
+        # backward) go to pinned host memory behind events and are checked at
+        # the next call into this solver or at check(), instead of at a
+        # synchronisation per backward; consecutive training steps then queue
+        # without a host round trip.  The reference's raise points (forward /
+        # backward) are the default.
+        self.lazy_checks = False
+        self._pending = []
+
+    def _defer(self, check, buf, *rest):
+        """Queue a status check: check(buf, *rest) once the stream has
+        produced buf (an event recorded now)."""
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        self._pending.append((ev, check, buf, rest))
+
+    def poll(self):
+        """Run the queued status checks whose data has arrived (no wait)."""
+        while self._pending and self._pending[0][0].query():
+            ev, check, buf, rest = self._pending.pop(0)
+            check(buf, *rest)
+
+    def check(self):
+        """Wait for and run every queued status check (lazy_checks)."""
+        while self._pending:
+            ev, check, buf, rest = self._pending.pop(0)
+            ev.synchronize()
+            check(buf, *rest)
 
     def _timed(self, name, fn, B, iters=None):
         if self.event_log is None:

@@ -24,7 +24,7 @@ from .solver import GGASolver
 
 class Calibration:
     def __init__(self, B=1024, seed=1, device="cuda", accuracy=1e-6, polish_steps=3,
-                 n_sensors=30, net=None):
+                 n_sensors=30, net=None, lazy_checks=False):
         self.net = planar_network(seed) if net is None else net
         net = self.net
         self.B = B
@@ -51,6 +51,9 @@ class Calibration:
             out = s.solve(self.d0, self.rh0, ke_int=self.ke_of(self.Ce0), accuracy=1e-12,
                           status_machine=True, r_hw=r_true)
             self.H_true = out["head_ft"].index_select(1, self.sensors).clone()
+        # status checks at the next step (or check()) instead of a
+        # synchronisation per step
+        s.lazy_checks = bool(lazy_checks)
 
     def r_of(self, C):
         return hw_resistance(C, self.len_ft, self.diam_ft, mask=self.pipe)
@@ -70,6 +73,10 @@ class Calibration:
                                  status_machine=True)
         Hs = H.index_select(1, self.sensors)
         return ((Hs - self.H_true) ** 2).sum()
+
+    def check(self):
+        """Wait for the queued steps and raise any solver error they hit."""
+        self.solver.check()
 
     def step(self):
         """One forward + adjoint pass; returns the four gradients."""
