@@ -255,6 +255,28 @@ typedef struct wds_adjoint_args {
 
 int wds_adjoint(wds_plan* plan, void* stream, const wds_adjoint_args* args);
 
+/* Ragged batches: several networks, each with its own plan, scenario count
+ * and arrays, solved in one persistent launch per kernel mode.  No
+ * counterpart in the reference (whose solver serves one network per call,
+ * dgga/solver.py:2703-3212); it serves workloads that solve many networks
+ * at once.  wds_multi_create uploads the plans and fixes, per network, the
+ * kernel mode: LDS mode when its layout (wds_plan_stats.lds_bytes) is at
+ * most lds_budget bytes, arena mode otherwise (lds_budget <= 0: every
+ * network in arena mode), and the work list of (network, scenario) items,
+ * costliest networks first.  Each network's args (B[net] scenarios, its own
+ * workspace of wds_multi_workspace_bytes(multi, net) bytes) are those of
+ * wds_solve / wds_adjoint; the results are the same bits as per-network
+ * calls.  wds_multi_info reports the scenarios in each mode and the LDS
+ * launch's dynamic LDS. */
+typedef struct wds_multi wds_multi;
+int wds_multi_create(wds_plan* const* plans, int32_t nnet, const int32_t* B,
+                     int64_t lds_budget, wds_multi** out);
+void wds_multi_destroy(wds_multi* multi);
+int64_t wds_multi_workspace_bytes(const wds_multi* multi, int32_t net);
+int wds_multi_info(const wds_multi* multi, int32_t* n_lds, int32_t* n_arena, int64_t* lds_bytes);
+int wds_multi_solve(wds_multi* multi, void* stream, const wds_solve_args* args /* [nnet] */);
+int wds_multi_adjoint(wds_multi* multi, void* stream, const wds_adjoint_args* args /* [nnet] */);
+
 int wds_counters_get(wds_plan* plan, wds_counters* out, int32_t reset);
 const char* wds_last_error(void);
 

@@ -1,0 +1,151 @@
+# -*- coding: utf-8 -*-
+"""The ragged batch: 64 seeded networks (50-5000 junctions, Hazen-Williams
+and Darcy-Weisbach, pumps, PRVs, TCVs, 16-256 scenarios each, about 4100
+solves) in one persistent launch per kernel mode (MultiSolver).  The multi
+launch gives every network the bits of its own per-network call; every
+network's solve matches the oracle; the adjoint matches per-network calls
+and, on a sample of networks, the oracle's full-Jacobian adjoint."""
+import ctypes as C
+
+import numpy as np
+import pytest
+import torch
+
+from oracle import adjoint as adj
+from oracle import gga
+from wdsgpu_amd import _lib
+from wdsgpu_amd.generators import ragged_batch, scenario_batch
+from wdsgpu_amd.multi import MultiSolver
+from wdsgpu_amd.solver import GGASolver
+
+pytestmark = pytest.mark.gpu
+
+TOL_H_FT = 1e-10 / 0.3048
+
+
+@pytest.fixture(scope="module")
+def ragged(cuda):
+    rb = ragged_batch(0)
+    solvers = [GGASolver(net, device=cuda, dense_status_machine=True) for net, _ in rb]
+    inputs = []
+    for i, (net, B) in enumerate(rb):
+        d, rh = scenario_batch(net, B, seed=100 + i)
+        inputs.append((torch.tensor(d, device=cuda), torch.tensor(rh[0], device=cuda),
+                       torch.tensor(net.node_ke, device=cuda), None))
+    print("ragged batch: %d networks, %d scenarios, Nj %d..%d" % (
+        len(rb), sum(b for _, b in rb), min(n.Nj for n, _ in rb), max(n.Nj for n, _ in rb)),
+        flush=True)
+    return rb, solvers, inputs
+
+
+def _single(s, inp, polish, want_factor):
+    d, rh, ke, r = inp
+    a, out, keep, single, B = s._args(d, rh, ke, None, None, 200, 1e-6, True, r_hw=r,
+                                      polish_steps=polish, want_factor=want_factor)
+    with torch.cuda.device(s.device):
+        _lib.check(s._lib.wds_solve(s._plan, s._stream(), C.byref(a)), "wds_solve")
+    return out, keep
+
+
+@pytest.mark.parametrize("budget", [0, 96 << 10])
+def test_multi_forward_gives_the_per_network_bits(ragged, budget):
+    rb, solvers, inputs = ragged
+    ms = MultiSolver(solvers, [b for _, b in rb], lds_budget=budget)
+    print("budget %d: LDS/arena scenarios and LDS bytes %s" % (budget, ms.info()), flush=True)
+    outs, keeps = ms.forward(inputs, accuracy=1e-6, polish_steps=3)
+    ms.check_forward(outs)
+    for i, s in enumerate(solvers):
+        ref, _ = _single(s, inputs[i], 3, True)
+        for k in ("head", "flow", "emitter", "st", "status", "factor", "resid"):
+            assert torch.equal(outs[i][k], ref[k]), (i, k)
+
+
+def test_every_network_matches_oracle(ragged):
+    # unpolished forward of the whole batch; each network's first and last
+    # scenarios against the oracle's correctly rounded iterate
+    rb, solvers, inputs = ragged
+    ms = MultiSolver(solvers, [b for _, b in rb], lds_budget=0)
+    outs, _ = ms.forward(inputs, accuracy=1e-6, polish_steps=-1, want_factor=False)
+    ms.check_forward(outs)
+    worst = 0.0
+    for i, ((net, B), o) in enumerate(zip(rb, outs)):
+        m = gga.GGAModel(net)
+        pick = [0, B - 1]
+        d = inputs[i][0][pick].cpu().numpy()
+        rh = np.broadcast_to(inputs[i][1].cpu().numpy(), (2, net.N)).copy()
+        ext = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-6, status_machine=True,
+                        extended=True)
+        it = o["iters"][pick].cpu().numpy()
+        assert it.tolist() == ext["iters"].tolist(), i
+        assert np.array_equal(o["status"][pick].cpu().numpy(), ext["status"]), i
+        dh = np.abs(o["head"][pick].cpu().numpy() - ext["head_ft"]).max()
+        worst = max(worst, dh)
+        assert dh <= TOL_H_FT, (i, dh)
+    print("64 networks: max|dH| vs oracle %.3e ft" % worst, flush=True)
+
+
+def test_multi_adjoint_matches_per_network_calls_and_oracle(ragged):
+    rb, solvers, inputs = ragged
+    ms = MultiSolver(solvers, [b for _, b in rb], lds_budget=0)
+    outs, keeps = ms.forward(inputs, accuracy=1e-8, polish_steps=3)
+    ms.check_forward(outs)
+    g = np.random.default_rng(4)
+    cot = []
+    for (net, B), s in zip(rb, solvers):
+        W = np.zeros((B, net.N))
+        W[:, s.junc_nodes[::7]] = g.normal(size=(B, s.junc_nodes[::7].size))
+        cot.append((torch.tensor(W, device=s.device), None, None))
+    grads, infos = ms.adjoint(outs, keeps, cot)
+    assert int(torch.cat(infos).abs().sum()) == 0
+    rel = lambda a, b: float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+    sample = [i for i, (n, _) in enumerate(rb) if n.Nj < 1500][::6]
+    for i, s in enumerate(solvers):
+        B = rb[i][1]
+        # the per-network adjoint on the same forward state
+        gd, grh, gke = (torch.empty(B, s.N, dtype=s.dtype, device=s.device) for _ in range(3))
+        gr = torch.empty(B, s.L, dtype=s.dtype, device=s.device)
+        info = torch.empty(B, dtype=torch.int32, device=s.device)
+        o, k = outs[i], keeps[i]
+        a = _lib.AdjointArgs(B=B, demand=_lib.ptr(k[0]), ke=_lib.ptr(k[2]),
+                             ke_stride=0, r_hw=_lib.ptr(k[3]), r_stride=0,
+                             head=_lib.ptr(o["head"]), flow=_lib.ptr(o["flow"]),
+                             emitter=_lib.ptr(o["emitter"]), status=_lib.ptr(o["status"]),
+                             factor=_lib.ptr(o["factor"]), gH=_lib.ptr(cot[i][0]),
+                             gQ=None, gE=None, n_refine=2, g_demand=_lib.ptr(gd),
+                             g_res_head=_lib.ptr(grh), g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr),
+                             info=_lib.ptr(info), workspace=_lib.ptr(ms._ws[i]))
+        with torch.cuda.device(s.device):
+            _lib.check(s._lib.wds_adjoint(s._plan, s._stream(), C.byref(a)), "wds_adjoint")
+        for x, y in zip(grads[i], (gd, grh, gke, gr)):
+            assert torch.equal(x, y), i
+        if i in sample:
+            net = rb[i][0]
+            m = gga.GGAModel(net)
+            pick = [0, B - 1]
+            d = k[0][pick].cpu().numpy()
+            rh = np.broadcast_to(k[1].cpu().numpy(), (2, net.N)).copy()
+            ke = np.broadcast_to(net.node_ke, (2, net.N)).copy()
+            sol = adj.solve_polished(m, d, rh, ke, m.r_hw, accuracy=1e-8, linsolve="sparse")
+            W = cot[i][0][pick].cpu().numpy()
+            cd, crh, cke, cr = adj.cpu_adjoint(m, sol, W, np.zeros((2, net.L)),
+                                               np.zeros((2, net.N)))
+            e = (rel(grads[i][0][pick].cpu().numpy(), cd),
+                 rel(grads[i][1][pick].cpu().numpy(), crh),
+                 rel(grads[i][3][pick].cpu().numpy(), cr))
+            print("network %d (Nj %d, %s, %d pumps, %d PRVs, %d TCVs): adjoint vs oracle %s"
+                  % (i, net.Nj, net.meta["headloss"], net.meta["pumps"], net.meta["prvs"],
+                     net.meta["tcvs"], ["%.1e" % x for x in e]), flush=True)
+            assert max(e) < 1e-9, (i, e)
+
+
+def test_autotuner_keeps_the_bits(ragged):
+    rb, solvers, inputs = ragged
+    ms = MultiSolver(solvers, [b for _, b in rb], lds_budget=0)
+    ref, _ = ms.forward(inputs, accuracy=1e-6, polish_steps=-1, want_factor=False)
+    t = ms.autotune(inputs, accuracy=1e-6, polish_steps=-1, want_factor=False)
+    print("autotune (budget bytes: ms):", {k: round(v, 2) for k, v in t.items()},
+          "picked", ms.lds_budget, flush=True)
+    assert ms.lds_budget in t and t[ms.lds_budget] == min(t.values())
+    got, _ = ms.forward(inputs, accuracy=1e-6, polish_steps=-1, want_factor=False)
+    for a, b in zip(ref, got):
+        assert torch.equal(a["head"], b["head"])

@@ -100,6 +100,14 @@ SIGNATURES = {
                                C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "wds_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
     "wds_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
+    "wds_multi_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32),
+                                   C.c_int64, C.POINTER(C.c_void_p)]),
+    "wds_multi_destroy": (None, [C.c_void_p]),
+    "wds_multi_workspace_bytes": (C.c_int64, [C.c_void_p, C.c_int32]),
+    "wds_multi_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                 C.POINTER(C.c_int64)]),
+    "wds_multi_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
+    "wds_multi_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdjointArgs)]),
     "wds_newton_iter": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs),
                                   C.c_int32, C.c_void_p]),
     "wds_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdjointArgs)]),

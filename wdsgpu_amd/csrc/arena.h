@@ -22,5 +22,9 @@ void arena_factor(const DevPlan& pl, const double* csr, double* factor_out, int3
 void arena_trisolve(const DevPlan& pl, const double* factor, const double* csr, const double* F,
                     double* H, int n_refine, double* ws, int B, hipStream_t st);
 void arena_adjoint(const DevPlan& pl, const AdjParams& ap, int B, hipStream_t st);
+void arena_solve_multi(const DevPlan* plans, const SolveParams* params, const int2* items,
+                       int nitems, int32_t* counter, int grid, hipStream_t st);
+void arena_adjoint_multi(const DevPlan* plans, const AdjParams* params, const int2* items,
+                         int nitems, int32_t* counter, int grid, hipStream_t st);
 
 }  // namespace wds

@@ -307,7 +307,8 @@ static int check_ready(wds_plan* P) {
   static bool lds_checked = false;
   if (!lds_checked) {
     const void* ks[] = {(const void*)k_solve, (const void*)k_newton_iter,
-                        (const void*)k_factor_solve, (const void*)k_adjoint};
+                        (const void*)k_factor_solve, (const void*)k_adjoint,
+                        (const void*)k_solve_multi, (const void*)k_adjoint_multi};
     for (const void* k : ks) {
       hipFuncAttributes at{};
       hipError_t e = hipFuncGetAttributes(&at, k);
@@ -527,6 +528,242 @@ extern "C" int wds_solve_noalloc(wds_plan* P, void* stream, const wds_solve_args
   wds_solve_args b = *a;
   b.workspace = ptr;
   return wds_solve(P, stream, &b);
+}
+
+// ---------------------------------------------------------------------------
+// Ragged batches of networks (wds_multi_*): one persistent launch per kernel
+// mode over a work list of (network, scenario) items.
+struct wds_multi {
+  std::vector<wds_plan*> plans;
+  std::vector<int32_t> B;
+  std::vector<int32_t> arena;     // per network: 1 arena mode, 0 LDS mode
+  std::vector<int64_t> ws_bytes;  // per network workspace (mode-aware)
+  int64_t lds_bytes = 0;          // dynamic LDS of the LDS-mode launch
+  int32_t n_lds = 0, n_arena = 0, grid_lds = 0, grid_arena = 0;
+  int device = -1;
+  void* d_mem = nullptr;
+  DevPlan* d_plans = nullptr;
+  SolveParams* d_sp = nullptr;
+  AdjParams* d_ap = nullptr;
+  int2* d_items = nullptr;
+  int32_t* d_counter = nullptr;
+  std::vector<SolveParams> h_sp;
+  std::vector<AdjParams> h_ap;
+};
+
+namespace {
+__global__ void k_reset_counters(int32_t* c) {
+  if (threadIdx.x < 2) c[threadIdx.x] = 0;
+}
+
+// latency proxy of one scenario of a network: program steps of a Newton
+// iteration (factor + three solve pairs) plus its link and row sweeps
+int64_t scenario_cost(const wds_plan* P) {
+  const wds_plan_stats& st = P->stats;
+  return (int64_t)st.factor_steps + 3 * ((int64_t)st.fwd_steps + st.bwd_steps) +
+         ((int64_t)P->L + P->Nj) / 64;
+}
+}  // namespace
+
+extern "C" int wds_multi_create(wds_plan* const* plans, int32_t nnet, const int32_t* B,
+                                int64_t lds_budget, wds_multi** out) {
+  if (!plans || !B || !out || nnet <= 0) {
+    set_error("wds_multi_create: bad argument");
+    return WDS_EINVAL;
+  }
+  int dev = -1;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+  wds_multi* M = new wds_multi();
+  M->device = dev;
+  std::vector<DevPlan> dp(nnet);
+  std::vector<int64_t> cost(nnet);
+  for (int i = 0; i < nnet; ++i) {
+    wds_plan* P = plans[i];
+    if (!P || B[i] <= 0) {
+      delete M;
+      set_error("wds_multi_create: null plan or non-positive batch");
+      return WDS_EINVAL;
+    }
+    int r = wds_plan_upload(P);
+    if (r) {
+      delete M;
+      return r;
+    }
+    const bool ar = P->arena > 0 || lds_budget <= 0 || P->stats.lds_bytes > lds_budget;
+    dp[i] = P->dev;
+    dp[i].arena = ar ? (P->arena > 0 ? P->arena : (P->stats.lds_bytes / 8 + 31) / 32 * 32) : 0;
+    M->plans.push_back(P);
+    M->B.push_back(B[i]);
+    M->arena.push_back(ar ? 1 : 0);
+    M->ws_bytes.push_back((int64_t)B[i] * ws_stride(dp[i]) * 8);
+    if (!ar) M->lds_bytes = std::max<int64_t>(M->lds_bytes, P->stats.lds_bytes);
+    cost[i] = scenario_cost(P);
+  }
+  // items: the LDS-mode group first, then the arena-mode group, each with its
+  // costliest networks' scenarios first
+  std::vector<int> order(nnet);
+  for (int i = 0; i < nnet; ++i) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<int2> items;
+  for (int pass = 0; pass < 2; ++pass)
+    for (int i : order)
+      if (M->arena[i] == pass)
+        for (int b = 0; b < B[i]; ++b) items.push_back(make_int2(i, b));
+  for (int i = 0; i < nnet; ++i) (M->arena[i] ? M->n_arena : M->n_lds) += B[i];
+  hipDeviceProp_t prop;
+  e = hipGetDeviceProperties(&prop, dev);
+  if (e != hipSuccess) {
+    delete M;
+    return hip_fail(e, "hipGetDeviceProperties");
+  }
+  const int cus = prop.multiProcessorCount;
+  // one wavefront per SIMD (four per CU), fewer where the LDS layout is
+  // larger than a quarter of the CU's LDS
+  const int per_cu_lds =
+      M->lds_bytes > 0 ? (int)std::max<int64_t>(1, std::min<int64_t>(4, kMaxLds / M->lds_bytes))
+                       : 4;
+  M->grid_lds = std::min(M->n_lds, cus * per_cu_lds);
+  M->grid_arena = std::min(M->n_arena, cus * 4);
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  const size_t o_plans = 0, o_sp = al(o_plans + nnet * sizeof(DevPlan)),
+               o_ap = al(o_sp + nnet * sizeof(SolveParams)),
+               o_items = al(o_ap + nnet * sizeof(AdjParams)),
+               o_cnt = al(o_items + items.size() * sizeof(int2)), total = o_cnt + 256;
+  e = hipMalloc(&M->d_mem, total);
+  if (e != hipSuccess) {
+    delete M;
+    return hip_fail(e, "hipMalloc(multi)");
+  }
+  char* base = (char*)M->d_mem;
+  M->d_plans = (DevPlan*)(base + o_plans);
+  M->d_sp = (SolveParams*)(base + o_sp);
+  M->d_ap = (AdjParams*)(base + o_ap);
+  M->d_items = (int2*)(base + o_items);
+  M->d_counter = (int32_t*)(base + o_cnt);
+  e = hipMemcpy(M->d_plans, dp.data(), nnet * sizeof(DevPlan), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(M->d_items, items.data(), items.size() * sizeof(int2), hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(M->d_mem);
+    delete M;
+    return hip_fail(e, "hipMemcpy(multi)");
+  }
+  M->h_sp.resize(nnet);
+  M->h_ap.resize(nnet);
+  *out = M;
+  return WDS_OK;
+}
+
+extern "C" void wds_multi_destroy(wds_multi* M) {
+  if (!M) return;
+  if (M->d_mem) (void)hipFree(M->d_mem);
+  delete M;
+}
+
+extern "C" int64_t wds_multi_workspace_bytes(const wds_multi* M, int32_t net) {
+  if (!M || net < 0 || net >= (int32_t)M->plans.size()) return WDS_EINVAL;
+  return M->ws_bytes[net];
+}
+
+extern "C" int wds_multi_info(const wds_multi* M, int32_t* n_lds, int32_t* n_arena,
+                              int64_t* lds_bytes) {
+  if (!M) return WDS_EINVAL;
+  if (n_lds) *n_lds = M->n_lds;
+  if (n_arena) *n_arena = M->n_arena;
+  if (lds_bytes) *lds_bytes = M->lds_bytes;
+  return WDS_OK;
+}
+
+static int multi_check(wds_multi* M) {
+  if (!M) {
+    set_error("null multi batch");
+    return WDS_EINVAL;
+  }
+  int dev = -1;
+  (void)hipGetDevice(&dev);
+  if (dev != M->device) {
+    set_error("multi batch built on another device");
+    return WDS_EINVAL;
+  }
+  for (wds_plan* P : M->plans) {
+    int r = check_ready(P);
+    if (r && !(P->arena == 0 && P->stats.lds_bytes > kMaxLds)) return r;
+  }
+  return WDS_OK;
+}
+
+extern "C" int wds_multi_solve(wds_multi* M, void* stream, const wds_solve_args* args) {
+  int r = multi_check(M);
+  if (r) return r;
+  const int nnet = (int)M->plans.size();
+  for (int i = 0; i < nnet; ++i) {
+    const wds_solve_args* a = &args[i];
+    wds_plan* P = M->plans[i];
+    if (a->B != M->B[i] || !a->demand || !a->res_head || !a->ke || !a->r_hw || !a->head ||
+        !a->flow || !a->emitter || !a->iters || !a->relerr || !a->status || !a->info ||
+        !a->workspace || a->max_iter < 0 || a->n_refine < 0) {
+      set_error("wds_multi_solve: bad argument for network " + std::to_string(i));
+      return WDS_EINVAL;
+    }
+    if ((P->p > 0 || P->has_cv) && !a->status_machine) {
+      set_error("networks with PRVs or CV pipes need status_machine=1");
+      return WDS_EUNSUPPORTED;
+    }
+    r = check_sched(P, a);
+    if (r) return r;
+    M->h_sp[i] = make_params(a, P->N);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(M->d_sp, M->h_sp.data(), nnet * sizeof(SolveParams),
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(multi params)");
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, M->d_counter);
+  if (M->n_lds) {
+    set_lds(k_solve_multi, M->lds_bytes);
+    hipLaunchKernelGGL(k_solve_multi, dim3(M->grid_lds), dim3(64), M->lds_bytes, st, M->d_plans,
+                       M->d_sp, M->d_items, M->n_lds, M->d_counter);
+  }
+  if (M->n_arena)
+    arena_solve_multi(M->d_plans, M->d_sp, M->d_items + M->n_lds, M->n_arena,
+                      M->d_counter + 1, M->grid_arena, st);
+  for (wds_plan* P : M->plans) P->cnt.launches += 1;
+  return launch_check("k_solve_multi");
+}
+
+extern "C" int wds_multi_adjoint(wds_multi* M, void* stream, const wds_adjoint_args* args) {
+  int r = multi_check(M);
+  if (r) return r;
+  const int nnet = (int)M->plans.size();
+  for (int i = 0; i < nnet; ++i) {
+    const wds_adjoint_args* a = &args[i];
+    if (a->B != M->B[i] || !a->factor || !a->g_demand || !a->g_res_head || !a->g_ke ||
+        !a->g_r || !a->info || !a->workspace || !a->head || !a->flow || !a->emitter ||
+        !a->status || !a->ke || !a->r_hw || a->n_refine < 0) {
+      set_error("wds_multi_adjoint: bad argument for network " + std::to_string(i));
+      return WDS_EINVAL;
+    }
+    M->h_ap[i] = make_adj_params(a);
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipError_t e = hipMemcpyAsync(M->d_ap, M->h_ap.data(), nnet * sizeof(AdjParams),
+                                hipMemcpyHostToDevice, st);
+  if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(multi params)");
+  hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, M->d_counter);
+  if (M->n_lds) {
+    set_lds(k_adjoint_multi, M->lds_bytes);
+    hipLaunchKernelGGL(k_adjoint_multi, dim3(M->grid_lds), dim3(64), M->lds_bytes, st,
+                       M->d_plans, M->d_ap, M->d_items, M->n_lds, M->d_counter);
+  }
+  if (M->n_arena)
+    arena_adjoint_multi(M->d_plans, M->d_ap, M->d_items + M->n_lds, M->n_arena,
+                        M->d_counter + 1, M->grid_arena, st);
+  for (wds_plan* P : M->plans) {
+    P->cnt.bwd_solve += 1;
+    P->cnt.bwd_reuse += 1;
+    P->cnt.launches += 1;
+  }
+  return launch_check("k_adjoint_multi");
 }
 
 extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* a, int32_t it,

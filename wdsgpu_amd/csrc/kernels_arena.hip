@@ -45,4 +45,16 @@ void arena_adjoint(const DevPlan& pl, const AdjParams& ap, int B, hipStream_t st
   hipLaunchKernelGGL(k_adjoint_arena, dim3(B), dim3(64), 0, st, pl, ap);
 }
 
+void arena_solve_multi(const DevPlan* plans, const SolveParams* params, const int2* items,
+                       int nitems, int32_t* counter, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_solve_multi_arena, dim3(grid), dim3(64), 0, st, plans, params, items,
+                     nitems, counter);
+}
+
+void arena_adjoint_multi(const DevPlan* plans, const AdjParams* params, const int2* items,
+                         int nitems, int32_t* counter, int grid, hipStream_t st) {
+  hipLaunchKernelGGL(k_adjoint_multi_arena, dim3(grid), dim3(64), 0, st, plans, params, items,
+                     nitems, counter);
+}
+
 }  // namespace wds

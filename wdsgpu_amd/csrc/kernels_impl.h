@@ -178,10 +178,10 @@ __host__ __device__ inline int64_t ws_stride(const DevPlan& pl) {
 // the scenario's LDS layout: the workgroup's dynamic LDS, or (arena mode) the
 // arena at the end of the scenario's workspace slice
 #ifndef WDS_GLOBAL_ARENA
-#define WDS_SCENARIO_MEM(ws) extern __shared__ double lds[]
+#define WDS_SCENARIO_MEM(ws, b) extern __shared__ double lds[]
 #else
-#define WDS_SCENARIO_MEM(ws)                                                  \
-  double* const lds = (ws) + (int64_t)blockIdx.x * ws_stride(pl) +           \
+#define WDS_SCENARIO_MEM(ws, b)                                               \
+  double* const lds = (ws) + (int64_t)(b) * ws_stride(pl) +                  \
                       ws_doubles(pl.L, pl.Nj, pl.na, pl.wp_tot, pl.np)
 #endif
 
@@ -811,10 +811,9 @@ struct Ctrl {
 
 // status-machine bookkeeping after an iteration (dgga/solver.py:3113-3179);
 // returns true when the scenario is done
-__device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams& sp, int it,
-                                    double relerr, Ctrl& st) {
+__device__ inline bool control_step(const DevPlan& pl, Scn& c, const SolveParams& sp, int b,
+                                    int it, double relerr, Ctrl& st) {
   if (!sp.sm) return !(relerr > sp.hacc);
-  const int b = blockIdx.x;
   SchedLog lg{sp.sched_events ? sp.sched_events + (int64_t)b * sp.sched_cap : nullptr,
               sp.sched_cap, it, st.nev};
   bool vchg = false;
@@ -892,8 +891,8 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor_solve)(DevPlan pl, const doub
                                                      const double* F, double* H, int n_refine,
                                                      double* factor_out, int32_t* info,
                                                      double* ws) {
-  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
+  WDS_SCENARIO_MEM(ws, b);
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, true);
   const double* Fb = F + (int64_t)b * pl.Nj;
@@ -918,8 +917,8 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor_solve)(DevPlan pl, const doub
 // dgga/solver.py:2630-2640): factor_out [B][nnzL], info [B].
 __global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor)(DevPlan pl, const double* csr,
                                                double* factor_out, int32_t* info, double* ws) {
-  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
+  WDS_SCENARIO_MEM(ws, b);
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, true);
   if (lane == 0) c.V[pl.nnzL] = 0.0;
@@ -936,8 +935,8 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_factor)(DevPlan pl, const double* cs
 __global__ WDS_SCENARIO_KERNEL void WDS_K(k_trisolve)(DevPlan pl, const double* factor,
                                                  const double* csr, const double* F, double* H,
                                                  int n_refine, double* ws) {
-  WDS_SCENARIO_MEM(ws);
   const int b = blockIdx.x, lane = lane_id();
+  WDS_SCENARIO_MEM(ws, b);
   Scn c = make_scn(pl, lds, ws, b, nullptr, nullptr, nullptr, 0);
   load_csr(pl, c, csr, b, false);
   for (int t = lane; t < pl.nnzL; t += 64) c.V[t] = factor[(int64_t)b * pl.nnzL + t];
@@ -995,10 +994,10 @@ __device__ inline void init_state(const DevPlan& pl, Scn& c, const SolveParams& 
   }
 }
 
-// Fused forward: the whole Newton loop of scenario blockIdx.x.
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
-  WDS_SCENARIO_MEM(sp.ws);
-  const int b = blockIdx.x, lane = lane_id();
+// Fused forward: the whole Newton loop of scenario b, its layout at lds.
+__device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolveParams& sp, int b,
+                                      double* lds) {
+  const int lane = lane_id();
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
   load_inputs(pl, c, sp, b);
   init_state(pl, c, sp, b);
@@ -1020,7 +1019,7 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
     st.iters = it;
     relerr_out = relerr;
     WDS_T0();
-    const bool done = control_step(pl, c, sp, it, relerr, st);
+    const bool done = control_step(pl, c, sp, b, it, relerr, st);
     WDS_TACC(c, 5);
     if (done) {
       st.active = 0;
@@ -1046,13 +1045,20 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
   }
 }
 
+// one scenario per workgroup (scenario blockIdx.x)
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
+  const int b = blockIdx.x;
+  WDS_SCENARIO_MEM(sp.ws, b);
+  solve_scenario(pl, sp, b, lds);
+}
+
 // One Newton iteration per launch (the unfused path).  ctrl[b] = {active,
 // iters, nextcheck, info}; heads/flows/emitters/statuses persist in the
 // output arrays between launches (it == 1 initialises them).
 __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams sp, int it,
                                                     int32_t* ctrl) {
-  WDS_SCENARIO_MEM(sp.ws);
   const int b = blockIdx.x, lane = lane_id();
+  WDS_SCENARIO_MEM(sp.ws, b);
   int32_t* cb = ctrl + 4 * b;
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
   load_inputs(pl, c, sp, b);
@@ -1095,7 +1101,7 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams
     st.info = it;
   } else {
     st.iters = it;
-    if (control_step(pl, c, sp, it, relerr, st)) st.active = 0;
+    if (control_step(pl, c, sp, b, it, relerr, st)) st.active = 0;
     if (st.active && it == maxtrials) st.iters = maxtrials + 1;
   }
   write_outputs(pl, c, sp, b);
@@ -1109,6 +1115,58 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams
     sp.info[b] = st.info;
     if (sp.diag_ratio && pl.p > 0) sp.diag_ratio[b] = dratio;
     if (sp.sched_count) sp.sched_count[b] = st.nev;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ragged batches: many networks, each with its own plan and arrays, in one
+// persistent launch.  The work items (network, scenario) are in the order the
+// host chose (longest first); every wavefront takes the next item from a
+// global counter, solves it with the network's plan, and exits when the list
+// is exhausted.  In LDS mode the dynamic LDS is the largest layout of the
+// group and every item reuses it; in arena mode each item's layout is its
+// arena.
+__device__ __forceinline__ int next_item(int32_t* counter) {
+  int i = 0;
+  if (lane_id() == 0) i = atomicAdd(counter, 1);
+  return __shfl(i, 0, 64);
+}
+
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve_multi)(const DevPlan* plans,
+                                                         const SolveParams* params,
+                                                         const int2* items, int32_t nitems,
+                                                         int32_t* counter) {
+  for (;;) {
+    const int i = next_item(counter);
+    if (i >= nitems) break;
+    const int2 w = items[i];
+    const int net = __builtin_amdgcn_readfirstlane(w.x);
+    const int b = __builtin_amdgcn_readfirstlane(w.y);
+    const DevPlan& pl = plans[net];
+    const SolveParams& sp = params[net];
+    WDS_SCENARIO_MEM(sp.ws, b);
+    solve_scenario(pl, sp, b, lds);
+    mem_sync();
+    lds_sync();
+  }
+}
+
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint_multi)(const DevPlan* plans,
+                                                           const AdjParams* params,
+                                                           const int2* items, int32_t nitems,
+                                                           int32_t* counter) {
+  for (;;) {
+    const int i = next_item(counter);
+    if (i >= nitems) break;
+    const int2 w = items[i];
+    const int net = __builtin_amdgcn_readfirstlane(w.x);
+    const int b = __builtin_amdgcn_readfirstlane(w.y);
+    const DevPlan& pl = plans[net];
+    const AdjParams& ap = params[net];
+    WDS_SCENARIO_MEM(ap.ws, b);
+    adjoint_scenario(pl, ap, b, lds);
+    mem_sync();
+    lds_sync();
   }
 }
 

@@ -647,9 +647,9 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
 
 // Backward of the implicit solve (ImplicitGGASolveGPU.backward,
 // dgga/autodiff.py:1245-1312), one wavefront per scenario.
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
-  WDS_SCENARIO_MEM(ap.ws);
-  const int b = blockIdx.x, lane = lane_id();
+__device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjParams& ap, int b,
+                                        double* lds) {
+  const int lane = lane_id();
   Scn c = make_scn(pl, lds, ap.ws, b, const_cast<double*>(ap.flow),
                    const_cast<int8_t*>(ap.status), ap.r, ap.r_stride);
   const double* ke = ap.ke + (int64_t)b * ap.ke_stride;
@@ -833,6 +833,12 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
     gke[node] = 0.0;
   }
   if (lane == 0) ap.info[b] = 0;
+}
+
+__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
+  const int b = blockIdx.x;
+  WDS_SCENARIO_MEM(ap.ws, b);
+  adjoint_scenario(pl, ap, b, lds);
 }
 
 }  // namespace wds

@@ -356,14 +356,14 @@ def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,
     g.shuffle(trunk)
     prv_idx, used_up, used_dn = [], set(), set()
     for i in trunk:
+        if len(prv_idx) == n_prv:
+            break
         p, v = tree[i]
         if v in used_dn or v in used_up or p in used_dn or p in used_up:
             continue
         prv_idx.append(i)
         used_up.add(p)
         used_dn.add(v)
-        if len(prv_idx) == n_prv:
-            break
     assert len(prv_idx) == n_prv
     prv_set = set(prv_idx)
     # pressure zones: a junction belongs to the zone of its nearest PRV
@@ -418,6 +418,114 @@ def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,
                 seed=int(seed), workload="planar%d" % n_junc,
                 leak_nodes=np.sort(leaks).tolist(), emitter_c=c_leak)
     return _mk_net(ntype, elev, demand, links, meta, ke=ke)
+
+
+def ragged_member(seed):
+    """One member of the ragged multi-network batch: a seeded planar network
+    (planar_network) of log-uniform size Nj in [50, 5000], Hazen-Williams or
+    Darcy-Weisbach friction (roughness U[0.01, 0.5] mm), 0-4 pumps lifting
+    from low reservoirs (head U[10, 30] m) into random junctions, each with a
+    3-point head curve (0, h0), (qd, 0.75 h0), (2 qd, 0), shutoff h0 U[40, 80]
+    m and design flow qd U[20, 200] L/s, and 0-3 valves: PRVs on trunk mains
+    and TCVs (loss coefficient U[2, 10]) on random pipes."""
+    from .network import TCV as _TCV
+    g = np.random.default_rng(10_000 + int(seed))
+    nj = int(round(np.exp(g.uniform(np.log(50.0), np.log(5000.0)))))
+    headloss = "D-W" if g.random() < 0.5 else "H-W"
+    n_pumps = int(g.integers(0, 5))
+    n_valves = int(g.integers(0, 4))
+    n_prv = int(g.integers(0, n_valves + 1))
+    n_tcv = n_valves - n_prv
+    while True:
+        try:
+            net = planar_network(int(seed), n_junc=nj, n_pipes=int(round(nj * 905 / 782)),
+                                 n_prv=n_prv, n_leaks=min(10, max(1, nj // 50)))
+            break
+        except AssertionError:      # too few trunk mains for the PRVs
+            if n_prv == 0:
+                raise
+            n_prv -= 1
+            n_tcv += 1
+    lt = np.asarray(net.link_type).copy()
+    st = np.asarray(net.init_status).copy()
+    setting = np.asarray(net.valve_setting_user, dtype=np.float64).copy()
+    rough = np.asarray(net.roughness, dtype=np.float64).copy()
+    r = np.asarray(net.r_hw, dtype=np.float64).copy()
+    pipes = np.nonzero(lt == PIPE)[0]
+    for k in g.choice(pipes, size=n_tcv, replace=False):
+        lt[k] = TCV
+        st[k] = 2
+        setting[k] = g.uniform(2.0, 10.0)
+        rough[k] = 0.0
+        r[k] = 0.0
+    pipe = lt == PIPE
+    if headloss == "D-W":
+        rough = np.where(pipe, g.uniform(0.01, 0.5, lt.size) / (1000.0 * M_PER_FT), 0.0)
+        r = np.where(pipe, dw_resistance(np.where(pipe, net.len_ft, 1.0),
+                                         np.where(pipe, net.diam_ft, 1.0)), 0.0)
+    nt = list(np.asarray(net.node_type))
+    elev = list(np.asarray(net.elev_ft))
+    dem = list(np.asarray(net.demand_cfs))
+    ke = list(np.asarray(net.node_ke))
+    n1, n2 = list(net.link_n1), list(net.link_n2)
+    diam, length, km = list(net.diam_ft), list(net.len_ft), list(net.km_int)
+    lt, st, setting, rough, r = list(lt), list(st), list(setting), list(rough), list(r)
+    link_id, node_id = list(net.link_id), list(net.node_id)
+    curves = []
+    junc = np.nonzero(np.asarray(net.node_type) == JUNCTION)[0]
+    for j in g.choice(junc, size=n_pumps, replace=False):
+        res = len(nt)
+        nt.append(RESERVOIR)
+        elev.append(g.uniform(10.0, 30.0) / M_PER_FT)
+        dem.append(0.0)
+        ke.append(0.0)
+        node_id.append("RP%d" % res)
+        h0 = g.uniform(40.0, 80.0)
+        qd = g.uniform(20.0, 200.0)
+        curves.append((len(lt), pump_power_curve(h0, 0.75 * h0, qd, 0.0, 2.0 * qd)))
+        n1.append(res)
+        n2.append(int(j))
+        lt.append(PUMP)
+        st.append(1)
+        setting.append(1.0)
+        for arr, v in ((diam, 0.0), (length, 0.0), (km, 0.0), (rough, 0.0), (r, 0.0)):
+            arr.append(v)
+        link_id.append("PU%d" % len(link_id))
+    kw = {}
+    if curves:
+        kw = dict(pump_link=np.array([k for k, _ in curves], dtype=np.int32),
+                  pump_ptype=np.full(len(curves), PUMP_POWER_FUNC, dtype=np.int8),
+                  pump_h0=np.array([c["h0"] for _, c in curves]),
+                  pump_r=np.array([c["r"] for _, c in curves]),
+                  pump_n=np.array([c["n"] for _, c in curves]),
+                  pump_q0=np.array([c["q0"] for _, c in curves]),
+                  pump_qmax=np.array([c["qmax"] for _, c in curves]),
+                  pump_hmax=np.array([c["hmax"] for _, c in curves]))
+    meta = dict(net.meta)
+    meta.update(headloss=headloss, workload="ragged%d-%s" % (nj, headloss), seed=int(seed),
+                pumps=n_pumps, prvs=n_prv, tcvs=n_tcv)
+    c_leak = np.concatenate([np.asarray(meta.get("emitter_c", np.zeros(net.N))),
+                             np.zeros(n_pumps)])
+    meta["emitter_c"] = c_leak
+    return Net(node_id=node_id, node_type=np.asarray(nt, dtype=np.int8),
+               elev_ft=np.asarray(elev), node_ke=np.asarray(ke), demand_cfs=np.asarray(dem),
+               link_id=link_id, link_type=np.asarray(lt, dtype=np.int8),
+               link_n1=np.asarray(n1, dtype=np.int32), link_n2=np.asarray(n2, dtype=np.int32),
+               diam_ft=np.asarray(diam), len_ft=np.asarray(length), roughness=np.asarray(rough),
+               km_int=np.asarray(km), r_hw=np.asarray(r),
+               init_status=np.asarray(st, dtype=np.int8),
+               valve_setting_user=np.asarray(setting), meta=meta, **kw)
+
+
+def ragged_batch(seed=0, n_net=64, total=4096, b_range=(16, 256)):
+    """The ragged workload: n_net ragged_member networks (seeds seed*1000 + i)
+    with scenario counts log-uniform in b_range, scaled to about ``total``
+    solves.  Returns [(net, B)]."""
+    g = np.random.default_rng(seed)
+    nets = [ragged_member(seed * 1000 + i) for i in range(n_net)]
+    b = np.exp(g.uniform(np.log(b_range[0]), np.log(b_range[1]), n_net))
+    b = np.clip(np.round(b * total / b.sum()), b_range[0], b_range[1]).astype(int)
+    return list(zip(nets, b.tolist()))
 
 
 def scenario_batch(net, B, seed, sigma=0.3):

@@ -476,7 +476,7 @@ class GGASolver:
 
     def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
               status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True,
-              status0=None):
+              status0=None, ws=None):
         d = self._prep(demand_cfs)
         rh = self._prep(res_head_ft)
         single = d.dim() == 1
@@ -542,7 +542,8 @@ class GGASolver:
             if bool(((s0t < 0) | (s0t > 7)).any()):
                 raise ValueError("link_status holds internal status codes 0..7")
             s0t = s0t.expand(B, -1).contiguous()
-        ws = self._workspace(B)
+        if ws is None:
+            ws = self._workspace(B)
         keep = (d.contiguous(), rh, ke, r, q0t, e0t, ws, s0t)
         a = _lib.SolveArgs(
             B=B, demand=_lib.ptr(keep[0]), res_head=_lib.ptr(rh), ke=_lib.ptr(ke),
