@@ -41,6 +41,13 @@ def main():
     ap.add_argument("--stages", action="store_true",
                     help="also time the standalone factor (k_factor) and triangular-solve "
                          "(k_trisolve) kernels at the same B against their algorithmic bytes")
+    ap.add_argument("--workload", choices=("calibration", "ragged", "eps"),
+                    default="calibration",
+                    help="calibration: B scenarios of the 782-junction network (the headline "
+                         "metric); ragged: the 64-network ragged batch in one launch per "
+                         "kernel mode; eps: the differentiable 24-step EPS of the "
+                         "20000-junction network at B=256 with checkpointed adjoint")
+    ap.add_argument("--checkpoint-every", type=int, default=4)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run one plain 512 MiB read + 512 MiB write elementwise "
                          "kernel, the reference point that scales rocprofv3 FETCH_SIZE / "
@@ -52,6 +59,10 @@ def main():
               "on a single MI355X (--gpus 1); multi-GPU runs are not part of it.",
               file=sys.stderr)
         sys.exit(2)
+    if args.workload == "ragged":
+        return bench_ragged(args)
+    if args.workload == "eps":
+        return bench_eps(args)
     # the CPU baseline runs first: its worker processes are forked before
     # this process touches the GPU
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample, args.B)
@@ -312,6 +323,265 @@ def cpu_baseline(n_single, B):
             "single_thread": {"value": round(n_single / t1, 3), "unit": "scenarios/s",
                               "cores": 1, "sample": "%d scenarios" % n_single,
                               "seconds": round(t1, 2)},
+            "host_cpus": os.cpu_count()}
+
+
+RAGGED_METRIC = "fwd+adjoint solves/s, ragged 64-network batch in one launch per kernel mode"
+
+
+def bench_ragged(args):
+    """The ragged batch (64 seeded networks, 50-5000 junctions, 4114 solves):
+    a step is the forward (accuracy 1e-6, polish 3) plus the implicit adjoint
+    of every solve through the multi-network entry points, LDS budget picked
+    by the autotuner.  value = solves per second over the whole batch."""
+    from wdsgpu_amd.generators import ragged_batch
+    batch = ragged_batch(0)
+    cpu = None if args.no_cpu_baseline else cpu_baseline_ragged(batch)
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    from wdsgpu_amd.roofline import adjoint_bytes, forward_bytes
+    from wdsgpu_amd.workloads import Ragged
+    dev = torch.device("cuda:0")
+    log("building the ragged workload and autotuning")
+    wl = Ragged(device=dev, batch=batch)
+    ms = wl.multi
+    n_lds, n_arena, lds_bytes = ms.info()
+    log("autotune (budget: ms) %s, picked %d" % (
+        {k: round(v, 2) for k, v in wl.tuning.items()}, ms.lds_budget))
+    for _ in range(args.warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    wl.check()
+    ms.event_log = []
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        outs, grads = wl.step()
+    torch.cuda.synchronize()
+    wl.check()
+    t1 = time.perf_counter()
+    n = wl.n_solves
+    value = n * args.steps / (t1 - t0)
+    ms_per_step = (t1 - t0) * 1e3 / args.steps
+    ev = ms.event_log
+    ms.event_log = None
+    fw_ms = float(np.mean([a.elapsed_time(b) for (nm, a, b) in ev if nm == "wds_multi_solve"]))
+    bw_ms = float(np.mean([a.elapsed_time(b) for (nm, a, b) in ev
+                           if nm == "wds_multi_adjoint"]))
+    fbytes = abytes = 0
+    iters_total = 0
+    for s, o, (net, B) in zip(wl.solvers, outs, wl.batch):
+        st = s.stats
+        ne = int(np.count_nonzero(net.node_ke > 0))
+        it = int(o["iters"].sum().item())
+        iters_total += it
+        fbytes += forward_bytes(st, it, wl.polish_steps, B, ne)
+        abytes += adjoint_bytes(st["L"], st["Nj"], st["nnz_lower"], st["nnzL"], ne,
+                                st["N"] - st["Nj"]) * B
+    achieved = fbytes / (fw_ms * 1e-3) / 1e9
+    kern = "k_solve_multi_arena" if n_lds == 0 else "k_solve_multi + k_solve_multi_arena"
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": kern, "kernel_ms": round(fw_ms, 4), "algorithmic_bytes": int(fbytes),
+                "adjoint_kernel_ms": round(bw_ms, 4),
+                "adjoint_achieved_GBps": round(abytes / (bw_ms * 1e-3) / 1e9, 1),
+                "mean_newton_iters": round(iters_total / n, 3)}
+    # the same work as one forward and one adjoint call per network
+    for _ in range(2):
+        wl.step(per_network=True)
+    torch.cuda.synchronize()
+    wl.check()
+    t2 = time.perf_counter()
+    for _ in range(args.steps):
+        wl.step(per_network=True)
+    torch.cuda.synchronize()
+    wl.check()
+    per_net = n * args.steps / (time.perf_counter() - t2)
+    nets = [net for net, _ in wl.batch]
+    out = {"metric": RAGGED_METRIC, "value": round(value, 1), "unit": "solves/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(ms_per_step, 3), "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (64 seeded networks, lognormal demand scenarios)",
+           "config": {"workload": "ragged batch: 64 networks, forward (polish 3) + implicit "
+                                  "adjoint of sensor-head functionals",
+                      "networks": len(nets), "solves": n,
+                      "Nj_range": [min(x.Nj for x in nets), max(x.Nj for x in nets)],
+                      "D-W_networks": sum(x.meta["headloss"] == "D-W" for x in nets),
+                      "pumps": sum(x.meta["pumps"] for x in nets),
+                      "prvs": sum(x.meta["prvs"] for x in nets),
+                      "tcvs": sum(x.meta["tcvs"] for x in nets),
+                      "accuracy": wl.accuracy, "polish_steps": wl.polish_steps,
+                      "lds_budget": ms.lds_budget, "lds_solves": n_lds,
+                      "arena_solves": n_arena,
+                      "autotune_ms": {str(k): round(v, 2) for k, v in wl.tuning.items()}},
+           "roofline": roofline,
+           "per_network_calls": {"value": round(per_net, 1), "unit": "solves/s",
+                                 "launches_per_step": 2 * len(nets)}}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+EPS_METRIC = "differentiable EPS fwd+adjoint scenario-steps/s (24 h, checkpointed adjoint)"
+
+
+def bench_eps(args):
+    """The extended-period calibration workload: the seeded 20000-junction
+    geometric network (4 reservoirs, 6 tanks, 8 pumps with level controls),
+    B = 256 scenarios (lognormal demand factors) over 24 hourly steps; loss
+    = squared misfit of 30 sensor heads and the 6 tank levels at every step
+    against a run with perturbed roughness (C + N(0, 5)); gradients with
+    respect to every demand, pipe C, reservoir head, initial tank level and
+    leak C.  A step of the bench is one forward + backward of the whole
+    24-hour simulation; value = B * 24 / seconds per step."""
+    from wdsgpu_amd.workloads import EpsCalibration
+    B = 256 if args.B == 1024 else args.B
+    cpu = None if args.no_cpu_baseline else cpu_baseline_eps()
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    from wdsgpu_amd.roofline import adjoint_bytes, forward_bytes
+    dev = torch.device("cuda:0")
+    log("building the EPS workload B=%d" % B)
+    wl = EpsCalibration(B=B, device=dev, checkpoint_every=args.checkpoint_every)
+    m, s = wl.model, wl.model.s
+    st = s.stats
+    for _ in range(args.warmup):
+        wl.step()
+    torch.cuda.synchronize()
+    torch.cuda.reset_peak_memory_stats(dev)
+    base = torch.cuda.memory_allocated(dev)
+    s.event_log = []
+    solves0 = m.solves
+    log("timing %d steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, grads = wl.step()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    peak = torch.cuda.max_memory_allocated(dev)
+    ev = s.event_log
+    s.event_log = None
+    fw = [(a.elapsed_time(b), it) for (n, a, b, _, it) in ev if n == "k_solve"]
+    bw = [a.elapsed_time(b) for (n, a, b, _, _) in ev if n == "k_adjoint"]
+    fw_ms = float(np.mean([x[0] for x in fw]))
+    bw_ms = float(np.mean(bw))
+    iters = float(np.mean([int(x[1].sum()) for x in fw]))
+    ne = int(np.count_nonzero(wl.net.node_ke > 0))
+    fbytes = forward_bytes(st, iters, wl.polish_steps, B, ne)
+    abytes = adjoint_bytes(st["L"], st["Nj"], st["nnz_lower"], st["nnzL"], ne,
+                           st["N"] - st["Nj"]) * B
+    achieved = fbytes / (fw_ms * 1e-3) / 1e9
+    sec = (t1 - t0) / args.steps
+    out = {"metric": EPS_METRIC, "value": round(B * m.T / sec, 1), "unit": "scenario-steps/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": round(sec * 1e3, 3), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded geometric network, lognormal demand scenarios, "
+                   "diurnal pattern)",
+           "config": {"workload": "EPS calibration: 24 hourly steps, tank-level and "
+                                  "sensor-head misfit, checkpointed discrete adjoint",
+                      "B": B, "steps_per_simulation": m.T, "Nj": st["Nj"], "L": st["L"],
+                      "nnzL": st["nnzL"], "tanks": m.nt, "pumps": len(wl.net.pump_link),
+                      "controls": len(wl.net.ctl_link),
+                      "checkpoint_every": m.k, "accuracy": m.accuracy,
+                      "polish_steps": m.polish_steps,
+                      "solves_per_step": (m.solves - solves0) // args.steps,
+                      "arena_bytes_per_scenario": st["arena_bytes"],
+                      "peak_device_MiB": round(peak / 2 ** 20, 1),
+                      "peak_above_inputs_MiB": round((peak - base) / 2 ** 20, 1)},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                        "traffic": None, "kernel": "k_solve_arena", "kernel_ms": round(fw_ms, 4),
+                        "algorithmic_bytes": int(fbytes), "adjoint_kernel_ms": round(bw_ms, 4),
+                        "adjoint_achieved_GBps": round(abytes / (bw_ms * 1e-3) / 1e9, 1),
+                        "mean_newton_iters": round(iters / B, 3)}}
+    if cpu is not None:
+        out["cpu_baseline"] = cpu
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline_eps(B=1, T=2):
+    """The oracle's fixed-step EPS (sparse SuperLU route) and its discrete
+    adjoint on the same 20000-junction network: one scenario, the first T
+    steps, one thread; extrapolated per scenario-step.  Runs before this
+    process touches the GPU."""
+    from threadpoolctl import threadpool_limits
+    from oracle import eps_adjoint as ea
+    from wdsgpu_amd.generators import geometric_network
+    net = geometric_network(0)
+    dem = net.demand_cfs[None, :]
+    W = np.zeros((T, B, net.N))
+    W[:, :, net.tank_node] = 1.0
+    t0 = time.perf_counter()
+    with threadpool_limits(1):
+        fw = ea.forward(net, dem, np.nan_to_num(net.reservoir_head_ft()), net.node_ke,
+                        net.r_hw, net.tank_h0[None, :], T, accuracy=1e-6, polish_steps=3)
+        ea.adjoint(net, fw, W)
+    t = time.perf_counter() - t0
+    return {"value": round(B * T / t, 4), "unit": "scenario-steps/s", "cores": 1, "kind": "port",
+            "sample": "%d scenario x %d steps of the 20000-junction EPS (forward + polish + "
+                      "discrete adjoint, sparse route), one thread" % (B, T),
+            "seconds": round(t, 2), "host_cpus": os.cpu_count()}
+
+
+def _ragged_chunk(job):
+    """Forward + polish + adjoint of scenarios [a, b) of ragged network i,
+    one thread; returns (i, b - a, seconds)."""
+    i, a, b = job
+    from threadpoolctl import threadpool_limits
+    from oracle import adjoint as adj
+    from oracle import gga
+    from wdsgpu_amd.generators import scenario_batch, sensor_nodes
+    net, B = _CPU[i]
+    m = gga.GGAModel(net)
+    d, rh = scenario_batch(net, B, seed=100 + i)
+    ke = np.broadcast_to(net.node_ke, (B, net.N)).copy()
+    W = np.zeros((b - a, net.N))
+    W[:, sensor_nodes(net, min(30, max(1, net.Nj // 10)), 1000 + i)] = 1.0
+    t0 = time.perf_counter()
+    with threadpool_limits(1):
+        sol = adj.solve_polished(m, d[a:b], rh[a:b], ke[a:b], m.r_hw, accuracy=1e-6,
+                                 polish_steps=3, linsolve="sparse")
+        adj.cpu_adjoint(m, sol, W, np.zeros((b - a, net.L)), np.zeros((b - a, net.N)))
+    return i, b - a, time.perf_counter() - t0
+
+
+def cpu_baseline_ragged(batch, per_net=2, chunk=32):
+    """The oracle (sparse SuperLU route) on the ragged batch: (i) one thread
+    on the first per_net scenarios of every network, extrapolated to the
+    batch by each network's scenario count; (ii) the whole batch in chunks of
+    at most ``chunk`` scenarios over the host's cores (at most 16), one
+    single-threaded worker process each.  Runs before this process touches
+    the GPU."""
+    import multiprocessing as mp
+    global _CPU
+    _CPU = batch
+    t_single = 0.0
+    for i, (net, B) in enumerate(batch):
+        _, k, t = _ragged_chunk((i, 0, per_net))
+        t_single += t / k * B
+    n = sum(B for _, B in batch)
+    cores = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+                       or (os.cpu_count() or 1)))
+    jobs = [(i, a, min(B, a + chunk)) for i, (_, B) in enumerate(batch)
+            for a in range(0, B, chunk)]
+    jobs.sort(key=lambda j: -batch[j[0]][0].Nj * (j[2] - j[1]))
+    t0 = time.perf_counter()
+    with mp.get_context("fork").Pool(cores) as pool:
+        done = sum(r[1] for r in pool.imap_unordered(_ragged_chunk, jobs))
+    t2 = time.perf_counter() - t0
+    return {"value": round(done / t2, 3), "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": "all %d solves of the ragged batch (forward + polish + adjoint, sparse "
+                      "route) in chunks of <= %d over %d single-threaded worker processes"
+                      % (done, chunk, cores),
+            "seconds": round(t2, 2),
+            "single_thread": {"value": round(n / t_single, 3), "unit": "solves/s",
+                              "cores": 1,
+                              "sample": "first %d scenarios of each of the %d networks, "
+                                        "extrapolated by scenario count" % (per_net, len(batch))},
             "host_cpus": os.cpu_count()}
 
 

@@ -162,8 +162,10 @@ typedef struct wds_solve_args {
   double accuracy;
   int32_t status_machine;      /* 1: batched status machine (dgga/solver.py:3126-3179) */
   int32_t n_refine;            /* refinement steps per linear solve (reference: 2) */
-  int32_t polish_steps;        /* >0: reduced Newton polish after convergence
-                                  (dgga/autodiff.py:1053-1104) */
+  int32_t polish_steps;        /* >=0: that many reduced Newton polish steps after
+                                  convergence, then the terminal factorisation
+                                  (kept in factor when non-NULL) and the residual
+                                  (dgga/autodiff.py:1053-1104); <0: none */
   /* outputs */
   double* head;                /* [B][N] */
   double* flow;                /* [B][L] */

@@ -209,15 +209,17 @@ def polish(m, d, rh, ke, r_hw, q, e_j, Hj, status, steps=3):
 
 
 def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_steps=3,
-                   linsolve="dense"):
+                   linsolve="dense", q0=None, e0=None, status0=None):
     """Dense GGA with the status machine, then per-scenario polish with the
-    converged status frozen (dgga/autodiff.py:539-603)."""
+    converged status frozen (dgga/autodiff.py:539-603).  q0, e0, status0:
+    an optional warm start (flows [B,L], emitter flows [B,N], internal
+    statuses [B,L]), as gga.solve takes it."""
     d = np.atleast_2d(d)
     B = d.shape[0]
     rh = np.broadcast_to(np.atleast_2d(rh), (B, m.N))
     ke = np.broadcast_to(np.atleast_2d(ke), (B, m.N))
     out = gga.solve(m, d, rh, ke=ke, r_hw=r_hw, accuracy=accuracy, max_iter=max_iter,
-                    status_machine=True, linsolve=linsolve)
+                    status_machine=True, linsolve=linsolve, q0=q0, e0=e0, status0=status0)
     q_all = out["flow_cfs"].copy()
     e_all = out["emitter_cfs"][:, m.junc_nodes].copy()
     H_all = out["head_ft"].copy()

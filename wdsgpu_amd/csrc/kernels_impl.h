@@ -1028,7 +1028,7 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
   }
   if (st.active && st.info == 0) st.iters = maxtrials + 1;
   double resid = 0.0;
-  if (st.info == 0 && sp.polish_steps >= 0 && sp.factor) {
+  if (st.info == 0 && sp.polish_steps >= 0) {
     WDS_T0();
     const int pinfo = polish_and_factor(pl, c, sp, b, resid);
     WDS_TACC(c, 6);

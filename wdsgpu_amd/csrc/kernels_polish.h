@@ -597,8 +597,10 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
   polish_coeffs(pl, c);
   polish_assemble(pl, c);
   if (k3_factor(pl, c)) return -99;
-  double* fo = sp.factor + (int64_t)b * pl.nnzL;
-  for (int t = lane; t < pl.nnzL; t += 64) fo[t] = c.V[t];
+  if (sp.factor) {
+    double* fo = sp.factor + (int64_t)b * pl.nnzL;
+    for (int t = lane; t < pl.nnzL; t += 64) fo[t] = c.V[t];
+  }
   resid = polish_residuals(pl, c, false);
   WDS_TACC(c, 109);
   return 0;

@@ -420,6 +420,225 @@ def planar_network(seed=1, n_junc=782, n_res=2, n_pipes=905, n_prv=3,
     return _mk_net(ntype, elev, demand, links, meta, ke=ke)
 
 
+SIZES_MM = np.array([100.0, 150.0, 200.0, 250.0, 300.0, 400.0, 500.0, 600.0, 800.0, 1000.0,
+                     1200.0, 1500.0])
+
+
+def _hw_loss_m(q_m3s, len_m, d_m, c):
+    """Hazen-Williams head loss in m (SI form 10.67 L Q^1.852 / C^1.852 /
+    D^4.871); only used to place tanks and pump suctions."""
+    return 10.67 * len_m * np.abs(q_m3s) ** 1.852 / (c ** 1.852 * d_m ** 4.871)
+
+
+def geometric_network(seed=0, n_junc=20000, n_res=4, n_tanks=6, n_pumps=8, n_links=None,
+                      k=3, n_leaks=10, hours=24):
+    """Seeded extended-period network on a random geometric graph.
+
+    * junctions: uniform points in a square of side 120 m * sqrt(n_junc),
+      elevation U[0,50] m, base demand U[0.05,0.5] L/s following one seeded
+      diurnal pattern (24 hourly factors in [0.6, 1.4]);
+    * pipes: the Euclidean minimum spanning tree of the points (connected),
+      plus chords drawn from the k nearest-neighbour pairs until the network
+      has ``n_links`` links (default 1.2 n_junc); lengths are the point
+      distances, C U[90,140];
+    * n_res reservoirs (head U[90,110] m) on the square's boundary, each
+      feeding its nearest junction through a 100 m main;
+    * diameters: a multi-source breadth-first tree from the reservoir mains
+      carries each junction's demand to its source; a tree pipe gets the
+      smallest standard size (100-1500 mm) with velocity <= 1 m/s at its
+      subtree demand, chords 100-200 mm;
+    * n_pumps pumps lift from low reservoirs into random junctions through a
+      3-point curve (0, h0), (qd, 0.75 h0), (2 qd, 0), shutoff h0 U[40,80] m,
+      design flow qd U[20,200] L/s; the suction head is the junction's head
+      estimate (tree path losses from its source) minus 0.75 h0;
+    * n_tanks cylindrical tanks (diameter U[10,30] m, initial level U[2,6] m,
+      levels 0.5..10 m) joined to random junctions by a 500 m, 150 mm pipe,
+      the bottom placed at the junction's head estimate minus the initial
+      level;
+    * controls: pump i is opened when tank i mod n_tanks falls 0.5 m below
+      its initial level (LOWLEVEL) and closed when it rises 0.5 m above it
+      (HILEVEL);
+    * n_leaks junctions get emitter C U[0,0.5] L/s/m^0.5.
+    Hydraulic and pattern steps 1 h, duration ``hours``."""
+    from scipy.sparse import coo_matrix
+    from scipy.sparse.csgraph import minimum_spanning_tree
+    from scipy.spatial import Delaunay, cKDTree
+
+    from .network import ST_CLOSED, ST_OPEN
+    g = np.random.default_rng(seed)
+    nj = int(n_junc)
+    n_links = int(round(1.2 * nj)) if n_links is None else int(n_links)
+    side = 120.0 * np.sqrt(nj)
+    pts = g.uniform(0.0, side, (nj, 2))
+    # Euclidean MST: a subgraph of the Delaunay triangulation
+    tri = Delaunay(pts)
+    e = np.concatenate([tri.simplices[:, [0, 1]], tri.simplices[:, [1, 2]],
+                        tri.simplices[:, [0, 2]]])
+    e = np.unique(np.sort(e, axis=1), axis=0)
+    w = np.hypot(*(pts[e[:, 0]] - pts[e[:, 1]]).T)
+    mst = minimum_spanning_tree(coo_matrix((w, (e[:, 0], e[:, 1])), shape=(nj, nj))).tocoo()
+    tree_e = np.sort(np.stack([mst.row, mst.col], 1), axis=1)
+    # chords from the k-nearest-neighbour pairs
+    _, nb = cKDTree(pts).query(pts, k + 1)
+    knn = np.sort(np.stack([np.repeat(np.arange(nj), k), nb[:, 1:].ravel()], 1), axis=1)
+    knn = np.unique(knn, axis=0)
+    tkey = set(map(tuple, tree_e.tolist()))
+    cand = np.array([p for p in knn.tolist() if tuple(p) not in tkey], dtype=np.int64)
+    n_fixed_links = n_res + n_pumps + n_tanks
+    n_ch = max(0, min(len(cand), n_links - n_fixed_links - len(tree_e)))
+    chords = cand[np.sort(g.choice(len(cand), size=n_ch, replace=False))]
+    pipes = np.concatenate([tree_e, chords])
+    is_tree = np.arange(len(pipes)) < len(tree_e)
+    adj = [[] for _ in range(nj)]
+    for i, (a, b) in enumerate(pipes.tolist()):
+        adj[a].append((b, i))
+        adj[b].append((a, i))
+    # sources on the boundary
+    roots = []
+    for r in range(n_res):
+        t = g.uniform(0.1, 0.9) * side
+        bp = [(t, 0.0), (side, t), (t, side), (0.0, t)][r % 4]
+        d2 = ((pts - np.asarray(bp)) ** 2).sum(1)
+        roots.append(int(next(j for j in np.argsort(d2) if int(j) not in roots)))
+    res_head = g.uniform(90.0, 110.0, n_res)
+    elev = g.uniform(0.0, 50.0, nj)
+    dem = g.uniform(0.05, 0.5, nj)
+    C = g.uniform(90.0, 140.0, len(pipes))
+    # multi-source BFS over the pipe graph
+    parent = np.full(nj, -2, dtype=np.int64)
+    plink = np.full(nj, -1, dtype=np.int64)
+    src = np.full(nj, -1, dtype=np.int64)
+    order = []
+    frontier = []
+    for r, j in enumerate(roots):
+        parent[j], src[j] = -1, r
+        frontier.append(j)
+    while frontier:
+        order.extend(frontier)
+        nxt = []
+        for u in frontier:
+            for v, i in adj[u]:
+                if parent[v] == -2:
+                    parent[v], plink[v], src[v] = u, i, src[u]
+                    nxt.append(v)
+        frontier = nxt
+    assert (parent != -2).all(), "the spanning tree reaches every junction"
+    sub = dem.copy()
+    for v in reversed(order):
+        if parent[v] >= 0:
+            sub[parent[v]] += sub[v]
+    # sizes: velocity <= 1 m/s at the subtree demand (BFS-tree pipes)
+    diam = g.choice(SIZES_MM[:3], size=len(pipes))
+    qsub = np.zeros(len(pipes))
+    for v in range(nj):
+        if plink[v] >= 0:
+            qsub[plink[v]] = sub[v] / 1000.0
+    in_bfs = qsub > 0.0
+    need = np.sqrt(4.0 * qsub / np.pi) * 1000.0
+    idx = np.minimum(np.searchsorted(SIZES_MM, need), len(SIZES_MM) - 1)
+    diam = np.where(in_bfs, np.maximum(SIZES_MM[idx], diam), diam)
+    length = np.hypot(*(pts[pipes[:, 0]] - pts[pipes[:, 1]]).T)
+    length = np.maximum(length, 1.0)
+    # head estimate along the BFS tree
+    main_q = np.array([sub[j] for j in roots]) / 1000.0
+    main_d = SIZES_MM[np.minimum(np.searchsorted(SIZES_MM, np.sqrt(4.0 * main_q / np.pi)
+                                                 * 1000.0), len(SIZES_MM) - 1)]
+    h_est = np.zeros(nj)
+    for r, j in enumerate(roots):
+        h_est[j] = res_head[r] - _hw_loss_m(main_q[r], 100.0, main_d[r] / 1000.0, 130.0)
+    for v in order:
+        p = parent[v]
+        if p >= 0:
+            i = plink[v]
+            h_est[v] = h_est[p] - _hw_loss_m(sub[v] / 1000.0, length[i], diam[i] / 1000.0, C[i])
+    # node and link lists
+    ntype = [JUNCTION] * nj
+    elev_l = list(elev)
+    dem_l = list(dem)
+    links = []
+    for i, (a, b) in enumerate(pipes.tolist()):
+        if g.random() < 0.5:
+            a, b = b, a
+        links.append(dict(n1=a, n2=b, type=PIPE, len_m=float(length[i]),
+                          diam_mm=float(diam[i]), C=float(C[i])))
+
+    def add_node(t, el):
+        ntype.append(t)
+        elev_l.append(el)
+        dem_l.append(0.0)
+        return len(ntype) - 1
+
+    for r, j in enumerate(roots):
+        n = add_node(RESERVOIR, float(res_head[r]))
+        links.append(dict(n1=n, n2=j, type=PIPE, len_m=100.0, diam_mm=float(main_d[r]),
+                          C=130.0))
+    interior = np.setdiff1d(np.arange(nj), roots)
+    pick = g.choice(interior, size=n_pumps + n_tanks, replace=False)
+    pumps = []
+    for i, j in enumerate(pick[:n_pumps]):
+        h0 = g.uniform(40.0, 80.0)
+        qd = g.uniform(20.0, 200.0)
+        n = add_node(RESERVOIR, float(h_est[j] - 0.75 * h0))
+        pumps.append((len(links), pump_power_curve(h0, 0.75 * h0, qd, 0.0, 2.0 * qd)))
+        links.append(dict(n1=n, n2=int(j), type=PUMP, diam_mm=0.0))
+    tanks, level0, tdiam, tmax = [], [], [], []
+    for i, j in enumerate(pick[n_pumps:]):
+        lv = g.uniform(2.0, 6.0)
+        n = add_node(TANK, float(h_est[j] - lv))
+        tanks.append(n)
+        level0.append(lv)
+        tdiam.append(g.uniform(10.0, 30.0))
+        links.append(dict(n1=n, n2=int(j), type=PIPE, len_m=500.0, diam_mm=150.0,
+                          C=float(g.uniform(90.0, 140.0))))
+    N = len(ntype)
+    leaks = g.choice(nj, size=min(n_leaks, nj), replace=False)
+    c_leak = np.zeros(N)
+    c_leak[leaks] = g.uniform(0.0, 0.5, leaks.size)
+    ke = emitter_ke(c_leak, "LPS", 2.0, M_PER_FT)
+    meta = dict(flow_units="LPS", headloss="H-W", accuracy=1e-3, trials=200, seed=int(seed),
+                workload="geometric%d" % nj, leak_nodes=np.sort(leaks).tolist(),
+                emitter_c=c_leak, duration_sec=3600 * int(hours), hyd_step_sec=3600,
+                pat_step_sec=3600, report_step_sec=3600, pat_start_sec=0, tstart_sec=0)
+    net = _mk_net(ntype, elev_l, dem_l, links, meta, ke=ke)
+    for k_, c in pumps:
+        net.valve_setting_user[k_] = 1.0
+    if pumps:
+        net.pump_link = np.array([k_ for k_, _ in pumps], dtype=np.int32)
+        net.pump_ptype = np.full(len(pumps), PUMP_POWER_FUNC, dtype=np.int8)
+        for key in ("h0", "r", "n", "q0", "qmax", "hmax"):
+            setattr(net, "pump_" + key, np.array([c[key] for _, c in pumps]))
+        net.pump_curve_ptr = np.zeros(len(pumps) + 1, dtype=np.int32)
+    if tanks:
+        el_t = np.asarray(elev_l)[tanks]
+        net.tank_node = np.asarray(tanks, dtype=np.int32)
+        net.tank_h0 = (el_t + np.asarray(level0)) / M_PER_FT
+        net.tank_hmin = (el_t + 0.5) / M_PER_FT
+        net.tank_hmax = (el_t + 10.0) / M_PER_FT
+        net.tank_area = np.pi * (np.asarray(tdiam) / M_PER_FT) ** 2 / 4.0
+        net.tank_overflow = np.zeros(len(tanks), dtype=np.int8)
+    base = 1.0 + 0.4 * np.sin(2.0 * np.pi * (np.arange(24) - 6.0) / 24.0)
+    net.patterns = [np.clip(base + g.uniform(-0.05, 0.05, 24), 0.6, 1.4)]
+    net.dem_pat = np.where(np.asarray(ntype) == JUNCTION, 0, -1).astype(np.int32)
+    if pumps and tanks:
+        cl, cn, ct, cg, cs = [], [], [], [], []
+        for i, (k_, _) in enumerate(pumps):
+            j = i % len(tanks)
+            el_j = float(np.asarray(elev_l)[tanks[j]])
+            for typ, dl, st in ((0, -0.5, ST_OPEN), (1, 0.5, ST_CLOSED)):
+                cl.append(k_)
+                cn.append(tanks[j])
+                ct.append(typ)
+                cg.append((el_j + level0[j] + dl) / M_PER_FT)
+                cs.append(st)
+        net.ctl_link = np.asarray(cl, dtype=np.int32)
+        net.ctl_node = np.asarray(cn, dtype=np.int32)
+        net.ctl_type = np.asarray(ct, dtype=np.int8)
+        net.ctl_grade = np.asarray(cg)
+        net.ctl_time = np.zeros(len(cl), dtype=np.int64)
+        net.ctl_status = np.asarray(cs, dtype=np.int8)
+    return net
+
+
 def ragged_member(seed):
     """One member of the ragged multi-network batch: a seeded planar network
     (planar_network) of log-uniform size Nj in [50, 5000], Hazen-Williams or

@@ -34,6 +34,9 @@ class MultiSolver:
         self._lib = _lib.load()
         self._h = None
         self.timings = {}
+        # a list to record (name, start event, end event) of every library
+        # call on the launching stream, or None
+        self.event_log = None
         self._build(lds_budget)
 
     def __del__(self):
@@ -65,12 +68,14 @@ class MultiSolver:
         return a.value, b.value, c.value
 
     def forward(self, inputs, accuracy=1e-6, max_iter=200, polish_steps=3,
-                status_machine=True, want_factor=True):
+                status_machine=True, want_factor=True, per_network=False):
         """inputs: per network (demand [B,N], res_head [N] or [B,N], ke [N] or
         [B,N] or None, r_hw [L] or [B,L] or None).  Returns per network the
         output dict of the solve (head, flow, emitter, iters, info, relerr,
         status, factor, ...) and the kept inputs, for ``adjoint``; the
-        status words are checked by ``check_forward``."""
+        status words are checked by ``check_forward``.  per_network=True
+        makes the same work one wds_solve call per network instead (the
+        comparison point for the single launch)."""
         if len(inputs) != len(self.solvers):
             raise ValueError("one input tuple per network")
         args, outs, keeps = [], [], []
@@ -85,11 +90,31 @@ class MultiSolver:
             args.append(a)
             outs.append(out)
             keeps.append(keep)
+        if per_network:
+            with torch.cuda.device(self.device):
+                for s, a in zip(self.solvers, args):
+                    self._timed("wds_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
+                                                                      C.byref(a)))
+            return outs, keeps
         arr = (_lib.SolveArgs * len(args))(*args)
         with torch.cuda.device(self.device):
-            _lib.check(self._lib.wds_multi_solve(self._h, self.solvers[0]._stream(), arr),
-                       "wds_multi_solve")
+            self._timed("wds_multi_solve",
+                        lambda: self._lib.wds_multi_solve(self._h, self.solvers[0]._stream(),
+                                                          arr))
         return outs, keeps
+
+    def _timed(self, name, fn):
+        if self.event_log is None:
+            _lib.check(fn(), name)
+            return
+        st = torch.cuda.current_stream(self.device)
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(st)
+        rc = fn()
+        e1.record(st)
+        _lib.check(rc, name)
+        self.event_log.append((name, e0, e1))
 
     def check_forward(self, outs, max_iter=200):
         """One host copy of every network's status words; RuntimeError
@@ -107,10 +132,11 @@ class MultiSolver:
                                                            info[bad[:8]].tolist(),
                                                            iters[bad[:8]].tolist()))
 
-    def adjoint(self, outs, keeps, cotangents, n_refine=2):
+    def adjoint(self, outs, keeps, cotangents, n_refine=2, per_network=False):
         """Implicit adjoint of every network: cotangents per network (gH
         [B,N], gQ [B,L], gE [B,N], each may be None).  Returns per network
-        (g_demand, g_res_head, g_ke, g_r) [B,*] and the info tensors."""
+        (g_demand, g_res_head, g_ke, g_r) [B,*] and the info tensors.
+        per_network=True: one wds_adjoint call per network instead."""
         args, grads, infos = [], [], []
         for i, (s, o, k, (gH, gQ, gE)) in enumerate(zip(self.solvers, outs, keeps, cotangents)):
             B = self.B[i]
@@ -134,10 +160,17 @@ class MultiSolver:
             args.append(a)
             grads.append((gd, grh, gke, gr, gHc, gQc, gEc))
             infos.append(info)
+        if per_network:
+            with torch.cuda.device(self.device):
+                for s, a in zip(self.solvers, args):
+                    self._timed("wds_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
+                                                                          C.byref(a)))
+            return [g[:4] for g in grads], infos
         arr = (_lib.AdjointArgs * len(args))(*args)
         with torch.cuda.device(self.device):
-            _lib.check(self._lib.wds_multi_adjoint(self._h, self.solvers[0]._stream(), arr),
-                       "wds_multi_adjoint")
+            self._timed("wds_multi_adjoint",
+                        lambda: self._lib.wds_multi_adjoint(self._h,
+                                                            self.solvers[0]._stream(), arr))
         return [g[:4] for g in grads], infos
 
     def autotune(self, inputs, cotangents=None, budgets=None, reps=2, **kw):

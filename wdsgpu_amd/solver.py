@@ -476,7 +476,7 @@ class GGASolver:
 
     def _args(self, demand_cfs, res_head_ft, ke_int, q0, e0, max_iter, accuracy,
               status_machine, r_hw=None, polish_steps=-1, want_factor=False, fused=True,
-              status0=None, ws=None):
+              status0=None, ws=None, status0_trusted=False):
         d = self._prep(demand_cfs)
         rh = self._prep(res_head_ft)
         single = d.dim() == 1
@@ -539,7 +539,9 @@ class GGASolver:
                 s0t = s0t.unsqueeze(0)
             if s0t.shape[-1] != self.L or s0t.shape[0] not in (1, B):
                 raise ValueError("link_status must be [L] or [B,L] internal status codes")
-            if bool(((s0t < 0) | (s0t > 7)).any()):
+            # statuses produced by a previous solve (status0_trusted) skip the
+            # range check and its host synchronisation
+            if not status0_trusted and bool(((s0t < 0) | (s0t > 7)).any()):
                 raise ValueError("link_status holds internal status codes 0..7")
             s0t = s0t.expand(B, -1).contiguous()
         if ws is None:
