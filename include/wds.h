@@ -228,9 +228,13 @@ int wds_newton_iter(wds_plan* plan, void* stream, const wds_solve_args* args,
                     int32_t it, int32_t* ctrl);
 
 /* Backward of the implicit solve (replaces ImplicitGGASolveGPU.backward,
- * dgga/autodiff.py:1245-1312): reuses the forward's terminal factor.  gH/gE
- * [B][N], gQ [B][L] (each may be NULL: a zero cotangent); outputs
- * g_demand/g_res_head/g_ke [B][N], g_r [B][L]. */
+ * dgga/autodiff.py:1245-1312, and the theta contraction of the CPU adjoint
+ * ImplicitGGASolve.backward, autodiff.py:658-732): reuses the forward's
+ * terminal factor.  gH/gE [B][N], gQ [B][L] (each may be NULL: a zero
+ * cotangent); outputs g_demand/g_res_head/g_ke [B][N], g_r [B][L] and, when
+ * g_pump is not NULL, g_pump [B][3][L]: d loss / d (speed, pump_h0, pump_r)
+ * of every pump link (pumpcoeff derivatives of _pump_coeffs_np,
+ * autodiff.py:122-218), 0 at other links. */
 typedef struct wds_adjoint_args {
   int32_t B;
   const double* demand;        /* [B][N] */
@@ -253,9 +257,22 @@ typedef struct wds_adjoint_args {
   double* g_r;                 /* [B][L] */
   int32_t* info;               /* [B] */
   void* workspace;
+  double* g_pump;              /* [B][3][L] or NULL */
 } wds_adjoint_args;
 
 int wds_adjoint(wds_plan* plan, void* stream, const wds_adjoint_args* args);
+
+/* Replace the pump parameters of the plan's pump links, batch-shared, as the
+ * reference's solve_polished(speed=, pump_h0=, pump_r=) does for a call
+ * (dgga/autodiff.py:539-603 with _override_pump): each array is [L] indexed
+ * by link and read at the pump links only, NULL keeps the parameter.
+ * speed: relative speed (LinkSetting; 0 closes the pump); h0, r: curve
+ * coefficients (internal units); hmax: the pumpstatus cut-off head
+ * speed^2 Hmax; q0: the initial flow speed Q0.  Stream-ordered: work queued
+ * on `stream` after this call sees the new values, work queued before it
+ * the old ones. */
+int wds_plan_set_pumps(wds_plan* plan, void* stream, const double* speed, const double* h0,
+                       const double* r, const double* hmax, const double* q0);
 
 /* Ragged batches: several networks, each with its own plan, scenario count
  * and arrays, solved in one persistent launch per kernel mode.  No

@@ -131,6 +131,78 @@ def pump_coeffs(m, q, hl, hg, dr):
     return hl, hg, dr
 
 
+def set_pump_theta(m, speed=None, h0=None, r=None):
+    """Replace the pump parameters of model m in place ([L] arrays read at
+    the pump links; None keeps one), as the reference's per-call speed /
+    pump_h0 / pump_r do (dgga/autodiff.py:539-603, _override_pump): the
+    speed also sets the pumpstatus cut-off speed^2 Hmax, the zero-speed
+    closure and the initial flow speed Q0."""
+    pl = np.asarray(m.pump_links, dtype=np.int64)
+    if speed is not None:
+        sp = np.asarray(speed, dtype=np.float64)
+        m.pump_speed = m.pump_speed.copy()
+        m.pump_speed[pl] = sp[pl]
+        m.init_setting = m.init_setting.copy()
+        m.init_setting[pl] = sp[pl]
+        m.sm_zero_speed = m.is_pump & (m.pump_speed == 0.0)
+        m.closed_dense = m.closed | m.sm_zero_speed
+        m.sm_pump_hmax = np.where(m.pump_chp, m.sm_pump_hmax,
+                                  m.pump_speed * m.pump_speed * m.pl_hmax)
+    if h0 is not None:
+        m.pl_h0 = m.pl_h0.copy()
+        m.pl_h0[pl] = np.asarray(h0, dtype=np.float64)[pl]
+    if r is not None:
+        m.pl_r = m.pl_r.copy()
+        m.pl_r[pl] = np.asarray(r, dtype=np.float64)[pl]
+
+
+def pump_dtheta(m, q, status):
+    """d hloss / d (speed, pump_h0, pump_r) per link, [L] each, at flows q
+    and statuses status (_pump_coeffs_np, dgga/autodiff.py:122-218): zero
+    for closed or zero-speed pumps, NOCURVE and the clamp branches of
+    CONST_HP; CUSTOM pumps 2 w H0 + R Q on the selected segment (speed
+    only); CONST_HP (2-n) R w^(1-n) / Q and w^(2-n) / Q; power function
+    2 w H0 + (2-n) R w^(1-n) |Q|^(n-1) Q, w^2 and w^(2-n) |Q|^(n-1) Q (RQtol
+    clamp: 2 w H0 and w^2); linear 2 w H0 + R Q, w^2 and w Q."""
+    dw = np.zeros(m.L)
+    dh0 = np.zeros(m.L)
+    drp = np.zeros(m.L)
+    for k in m.pump_links:
+        k = int(k)
+        sp_ = float(m.pump_speed[k])
+        if int(status[k]) <= ST_CLOSED or sp_ == 0.0 or m.pl_ptype[k] == 3:
+            continue
+        qk = float(q[k])
+        qa = abs(qk)
+        if m.pl_ptype[k] == 2:                        # CUSTOM
+            h0c, rc = m.curvecoeff(k, qa / sp_)
+            dw[k] = 2.0 * sp_ * (-h0c) + (-rc) * qk
+            continue
+        H0, R = float(m.pl_h0[k]), float(m.pl_r[k])
+        n = float(m.pump_n[k])
+        w2n = float(det_pow(sp_, 2.0 - n))
+        r = R * w2n
+        if m.pl_ptype[k] == 0:                        # CONST_HP
+            g = -r / qa / qa if qa > 0.0 else np.inf
+            if g > CBIG or g < m.rqtol:
+                continue
+            dw[k] = (2.0 - n) * R * float(det_pow(sp_, 1.0 - n)) / qk
+            drp[k] = w2n / qk
+        elif n != 1.0:                                # power function
+            pw = float(det_pow(qa, n - 1.0))
+            dh0[k] = sp_ * sp_
+            if n * r * pw < m.rqtol:
+                dw[k] = 2.0 * sp_ * H0
+            else:
+                dw[k] = 2.0 * sp_ * H0 + (2.0 - n) * R * float(det_pow(sp_, 1.0 - n)) * pw * qk
+                drp[k] = w2n * pw * qk
+        else:                                         # linear
+            dw[k] = 2.0 * sp_ * H0 + R * qk
+            dh0[k] = sp_ * sp_
+            drp[k] = sp_ * qk
+    return dw, dh0, drp
+
+
 def emitter_coeffs(m, e, ke):
     """dgga/autodiff.py:365-377."""
     ke_adj = np.maximum(CSMALL, ke)
@@ -235,16 +307,19 @@ def solve_polished(m, d, rh, ke, r_hw, accuracy=1e-12, max_iter=200, polish_step
                 resid_inf=resid, iters=out["iters"], d=d, rh=rh, ke=ke, r_hw=r_hw)
 
 
-def cpu_adjoint(m, sol, gH, gQ, gE):
+def cpu_adjoint(m, sol, gH, gQ, gE, pumps=False):
     """ImplicitGGASolve.backward (dgga/autodiff.py:658-732): per scenario
     splu(J).solve(v, trans='T') and the theta contraction.  Returns
-    (g_demand [B,N], g_res_head [B,N], g_ke [B,N], g_r [B,L])."""
+    (g_demand [B,N], g_res_head [B,N], g_ke [B,N], g_r [B,L]) and, with
+    pumps, (g_speed, g_pump_h0, g_pump_r) [B,L] per scenario (the reference
+    sums them over the batch, autodiff.py:713-718)."""
     B = sol["q"].shape[0]
     gH, gQ, gE = np.atleast_2d(gH), np.atleast_2d(gQ), np.atleast_2d(gE)
     gd = np.zeros((B, m.N))
     grh = np.zeros((B, m.N))
     gke = np.zeros((B, m.N))
     gr = np.zeros((B, m.L))
+    gp = np.zeros((3, B, m.L))
     L = m.L
     f1 = ~m.m1
     f2 = ~m.m2
@@ -266,4 +341,9 @@ def cpu_adjoint(m, sol, gH, gQ, gE):
         np.add.at(grh[b], m.n2[f2], -lam_l[f2])
         grh[b, m.fixed_nodes] += gH[b, m.fixed_nodes]
         gr[b] = -lam_l * dr
+        if pumps:
+            for i, dth in enumerate(pump_dtheta(m, q, sol["status"][b])):
+                gp[i, b] = -lam_l * dth
+    if pumps:
+        return gd, grh, gke, gr, gp[0], gp[1], gp[2]
     return gd, grh, gke, gr

@@ -80,7 +80,7 @@ class AdjointArgs(C.Structure):
                 ("n_refine", C.c_int32),
                 ("g_demand", C.c_void_p), ("g_res_head", C.c_void_p),
                 ("g_ke", C.c_void_p), ("g_r", C.c_void_p), ("info", C.c_void_p),
-                ("workspace", C.c_void_p)]
+                ("workspace", C.c_void_p), ("g_pump", C.c_void_p)]
 
 
 # every exported symbol of include/wds.h with its signature
@@ -99,6 +99,7 @@ SIGNATURES = {
     "wds_trisolve": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
                                C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p]),
     "wds_solve": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
+    "wds_plan_set_pumps": (C.c_int, [C.c_void_p, C.c_void_p, _dp, _dp, _dp, _dp, _dp]),
     "wds_solve_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(SolveArgs)]),
     "wds_multi_create": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.POINTER(C.c_int32),
                                    C.c_int64, C.POINTER(C.c_void_p)]),

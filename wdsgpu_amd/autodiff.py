@@ -63,6 +63,151 @@ def _check_step(binfo, fwd, out, B):
         raise RuntimeError("adjoint='gpu': the node adjoint failed for samples %s" % bad[:8])
 
 
+def _forward(ctx, demand, res_head, ke, r_hw, s, accuracy, max_iter, sm, polish_steps,
+             defer_check, theta, status0, single_dims):
+    """The implicit solve's forward: the fused HIP solve (status machine,
+    polish, terminal factor) with the pump parameters theta (or the
+    network's), its status checked now or at the backward's synchronisation.
+    single_dims: the input ranks that de-batch the outputs when all are 1."""
+    if s.dtype != torch.float64:
+        raise NotImplementedError("the implicit adjoint is float64 only")
+    if s.n_pcv:
+        shared, cascaded = _prv_topology(s)
+        if shared:
+            raise NotImplementedError(
+                "implicit adjoint: several PRVs share a downstream junction")
+        if cascaded:
+            raise NotImplementedError("implicit adjoint: cascaded PRVs")
+    d = demand.detach()
+    rh = res_head.detach()
+    ket = ke.detach()
+    rt = r_hw.detach()
+    if d.dim() == 1:
+        d = d.unsqueeze(0)
+    if rh.dim() == 1:
+        rh = rh.unsqueeze(0)
+    B = max(d.shape[0], rh.shape[0], ket.shape[0] if ket.dim() == 2 else 1,
+            rt.shape[0] if rt.dim() == 2 else 1)
+    if d.shape[0] not in (1, B):
+        raise ValueError("demand has %d scenarios, another input %d" % (d.shape[0], B))
+    d = d.expand(B, -1)
+    if s._pending:
+        s.poll()
+    s.set_pumps(theta)
+    try:
+        a, out, keep, single, B = s._args(d, rh, ket, None, None, max_iter, accuracy, sm,
+                                          r_hw=rt, polish_steps=int(polish_steps),
+                                          want_factor=True, status0=status0)
+        with torch.cuda.device(s.device):
+            _lib.check(s._timed("k_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
+                                                                   C.byref(a)),
+                                B, out["iters"]), "wds_solve")
+    finally:
+        s.set_pumps(None)
+    maxtrials = s.maxtrials(max_iter, sm)
+    ctx.fwd_check = None
+    if defer_check:
+        # a backward will follow: the solve's status goes to pinned host
+        # memory asynchronously and is checked at the backward's one
+        # synchronisation, so the loss and the backward queue behind the
+        # forward without a host round trip
+        buf = torch.empty(2 * B, dtype=torch.int32, pin_memory=True)
+        buf.copy_(out["st"], non_blocking=True)
+        ctx.fwd_check = (buf, maxtrials, float(accuracy))
+    else:
+        info, iters = s._host_status(out, B)
+        _check_forward(info, iters, maxtrials, out, accuracy)
+    # outputs that do not reach the loss arrive in backward as None and are
+    # passed to the kernel as null pointers (zero cotangents)
+    ctx.set_materialize_grads(False)
+    ctx.solver = s
+    ctx.keep = keep
+    ctx.out = out
+    ctx.theta = theta
+    ctx.dims = (demand.dim(), res_head.dim(), ke.dim(), r_hw.dim())
+    ctx.in_devs = (demand.device, res_head.device, ke.device, r_hw.device)
+    ctx.B = B
+    s._last_resid = out["resid"]
+    head, flow, emit = out["head"], out["flow"], out["emitter"]
+    if all(x == 1 for x in single_dims):
+        head, flow, emit = head[0], flow[0], emit[0]
+    return head, flow, emit
+
+
+def _backward(ctx, gH, gQ, gE, want_pump):
+    """One wds_adjoint launch on the forward's terminal factor; returns the
+    gradients (demand, res_head, ke, r_hw) fitted to the inputs' shapes and,
+    with want_pump, d loss / d (speed, pump_h0, pump_r) [L] summed over the
+    batch (batch-shared parameters, dgga/autodiff.py:713-718)."""
+    if torch.is_grad_enabled():
+        raise NotImplementedError(
+            "the implicit adjoint is first order only (create_graph=True is not supported)")
+    s = ctx.solver
+    out = ctx.out
+    B = ctx.B
+
+    def _g(g, ref):
+        if g is None:
+            return None
+        g = g.detach().to(dtype=s.dtype, device=s.device)
+        return (g.unsqueeze(0).expand_as(ref) if g.dim() == 1 else g).contiguous()
+
+    gHt = _g(gH, out["head"])
+    gQt = _g(gQ, out["flow"])
+    gEt = _g(gE, out["emitter"])
+    dkeep, rh, ke, r = ctx.keep[0], ctx.keep[1], ctx.keep[2], ctx.keep[3]
+    dev = s.device
+    gd = torch.empty(B, s.N, dtype=s.dtype, device=dev)
+    grh = torch.empty_like(gd)
+    gke = torch.empty_like(gd)
+    gr = torch.empty(B, s.L, dtype=s.dtype, device=dev)
+    gp = torch.empty(B, 3, s.L, dtype=s.dtype, device=dev) if want_pump else None
+    info = torch.empty(B, dtype=torch.int32, device=dev)
+    ws = s._workspace(B)
+    a = _lib.AdjointArgs(
+        B=B, demand=_lib.ptr(dkeep), ke=_lib.ptr(ke),
+        ke_stride=s.N if ke.dim() == 2 else 0, r_hw=_lib.ptr(r),
+        r_stride=s.L if r.dim() == 2 else 0, head=_lib.ptr(out["head"]),
+        flow=_lib.ptr(out["flow"]), emitter=_lib.ptr(out["emitter"]),
+        status=_lib.ptr(out["status"]), factor=_lib.ptr(out["factor"]),
+        gH=_lib.ptr(gHt), gQ=_lib.ptr(gQt), gE=_lib.ptr(gEt),
+        n_refine=int(s.cudss_grad_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
+        g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr), info=_lib.ptr(info), workspace=_lib.ptr(ws),
+        g_pump=_lib.ptr(gp))
+    # the adjoint's link coefficients use the forward's pump parameters
+    s.set_pumps(ctx.theta)
+    try:
+        with torch.cuda.device(dev):
+            _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
+                                                                       C.byref(a)), B),
+                       "wds_adjoint")
+    finally:
+        s.set_pumps(None)
+    binfo = torch.empty(B, dtype=torch.int32, pin_memory=True) if s.lazy_checks \
+        else s._pinned_info(B)
+    binfo.copy_(info, non_blocking=True)
+    dm, dv = ctx.dims, ctx.in_devs
+
+    def _fit(g, dim, dev_):
+        g = g.to(device=dev_)
+        return g.sum(0) if dim == 1 else g
+
+    # the batch reductions are queued behind the adjoint before the host
+    # waits for its status
+    grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
+             _fit(gr, dm[3], dv[3]))
+    pump = tuple(gp.sum(0).unbind(0)) if want_pump else (None, None, None)
+    fwd = ctx.fwd_check
+    ctx.fwd_check = None
+    if s.lazy_checks:
+        # checked at the next call into the solver or at s.check()
+        s._defer(_check_step, binfo, fwd, out, B)
+    else:
+        torch.cuda.current_stream(dev).synchronize()
+        _check_step(binfo, fwd, out, B)
+    return grads, pump
+
+
 class ImplicitGGASolveGPU(torch.autograd.Function):
     """(demand, res_head, ke, r_hw, solver, accuracy, max_iter,
     status_machine, assemble, linear_solver, polish_steps) ->
@@ -73,147 +218,73 @@ class ImplicitGGASolveGPU(torch.autograd.Function):
                 status_machine=None, assemble="dense", linear_solver="dense",
                 polish_steps=2, defer_check=False):
         s = solver
-        if s.dtype != torch.float64:
-            raise NotImplementedError("adjoint='gpu' is float64 only")
-        if s.n_pcv:
-            shared, cascaded = _prv_topology(s)
-            if shared:
-                raise NotImplementedError(
-                    "adjoint='gpu': several PRVs share a downstream junction")
-            if cascaded:
-                raise NotImplementedError("adjoint='gpu': cascaded PRVs")
-        dims = (demand.dim(), res_head.dim(), ke.dim(), r_hw.dim())
         sm = bool(s.dense_status_machine) if status_machine is None else bool(status_machine)
         s._validate(assemble, linear_solver if linear_solver != "dense" else "hip", sm,
                     None, None)
-        d = demand.detach()
-        rh = res_head.detach()
-        ket = ke.detach()
-        rt = r_hw.detach()
-        if d.dim() == 1:
-            d = d.unsqueeze(0)
-        if rh.dim() == 1:
-            rh = rh.unsqueeze(0)
-        B = max(d.shape[0], rh.shape[0], ket.shape[0] if ket.dim() == 2 else 1,
-                rt.shape[0] if rt.dim() == 2 else 1)
-        if d.shape[0] not in (1, B):
-            raise ValueError("demand has %d scenarios, another input %d" % (d.shape[0], B))
-        d = d.expand(B, -1)
-        if s._pending:
-            s.poll()
-        a, out, keep, single, B = s._args(d, rh, ket, None, None, max_iter, accuracy, sm,
-                                          r_hw=rt, polish_steps=int(polish_steps),
-                                          want_factor=True)
-        with torch.cuda.device(s.device):
-            _lib.check(s._timed("k_solve", lambda: s._lib.wds_solve(s._plan, s._stream(),
-                                                                   C.byref(a)),
-                                B, out["iters"]), "wds_solve")
-        maxtrials = s.maxtrials(max_iter, sm)
-        ctx.fwd_check = None
-        if defer_check and any(ctx.needs_input_grad[:4]):
-            # a backward will follow (implicit_solve under grad mode): the
-            # solve's status goes to pinned host memory asynchronously and is
-            # checked at the backward's one synchronisation, so the loss and
-            # the backward queue behind the forward without a host round trip
-            buf = torch.empty(2 * B, dtype=torch.int32, pin_memory=True)
-            buf.copy_(out["st"], non_blocking=True)
-            ctx.fwd_check = (buf, maxtrials, float(accuracy))
-        else:
-            info, iters = s._host_status(out, B)
-            _check_forward(info, iters, maxtrials, out, accuracy)
-        # outputs that do not reach the loss arrive in backward as None and
-        # are passed to the kernel as null pointers (zero cotangents)
-        ctx.set_materialize_grads(False)
-        ctx.solver = s
-        ctx.keep = keep
-        ctx.out = out
-        ctx.dims = dims
-        ctx.in_devs = (demand.device, res_head.device, ke.device, r_hw.device)
-        ctx.B = B
-        s._last_resid = out["resid"]
-        head, flow, emit = out["head"], out["flow"], out["emitter"]
-        if all(x == 1 for x in dims):
-            head, flow, emit = head[0], flow[0], emit[0]
-        return head, flow, emit
+        defer = bool(defer_check) and any(ctx.needs_input_grad[:4])
+        dims = (demand.dim(), res_head.dim(), ke.dim(), r_hw.dim())
+        return _forward(ctx, demand, res_head, ke, r_hw, s, accuracy, max_iter, sm,
+                        polish_steps, defer, None, None, dims)
 
     @staticmethod
     def backward(ctx, gH, gQ, gE):
-        if torch.is_grad_enabled():
-            raise NotImplementedError(
-                "adjoint='gpu' is first order only (create_graph=True is not supported)")
-        s = ctx.solver
-        out = ctx.out
-        B = ctx.B
-
-        def _g(g, ref):
-            if g is None:
-                return None
-            g = g.detach().to(dtype=s.dtype, device=s.device)
-            return (g.unsqueeze(0).expand_as(ref) if g.dim() == 1 else g).contiguous()
-
-        gHt = _g(gH, out["head"])
-        gQt = _g(gQ, out["flow"])
-        gEt = _g(gE, out["emitter"])
-        dkeep, rh, ke, r = ctx.keep[0], ctx.keep[1], ctx.keep[2], ctx.keep[3]
-        dev = s.device
-        gd = torch.empty(B, s.N, dtype=s.dtype, device=dev)
-        grh = torch.empty_like(gd)
-        gke = torch.empty_like(gd)
-        gr = torch.empty(B, s.L, dtype=s.dtype, device=dev)
-        info = torch.empty(B, dtype=torch.int32, device=dev)
-        ws = s._workspace(B)
-        a = _lib.AdjointArgs(
-            B=B, demand=_lib.ptr(dkeep), ke=_lib.ptr(ke),
-            ke_stride=s.N if ke.dim() == 2 else 0, r_hw=_lib.ptr(r),
-            r_stride=s.L if r.dim() == 2 else 0, head=_lib.ptr(out["head"]),
-            flow=_lib.ptr(out["flow"]), emitter=_lib.ptr(out["emitter"]),
-            status=_lib.ptr(out["status"]), factor=_lib.ptr(out["factor"]),
-            gH=_lib.ptr(gHt), gQ=_lib.ptr(gQt), gE=_lib.ptr(gEt),
-            n_refine=int(s.cudss_grad_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
-            g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr), info=_lib.ptr(info), workspace=_lib.ptr(ws))
-        with torch.cuda.device(dev):
-            _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
-                                                                       C.byref(a)), B),
-                       "wds_adjoint")
-        binfo = torch.empty(B, dtype=torch.int32, pin_memory=True) if s.lazy_checks \
-            else s._pinned_info(B)
-        binfo.copy_(info, non_blocking=True)
-        dm, dv = ctx.dims, ctx.in_devs
-
-        def _fit(g, dim, dev_):
-            g = g.to(device=dev_)
-            return g.sum(0) if dim == 1 else g
-
-        # the batch reductions are queued behind the adjoint before the host
-        # waits for its status
-        grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
-                 _fit(gr, dm[3], dv[3]))
-        fwd = ctx.fwd_check
-        ctx.fwd_check = None
-        if s.lazy_checks:
-            # checked at the next call into the solver or at s.check()
-            s._defer(_check_step, binfo, fwd, out, B)
-        else:
-            torch.cuda.current_stream(dev).synchronize()
-            _check_step(binfo, fwd, out, B)
+        grads, _ = _backward(ctx, gH, gQ, gE, False)
         return grads + (None, None, None, None, None, None, None, None)
+
+
+class ImplicitGGASolve(torch.autograd.Function):
+    """The reference's CPU-adjoint route (dgga/autodiff.py:609-732, the
+    default adjoint='cpu' of implicit_solve) on the same HIP kernels:
+    (demand, res_head, ke, r_hw, solver, accuracy, max_iter, polish_steps,
+    speed [L], pump_h0 [L], pump_r [L], status [L]) -> (head, flow,
+    emitter).  speed / pump_h0 / pump_r are batch-shared pump parameters
+    (read at the pump links), differentiable when passed as tensors;
+    status (internal codes) is the solve's initial link status.  The
+    statuses at which the adjoint is taken are those the status machine
+    converges to (the reference freezes the given, or initial, statuses
+    there; the two agree when status is the converged one, the reference's
+    recommended use)."""
+
+    @staticmethod
+    def forward(ctx, demand, res_head, ke, r_hw, solver, accuracy=1e-12, max_iter=200,
+                polish_steps=3, speed=None, pump_h0=None, pump_r=None, status=None,
+                defer_check=False):
+        s = solver
+        sm = bool(s.dense_status_machine)
+        s._validate("dense", "hip", sm, status, None)
+        pumps = (speed, pump_h0, pump_r)
+        theta = None
+        if any(x is not None for x in pumps):
+            if not s.n_pumps:
+                raise ValueError("speed / pump_h0 / pump_r given for a network without pumps")
+            theta = s.pump_theta(*pumps)
+        ctx.pump_in = tuple(isinstance(x, torch.Tensor) for x in pumps)
+        ctx.pump_devs = tuple(x.device if isinstance(x, torch.Tensor) else None for x in pumps)
+        defer = bool(defer_check) and (any(ctx.needs_input_grad[:4]) or
+                                       any(ctx.needs_input_grad[8:11]))
+        dims = (demand.dim(), res_head.dim(), ke.dim())
+        return _forward(ctx, demand, res_head, ke, r_hw, s, accuracy, max_iter, sm,
+                        polish_steps, defer, theta, status, dims)
+
+    @staticmethod
+    def backward(ctx, gH, gQ, gE):
+        want = any(ctx.pump_in) and any(ctx.needs_input_grad[8:11])
+        grads, pump = _backward(ctx, gH, gQ, gE, want)
+        pg = tuple(g.to(device=d) if (g is not None and inp) else None
+                   for g, inp, d in zip(pump, ctx.pump_in, ctx.pump_devs))
+        return grads + (None, None, None, None) + pg + (None, None)
 
 
 def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
                    max_iter=200, polish_steps=3, speed=None, pump_h0=None, pump_r=None,
-                   status=None, adjoint="gpu", status_machine=None, assemble="dense",
+                   status=None, adjoint="cpu", status_machine=None, assemble="dense",
                    linear_solver="dense"):
-    """dgga/autodiff.py:1315-1351.  adjoint must be 'gpu' (the scalar
-    SuperLU reference adjoint is not part of the MI355X path); pump theta
-    arguments are refused as on the reference's GPU adjoint."""
-    if adjoint == "cpu":
-        raise NotImplementedError(
-            "adjoint='cpu' (scipy SuperLU per scenario) is the reference's CPU path; "
-            "this package runs adjoint='gpu'")
-    if adjoint != "gpu":
-        raise ValueError("adjoint must be 'cpu' or 'gpu', got %r" % (adjoint,))
-    if speed is not None or pump_h0 is not None or pump_r is not None or status is not None:
-        raise NotImplementedError("adjoint='gpu' does not take speed/pump_h0/pump_r/status")
+    """dgga/autodiff.py:1315-1351 with the reference's defaults.  Both
+    routes run the HIP forward and the HIP implicit adjoint:
+    adjoint="cpu" (ImplicitGGASolve) takes the pump parameters speed /
+    pump_h0 / pump_r ([L], differentiable) and status; adjoint="gpu"
+    (ImplicitGGASolveGPU) takes status_machine / assemble / linear_solver
+    and refuses the pump arguments, as the reference's GPU route does."""
     dt, dev = solver.dtype, solver.device
     if ke is None:
         ke = solver.node_ke_default
@@ -222,8 +293,21 @@ def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
     t = lambda x: x.to(dtype=dt, device=dev) if isinstance(x, torch.Tensor) \
         else torch.as_tensor(np.asarray(x), dtype=dt, device=dev)
     args = (t(demand), t(res_head), t(ke), t(r_hw))
+    if adjoint == "cpu":
+        tn = lambda x: None if x is None else t(x)
+        pumps = (tn(speed), tn(pump_h0), tn(pump_r))
+        defer = torch.is_grad_enabled() and any(
+            x is not None and x.requires_grad for x in args + pumps)
+        return ImplicitGGASolve.apply(*args, solver, accuracy, max_iter, int(polish_steps),
+                                      *pumps, status, defer)
+    if adjoint != "gpu":
+        raise ValueError("adjoint must be 'cpu' or 'gpu', got %r" % (adjoint,))
+    if speed is not None or pump_h0 is not None or pump_r is not None or status is not None:
+        raise NotImplementedError(
+            "adjoint='gpu' does not take speed/pump_h0/pump_r/status (the reference's GPU "
+            "route refuses them too); use adjoint='cpu' for pump-parameter gradients")
     # with a backward to come, the forward's status check waits for the
-    # backward's synchronisation (ImplicitGGASolveGPU.forward)
+    # backward's synchronisation
     defer = torch.is_grad_enabled() and any(x.requires_grad for x in args)
     return ImplicitGGASolveGPU.apply(*args, solver, accuracy, max_iter, status_machine,
                                      assemble, linear_solver, int(polish_steps), defer)

@@ -724,6 +724,73 @@ __device__ __forceinline__ void pump_PY(const DevPlan& pl, int k, int f, double 
   Y = hl / hg;
 }
 
+// d hloss / d (speed, h0, r) of open pump link k at flow q (the pumpcoeff
+// derivatives of dgga/autodiff.py:122-218), on the branches pump_PY takes:
+// NOCURVE and the CBIG / RQtol clamps of CONST_HP carry none, the RQtol
+// clamp of a power curve none in r, a CUSTOM pump only the speed (its
+// segment line is fixed by the curve points).
+__device__ __forceinline__ void pump_dtheta(const DevPlan& pl, int k, int f, double q,
+                                            double& dw, double& dh0, double& drp) {
+  dw = 0.0;
+  dh0 = 0.0;
+  drp = 0.0;
+  if (f & LF_PUMP_NOCURVE) return;
+  const double spc = fmax(pl.pu_speed[k], 1e-12);
+  if (f & LF_PUMP_CUSTOM) {
+    // the segment of pump_custom: hloss = -h0 w^2 - r w q
+    const double qq = (fabs(q) / spc) * pl.ucf_flow;
+    const int a = pl.pu_cptr[k], npts = pl.pu_cptr[k + 1] - a;
+    const double* x = pl.pu_cx + a;
+    const double* y = pl.pu_cy + a;
+    int k2 = 0;
+    while (k2 < npts && x[k2] < qq) ++k2;
+    if (k2 == 0) k2 = 1;
+    else if (k2 == npts) k2 = npts - 1;
+    const int k1 = k2 - 1;
+    double r = (y[k2] - y[k1]) / (x[k2] - x[k1]);
+    double h0 = y[k1] - r * x[k1];
+    h0 = h0 / pl.ucf_head;
+    r = r * pl.ucf_flow / pl.ucf_head;
+    dw = -2.0 * spc * h0 - r * q;
+    return;
+  }
+  const double sgn = q < 0.0 ? -1.0 : 1.0;
+  const double qa = q * sgn;
+  const double n = pl.pu_n[k];
+  const double H0 = pl.pu_h0[k], R = pl.pu_r[k];
+  const double w2n = det_pow(spc, 2.0 - n);
+  const double re = R * w2n;
+  if (f & LF_PUMP_CHP) {
+    // hloss = R w^(2-n) / q
+    const double qa_c = fmax(qa, 1e-30);
+    const double raw = -re / qa_c / qa_c;
+    if (raw > CBIG || raw < pl.rqtol) return;
+    const double qs = sgn * qa_c;
+    dw = (2.0 - n) * R * det_pow(spc, 1.0 - n) / qs;
+    drp = w2n / qs;
+  } else if (f & LF_PUMP_LIN) {
+    // hloss = w^2 H0 + R w q
+    dw = 2.0 * spc * H0 + R * q;
+    dh0 = spc * spc;
+    drp = spc * q;
+  } else {
+    // hloss = w^2 H0 + R w^(2-n) |q|^(n-1) q, or w^2 H0 + RQtol q clamped
+    const double nm1 = n - 1.0;
+    double pw;
+    if (qa == 0.0)
+      pw = nm1 > 0.0 ? 0.0 : (nm1 < 0.0 ? __builtin_huge_val() : 1.0);
+    else
+      pw = det_pow(qa, nm1);
+    dh0 = spc * spc;
+    if (n * re * pw < pl.rqtol) {
+      dw = 2.0 * spc * H0;
+      return;
+    }
+    dw = 2.0 * spc * H0 + (2.0 - n) * R * det_pow(spc, 1.0 - n) * pw * q;
+    drp = w2n * pw * q;
+  }
+}
+
 // P, Y of link k at flow q and status st (pcv PRVs -> 0, handled later)
 // f = l_flags[k], km = l_km[k] preloaded by the caller
 __device__ __forceinline__ LinkPY link_PY_f(const DevPlan& pl, int k, int f, double km,

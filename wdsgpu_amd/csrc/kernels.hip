@@ -282,6 +282,54 @@ extern "C" int wds_counters_get(wds_plan* P, wds_counters* out, int32_t reset) {
   return WDS_OK;
 }
 
+static int check_ready(wds_plan* P);
+
+extern "C" int wds_plan_set_pumps(wds_plan* P, void* stream, const double* speed,
+                                  const double* h0, const double* r, const double* hmax,
+                                  const double* q0) {
+  int rc = check_ready(P);
+  if (rc) return rc;
+  const int L = P->L;
+  for (int k = 0; k < L; ++k) {
+    int32_t& f = P->l_flags[k];
+    if (!(f & LF_PUMP)) continue;
+    if (speed) {
+      if (!(speed[k] >= 0.0) || !std::isfinite(speed[k])) {
+        set_error("wds_plan_set_pumps: pump speeds must be finite and >= 0");
+        return WDS_EINVAL;
+      }
+      P->pu_speed[k] = speed[k];
+      f &= ~(LF_ZERO_SPEED | LF_PUMP_LIVE);
+      if (speed[k] == 0.0) f |= LF_ZERO_SPEED;
+      if (speed[k] > 0.0) f |= LF_PUMP_LIVE;
+    }
+    if (h0) P->pu_h0[k] = h0[k];
+    if (r) P->pu_r[k] = r[k];
+    if (hmax) P->pu_hmax[k] = hmax[k];
+    if (q0) P->l_q0[k] = q0[k];
+  }
+  // queued behind the work already on the stream; pageable sources are
+  // staged by the runtime before the call returns
+  hipStream_t st = (hipStream_t)stream;
+  const wds::DevPlan& D = P->dev;
+  struct {
+    const void* dst;
+    const void* src;
+    size_t bytes;
+  } cp[] = {{D.l_flags, P->l_flags.data(), sizeof(int32_t) * (size_t)L},
+            {D.pu_speed, P->pu_speed.data(), sizeof(double) * (size_t)L},
+            {D.pu_h0, P->pu_h0.data(), sizeof(double) * (size_t)L},
+            {D.pu_r, P->pu_r.data(), sizeof(double) * (size_t)L},
+            {D.pu_hmax, P->pu_hmax.data(), sizeof(double) * (size_t)L},
+            {D.l_q0, P->l_q0.data(), sizeof(double) * (size_t)L}};
+  for (const auto& c : cp) {
+    hipError_t e = hipMemcpyAsync(const_cast<void*>(c.dst), c.src, c.bytes,
+                                  hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "wds_plan_set_pumps");
+  }
+  return WDS_OK;
+}
+
 static int check_ready(wds_plan* P) {
   if (!P) {
     set_error("null plan");

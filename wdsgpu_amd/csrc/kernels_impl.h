@@ -103,6 +103,7 @@ struct AdjParams {
   double* gr;
   int32_t* info;
   double* ws;
+  double* gpump;  // [B][3][L] or null
 };
 
 inline AdjParams make_adj_params(const wds_adjoint_args* a) {
@@ -127,6 +128,7 @@ inline AdjParams make_adj_params(const wds_adjoint_args* a) {
   s.gr = a->g_r;
   s.info = a->info;
   s.ws = (double*)a->workspace;
+  s.gpump = a->g_pump;
   return s;
 }
 

@@ -786,6 +786,14 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
         if (hgr >= pl.rqtol) dr = det_pow(qa, pl.hexp) * sgn;
       }
       gr[k] = -(ll * dr);
+      if (ap.gpump) {
+        double dw = 0.0, dh0 = 0.0, drp = 0.0;
+        if ((f & LF_PUMP) && !closed_now) pump_dtheta(pl, k, f, qv[u], dw, dh0, drp);
+        double* gp = ap.gpump + (int64_t)b * 3 * pl.L + k;
+        gp[0] = -(ll * dw);
+        gp[pl.L] = -(ll * dh0);
+        gp[2 * pl.L] = -(ll * drp);
+      }
     }
   }
   for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {

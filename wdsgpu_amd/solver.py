@@ -158,9 +158,21 @@ class GGASolver:
                     cx.extend(np.asarray(net.pump_curve_q[a:b], dtype=np.float64).tolist())
                     cy.extend(np.asarray(net.pump_curve_h[a:b], dtype=np.float64).tolist())
                 cptr[kk + 1] = len(cx)
+        # pump parameters per link (batch-shared; set_pumps replaces them for
+        # a call, as the reference's speed / pump_h0 / pump_r arguments do)
+        self.pump_links = np.asarray(net.pump_link, dtype=np.int64)
+        self.pl_h0, self.pl_r = ph0.copy(), pr.copy()
+        self._pl_Hmax = np.zeros(self.L)
+        self._pl_chp = np.zeros(self.L, dtype=bool)
+        self._pl_q0 = pq0.copy()
+        if self.n_pumps:
+            self._pl_Hmax[pl] = np.asarray(net.pump_hmax, dtype=np.float64)
+            self._pl_chp[pl] = kinds == PUMP_CONST_HP
+        self._pumps_now = None   # (speed, h0, r) on the device when not the defaults
         self.ucf_flow = ucf_q
         self.ucf_head = M_PER_FT if str(net.meta["flow_units"]).upper() in SI_FLOW_UNITS else 1.0
         closed = st == 0
+        self._closed0 = closed
         q0 = PI_EPANET * diam ** 2 / 4.0
         if self.n_pumps:
             q0 = np.where(lt == PUMP, init_setting * pq0, q0)
@@ -279,6 +291,45 @@ class GGASolver:
     # ------------------------------------------------------------------
     def _stream(self):
         return C.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+
+    def pump_theta(self, speed=None, pump_h0=None, pump_r=None):
+        """(speed, pump_h0, pump_r) as float64 [L] numpy arrays, defaults
+        (the network's speeds and curve coefficients) where None."""
+        f = lambda x, d: d.copy() if x is None else np.array(
+            x.detach().cpu().numpy() if isinstance(x, torch.Tensor) else x,
+            dtype=np.float64).reshape(-1)
+        th = (f(speed, self.init_setting), f(pump_h0, self.pl_h0), f(pump_r, self.pl_r))
+        for a, what in zip(th, ("speed", "pump_h0", "pump_r")):
+            if a.shape != (self.L,):
+                raise ValueError("%s must be [L] indexed by link (L=%d)" % (what, self.L))
+        if self.n_pumps and (not np.isfinite(th[0][self.pump_links]).all() or
+                             (th[0][self.pump_links] < 0.0).any()):
+            raise ValueError("pump speeds must be finite and >= 0")
+        return th
+
+    def set_pumps(self, theta=None):
+        """Queue the pump parameters theta = (speed, pump_h0, pump_r) [L]
+        (pump_theta) on the solver's stream for the calls that follow; None
+        restores the network's own.  Like the reference's per-call
+        speed / pump_h0 / pump_r (dgga/autodiff.py:539-603): the cut-off
+        head of pumpstatus is speed^2 Hmax and the initial flow speed Q0."""
+        if not self.n_pumps:
+            return
+        if theta is None and self._pumps_now is None:
+            return
+        if theta is not None and self._pumps_now is not None and all(
+                np.array_equal(a, b) for a, b in zip(theta, self._pumps_now)):
+            return
+        sp, h0, r = theta if theta is not None else self.pump_theta()
+        hmax = np.where(self._pl_chp, BIG, sp * sp * self._pl_Hmax)
+        q0 = np.where(self.lt_np == PUMP, np.where(self._closed0, QZERO, sp * self._pl_q0), 0.0)
+        keep = [np.ascontiguousarray(a, dtype=np.float64) for a in (sp, h0, r, hmax, q0)]
+        P = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_plan_set_pumps(self._plan, self._stream(),
+                                                    *[P(a) for a in keep]),
+                       "wds_plan_set_pumps")
+        self._pumps_now = None if theta is None else tuple(a.copy() for a in theta)
 
     def _workspace(self, B):
         key = (int(B), self.dtype, self.device, 0)
