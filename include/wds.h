@@ -90,6 +90,9 @@ typedef struct wds_plan_stats {
   int64_t arena_bytes;         /* arena mode: global memory per scenario holding that
                                   layout (networks whose lds_bytes exceed a CU's
                                   160 KB); 0 in LDS mode */
+  int32_t ordering;            /* the ordering the plan uses (0, 1 or 3 + d) */
+  int32_t iteration_cost;      /* scheduler cost of one factorisation and three
+                                  solve pairs (model units) */
 } wds_plan_stats;
 
 /* Counters (GGASolver.cudss_counters, dgga/solver.py:2580-2593): batched
@@ -102,9 +105,12 @@ typedef struct wds_counters {
 /* Symbolic analysis, once per topology (replaces cuDSS plan(),
  * dgga/solver.py:2474-2480, and models genmmd/etree/col_counts,
  * dgga/smatrix.py:352-547, dgga/symbolic.py:101-230).  Host only: no GPU
- * call.  ordering: 0 minimum degree, 1 natural; | 0x100 runs the kernels in
- * arena mode (the per-scenario layout in global memory) even when it would
- * fit in LDS, which a network larger than a CU's LDS gets anyway. */
+ * call.  ordering: 0 minimum degree, 1 natural, 2 automatic (the cheapest of
+ * minimum degree and independent-set rounds with degree caps 3..6 by the
+ * programs' scheduled cost, at the best LDS occupancy any of them reaches),
+ * 3 + d independent-set rounds with degree cap d (0..8); | 0x100 runs the
+ * kernels in arena mode (the per-scenario layout in global memory) even when
+ * it would fit in LDS, which a network larger than a CU's LDS gets anyway. */
 int wds_plan_create(const wds_network_desc* net, int32_t ordering, wds_plan** out);
 void wds_plan_destroy(wds_plan* plan);
 int wds_plan_stats_get(const wds_plan* plan, wds_plan_stats* out);

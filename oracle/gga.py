@@ -250,6 +250,14 @@ class GGAModel:
     # rounding of the transcendental (the reference's self-jitter acceptance
     # practice, data/sparse_gpu_plan.md:404-445).
     pow_jitter = None
+    # When set to a numpy Generator, every linear solve's heads are perturbed
+    # by -1, 0 or +1 ulp at random: the oracle's own sensitivity to one
+    # rounding of each solve.  An intermediate Newton system with
+    # kappa * eps near 1 (a junction held only through closed links, whose
+    # conductance is 1/CBIG) has no reproducible solution in float64: the
+    # final heads then move by far more than a rounding under such a
+    # perturbation, and a comparison is held to a multiple of that motion.
+    solve_jitter = None
 
     def _pow(self, x, y):
         p = det_pow(x, y)
@@ -869,6 +877,9 @@ def solve(model, demand, res_head, ke=None, r_hw=None, q0=None, e0=None,
                 dr_now = dn.max(1) / np.maximum(dn.min(1), 1e-300)
                 diag_ratio = np.where(active, dr_now, diag_ratio)
             Hj = chol_solve_refined(A, F, n_refine, extended)
+        if m.solve_jitter is not None:
+            u = m.solve_jitter.integers(-1, 2, size=Hj.shape)
+            Hj = Hj * (1.0 + u * 2.0 ** -52)
         H_junc = np.where(active[:, None], Hj, H_junc)
         H = m.full_heads(H_junc, rh_fix)
         dh = H[:, m.n1] - H[:, m.n2]

@@ -62,12 +62,19 @@ def test_multi_forward_gives_the_per_network_bits(ragged, budget):
 
 def test_every_network_matches_oracle(ragged):
     # unpolished forward of the whole batch; each network's first and last
-    # scenarios against the oracle's correctly rounded iterate
+    # scenarios against the oracle's correctly rounded iterate: identical
+    # iteration counts and statuses, heads within 1e-10 m, or within 3x the
+    # oracle's own motion under a one-ulp perturbation of each linear solve
+    # where that motion exceeds 1e-10 m / 3 (a trajectory through an
+    # intermediate system with kappa * eps near 1, whose solution float64
+    # does not determine: the reference's cross <= 3x self-jitter
+    # acceptance, data/sparse_gpu_plan.md:404-445)
     rb, solvers, inputs = ragged
     ms = MultiSolver(solvers, [b for _, b in rb], lds_budget=0)
     outs, _ = ms.forward(inputs, accuracy=1e-6, polish_steps=-1, want_factor=False)
     ms.check_forward(outs)
     worst = 0.0
+    jittery = []
     for i, ((net, B), o) in enumerate(zip(rb, outs)):
         m = gga.GGAModel(net)
         pick = [0, B - 1]
@@ -79,9 +86,22 @@ def test_every_network_matches_oracle(ragged):
         assert it.tolist() == ext["iters"].tolist(), i
         assert np.array_equal(o["status"][pick].cpu().numpy(), ext["status"]), i
         dh = np.abs(o["head"][pick].cpu().numpy() - ext["head_ft"]).max()
-        worst = max(worst, dh)
-        assert dh <= TOL_H_FT, (i, dh)
-    print("64 networks: max|dH| vs oracle %.3e ft" % worst, flush=True)
+        tol = TOL_H_FT
+        if dh > TOL_H_FT:
+            m.solve_jitter = np.random.default_rng(7)
+            jit = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-6, status_machine=True,
+                            extended=True)
+            m.solve_jitter = None
+            noise = np.abs(jit["head_ft"] - ext["head_ft"]).max()
+            tol = max(TOL_H_FT, 3.0 * noise)
+            jittery.append((i, net.Nj, dh, noise))
+        else:
+            worst = max(worst, dh)
+        assert dh <= tol, (i, dh, tol)
+    print("64 networks: max|dH| vs oracle %.3e ft on %d networks; beyond 1e-10 m "
+          "(network, Nj, |dH|, oracle 1-ulp solve jitter): %s" % (
+              worst, 64 - len(jittery), jittery), flush=True)
+    assert len(jittery) <= 2
 
 
 def test_multi_adjoint_matches_per_network_calls_and_oracle(ragged):

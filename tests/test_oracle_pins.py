@@ -343,3 +343,29 @@ def test_sparse_refined_solve_is_correctly_rounded():
     # the dense Cholesky path with plain refinement lands close by
     xd = gga.chol_solve_refined(m.csr_to_dense(csr), F)
     assert np.all(np.abs(xd - x) <= 1e-12 * np.abs(x))
+
+
+def test_solve_jitter_separates_ill_conditioned_trajectories():
+    # The oracle's one-ulp solve jitter (accuracy 1e-6, so the final heads
+    # carry the trajectory's rounding): on a network without PRVs the final
+    # heads move by about 1e-10 ft; on the ragged-batch member whose third
+    # Newton system has kappa ~ 2e15 (a PRV closed, a junction held by
+    # 1/CBIG links) by several 1e-9 ft: no float64 solver determines that
+    # trajectory to 1e-10 m, and the ragged GPU test holds it to 3x this
+    # motion instead
+    from wdsgpu_amd.generators import grid_network, ragged_batch, scenario_batch
+    cases = []
+    member, B = ragged_batch(0)[58]
+    for net, B, seed in ((grid_network(0), 2, 5), (member, B, 158)):
+        m = gga.GGAModel(net)
+        d, rh = scenario_batch(net, B, seed=seed)
+        d, rh = d[[0, B - 1]], np.nan_to_num(rh[[0, B - 1]])
+        base = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-6, extended=True)
+        m.solve_jitter = np.random.default_rng(7)
+        jit = gga.solve(m, d, rh, ke=net.node_ke, accuracy=1e-6, extended=True)
+        m.solve_jitter = None
+        cases.append(np.abs(jit["head_ft"] - base["head_ft"]).max())
+    print("1-ulp solve jitter of the final heads (ft): grid %.2e, ragged member 58 %.2e"
+          % tuple(cases))
+    assert cases[0] < 1e-9
+    assert cases[1] > 2e-9

@@ -45,7 +45,8 @@ class PlanStats(C.Structure):
                 ("fwd_steps", C.c_int32), ("bwd_steps", C.c_int32),
                 ("factor_instr", C.c_int64), ("fwd_instr", C.c_int64),
                 ("bwd_instr", C.c_int64), ("lds_bytes", C.c_int64),
-                ("arena_bytes", C.c_int64)]
+                ("arena_bytes", C.c_int64), ("ordering", C.c_int32),
+                ("iteration_cost", C.c_int32)]
 
 
 class Counters(C.Structure):

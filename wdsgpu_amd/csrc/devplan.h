@@ -41,10 +41,11 @@ enum LinkFlag : int32_t {
 WDS_HD inline int64_t lds_sink_index(int64_t nnzL, int64_t Nj, int64_t Nf) {
   return nnzL + 2 + (Nj + Nf) + 2 * Nj + 2 * kMaxPrv;
 }
-// after the sinks: the Woodbury block (valve-end paths y, z, capacitance,
-// the small dense system, its right-hand side, probes, valve residuals)
+// after the sinks: the Woodbury block sized by the plan's np = 2p valve ends
+// (valve-end paths y, z [wp_tot] each, capacitance [np^2], the small dense
+// system [np^2], its right-hand side [np], probes [np], valve residuals [p])
 WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
-  return 2 * wp_tot + np * np + 4 * kMaxPrv * kMaxPrv + 2 * kMaxPrv + 2 * kMaxPrv + kMaxPrv;
+  return 2 * wp_tot + 2 * np * np + 2 * np + np / 2;
 }
 
 // solve programs: the store word of a level's last step carries the level

@@ -159,10 +159,10 @@ struct Scn {
   double* Yp;   // LDS [wp_tot] y_u = L^-1 e_u on the valve-end paths
   double* Zp;   // LDS [wp_tot] z_u = D^-1 y_u
   double* Cm;   // LDS [np*np] e_u^T A-hat^-1 e_v between valve ends
-  double* sM;   // LDS [(2 kMaxPrv)^2] small dense (capacitance) system
-  double* sw;   // LDS [2 kMaxPrv] its right-hand side / solution
-  double* sx0;  // LDS [2 kMaxPrv] probes e_u^T A-hat^-1 r
-  double* sprv; // LDS [kMaxPrv] valve residuals (polish) / valve cotangents (adjoint)
+  double* sM;   // LDS [np^2] small dense (capacitance) system, np = 2p
+  double* sw;   // LDS [np] its right-hand side / solution
+  double* sx0;  // LDS [np] probes e_u^T A-hat^-1 r
+  double* sprv; // LDS [p] valve residuals (polish) / valve cotangents (adjoint)
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
   // per-scenario inputs / state
@@ -235,11 +235,11 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.Cm = sm;
   sm += pl.np * pl.np;
   c.sM = sm;
-  sm += 4 * kMaxPrv * kMaxPrv;
+  sm += pl.np * pl.np;
   c.sw = sm;
-  sm += 2 * kMaxPrv;
+  sm += pl.np;
   c.sx0 = sm;
-  sm += 2 * kMaxPrv;
+  sm += pl.np;
   c.sprv = sm;
   c.st0 = reinterpret_cast<int8_t*>(w);
   w += ((int64_t)L + 7) / 8;

@@ -518,6 +518,30 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
                  ? (st.lds_bytes / 8 + 31) / 32 * 32
                  : 0;
   st.arena_bytes = 8 * P->arena;
+  st.ordering = ordering & 0xff;
+  st.iteration_cost = (int32_t)std::min<int64_t>(iteration_cost(S), INT32_MAX);
+}
+
+// Scenarios of the plan one CU holds at a time in LDS mode (one wavefront per
+// SIMD at most); arena mode is not LDS-limited.
+static int lds_occupancy(const wds_plan* P) {
+  if (P->arena > 0) return 4;
+  return (int)std::min<int64_t>(4, kMaxLdsBytes / std::max<int64_t>(1, P->stats.lds_bytes));
+}
+
+// ordering 2: build the plan under each candidate ordering and keep the one
+// with the best LDS occupancy, then the lowest scheduled iteration cost
+// (minimum degree on ties)
+static void build_auto(const wds_network_desc* d, int ordering, wds_plan* P) {
+  const int flags = ordering & ~0xff;
+  build(d, 0 | flags, P);
+  for (int cap = 3; cap <= 6; ++cap) {
+    wds_plan C;
+    build(d, (kOrderRounds + cap) | flags, &C);
+    const int oc = lds_occupancy(&C), op = lds_occupancy(P);
+    if (oc > op || (oc == op && C.stats.iteration_cost < P->stats.iteration_cost))
+      *P = std::move(C);
+  }
 }
 
 }  // namespace wds
@@ -529,7 +553,10 @@ extern "C" int wds_plan_create(const wds_network_desc* net, int32_t ordering, wd
   }
   wds_plan* P = new wds_plan();
   try {
-    wds::build(net, ordering, P);
+    if ((ordering & 0xff) == 2)
+      wds::build_auto(net, ordering, P);
+    else
+      wds::build(net, ordering, P);
   } catch (const std::domain_error& e) {
     wds::set_error(e.what());
     delete P;

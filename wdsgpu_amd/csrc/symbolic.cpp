@@ -62,6 +62,65 @@ static void minimum_degree(int n, const std::vector<std::vector<int32_t>>& adj,
   }
 }
 
+// Depth-limited elimination: rounds of independent sets.  Every round
+// eliminates a maximal independent set (in the current elimination graph,
+// fill included) of the vertices whose degree is at most max(dmin, dcap),
+// taken in ascending (degree, index) order.  No two vertices of one round
+// are adjacent, so none is an elimination-tree descendant of another and the
+// tree's height is at most the number of rounds; the price is fill from
+// eliminating vertices of degree up to dcap before the minimum-degree
+// vertices alone would.  On tree-like networks (long chains of degree-2
+// vertices) a round halves every chain, as cyclic reduction does, where
+// minimum degree eats chains from their ends one level at a time.
+static void independent_rounds(int n, const std::vector<std::vector<int32_t>>& adj, int dcap,
+                               std::vector<int32_t>& order) {
+  std::vector<std::set<int32_t>> g(n);
+  for (int v = 0; v < n; ++v)
+    for (int32_t u : adj[v])
+      if (u != v) {
+        g[v].insert(u);
+        g[u].insert(v);
+      }
+  std::vector<char> alive(n, 1), blocked(n, 0);
+  order.clear();
+  order.reserve(n);
+  std::vector<std::pair<int32_t, int32_t>> cand;
+  std::vector<int32_t> sel, nb;
+  int left = n;
+  while (left > 0) {
+    int dmin = n;
+    for (int v = 0; v < n; ++v)
+      if (alive[v]) dmin = std::min(dmin, (int)g[v].size());
+    const int thr = std::max(dmin, dcap);
+    cand.clear();
+    for (int v = 0; v < n; ++v)
+      if (alive[v] && (int)g[v].size() <= thr) cand.push_back({(int32_t)g[v].size(), v});
+    std::sort(cand.begin(), cand.end());
+    sel.clear();
+    for (const auto& c : cand) {
+      const int v = c.second;
+      if (blocked[v]) continue;
+      sel.push_back(v);
+      blocked[v] = 1;
+      for (int32_t u : g[v]) blocked[u] = 1;
+    }
+    for (const auto& c : cand) blocked[c.second] = 0;
+    for (int32_t v : sel)
+      for (int32_t u : g[v]) blocked[u] = 0;
+    for (int32_t v : sel) {
+      order.push_back(v);
+      alive[v] = 0;
+      --left;
+      nb.assign(g[v].begin(), g[v].end());
+      for (int32_t u : nb) g[u].erase(v);
+      for (size_t x = 0; x < nb.size(); ++x)
+        for (size_t y = 0; y < nb.size(); ++y)
+          if (x != y) g[nb[x]].insert(nb[y]);
+      g[v].clear();
+    }
+  }
+}
+
 // Schedule the tasks of one level onto the 64 lanes and append its steps.
 //
 // A level of S steps gives every task the smallest power-of-two group of g
@@ -201,6 +260,7 @@ static void emit_level(const std::vector<Task>& tasks, Program& prog, int32_t ze
     if (cur.lw_max == 0) break;  // no groups: a longer level only costs more
   }
   if (best_cost < 0) throw std::runtime_error("symbolic: a level does not fit the wavefront");
+  prog.cost += best_cost;
   for (size_t st = 0; st < best.steps; ++st)
     for (int l = 0; l < kLanes; ++l) prog.instr.push_back(best.lane[l][st]);
   const int32_t prev = prog.level_end.empty() ? 0 : prog.level_end.back();
@@ -214,6 +274,8 @@ void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
   if (n <= 0) throw std::runtime_error("empty matrix");
   if (ordering == 0) {
     minimum_degree(n, adj, s.perm);
+  } else if (ordering >= 3 && ordering <= 3 + 8) {
+    independent_rounds(n, adj, ordering - 3, s.perm);
   } else {
     s.perm.resize(n);
     for (int i = 0; i < n; ++i) s.perm[i] = i;

@@ -49,6 +49,7 @@ struct Program {
   std::vector<Instr> instr;            // step-major, nsteps * 64
   std::vector<int32_t> level_end;      // cumulative steps at each level end
   int64_t useful = 0;                  // non-padding instructions
+  int64_t cost = 0;                    // scheduler cost model summed over the levels
   int32_t nsteps() const { return level_end.empty() ? 0 : level_end.back(); }
 };
 
@@ -67,8 +68,17 @@ struct Symbolic {
 };
 
 // adjacency: for each vertex (original index) its neighbours (no self loops,
-// duplicates allowed).  ordering: 0 = minimum degree, 1 = natural.
+// duplicates allowed).  ordering: 0 = minimum degree, 1 = natural, 3 + d =
+// independent-set rounds with degree cap d (0 <= d <= 8).
+constexpr int kOrderRounds = 3;
 void analyse(int n, const std::vector<std::vector<int32_t>>& adj, int ordering,
              Symbolic& s);
+
+// weighted scheduler cost of one Newton iteration's programs: one
+// factorisation and three forward/backward solve pairs (a solve and two
+// refinement steps)
+inline int64_t iteration_cost(const Symbolic& s) {
+  return s.factor.cost + 3 * (s.fwd.cost + s.bwd.cost);
+}
 
 }  // namespace wds

@@ -43,8 +43,11 @@ def _f64(a):
 class GGASolver:
     """Batched GGA forward solver (dgga/solver.py:140-495 interface).
 
-    Beyond the reference's arguments: ``ordering`` ("mmd" minimum degree or
-    "natural") and ``arena`` (run the scenario kernels in arena mode, with
+    Beyond the reference's arguments: ``ordering`` ("auto": the cheaper by
+    scheduled program cost of minimum degree and depth-limited
+    independent-set rounds; "mmd" minimum degree; "rounds3".."rounds6"
+    independent-set rounds with that degree cap; "natural") and ``arena``
+    (run the scenario kernels in arena mode, with
     the per-scenario factor and vectors in global memory, even for a network
     that fits a CU's LDS; used to test that mode)."""
 
@@ -54,7 +57,7 @@ class GGASolver:
 
     def __init__(self, net, device="cuda", dtype=torch.float64, mode="dense",
                  inp_path=None, dense_tank_bound_check=True,
-                 dense_status_machine=False, ordering="mmd", arena=False):
+                 dense_status_machine=False, ordering="auto", arena=False):
         if mode != "dense":
             raise NotImplementedError(
                 "mode=%r: the MI355X path is the batched dense-mode solver "
@@ -216,7 +219,11 @@ class GGASolver:
         # arena=True runs the kernels with each scenario's factor and vectors in
         # a global-memory arena even when they would fit in LDS (networks
         # larger than a CU's LDS run that way regardless)
-        order = {"mmd": 0, "natural": 1}[ordering] | (0x100 if arena else 0)
+        codes = {"mmd": 0, "natural": 1, "auto": 2}
+        codes.update({"rounds%d" % d: 3 + d for d in range(9)})
+        if ordering not in codes:
+            raise ValueError("ordering must be one of %s" % sorted(codes))
+        order = codes[ordering] | (0x100 if arena else 0)
         _lib.check(self._lib.wds_plan_create(C.byref(desc), order, C.byref(h)),
                    "wds_plan_create")
         self._plan = h
