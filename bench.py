@@ -198,18 +198,35 @@ def stage_rooflines(s, B, reps=20):
     fac = s.factor(csr)
     s.trisolve(fac, csr, F)
     torch.cuda.synchronize()
+    # the kernels alone: preallocated outputs and workspace, HIP events on
+    # the launching stream right around each C-ABI call (the Python methods
+    # also allocate and read the pivot status back)
+    from wdsgpu_amd import _lib
+    lib, plan = s._lib, s._plan
+    info = torch.zeros(B, dtype=torch.int32, device=dev)
+    H = torch.empty_like(F)
+    ws = s._workspace(B)
+    strm = torch.cuda.current_stream(dev)
+    sp = s._stream()
+    calls = (
+        ("k_factor", lambda: lib.wds_factor(plan, sp, B, _lib.ptr(csr), _lib.ptr(fac),
+                                            _lib.ptr(info), _lib.ptr(ws)),
+         8 * (st["nnz_lower"] + st["nnzL"])),
+        ("k_trisolve", lambda: lib.wds_trisolve(plan, sp, B, _lib.ptr(fac), _lib.ptr(csr),
+                                                _lib.ptr(F), _lib.ptr(H), 2, _lib.ptr(ws)),
+         8 * (2 * st["nnzL"] + 2 * Nj) * 3 + 8 * (st["nnz_lower"] + 3 * Nj) * 2))
     out = {}
-    for name, fn, nbytes in (
-            ("k_factor", lambda: s.factor(csr),
-             8 * (st["nnz_lower"] + st["nnzL"])),
-            ("k_trisolve", lambda: s.trisolve(fac, csr, F),
-             8 * (2 * st["nnzL"] + 2 * Nj) * 3 + 8 * (st["nnz_lower"] + 3 * Nj) * 2)):
-        s.event_log = []
+    for name, fn, nbytes in calls:
+        evs = []
         for _ in range(reps):
-            s._timed(name, fn, B)
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record(strm)
+            _lib.check(fn(), name)
+            e1.record(strm)
+            evs.append((e0, e1))
         torch.cuda.synchronize()
-        ms = float(np.mean([e0.elapsed_time(e1) for (_, e0, e1, _, _) in s.event_log]))
-        s.event_log = None
+        ms = float(np.mean([e0.elapsed_time(e1) for e0, e1 in evs]))
         gbs = nbytes * B / (ms * 1e-3) / 1e9
         out[name] = {"kernel_ms": round(ms, 4), "algorithmic_bytes": int(nbytes * B),
                      "achieved_GBps": round(gbs, 1), "frac": round(gbs / HBM_PEAK_GBS, 4)}
