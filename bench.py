@@ -41,12 +41,14 @@ def main():
     ap.add_argument("--stages", action="store_true",
                     help="also time the standalone factor (k_factor) and triangular-solve "
                          "(k_trisolve) kernels at the same B against their algorithmic bytes")
-    ap.add_argument("--workload", choices=("calibration", "ragged", "eps"),
+    ap.add_argument("--workload", choices=("calibration", "ragged", "eps", "leak"),
                     default="calibration",
                     help="calibration: B scenarios of the 782-junction network (the headline "
                          "metric); ragged: the 64-network ragged batch in one launch per "
                          "kernel mode; eps: the differentiable 24-step EPS of the "
-                         "20000-junction network at B=256 with checkpointed adjoint")
+                         "20000-junction network at B=256 with checkpointed adjoint; leak: "
+                         "the Adam leak-inversion training loop (60 candidate junctions, "
+                         "3 true leaks, 33 sensors) on B scenarios of the 782-junction network")
     ap.add_argument("--checkpoint-every", type=int, default=4)
     ap.add_argument("--copy-calibrate", action="store_true",
                     help="also run one plain 512 MiB read + 512 MiB write elementwise "
@@ -63,6 +65,8 @@ def main():
         return bench_ragged(args)
     if args.workload == "eps":
         return bench_eps(args)
+    if args.workload == "leak":
+        return bench_leak(args)
     # the CPU baseline runs first: its worker processes are forked before
     # this process touches the GPU
     cpu = None if args.no_cpu_baseline else cpu_baseline(args.cpu_sample, args.B)
@@ -344,6 +348,61 @@ def cpu_baseline(n_single, B):
 
 
 RAGGED_METRIC = "fwd+adjoint solves/s, ragged 64-network batch in one launch per kernel mode"
+
+
+def bench_leak(args):
+    """The Adam leak-inversion training loop (the reference's training-loop
+    benchmark, data/mainline_evidence.md §5.4): B scenarios of the
+    782-junction planar network, emitter coefficients of 60 candidate
+    junctions as softplus parameters, 3 true leaks of C = 1.5, 33 sensors,
+    loss = head MSE + 1e-4 sum C, Adam(lr = 0.1).  A step is one forward +
+    implicit adjoint + Adam update; value = B / seconds per step.  The warmup
+    steps and the timed steps are consecutive steps of one optimisation."""
+    if not torch.cuda.is_available():
+        print("bench.py needs a HIP device", file=sys.stderr)
+        sys.exit(2)
+    from wdsgpu_amd.workloads import LeakInversion
+    dev = torch.device("cuda:0")
+    B = args.B
+    log("building the leak-inversion workload B=%d" % B)
+    wl = LeakInversion(B=B, device=dev)
+    s = wl.solver
+    losses = []
+    for _ in range(args.warmup):
+        losses.append(wl.step())
+    torch.cuda.synchronize()
+    s.event_log = []
+    log("timing %d Adam steps" % args.steps)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        losses.append(wl.step())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ev = s.event_log
+    s.event_log = None
+    fw = [a.elapsed_time(b) for (n, a, b, _, _) in ev if n == "k_solve"]
+    bw = [a.elapsed_time(b) for (n, a, b, _, _) in ev if n == "k_adjoint"]
+    ms = (t1 - t0) * 1e3 / args.steps
+    L = [float(x) for x in losses]
+    rank = wl.ranking()
+    out = {"metric": "Adam leak-inversion training steps (forward + implicit adjoint + "
+                     "update), scenarios/s",
+           "value": round(B * args.steps / (t1 - t0), 1), "unit": "scenarios/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms, 3),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+           "data": "synthetic (seeded planar network, lognormal demand scenarios, seeded "
+                   "true leaks)",
+           "config": {"workload": "leak inversion: Adam(lr=0.1) on softplus emitter C of 60 "
+                                  "candidate junctions, 3 true leaks (C=1.5), 33 sensors, "
+                                  "loss = head MSE + 1e-4 sum C",
+                      "B": B, "Nj": s.Nj, "L": s.L, "accuracy": wl.accuracy,
+                      "polish_steps": wl.polish_steps},
+           "kernel_ms": {"k_solve": round(float(np.mean(fw)), 4),
+                         "k_adjoint": round(float(np.mean(bw)), 4)},
+           "loss": {"first": L[0], "last": L[-1], "steps_taken": len(L)},
+           "true_leaks": [int(x) for x in wl.true_leaks],
+           "top3_candidates": [int(x) for x in rank[:3]]}
+    print(json.dumps(out), flush=True)
 
 
 def bench_ragged(args):

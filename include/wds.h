@@ -268,6 +268,20 @@ typedef struct wds_adjoint_args {
 
 int wds_adjoint(wds_plan* plan, void* stream, const wds_adjoint_args* args);
 
+/* Link coefficients at a given state, as the kernels compute them
+ * (replaces _link_coeffs_np, dgga/autodiff.py:239-362, for the consumers
+ * that read the converged link gradients: the Schur-diagonal preconditioner
+ * of dgga/optim2.py:137-215 and the RQtol clamp mask of dgga/calib.py:264-300).
+ * q, status [B][L]; r_hw [B][L] (r_stride L) or [L] (r_stride 0).  P_out
+ * [B][L] = 1 / hgrad as assembled into the GGA matrix (1/CBIG for a closed
+ * link, 0 for an ACTIVE PRV, whose row is replaced); hloss [B][L] (optional)
+ * the head loss;
+ * dr [B][L] (optional) d hloss / d r of an open pipe, 0 on the RQtol clamp
+ * and at other links.  The pump parameters are the plan's current ones. */
+int wds_link_coeffs(wds_plan* plan, void* stream, int32_t B, const double* q,
+                    const double* r_hw, int32_t r_stride, const int8_t* status,
+                    double* P_out, double* hloss, double* dr);
+
 /* Replace the pump parameters of the plan's pump links, batch-shared, as the
  * reference's solve_polished(speed=, pump_h0=, pump_r=) does for a call
  * (dgga/autodiff.py:539-603 with _override_pump): each array is [L] indexed

@@ -115,6 +115,8 @@ SIGNATURES = {
     "wds_adjoint": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(AdjointArgs)]),
     "wds_counters_get": (C.c_int, [C.c_void_p, C.POINTER(Counters), C.c_int32]),
     "wds_last_error": (C.c_char_p, []),
+    "wds_link_coeffs": (C.c_int, [C.c_void_p, C.c_void_p, C.c_int32, C.c_void_p, C.c_void_p,
+                                  C.c_int32, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
     "wds_hw_resistance": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.c_void_p, C.c_void_p,
                                     C.c_void_p, C.c_void_p, C.c_void_p]),
     "wds_hw_resistance_grad": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,

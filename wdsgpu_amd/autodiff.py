@@ -313,6 +313,133 @@ def implicit_solve(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
                                      assemble, linear_solver, int(polish_steps), defer)
 
 
+class _State:
+    """What a forward keeps for its adjoints, outside autograd."""
+
+    def set_materialize_grads(self, value):
+        pass
+
+
+def forward_state(solver, demand, res_head, ke=None, r_hw=None, accuracy=1e-12,
+                  max_iter=200, polish_steps=3, speed=None, pump_h0=None, pump_r=None,
+                  status=None):
+    """The implicit solve's forward outside autograd (the forward of
+    ImplicitGGASolve, dgga/autodiff.py:539-603 + 1171-1243): the fused HIP
+    solve with the status machine, the polish and the terminal factor, its
+    status checked now.  Returns the state ``adjoint_rows`` takes."""
+    s = solver
+    dt, dev = s.dtype, s.device
+    t = lambda x: x.detach().to(dtype=dt, device=dev) if isinstance(x, torch.Tensor) \
+        else torch.as_tensor(np.asarray(x), dtype=dt, device=dev)
+    ke = s.node_ke_default if ke is None else ke
+    r_hw = s.r_hw if r_hw is None else r_hw
+    d, rh, ket, rt = t(demand), t(res_head), t(ke), t(r_hw)
+    sm = bool(s.dense_status_machine)
+    s._validate("dense", "hip", sm, status, None)
+    theta = None
+    if speed is not None or pump_h0 is not None or pump_r is not None:
+        if not s.n_pumps:
+            raise ValueError("speed / pump_h0 / pump_r given for a network without pumps")
+        theta = s.pump_theta(speed, pump_h0, pump_r)
+    st = _State()
+    with torch.no_grad():
+        _forward(st, d, rh, ket, rt, s, accuracy, max_iter, sm, polish_steps, False, theta,
+                 status, (2,))
+    return st
+
+
+def adjoint_rows(st, rows, gH=None, gQ=None, gE=None, want_pump=False):
+    """One wds_adjoint launch over the forward scenarios ``rows`` (1-D
+    indices into the forward's batch, repeats allowed) with cotangents
+    gH/gE [R,N], gQ [R,L] (None: zero), each row reusing its scenario's
+    terminal factor (no new factorisation; the reference solves one
+    factorised Jacobian for many right-hand sides the same way,
+    dgga/sensitivity.py:59-160).  Returns per-row (g_demand [R,N],
+    g_res_head [R,N], g_ke [R,N], g_r [R,L]) and, with want_pump,
+    g_pump [R,3,L]; the status of the launch is checked before returning."""
+    s = st.solver
+    dev = s.device
+    rows = torch.as_tensor(rows, dtype=torch.int64, device=dev).reshape(-1)
+    R = int(rows.numel())
+    if R < 1 or int(rows.min()) < 0 or int(rows.max()) >= st.B:
+        raise ValueError("rows must index the forward's %d scenarios" % st.B)
+    out, keep = st.out, st.keep
+    pick = lambda x: x.index_select(0, rows).contiguous()
+    dk, ke, r = keep[0], keep[2], keep[3]
+    ke = pick(ke) if ke.dim() == 2 and ke.shape[0] == st.B and st.B > 1 else ke
+    r = pick(r) if r.dim() == 2 and r.shape[0] == st.B and st.B > 1 else r
+    dk = pick(dk.expand(st.B, -1))
+    head, flow, emit = pick(out["head"]), pick(out["flow"]), pick(out["emitter"])
+    status, factor = pick(out["status"]), pick(out["factor"])
+
+    def _g(g, n):
+        if g is None:
+            return None
+        g = g.detach().to(dtype=s.dtype, device=dev).contiguous()
+        if tuple(g.shape) != (R, n):
+            raise ValueError("cotangent must be [%d,%d]" % (R, n))
+        return g
+
+    gHt, gQt, gEt = _g(gH, s.N), _g(gQ, s.L), _g(gE, s.N)
+    gd = torch.empty(R, s.N, dtype=s.dtype, device=dev)
+    grh = torch.empty_like(gd)
+    gke = torch.empty_like(gd)
+    gr = torch.empty(R, s.L, dtype=s.dtype, device=dev)
+    gp = torch.empty(R, 3, s.L, dtype=s.dtype, device=dev) if want_pump else None
+    info = torch.empty(R, dtype=torch.int32, device=dev)
+    ws = s._workspace(R)
+    a = _lib.AdjointArgs(
+        B=R, demand=_lib.ptr(dk), ke=_lib.ptr(ke), ke_stride=s.N if ke.dim() == 2 else 0,
+        r_hw=_lib.ptr(r), r_stride=s.L if r.dim() == 2 else 0, head=_lib.ptr(head),
+        flow=_lib.ptr(flow), emitter=_lib.ptr(emit), status=_lib.ptr(status),
+        factor=_lib.ptr(factor), gH=_lib.ptr(gHt), gQ=_lib.ptr(gQt), gE=_lib.ptr(gEt),
+        n_refine=int(s.cudss_grad_refine), g_demand=_lib.ptr(gd), g_res_head=_lib.ptr(grh),
+        g_ke=_lib.ptr(gke), g_r=_lib.ptr(gr), info=_lib.ptr(info), workspace=_lib.ptr(ws),
+        g_pump=_lib.ptr(gp))
+    s.set_pumps(st.theta)
+    try:
+        with torch.cuda.device(dev):
+            _lib.check(s._timed("k_adjoint", lambda: s._lib.wds_adjoint(s._plan, s._stream(),
+                                                                       C.byref(a)), R),
+                       "wds_adjoint")
+    finally:
+        s.set_pumps(None)
+    bad = torch.nonzero(info != 0).flatten().tolist()
+    if bad:
+        raise RuntimeError("adjoint: the node adjoint failed for rows %s" % bad[:8])
+    res = (gd, grh, gke, gr)
+    return res + (gp,) if want_pump else res
+
+
+def solve_polished(solver, d, rh, ke=None, r_hw=None, accuracy=1e-12, max_iter=200,
+                   polish_steps=3, speed=None, status=None, pump_h0=None, pump_r=None,
+                   assemble="dense", linear_solver="dense"):
+    """dgga/autodiff.py:539-603: converged and polished states as numpy
+    (q [B,L], e_j [B,Nj], head [B,N], emitter [B,N], d, rh, ke [B,N], r_hw
+    [L], resid_inf [B], speed / status / h0p / rp [L] as resolved, iters
+    [B]), computed by the HIP forward (status machine, reduced Newton
+    polish).  ``status`` holds the links' initial internal statuses."""
+    s = solver
+    s._validate(assemble, linear_solver if linear_solver != "dense" else "hip",
+                bool(s.dense_status_machine), status, None)
+    st = forward_state(s, d, rh, ke, r_hw, accuracy, max_iter, polish_steps, speed, pump_h0,
+                       pump_r, status)
+    out, keep = st.out, st.keep
+    B = st.B
+    np_ = lambda x: x.detach().cpu().numpy()
+    head = np_(out["head"])
+    em = np_(out["emitter"])
+    kk = np.broadcast_to(np.atleast_2d(np_(keep[2])), (B, s.N)).copy()
+    th = s.pump_theta(*(st.theta if st.theta is not None else (None, None, None)))
+    return dict(q=np_(out["flow"]), e_j=em[:, s.junc_nodes].copy(), head=head, emitter=em,
+                d=np.broadcast_to(np.atleast_2d(np_(keep[0])), (B, s.N)).copy(),
+                rh=np.broadcast_to(np.atleast_2d(np_(keep[1])), (B, s.N)).copy(), ke=kk,
+                r_hw=np_(keep[3]).reshape(-1, s.L)[0].copy() if keep[3].dim() == 2
+                and keep[3].shape[0] == 1 else np_(keep[3]),
+                resid_inf=np_(out["resid"]), speed=th[0], status=np_(out["status"]),
+                h0p=th[1], rp=th[2], iters=np_(out["iters"]).astype(np.int64))
+
+
 def _stream_of(t):
     return C.c_void_p(torch.cuda.current_stream(t.device).cuda_stream)
 

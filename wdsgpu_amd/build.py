@@ -33,7 +33,7 @@ def build(verbose=False, arch="gfx950"):
     os.makedirs(objdir, exist_ok=True)
     common = ["-O3", "-fPIC", "-std=c++17", "-Wall", "-Wno-unused-function",
               "-I", os.path.join(HERE, "..", "include")]
-    objs = []
+    objs, cmds = [], []
     for src in SOURCES:
         obj = os.path.join(objdir, src + ".o")
         cmd = [hipcc()] + common + ["-c", os.path.join(CSRC, src), "-o", obj]
@@ -55,8 +55,13 @@ def build(verbose=False, arch="gfx950"):
             cmd[cmd.index("-O3")] = "-O2"
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # the sources compile independently: run them side by side
+    procs = [subprocess.Popen(c) for c in cmds]
+    bad = [c[c.index("-c") + 1] for c, p in zip(cmds, procs) if p.wait() != 0]
+    if bad:
+        raise subprocess.CalledProcessError(1, "hipcc", "failed: %s" % ", ".join(bad))
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "-shared", "-fPIC", "--offload-arch=" + arch, "-o", tmp] + objs
     if verbose:

@@ -865,6 +865,74 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
 }
 
 // ---------------------------------------------------------------------------
+// Link coefficients at a given state (dgga/autodiff.py:239-362
+// _link_coeffs_np): one lane per (scenario, link), the device functions the
+// Newton loop and the adjoint use.  P = 1 / hgrad as assembled into the GGA
+// matrix (1/CBIG closed, 0 for an ACTIVE PRV whose row is replaced),
+// hloss = Y / P where P > 0 (0 otherwise), dr = d hloss / d r of an open pipe
+// (0 on the RQtol clamp and at every other link).
+namespace {
+__global__ void k_link_coeffs(DevPlan pl, int32_t B, const double* q, const double* r,
+                              int32_t r_stride, const int8_t* status, double* P_out,
+                              double* hl_out, double* dr_out) {
+  const int64_t n = (int64_t)B * pl.L;
+  for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n;
+       i += (int64_t)gridDim.x * blockDim.x) {
+    const int b = (int)(i / pl.L);
+    const int k = (int)(i - (int64_t)b * pl.L);
+    const int f = pl.l_flags[k];
+    const double qk = q[i];
+    const double rk = r[(int64_t)b * r_stride + k];
+    const int8_t st = status[i];
+    LinkPY py = link_PY(pl, k, qk, rk, st);
+    if (f & LF_PCV) {
+      // a PRV with a setting, as the PRV stage assembles it: 0 when ACTIVE
+      // (its row is replaced), closed, or the open valve's coefficients
+      if (st == ST_ACTIVE) {
+        py = LinkPY{0.0, 0.0};
+      } else if (st <= ST_CLOSED) {
+        py = LinkPY{1.0 / CBIG, qk};
+      } else {
+        valve_open(pl, qk, pl.l_km_valve[k], py.P, py.Y);
+      }
+    }
+    P_out[i] = py.P;
+    if (hl_out) hl_out[i] = py.P > 0.0 ? py.Y / py.P : 0.0;
+    if (!dr_out) continue;
+    double dr = 0.0;
+    const bool closed_now = (st <= ST_CLOSED || (f & LF_ZERO_SPEED)) && !(f & LF_PCV);
+    if ((f & LF_PIPE) && !closed_now && pl.headloss != 0) {
+      dr = dw_dr(pl, qk, pl.l_diam[k], pl.l_kc[k]);
+    } else if ((f & LF_PIPE) && !closed_now) {
+      const double sgn = qk < 0.0 ? -1.0 : 1.0;
+      const double qa = qk * sgn;
+      const double hgr = pl.hexp * rk * det_pow(fmax(qa, 1e-30), pl.hexp - 1.0);
+      if (hgr >= pl.rqtol) dr = det_pow(qa, pl.hexp) * sgn;
+    }
+    dr_out[i] = dr;
+  }
+}
+}  // namespace
+
+extern "C" int wds_link_coeffs(wds_plan* P, void* stream, int32_t B, const double* q,
+                               const double* r_hw, int32_t r_stride, const int8_t* status,
+                               double* P_out, double* hloss, double* dr) {
+  int rc = check_ready(P);
+  if (rc) return rc;
+  if (B <= 0 || !q || !r_hw || !status || !P_out || (r_stride != 0 && r_stride != P->L)) {
+    set_error("wds_link_coeffs: bad argument");
+    return WDS_EINVAL;
+  }
+  const int64_t n = (int64_t)B * P->L;
+  const int64_t g = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_link_coeffs, dim3((unsigned)std::max<int64_t>(g, 1)), dim3(256), 0,
+                     (hipStream_t)stream, P->dev, B, q, r_hw, r_stride, status, P_out, hloss,
+                     dr);
+  P->cnt.launches += 1;
+  return launch_check("k_link_coeffs");
+}
+
+// ---------------------------------------------------------------------------
 // Parameter maps (dgga/calib.py:64-108, dgga/parse.py:557-568): elementwise,
 // one lane per entry, grid-stride over n.
 namespace {

@@ -464,6 +464,42 @@ class GGASolver:
             raise RuntimeError("batched LDL^T failed: samples %s are not positive definite" % bad)
         return (H, fac) if want_factor else H
 
+    def link_coeffs(self, q, r_hw=None, status=None):
+        """The kernels' link coefficients at flows q [B,L] (or [L]) with
+        resistances r_hw ([L] or [B,L]; None: the solver's) and internal
+        statuses status ([L] or [B,L]; None: the initial ones), as
+        _link_coeffs_np returns them (dgga/autodiff.py:239-362): (P, hloss,
+        dr) [B,L] tensors with P = 1/hgrad as assembled (1/CBIG closed, 0
+        for an ACTIVE PRV) and dr = d hloss / d r of open pipes."""
+        qt = self._prep(q)
+        if qt.dim() == 1:
+            qt = qt.unsqueeze(0)
+        B = int(qt.shape[0])
+        if qt.dim() != 2 or qt.shape[1] != self.L:
+            raise ValueError("q must be [L] or [B,L] with L=%d" % self.L)
+        qt = qt.contiguous()
+        r = self.r_hw if r_hw is None else self._prep(r_hw)
+        self._check_shape(r, self.L, B, "r_hw")
+        if r.dim() == 2 and r.shape[0] == 1:
+            r = r[0]
+        r = r.contiguous()
+        st = torch.as_tensor(self.init_status_int if status is None else status,
+                             dtype=torch.int8, device=self.device)
+        if st.dim() == 1:
+            st = st.unsqueeze(0)
+        if st.shape[-1] != self.L or st.shape[0] not in (1, B):
+            raise ValueError("status must be [L] or [B,L]")
+        st = st.expand(B, -1).contiguous()
+        P = torch.empty(B, self.L, dtype=self.dtype, device=self.device)
+        hl = torch.empty_like(P)
+        dr = torch.empty_like(P)
+        with torch.cuda.device(self.device):
+            _lib.check(self._lib.wds_link_coeffs(
+                self._plan, self._stream(), B, _lib.ptr(qt), _lib.ptr(r),
+                self.L if r.dim() == 2 else 0, _lib.ptr(st), _lib.ptr(P), _lib.ptr(hl),
+                _lib.ptr(dr)), "wds_link_coeffs")
+        return P, hl, dr
+
     def factor(self, csr_data):
         """Numeric LDL^T of every scenario's CSR matrix [B,nnz] (the cuDSS
         factorize phase, dgga/solver.py:2630-2640); returns the factor

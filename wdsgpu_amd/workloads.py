@@ -209,3 +209,75 @@ class EpsCalibration:
         loss = ((H - self.H_true) ** 2).sum()
         loss.backward()
         return loss.detach(), [x.grad for x in leaves]
+
+
+class LeakInversion:
+    """The reference's training-loop workload (data/mainline_evidence.md
+    §5.4, scripts/demo_leak_inversion.py): leak localisation by Adam on the
+    emitter coefficients of candidate junctions.  B scenarios of the
+    782-junction planar network (lognormal demand factors, sigma 0.3), no
+    emitters but n_leaks true leaks of user coefficient leak_C among
+    n_candidates random junctions; observations: the heads at n_sensors
+    sensor junctions of the same batch solved with the true leaks.
+    Parameters: C = softplus(theta) at the candidates (Ke = (Ucf_Q^Qexp /
+    Ucf_P) / C^Qexp, the HIP emitter map); loss = mean squared head misfit
+    (ft^2) + reg * sum C; Adam(lr).  A step is one forward + implicit
+    adjoint of the batch and one Adam update."""
+
+    def __init__(self, B=1024, seed=1, device="cuda", n_candidates=60, n_leaks=3, leak_C=1.5,
+                 n_sensors=33, lr=0.1, reg=1e-4, accuracy=1e-6, polish_steps=3,
+                 theta0=-3.0):
+        net = planar_network(seed)
+        net.node_ke = np.zeros(net.N)
+        self.net = net
+        self.B = B
+        self.device = torch.device(device)
+        self.accuracy, self.polish_steps, self.reg = accuracy, polish_steps, reg
+        self.solver = GGASolver(net, device=device, dense_status_machine=True)
+        s = self.solver
+        dev = self.device
+        f64 = lambda a: torch.as_tensor(np.asarray(a, dtype=np.float64), device=dev)
+        rng = np.random.default_rng(seed + 11)
+        junc = np.nonzero(np.asarray(net.node_type) == 0)[0]
+        self.candidates = np.sort(rng.choice(junc, n_candidates, replace=False))
+        self.true_leaks = np.sort(rng.choice(self.candidates, n_leaks, replace=False))
+        self.cand_t = torch.as_tensor(self.candidates, device=dev)
+        self.sensors = torch.as_tensor(sensor_nodes(net, n_sensors, seed + 12), device=dev)
+        d, _ = scenario_batch(net, B, seed=seed + 13)
+        self.d0 = f64(d)
+        self.rh0 = f64(np.nan_to_num(net.reservoir_head_ft()))
+        C_true = np.zeros(net.N)
+        C_true[self.true_leaks] = leak_C
+        with torch.no_grad():
+            out = s.solve(self.d0, self.rh0, ke_int=emitter_ke(f64(C_true)), accuracy=1e-12,
+                          status_machine=True)
+            self.H_obs = out["head_ft"].index_select(1, self.sensors).clone()
+        self.theta = torch.full((n_candidates,), float(theta0), dtype=torch.float64,
+                                device=dev, requires_grad=True)
+        self.opt = torch.optim.Adam([self.theta], lr=lr)
+
+    def C(self):
+        return torch.nn.functional.softplus(self.theta)
+
+    def loss(self):
+        C = self.C()
+        ke = torch.zeros(self.net.N, dtype=torch.float64, device=self.device).index_copy(
+            0, self.cand_t, emitter_ke(C))
+        H, _, _ = implicit_solve(self.solver, self.d0, self.rh0, ke=ke, accuracy=self.accuracy,
+                                 max_iter=200, polish_steps=self.polish_steps, adjoint="gpu",
+                                 status_machine=True)
+        mis = H.index_select(1, self.sensors) - self.H_obs
+        return (mis * mis).mean() + self.reg * C.sum()
+
+    def step(self):
+        """One Adam step; returns the loss before the update (a tensor)."""
+        self.opt.zero_grad(set_to_none=True)
+        L = self.loss()
+        L.backward()
+        self.opt.step()
+        return L.detach()
+
+    def ranking(self):
+        """Candidate junctions by current C, largest first."""
+        C = self.C().detach().cpu().numpy()
+        return self.candidates[np.argsort(-C)]
