@@ -631,3 +631,173 @@ def _read_controls(net, rows, idx, lidx, hcf, pcf, path):
     net.ctl_grade = np.asarray(cg, dtype=np.float64)
     net.ctl_time = np.asarray(ctm, dtype=np.int64)
     net.ctl_status = np.asarray(cs, dtype=np.int8)
+
+
+def _g(x):
+    """Shortest text that reads back as the same double."""
+    return repr(float(x))
+
+
+def _time_field(sec):
+    return "%d SEC" % int(sec)
+
+
+def write_inp(net, path):
+    """Write a Net as an EPANET INP file that ``read_inp`` reads back to the
+    same network (the INP writer half of dgga/parse.py:112-1042; section and
+    keyword names are EPANET's).  Internal units go back to the file's flow
+    units (meta flow_units) and pressure units (meta press_units); values are
+    written with 17 significant digits.  Power-function pump curves are
+    written as 3-point curves through (0, shutoff), the design flow and twice
+    it, which the reader fits back to the same coefficients; CUSTOM curves
+    keep their points; CONST_HP pumps are written as POWER."""
+    m = net.meta
+    units = str(m.get("flow_units", "LPS")).upper()
+    si = units in SI_FLOW_UNITS
+    qcf = FLOW_UCF[units]
+    hcf = M_PER_FT if si else 1.0
+    dcf = 1000.0 * M_PER_FT if si else 12.0
+    pcf = net.ucf_pressure()
+    headloss = str(m.get("headloss", "H-W")).upper()
+    nt = np.asarray(net.node_type)
+    lt = np.asarray(net.link_type)
+    el = np.asarray(net.elev_ft, dtype=np.float64)
+    qexp = float(m.get("qexp", 2.0))
+    out = ["[TITLE]", "written by wdsgpu_amd.network.write_inp", ""]
+    press = str(m.get("press_units", "") or ("METERS" if si else "PSI")).upper()
+    out += ["[OPTIONS]", "UNITS %s" % units, "HEADLOSS %s" % headloss,
+            "PRESSURE %s" % press, "TRIALS %d" % int(m.get("trials", 200)),
+            "ACCURACY %s" % _g(m.get("accuracy", 1e-3)),
+            "EMITTER EXPONENT %s" % _g(1.0 / qexp),
+            "SPECIFIC GRAVITY %s" % _g(m.get("spgrav", 1.0)),
+            "DEMAND MULTIPLIER %s" % _g(m.get("demand_multiplier", 1.0)),
+            "CHECKFREQ %d" % int(m.get("checkfreq", 2)),
+            "MAXCHECK %d" % int(m.get("maxcheck", 10)),
+            "VISCOSITY %s" % _g(float(m.get("viscosity", VISCOS_DEFAULT)) / VISCOS_DEFAULT)]
+    extra = int(m.get("extra_iter", -1))
+    out.append("UNBALANCED STOP" if extra < 0 else "UNBALANCED CONTINUE %d" % extra)
+    # a pattern name no pattern has: junctions without one stay without
+    out += ["PATTERN __none__", ""]
+    times = [("DURATION", "duration_sec"), ("HYDRAULIC TIMESTEP", "hyd_step_sec"),
+             ("PATTERN TIMESTEP", "pat_step_sec"), ("PATTERN START", "pat_start_sec"),
+             ("REPORT TIMESTEP", "report_step_sec"), ("START CLOCKTIME", "tstart_sec")]
+    rows = ["%s %s" % (k, _time_field(m[f])) for k, f in times if f in m]
+    if rows:
+        out += ["[TIMES]"] + rows + [""]
+    pname = ["P%d" % (i + 1) for i in range(len(net.patterns))]
+    if net.patterns:
+        out.append("[PATTERNS]")
+        for name, pat in zip(pname, net.patterns):
+            out.append(name + " " + " ".join(_g(v) for v in np.asarray(pat)))
+        out.append("")
+    ids = net.node_id
+    dem_pat = np.asarray(net.dem_pat)
+    out.append("[JUNCTIONS]")
+    for i in np.nonzero(nt == JUNCTION)[0]:
+        row = "%s %s %s" % (ids[i], _g(el[i] * hcf), _g(net.demand_cfs[i] * qcf))
+        if dem_pat[i] >= 0:
+            row += " " + pname[dem_pat[i]]
+        out.append(row)
+    out += ["", "[RESERVOIRS]"]
+    for i in np.nonzero(nt == RESERVOIR)[0]:
+        out.append("%s %s" % (ids[i], _g(el[i] * hcf)))
+    out.append("")
+    if len(net.tank_node):
+        out.append("[TANKS]")
+        for j, i in enumerate(net.tank_node):
+            dd = np.sqrt(4.0 * float(net.tank_area[j]) / PI_EPANET)
+            out.append("%s %s %s %s %s %s 0 * %s" % (
+                ids[i], _g(el[i] * hcf), _g((net.tank_h0[j] - el[i]) * hcf),
+                _g((net.tank_hmin[j] - el[i]) * hcf), _g((net.tank_hmax[j] - el[i]) * hcf),
+                _g(dd * hcf), "YES" if net.tank_overflow[j] else "NO"))
+        out.append("")
+    lids = net.link_id
+    n1, n2 = np.asarray(net.link_n1), np.asarray(net.link_n2)
+    st = np.asarray(net.init_status)
+    dia = np.asarray(net.diam_ft, dtype=np.float64)
+    K = np.where(dia > 0, np.asarray(net.km_int) * dia ** 4 / 0.02517, 0.0)
+    out.append("[PIPES]")
+    for k in np.nonzero(lt <= PIPE)[0]:
+        rough = net.roughness[k] if headloss != "D-W" else net.roughness[k] * 1000.0 * hcf
+        word = "CV" if lt[k] == CVPIPE else ("CLOSED" if st[k] == 0 else "OPEN")
+        out.append("%s %s %s %s %s %s %s %s" % (lids[k], ids[n1[k]], ids[n2[k]],
+                                                 _g(net.len_ft[k] * hcf), _g(dia[k] * dcf),
+                                                 _g(rough), _g(K[k]), word))
+    out.append("")
+    curves = []
+    status_rows = []
+    if len(net.pump_link):
+        out.append("[PUMPS]")
+        ptr = np.asarray(net.pump_curve_ptr)
+        for j, k in enumerate(net.pump_link):
+            kind = int(net.pump_ptype[j])
+            speed = float(net.valve_setting_user[k])
+            if kind == PUMP_CONST_HP:
+                p = -float(net.pump_r[j]) * (0.7457 if si else 1.0) / 8.814
+                spec = "POWER %s" % _g(p)
+            elif kind == PUMP_CUSTOM:
+                cname = "C_%s" % lids[k]
+                curves += ["%s %s %s" % (cname, _g(x), _g(y)) for x, y in
+                           zip(net.pump_curve_q[ptr[j]:ptr[j + 1]],
+                               net.pump_curve_h[ptr[j]:ptr[j + 1]])]
+                spec = "HEAD %s" % cname
+            elif kind == PUMP_POWER_FUNC:
+                cname = "C_%s" % lids[k]
+                a = -float(net.pump_h0[j]) * hcf
+                n = float(net.pump_n[j])
+                b = -float(net.pump_r[j]) * hcf / qcf ** n
+                q1 = float(net.pump_q0[j]) * qcf
+                pts = [(0.0, a), (q1, a + b * q1 ** n), (2.0 * q1, a + b * (2.0 * q1) ** n)]
+                curves += ["%s %s %s" % (cname, _g(x), _g(y)) for x, y in pts]
+                spec = "HEAD %s" % cname
+            else:
+                raise NotImplementedError("write_inp: pump %s has no curve to write" % lids[k])
+            if speed != 1.0:
+                spec += " SPEED %s" % _g(speed)
+            out.append("%s %s %s %s" % (lids[k], ids[n1[k]], ids[n2[k]], spec))
+            if st[k] == 0:
+                status_rows.append("%s CLOSED" % lids[k])
+        out.append("")
+    vname = {PRV: "PRV", PSV: "PSV", PBV: "PBV", FCV: "FCV", TCV: "TCV", GPV: "GPV"}
+    vk = np.nonzero(lt >= PRV)[0]
+    if vk.size:
+        out.append("[VALVES]")
+        for k in vk:
+            out.append("%s %s %s %s %s %s %s" % (lids[k], ids[n1[k]], ids[n2[k]],
+                                                 _g(dia[k] * dcf), vname[int(lt[k])],
+                                                 _g(net.valve_setting_user[k]), _g(K[k])))
+            if st[k] == 0:
+                status_rows.append("%s CLOSED" % lids[k])
+            elif st[k] == 1:
+                status_rows.append("%s OPEN" % lids[k])
+        out.append("")
+    if status_rows:
+        out += ["[STATUS]"] + status_rows + [""]
+    if curves:
+        out += ["[CURVES]"] + curves + [""]
+    ke = np.asarray(net.node_ke, dtype=np.float64)
+    em = np.nonzero((ke > 0) & (nt == JUNCTION))[0]
+    if em.size:
+        out.append("[EMITTERS]")
+        for i in em:
+            out.append("%s %s" % (ids[i], _g(((qcf ** qexp / pcf) / ke[i]) ** (1.0 / qexp))))
+        out.append("")
+    if len(net.ctl_link):
+        out.append("[CONTROLS]")
+        for c in range(len(net.ctl_link)):
+            k = int(net.ctl_link[c])
+            word = "OPEN" if int(net.ctl_status[c]) == ST_OPEN else "CLOSED"
+            ct = int(net.ctl_type[c])
+            if ct >= 2:
+                cond = "AT %s %s" % ("TIME" if ct == 2 else "CLOCKTIME",
+                                     _time_field(net.ctl_time[c]))
+            else:
+                i = int(net.ctl_node[c])
+                f = hcf if nt[i] != JUNCTION else pcf
+                cond = "IF NODE %s %s %s" % (ids[i], "BELOW" if ct == 0 else "ABOVE",
+                                            _g((net.ctl_grade[c] - el[i]) * f))
+            out.append("LINK %s %s %s" % (lids[k], word, cond))
+        out.append("")
+    out.append("[END]")
+    with open(path, "w", encoding="utf-8") as fh:
+        fh.write("\n".join(out) + "\n")
