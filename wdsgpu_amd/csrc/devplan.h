@@ -153,6 +153,7 @@ struct DevPlan {
   const int32_t* prv_pos_off; // [p] L position of (j1,j2) lower
   const double* prv_hset;     // [p]
   const double* prv_km;       // [p]
+  int32_t prv_disjoint;       // 1: no two valves share an end node
   // elimination-tree paths of the valve ends (path a < p: n2 of valve a,
   // path p + a: n1 of valve a).  y_u = L^-1 e_u is nonzero only on the path
   // from u to its root, so unit solves with the factor cost O(height^2).

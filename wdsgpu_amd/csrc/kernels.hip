@@ -154,6 +154,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.Nf = P->Nf;
   D.nnzL = (int32_t)S.nnzL;
   D.p = P->p;
+  D.prv_disjoint = P->prv_disjoint;
   D.ntl = (int32_t)P->tl_k.size();
   D.headloss = P->headloss;
   D.hexp = P->hexp;

@@ -438,44 +438,54 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
 
 // prvcoeff for every PRV with a setting, in valve order, by one lane
 // (dgga/solver.py:850-912).  Must follow k2_assemble (reads Xflow[j2]).
+// the prvcoeff terms of valve i (A, V, F, P, Y)
+__device__ __forceinline__ void prv_apply(const DevPlan& pl, Scn& c, int i) {
+  const int k = pl.prv_k[i], j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
+  const double qk = c.q[k];
+  const int8_t sk = c.S[k];
+  const bool act = sk == ST_ACTIVE, cl = sk <= ST_CLOSED;
+  double pk_o, yk_o;
+  valve_open(pl, qk, pl.prv_km[i], pk_o, yk_o);
+  const double pk_na = cl ? 1.0 / CBIG : pk_o;
+  const double yk_na = cl ? qk : yk_o;
+  const double mna = act ? 0.0 : 1.0, mact = act ? 1.0 : 0.0;
+  const double xf2 = c.R[j2];
+  const double pna = pk_na * mna;
+  const int po = pl.prv_pos_off[i], d1 = pl.diag_pos[j1], d2 = pl.diag_pos[j2];
+  const int4 ae = pl.prv_ae[i];
+  // V holds the same values as A here (k2_assemble writes both)
+  double v;
+  v = c.V[po] + (-pna);
+  c.A[ae.x] = v;
+  c.A[ae.y] = v;
+  c.V[po] = v;
+  v = c.V[d1] + pna;
+  c.A[ae.z] = v;
+  c.V[d1] = v;
+  v = c.V[d2] + pna;
+  v = v + CBIG * mact;
+  c.A[ae.w] = v;
+  c.V[d2] = v;
+  const double dy = (yk_na - qk) * mna;
+  const double f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
+  const double f2 = -dy + pl.prv_hset[i] * CBIG * mact;
+  c.F[j1] = c.F[j1] + f1;
+  c.F[j2] = c.F[j2] + f2;
+  c.P[k] = pna;
+  c.Y[k] = act ? qk + xf2 : yk_na;
+}
+
 __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
   if (pl.p == 0) return;
-  if (lane_id() == 0) {
-    for (int i = 0; i < pl.p; ++i) {
-      const int k = pl.prv_k[i], j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
-      const double qk = c.q[k];
-      const int8_t sk = c.S[k];
-      const bool act = sk == ST_ACTIVE, cl = sk <= ST_CLOSED;
-      double pk_o, yk_o;
-      valve_open(pl, qk, pl.prv_km[i], pk_o, yk_o);
-      const double pk_na = cl ? 1.0 / CBIG : pk_o;
-      const double yk_na = cl ? qk : yk_o;
-      const double mna = act ? 0.0 : 1.0, mact = act ? 1.0 : 0.0;
-      const double xf2 = c.R[j2];
-      const double pna = pk_na * mna;
-      const int po = pl.prv_pos_off[i], d1 = pl.diag_pos[j1], d2 = pl.diag_pos[j2];
-      const int4 ae = pl.prv_ae[i];
-      // V holds the same values as A here (k2_assemble writes both)
-      double v;
-      v = c.V[po] + (-pna);
-      c.A[ae.x] = v;
-      c.A[ae.y] = v;
-      c.V[po] = v;
-      v = c.V[d1] + pna;
-      c.A[ae.z] = v;
-      c.V[d1] = v;
-      v = c.V[d2] + pna;
-      v = v + CBIG * mact;
-      c.A[ae.w] = v;
-      c.V[d2] = v;
-      const double dy = (yk_na - qk) * mna;
-      const double f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
-      const double f2 = -dy + pl.prv_hset[i] * CBIG * mact;
-      c.F[j1] = c.F[j1] + f1;
-      c.F[j2] = c.F[j2] + f2;
-      c.P[k] = pna;
-      c.Y[k] = act ? qk + xf2 : yk_na;
-    }
+  const int lane = lane_id();
+  if (pl.prv_disjoint) {
+    // no two valves share an end node: their terms touch disjoint entries of
+    // A, V and F, so every valve goes on its own lane at once
+    if (lane < pl.p) prv_apply(pl, c, lane);
+  } else if (lane == 0) {
+    // valves in valve order: shared diagonal and right-hand-side entries are
+    // summed in the reference's order
+    for (int i = 0; i < pl.p; ++i) prv_apply(pl, c, i);
   }
   mem_sync();
   lds_sync();
@@ -716,20 +726,23 @@ __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
   return wave_any(chg);
 }
 
-// diag ratio proxy max|diag| / min|diag| (dgga/solver.py:3026-3032)
+// diag ratio proxy max|diag| / min|diag| (dgga/solver.py:3026-3032) of the
+// matrix last assembled (with the PRV terms), read from its global copy A:
+// the loop reports the last iteration's value, so it is taken once after it
 __device__ inline double diag_ratio_now(const DevPlan& pl, Scn& c) {
   double mx = 0.0, mn = __builtin_huge_val();
-  const lds_f64* Vl = (const lds_f64*)c.V;
   for (int j0 = lane_id(); j0 < pl.Nj; j0 += 64 * kRowG) {
-    int dp[kRowG];
+    int da[kRowG];
 #pragma unroll
-    for (int u = 0; u < kRowG; ++u) dp[u] = pl.diag_pos[min(j0 + 64 * u, pl.Nj - 1)];
+    for (int u = 0; u < kRowG; ++u) da[u] = pl.diag_ae[min(j0 + 64 * u, pl.Nj - 1)];
+    double v[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) v[u] = c.A[da[u]];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u) {
       if (j0 + 64 * u >= pl.Nj) break;
-      const double v = fabs(Vl[dp[u]]);
-      mx = fmax(mx, v);
-      mn = fmin(mn, v);
+      mx = fmax(mx, fabs(v[u]));
+      mn = fmin(mn, fabs(v[u]));
     }
   }
   mx = wave_max(mx);
@@ -784,14 +797,14 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
 
 // one Newton iteration; returns relerr, sets bad on a non-positive pivot
 __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const SolveParams& sp,
-                                          bool& bad, double* dratio) {
+                                          bool& bad) {
   WDS_T0();
   k1_coeffs(pl, c);
   WDS_TACC(c, 0);
   k2_assemble<true>(pl, c);
+  WDS_TACC(c, 104);
   k2_prv(pl, c);
-  if (pl.p > 0 && dratio) *dratio = diag_ratio_now(pl, c);
-  WDS_TACC(c, 1);
+  WDS_TACC(c, 105);
   bad = k3_factor(pl, c);
   WDS_TACC(c, 2);
   if (bad) return 0.0;
@@ -1005,6 +1018,8 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
   init_state(pl, c, sp, b);
 #ifdef WDS_PROFILE_STAGES
   for (int i = lane; i < 112; i += 64) c.misc[400 + i] = 0.0;
+  // marks the counter block for the host that reads it back
+  if (lane == 0) c.misc[511] = 12345.0;
 #endif
   mem_sync();
   lds_sync();
@@ -1013,7 +1028,7 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
   double relerr_out = 0.0, dratio = 0.0;
   for (int it = 1; it <= maxtrials; ++it) {
     bool bad = false;
-    const double relerr = newton_iteration(pl, c, sp, bad, &dratio);
+    const double relerr = newton_iteration(pl, c, sp, bad);
     if (bad) {
       st.info = it;
       break;
@@ -1029,6 +1044,8 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
     }
   }
   if (st.active && st.info == 0) st.iters = maxtrials + 1;
+  // the last assembled matrix is still in A (the polish assembles its own)
+  if (pl.p > 0 && sp.diag_ratio) dratio = diag_ratio_now(pl, c);
   double resid = 0.0;
   if (st.info == 0 && sp.polish_steps >= 0) {
     WDS_T0();
@@ -1098,7 +1115,8 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams
   const int maxtrials = maxtrials_of(pl, sp);
   bool bad = false;
   double dratio = 0.0;
-  const double relerr = newton_iteration(pl, c, sp, bad, &dratio);
+  const double relerr = newton_iteration(pl, c, sp, bad);
+  if (pl.p > 0 && sp.diag_ratio) dratio = diag_ratio_now(pl, c);
   if (bad) {
     st.info = it;
   } else {

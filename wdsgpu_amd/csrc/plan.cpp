@@ -337,6 +337,12 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     P->prv_km.push_back(d->km_valve[k]);
   }
   P->p = (int32_t)P->prv_k.size();
+  {
+    std::vector<int32_t> ends(P->prv_s1);
+    ends.insert(ends.end(), P->prv_s2.begin(), P->prv_s2.end());
+    std::sort(ends.begin(), ends.end());
+    P->prv_disjoint = std::adjacent_find(ends.begin(), ends.end()) == ends.end() ? 1 : 0;
+  }
   if (P->p > kMaxPrv)
     throw std::domain_error("more than " + std::to_string(kMaxPrv) + " PRVs with a setting");
   // ---- slots of A in the row-block layout ----

@@ -49,6 +49,7 @@ struct wds_plan {
   // scalars
   int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
   int32_t has_cv = 0, has_pump = 0;
+  int32_t prv_disjoint = 0;  // no two PRVs with a setting share an end node
   double hexp = 1.852, qexp = 2.0, rqtol = 1e-7, viscos = 1.1e-5, htol = 5e-4, qtol = 1e-4;
   // arena mode (doubles per scenario of the global-memory copy of the LDS
   // layout; 0 in LDS mode)
