@@ -44,7 +44,8 @@ typedef struct wds_network_desc {
   const double* elev;          /* [N] ft */
   const int32_t* link_n1;      /* [L] */
   const int32_t* link_n2;      /* [L] */
-  const int32_t* link_type;    /* [L] EN_LinkType: 0 CVPIPE 1 PIPE 2 PUMP 3 PRV 7 TCV */
+  const int32_t* link_type;    /* [L] EN_LinkType: 0 CVPIPE 1 PIPE 2 PUMP 3 PRV 4 PSV 6 FCV
+                                  7 TCV (control valves need junction ends) */
   const int32_t* init_status;  /* [L] internal status code */
   const double* diam;          /* [L] ft */
   const double* km;            /* [L] pipe minor loss 0.02517 K / D^4 */

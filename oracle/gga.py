@@ -26,7 +26,7 @@ from wdsgpu_amd.network import (
     CSMALL, CBIG, BIG, TINY, MISSING, QZERO, HEXP_HW, PI_EPANET,
     DW_A1, DW_A2, DW_A8, DW_A9, DW_AB, DW_AC,
     CVPIPE, PIPE, PUMP, PRV, PSV, PBV, FCV, TCV,
-    ST_XHEAD, ST_TEMPCLOSED, ST_CLOSED, ST_OPEN, ST_ACTIVE, ST_XPRESSURE,
+    ST_XHEAD, ST_TEMPCLOSED, ST_CLOSED, ST_OPEN, ST_ACTIVE, ST_XFCV, ST_XPRESSURE,
     PUMP_CONST_HP, PUMP_POWER_FUNC, PUMP_CUSTOM, PUMP_NOCURVE, JUNCTION)
 
 
@@ -37,7 +37,7 @@ class GGAModel:
         self.net = net
         lt = np.asarray(net.link_type, dtype=np.int64)
         nt = np.asarray(net.node_type)
-        allowed = {CVPIPE, PIPE, TCV, PUMP, PRV}
+        allowed = {CVPIPE, PIPE, TCV, PUMP, PRV, PSV, FCV}
         bad = set(lt.tolist()) - allowed
         if bad:
             raise NotImplementedError(f"link types {sorted(bad)} not on the dense path")
@@ -208,8 +208,9 @@ class GGAModel:
         self.sm_zero_speed = self.is_pump & (sp == 0.0)
         self.sm_pump_live = self.is_pump & (sp > 0.0)
         self.sm_pump_hmax = np.where(self.pump_chp, BIG, sp * sp * self.pl_hmax)
-        # ---- PRVs (745-832) ----
-        prv_all = np.where(lt == PRV)[0]
+        # ---- control valves: PRVs (745-832), PSVs and FCVs (the scalar
+        # replica's valvecoeffs, 1616-1688), in valve (file) order ----
+        prv_all = np.where((lt == PRV) | (lt == PSV) | (lt == FCV))[0]
         pcv = prv_all[init_setting[prv_all] != MISSING]
         fixedv = prv_all[init_setting[prv_all] == MISSING]
         self.n_pcv = int(pcv.size)
@@ -224,11 +225,21 @@ class GGAModel:
                 raise NotImplementedError("PRV with a fixed-head end")
             self.prv_n1, self.prv_n2 = pn1, pn2
             self.prv_j1, self.prv_j2 = junc_row[pn1], junc_row[pn2]
-            self.prv_hset = self.elev[pn2] + init_setting[pcv]
+            # 0 PRV, 1 PSV, 2 FCV; hset: PRV El[n2] + setting (prvcoeff),
+            # PSV El[n1] + setting (psvcoeff), FCV the flow setting (fcvcoeff)
+            self.prv_kind = np.where(lt[pcv] == PRV, 0, np.where(lt[pcv] == PSV, 1, 2))
+            self.prv_hset = np.where(self.prv_kind == 0, self.elev[pn2] + init_setting[pcv],
+                                     np.where(self.prv_kind == 1,
+                                              self.elev[pn1] + init_setting[pcv],
+                                              init_setting[pcv]))
             self.prv_km = self.km_valve_ml[pcv].copy()
             pj1, pj2 = self.prv_j1, self.prv_j2
+            # per valve, in the order the terms are added: the off-diagonal
+            # pair, d1 (+ P, then + CBIG of an ACTIVE PSV), d2 (+ P, then
+            # + CBIG of an ACTIVE PRV)
             self.prv_A_idx = np.stack([pj1 * Nj + pj2, pj2 * Nj + pj1, pj1 * Nj + pj1,
-                                       pj2 * Nj + pj2, pj2 * Nj + pj2], axis=1).reshape(-1)
+                                       pj1 * Nj + pj1, pj2 * Nj + pj2, pj2 * Nj + pj2],
+                                      axis=1).reshape(-1)
             self.prv_A_csr = np.searchsorted(csr_pos, self.prv_A_idx)
             self.prv_F_idx = np.stack([pj1, pj2], axis=1).reshape(-1)
 
@@ -525,12 +536,19 @@ class GGAModel:
         return A, F, Xflow, P, Y, em, hloss_e, hgrad_e
 
     def prvcoeffs(self, P, Y, F, Xflow, q, S, A, csr):
-        """dgga/solver.py:850-912 (valvecoeffs -> prvcoeff, batched)."""
+        """Control-valve terms in valve order: prvcoeff (dgga/solver.py:850-912,
+        batched), psvcoeff and fcvcoeff (the scalar replica's valvecoeffs,
+        dgga/solver.py:1616-1688).  An ACTIVE FCV moves its setting out of
+        Xflow[j1] into Xflow[j2] for the valves after it."""
         B = P.shape[0]
         a_cols, f_cols, p_cols, y_cols = [], [], [], []
+        Xf = Xflow.copy()
         for i in range(self.n_pcv):
             ki = int(self.prv_k[i])
+            kind = int(self.prv_kind[i])
+            j1 = int(self.prv_j1[i])
             j2 = int(self.prv_j2[i])
+            hset = float(self.prv_hset[i])
             qk = q[:, ki]
             sk = S[:, ki]
             m_act = sk == ST_ACTIVE
@@ -551,15 +569,33 @@ class GGAModel:
             yk_na = np.where(m_cl, qk, yk_o)
             mna = (~m_act).astype(np.float64)
             mact = m_act.astype(np.float64)
-            xf2 = Xflow[:, j2]
-            pna = pk_na * mna
-            a_cols += [-pna, -pna, pna, pna, CBIG * mact]
+            xf1 = Xf[:, j1].copy()
+            xf2 = Xf[:, j2].copy()
+            zero = np.zeros_like(qk)
             dy = (yk_na - qk) * mna
-            f1 = dy + np.where(m_act & (xf2 < 0.0), xf2, 0.0)
-            f2 = -dy + float(self.prv_hset[i]) * CBIG * mact
+            if kind == 0:       # PRV: pins its downstream row
+                pna = pk_na * mna
+                a_cols += [-pna, -pna, pna, zero, pna, CBIG * mact]
+                f1 = dy + np.where(m_act & (xf2 < 0.0), xf2, 0.0)
+                f2 = -dy + hset * CBIG * mact
+                y = np.where(m_act, qk + xf2, yk_na)
+            elif kind == 1:     # PSV: pins its upstream row
+                pna = pk_na * mna
+                a_cols += [-pna, -pna, pna, CBIG * mact, pna, zero]
+                f1 = dy + hset * CBIG * mact
+                f2 = -dy + np.where(m_act & (xf1 > 0.0), xf1, 0.0)
+                y = np.where(m_act, qk - xf1, yk_na)
+            else:               # FCV: ACTIVE cuts the link, flow = setting
+                pna = np.where(m_act, 1.0 / CBIG, pk_na)
+                a_cols += [-pna, -pna, pna, zero, pna, zero]
+                f1 = np.where(m_act, -hset, yk_na - qk)
+                f2 = np.where(m_act, hset, -(yk_na - qk))
+                y = np.where(m_act, qk - hset, yk_na)
+                Xf[:, j1] = np.where(m_act, xf1 - hset, xf1)
+                Xf[:, j2] = np.where(m_act, xf2 + hset, xf2)
             f_cols += [f1, f2]
             p_cols.append(pna)
-            y_cols.append(np.where(m_act, qk + xf2, yk_na))
+            y_cols.append(y)
         a_vals = np.stack(a_cols, axis=1)
         f_vals = np.stack(f_cols, axis=1)
         P = P.copy()
@@ -621,10 +657,26 @@ class GGAModel:
                              ~self.cvstatus_closed(False, dht, qk))
             hit = live & ((full & cut_f) | (empty & cut_e))
             s[:, k] = np.where(hit, ST_TEMPCLOSED, s[:, k])
+        if self.n_pcv and (self.prv_kind == 2).any():
+            # fcvstatus on the status before the reopening (the scalar
+            # replica's _linkstatus 1875-1877, _fcvstatus 1819-1831)
+            f = np.nonzero(self.prv_kind == 2)[0]
+            kf = self.prv_k[f]
+            s0 = st0[:, kf]
+            h1 = H[:, self.prv_n1[f]]
+            h2 = H[:, self.prv_n2[f]]
+            qf = q[:, kf]
+            new = np.where(h1 - h2 < -self.htol, ST_XFCV,
+                           np.where(qf < -self.qtol, ST_XFCV,
+                                    np.where((s0 == ST_XFCV) & (qf >= self.prv_hset[f]),
+                                             ST_ACTIVE, s0)))
+            s[:, kf] = new.astype(np.int8)
         return s, (s != st0).any(axis=1)
 
     def prvstatus(self, S, H, q):
-        """dgga/solver.py:914-947."""
+        """valvestatus: prvstatus (dgga/solver.py:914-947) for PRVs and
+        psvstatus (the scalar replica's 1740-1767) for PSVs; FCVs keep their
+        status here (fcvstatus runs in linkstatus)."""
         k = self.prv_k
         h1 = H[:, self.prv_n1]
         h2 = H[:, self.prv_n2]
@@ -639,10 +691,20 @@ class GGAModel:
         c_cl = np.where((h1 >= hset + htol) & (h2 < hset - htol), ST_ACTIVE,
                         np.where((h1 < hset - htol) & (h1 > h2 + htol), ST_OPEN, ST_CLOSED))
         c_xp = np.where(neg, ST_CLOSED, sk)
-        s_new = np.where(sk == ST_ACTIVE, c_act,
+        s_prv = np.where(sk == ST_ACTIVE, c_act,
                          np.where(sk == ST_OPEN, c_open,
                                   np.where(sk == ST_CLOSED, c_cl,
                                            np.where(sk == ST_XPRESSURE, c_xp, sk))))
+        p_act = np.where(neg, ST_CLOSED, np.where(h2 + hml > hset + htol, ST_OPEN, ST_ACTIVE))
+        p_open = np.where(neg, ST_CLOSED, np.where(h1 < hset - htol, ST_ACTIVE, ST_OPEN))
+        p_cl = np.where((h2 > hset + htol) & (h1 > h2 + htol), ST_OPEN,
+                        np.where((h1 >= hset + htol) & (h1 > h2 + htol), ST_ACTIVE, ST_CLOSED))
+        s_psv = np.where(sk == ST_ACTIVE, p_act,
+                         np.where(sk == ST_OPEN, p_open,
+                                  np.where(sk == ST_CLOSED, p_cl,
+                                           np.where(sk == ST_XPRESSURE, c_xp, sk))))
+        kind = self.prv_kind
+        s_new = np.where(kind == 0, s_prv, np.where(kind == 1, s_psv, sk))
         S_out = S.copy()
         S_out[:, k] = s_new
         return S_out.astype(np.int8), (s_new != sk).any(axis=1)

@@ -12,7 +12,7 @@ namespace wds {
 constexpr double CSMALL = 1.0e-6;
 constexpr double CBIG = 1.0e8;
 constexpr int8_t ST_XHEAD = 0, ST_TEMPCLOSED = 1, ST_CLOSED = 2, ST_OPEN = 3, ST_ACTIVE = 4,
-                 ST_XPRESSURE = 7;
+                 ST_XFCV = 6, ST_XPRESSURE = 7;
 // Darcy-Weisbach friction-factor constants (EPANET hydcoeffs.c; quoted in
 // dgga/solver.py:40-45)
 constexpr double DW_A1 = 3.14159265358979323850e+03;

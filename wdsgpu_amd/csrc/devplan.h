@@ -15,8 +15,9 @@ enum LinkFlag : int32_t {
   LF_CV = 1 << 1,
   LF_TCV = 1 << 2,
   LF_PUMP = 1 << 3,
-  LF_PRV_FIXED = 1 << 4,   // PRV whose setting is MISSING (fixed open/closed)
-  LF_PCV = 1 << 5,         // PRV with a setting: coefficients from prvcoeff
+  LF_PRV_FIXED = 1 << 4,   // PRV, PSV or FCV whose setting is MISSING (fixed open/closed)
+  LF_PCV = 1 << 5,         // PRV, PSV or FCV with a setting: coefficients from the
+                           // valve stage (prvcoeff, psvcoeff, fcvcoeff)
   LF_PUMP_CHP = 1 << 6,    // constant-horsepower pump
   LF_PUMP_LIN = 1 << 7,    // power-function pump with N == 1
   LF_PUMP_PF = 1 << 8,     // power-function pump
@@ -74,7 +75,7 @@ struct DevPlan {
   int32_t headloss;         // 0 Hazen-Williams, 1 Darcy-Weisbach
   double hexp, qexp, rqtol, viscos, htol, qtol;
   int32_t checkfreq, maxcheck, extra_iter;
-  int32_t has_cv, has_pump;
+  int32_t has_cv, has_pump, has_fcv;
   // nodes
   const int32_t* node_slot;   // [N] original node -> slot (junction slots
                               //     0..Nj-1 in elimination order, fixed Nj..)
@@ -146,12 +147,14 @@ struct DevPlan {
   const int2* off_ae;         // [noff]
   const int4* prv_ae;         // [p]
   const int32_t* csr_to_ell;  // [nnz_csr]
-  // PRVs with a setting (file order)
+  // control valves with a setting (PRV, PSV, FCV; file order)
   const int32_t* prv_k;       // [p]
+  const int32_t* prv_kind;    // [p] 0 PRV, 1 PSV, 2 FCV
   const int32_t* prv_s1;      // [p] slot of n1
   const int32_t* prv_s2;      // [p]
   const int32_t* prv_pos_off; // [p] L position of (j1,j2) lower
-  const double* prv_hset;     // [p]
+  const double* prv_hset;     // [p] PRV: El[n2] + setting, PSV: El[n1] + setting,
+                              //     FCV: the flow setting (cfs)
   const double* prv_km;       // [p]
   int32_t prv_disjoint;       // 1: no two valves share an end node
   // elimination-tree paths of the valve ends (path a < p: n2 of valve a,

@@ -126,7 +126,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   size_t o_dae = bb.add(P->diag_ae), o_oae = bb.add(P->off_ae), o_pae = bb.add(P->prv_ae),
          o_cae = bb.add(P->csr_to_ell);
   size_t o_pk = bb.add(P->prv_k), o_p1 = bb.add(P->prv_s1), o_p2 = bb.add(P->prv_s2),
-         o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km);
+         o_po = bb.add(P->prv_pos_off), o_hs = bb.add(P->prv_hset), o_pkm = bb.add(P->prv_km),
+         o_pkd = bb.add(P->prv_kind);
   size_t o_tk = bb.add(P->tl_k), o_tt = bb.add(P->tl_tank), o_to = bb.add(P->tl_other),
          o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
   size_t o_jgp = bb.add(P->jg_ptr), o_jg = bb.add(P->jg), o_jm = bb.add(P->jadj_mid);
@@ -168,6 +169,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.extra_iter = P->extra_iter;
   D.has_cv = P->has_cv;
   D.has_pump = P->has_pump;
+  D.has_fcv = P->has_fcv;
   D.node_slot = at<int32_t>(d, o_ns);
   D.slot_node = at<int32_t>(d, o_sn);
   D.el_slot = at<double>(d, o_el);
@@ -217,6 +219,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.prv_ae = at<int4>(d, o_pae);
   D.csr_to_ell = at<int32_t>(d, o_cae);
   D.prv_k = at<int32_t>(d, o_pk);
+  D.prv_kind = at<int32_t>(d, o_pkd);
   D.prv_s1 = at<int32_t>(d, o_p1);
   D.prv_s2 = at<int32_t>(d, o_p2);
   D.prv_pos_off = at<int32_t>(d, o_po);
@@ -456,6 +459,13 @@ extern "C" int wds_trisolve(wds_plan* P, void* stream, int32_t B, const double* 
 // record_schedule buffers: all or none, status machine on, event words wide
 // enough for the iteration and link numbers
 static int check_sched(const wds_plan* P, const wds_solve_args* a) {
+  // the polish and the terminal factor's adjoint solve the row-replaced
+  // systems of ACTIVE PRVs only
+  if (P->has_psv_fcv && a->polish_steps >= 0) {
+    set_error("the polish (and the implicit adjoint) of networks with PSVs or FCVs with a "
+              "setting is not built; solve without polish (polish_steps < 0)");
+    return WDS_EUNSUPPORTED;
+  }
   const bool any = a->sched_flags || a->sched_events || a->sched_count;
   if (!any) return WDS_OK;
   const int64_t maxtrials =
@@ -792,6 +802,10 @@ extern "C" int wds_multi_adjoint(wds_multi* M, void* stream, const wds_adjoint_a
       set_error("wds_multi_adjoint: bad argument for network " + std::to_string(i));
       return WDS_EINVAL;
     }
+    if (M->plans[i]->has_psv_fcv) {
+      set_error("the implicit adjoint of networks with PSVs or FCVs with a setting is not built");
+      return WDS_EUNSUPPORTED;
+    }
     M->h_ap[i] = make_adj_params(a);
   }
   hipStream_t st = (hipStream_t)stream;
@@ -845,6 +859,10 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
 extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a) {
   int r = check_ready(P);
   if (r) return r;
+  if (P->has_psv_fcv) {
+    set_error("the implicit adjoint of networks with PSVs or FCVs with a setting is not built");
+    return WDS_EUNSUPPORTED;
+  }
   if (!a || a->B <= 0 || !a->factor || !a->g_demand ||
       !a->g_res_head || !a->g_ke || !a->g_r || !a->info || !a->workspace || !a->head ||
       !a->flow || !a->emitter || !a->status || !a->ke || !a->r_hw || a->n_refine < 0) {

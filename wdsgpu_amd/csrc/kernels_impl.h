@@ -438,9 +438,13 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
 
 // prvcoeff for every PRV with a setting, in valve order, by one lane
 // (dgga/solver.py:850-912).  Must follow k2_assemble (reads Xflow[j2]).
-// the prvcoeff terms of valve i (A, V, F, P, Y)
+// the coefficient terms of control valve i (A, V, F, P, Y; the Xflow R of
+// an ACTIVE FCV): prvcoeff, psvcoeff, fcvcoeff (dgga/solver.py:850-912 for
+// the batched PRV; the scalar valvecoeffs 1616-1688 for PSV and FCV).  A
+// valve that is not ACTIVE takes valvecoeff's open (or closed) terms.
 __device__ __forceinline__ void prv_apply(const DevPlan& pl, Scn& c, int i) {
   const int k = pl.prv_k[i], j1 = pl.prv_s1[i], j2 = pl.prv_s2[i];
+  const int kind = pl.prv_kind[i];
   const double qk = c.q[k];
   const int8_t sk = c.S[k];
   const bool act = sk == ST_ACTIVE, cl = sk <= ST_CLOSED;
@@ -449,30 +453,53 @@ __device__ __forceinline__ void prv_apply(const DevPlan& pl, Scn& c, int i) {
   const double pk_na = cl ? 1.0 / CBIG : pk_o;
   const double yk_na = cl ? qk : yk_o;
   const double mna = act ? 0.0 : 1.0, mact = act ? 1.0 : 0.0;
-  const double xf2 = c.R[j2];
-  const double pna = pk_na * mna;
+  const double xf1 = c.R[j1], xf2 = c.R[j2];
+  const double hset = pl.prv_hset[i];
+  const bool fcv_act = kind == 2 && act;
+  // an ACTIVE FCV cuts its link: P = 1/CBIG, flow set to the setting
+  const double pna = fcv_act ? 1.0 / CBIG : pk_na * mna;
   const int po = pl.prv_pos_off[i], d1 = pl.diag_pos[j1], d2 = pl.diag_pos[j2];
   const int4 ae = pl.prv_ae[i];
-  // V holds the same values as A here (k2_assemble writes both)
+  // V holds the same values as A here (k2_assemble writes both); an ACTIVE
+  // PRV pins its downstream row, an ACTIVE PSV its upstream row
   double v;
   v = c.V[po] + (-pna);
   c.A[ae.x] = v;
   c.A[ae.y] = v;
   c.V[po] = v;
   v = c.V[d1] + pna;
+  if (kind == 1) v = v + CBIG * mact;
   c.A[ae.z] = v;
   c.V[d1] = v;
   v = c.V[d2] + pna;
-  v = v + CBIG * mact;
+  if (kind == 0) v = v + CBIG * mact;
   c.A[ae.w] = v;
   c.V[d2] = v;
   const double dy = (yk_na - qk) * mna;
-  const double f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
-  const double f2 = -dy + pl.prv_hset[i] * CBIG * mact;
+  double f1, f2, y;
+  if (kind == 0) {
+    f1 = dy + ((act && xf2 < 0.0) ? xf2 : 0.0);
+    f2 = -dy + hset * CBIG * mact;
+    y = act ? qk + xf2 : yk_na;
+  } else if (kind == 1) {
+    f1 = dy + hset * CBIG * mact;
+    f2 = -dy + ((act && xf1 > 0.0) ? xf1 : 0.0);
+    y = act ? qk - xf1 : yk_na;
+  } else {
+    f1 = fcv_act ? -hset : (yk_na - qk);
+    f2 = fcv_act ? hset : -(yk_na - qk);
+    y = fcv_act ? qk - hset : yk_na;
+    if (fcv_act) {
+      // the setting leaves j1 and enters j2 as fixed flows for the valves
+      // that follow (fcvcoeff's Xflow update)
+      c.R[j1] = xf1 - hset;
+      c.R[j2] = xf2 + hset;
+    }
+  }
   c.F[j1] = c.F[j1] + f1;
   c.F[j2] = c.F[j2] + f2;
   c.P[k] = pna;
-  c.Y[k] = act ? qk + xf2 : yk_na;
+  c.Y[k] = y;
 }
 
 __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
@@ -636,11 +663,13 @@ __device__ inline void sched_log(SchedLog& lg, int kind, bool chg, int link) {
   lg.nev += (int)__popcll(m);
 }
 
-// K6a: prvstatus for every PRV with a setting (dgga/solver.py:914-947)
+// K6a: valvestatus every iteration: prvstatus (dgga/solver.py:914-947) for
+// every PRV and psvstatus (the scalar replica's 1740-1767, hydstatus.c
+// psvstatus) for every PSV with a setting; FCVs change in linkstatus
 __device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
   const int i = lane_id();
   bool chg = false;
-  if (i < pl.p) {
+  if (i < pl.p && pl.prv_kind[i] != 2) {
     const int k = pl.prv_k[i];
     const double h1 = c.Hn[pl.prv_s1[i]], h2 = c.Hn[pl.prv_s2[i]];
     const double qk = c.q[k];
@@ -650,16 +679,29 @@ __device__ inline bool k6_prvstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
     const double htol = pl.htol;
     const bool neg = qk < -pl.qtol;
     int8_t sn = sk;
-    if (sk == ST_ACTIVE)
-      sn = neg ? ST_CLOSED : (h1 - hml < hset - htol ? ST_OPEN : ST_ACTIVE);
-    else if (sk == ST_OPEN)
-      sn = neg ? ST_CLOSED : (h2 >= hset + htol ? ST_ACTIVE : ST_OPEN);
-    else if (sk == ST_CLOSED)
-      sn = (h1 >= hset + htol && h2 < hset - htol)
-               ? ST_ACTIVE
-               : ((h1 < hset - htol && h1 > h2 + htol) ? ST_OPEN : ST_CLOSED);
-    else if (sk == ST_XPRESSURE)
-      sn = neg ? ST_CLOSED : sk;
+    if (pl.prv_kind[i] == 0) {
+      if (sk == ST_ACTIVE)
+        sn = neg ? ST_CLOSED : (h1 - hml < hset - htol ? ST_OPEN : ST_ACTIVE);
+      else if (sk == ST_OPEN)
+        sn = neg ? ST_CLOSED : (h2 >= hset + htol ? ST_ACTIVE : ST_OPEN);
+      else if (sk == ST_CLOSED)
+        sn = (h1 >= hset + htol && h2 < hset - htol)
+                 ? ST_ACTIVE
+                 : ((h1 < hset - htol && h1 > h2 + htol) ? ST_OPEN : ST_CLOSED);
+      else if (sk == ST_XPRESSURE)
+        sn = neg ? ST_CLOSED : sk;
+    } else {
+      if (sk == ST_ACTIVE)
+        sn = neg ? ST_CLOSED : (h2 + hml > hset + htol ? ST_OPEN : ST_ACTIVE);
+      else if (sk == ST_OPEN)
+        sn = neg ? ST_CLOSED : (h1 < hset - htol ? ST_ACTIVE : ST_OPEN);
+      else if (sk == ST_CLOSED)
+        sn = (h2 > hset + htol && h1 > h2 + htol)
+                 ? ST_OPEN
+                 : ((h1 >= hset + htol && h1 > h2 + htol) ? ST_ACTIVE : ST_CLOSED);
+      else if (sk == ST_XPRESSURE)
+        sn = neg ? ST_CLOSED : sk;
+    }
     chg = sn != sk;
     c.S[k] = sn;
   }
@@ -685,7 +727,7 @@ __device__ inline bool cv_closed(double htol, double qtol, bool s_is_open, doubl
 __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
   const int lane = lane_id();
   bool chg = false;
-  if (!pl.has_cv && !pl.has_pump && pl.ntl == 0) return false;
+  if (!pl.has_cv && !pl.has_pump && pl.ntl == 0 && !pl.has_fcv) return false;
   for (int k = lane; k < pl.L; k += 64) {
     const int8_t st0 = c.S[k];
     c.st0[k] = st0;
@@ -711,6 +753,23 @@ __device__ inline bool k6_linkstatus(const DevPlan& pl, Scn& c, SchedLog& lg) {
     const bool cut_f = is_pump ? (bool)(bits & 8) : cv_closed(pl.htol, pl.qtol, true, dht, qk);
     const bool cut_e = is_pump ? (bool)(bits & 16) : !cv_closed(pl.htol, pl.qtol, false, dht, qk);
     if (live && ((full && cut_f) || (empty && cut_e))) c.S[k] = ST_TEMPCLOSED;
+  }
+  // FCVs with a setting: fcvstatus from the status before the reopening
+  // (the scalar replica's _linkstatus, dgga/solver.py:1875-1877, and
+  // _fcvstatus 1819-1831): XFCV on a reversed head or flow, back to ACTIVE
+  // from XFCV once the flow reaches the setting
+  if (pl.has_fcv && lane < pl.p && pl.prv_kind[lane] == 2) {
+    const int i = lane, k = pl.prv_k[i];
+    const int8_t st0 = c.st0[k];
+    const double h1 = c.Hn[pl.prv_s1[i]], h2 = c.Hn[pl.prv_s2[i]], qk = c.q[k];
+    int8_t s = st0;
+    if (h1 - h2 < -pl.htol)
+      s = ST_XFCV;
+    else if (qk < -pl.qtol)
+      s = ST_XFCV;
+    else if (st0 == ST_XFCV && qk >= pl.prv_hset[i])
+      s = ST_ACTIVE;
+    c.S[k] = s;
   }
   mem_sync();
   if (lg.ev) {

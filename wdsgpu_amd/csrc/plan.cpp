@@ -37,10 +37,10 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
   if (Nj == 0) throw std::invalid_argument("network has no junctions");
   P->Nj = Nj;
   P->Nf = Nf;
-  // supported link types: CVPIPE 0, PIPE 1, PUMP 2, PRV 3, TCV 7
+  // supported link types: CVPIPE 0, PIPE 1, PUMP 2, PRV 3, PSV 4, FCV 6, TCV 7
   for (int k = 0; k < L; ++k) {
     const int t = d->link_type[k];
-    if (!(t == 0 || t == 1 || t == 2 || t == 3 || t == 7))
+    if (!(t == 0 || t == 1 || t == 2 || t == 3 || t == 4 || t == 6 || t == 7))
       throw std::domain_error("link type " + std::to_string(t) + " is not on the GGA dense path");
     if (d->link_n1[k] == d->link_n2[k]) throw std::invalid_argument("self-loop link");
     if (d->link_n1[k] < 0 || d->link_n1[k] >= N || d->link_n2[k] < 0 || d->link_n2[k] >= N)
@@ -98,7 +98,7 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       P->has_cv = 1;
     }
     if (t == 7) f |= LF_TCV;
-    if (t == 3) f |= (d->setting[k] == kMissing) ? LF_PRV_FIXED : LF_PCV;
+    if (t == 3 || t == 4 || t == 6) f |= (d->setting[k] == kMissing) ? LF_PRV_FIXED : LF_PCV;
     if (t == 2) {
       P->has_pump = 1;
       f |= LF_PUMP;
@@ -321,20 +321,28 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     }
   }
 
-  // ---- PRVs with a setting ----
+  // ---- control valves with a setting (PRV, PSV, FCV), in valve order ----
   P->prv_k.clear();
   for (int k = 0; k < L; ++k) {
     if (!(P->l_flags[k] & LF_PCV)) continue;
     const int n1 = d->link_n1[k], n2 = d->link_n2[k];
     if (junc_row[n1] < 0 || junc_row[n2] < 0)
-      throw std::domain_error("PRV with a fixed-head end (the batched PRV path needs junction ends)");
+      throw std::domain_error(
+          "control valve with a fixed-head end (the batched valve path needs junction ends)");
     const int s1 = P->node_slot[n1], s2 = P->node_slot[n2];
+    const int t = d->link_type[k];
     P->prv_k.push_back(k);
+    P->prv_kind.push_back(t == 3 ? 0 : (t == 4 ? 1 : 2));
     P->prv_s1.push_back(s1);
     P->prv_s2.push_back(s2);
     P->prv_pos_off.push_back((int32_t)S.pos(std::max(s1, s2), std::min(s1, s2)));
-    P->prv_hset.push_back(d->elev[n2] + d->setting[k]);
+    // PRV: downstream head El[n2] + setting; PSV: upstream head El[n1] +
+    // setting (prvcoeff, psvcoeff); FCV: the flow setting (fcvcoeff)
+    P->prv_hset.push_back(t == 3 ? d->elev[n2] + d->setting[k]
+                                 : (t == 4 ? d->elev[n1] + d->setting[k] : d->setting[k]));
     P->prv_km.push_back(d->km_valve[k]);
+    if (t != 3) P->has_psv_fcv = 1;
+    if (t == 6) P->has_fcv = 1;
   }
   P->p = (int32_t)P->prv_k.size();
   {

@@ -21,7 +21,7 @@ struct wds_plan {
   std::vector<int32_t> pu_cptr;
   double ucf_flow = 1.0, ucf_head = 1.0;
   std::vector<int32_t> jadj_ptr, jadj, fadj_ptr, fadj, off_pos, off_ptr, off_link, fill_pos,
-      spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, tl_k, tl_tank, tl_other,
+      spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, prv_kind, tl_k, tl_tank, tl_other,
       tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;
   std::vector<int32_t> spmv_ent;  // pairs (pos, col)
   // the same rows in 64-row blocks, entry-major within a block (ELL): entry e
@@ -49,7 +49,9 @@ struct wds_plan {
   // scalars
   int32_t headloss = 0, checkfreq = 2, maxcheck = 10, extra_iter = -1;
   int32_t has_cv = 0, has_pump = 0;
-  int32_t prv_disjoint = 0;  // no two PRVs with a setting share an end node
+  int32_t prv_disjoint = 0;  // no two valves with a setting share an end node
+  int32_t has_psv_fcv = 0;   // a PSV or FCV with a setting (no polish / adjoint)
+  int32_t has_fcv = 0;       // an FCV with a setting (fcvstatus in linkstatus)
   double hexp = 1.852, qexp = 2.0, rqtol = 1e-7, viscos = 1.1e-5, htol = 5e-4, qtol = 1e-4;
   // arena mode (doubles per scenario of the global-memory copy of the LDS
   // layout; 0 in LDS mode)

@@ -24,7 +24,8 @@ from __future__ import annotations
 
 import numpy as np
 
-from .network import (Net, JUNCTION, RESERVOIR, TANK, PIPE, CVPIPE, PUMP, PRV, TCV, M_PER_FT,
+from .network import (Net, JUNCTION, RESERVOIR, TANK, PIPE, CVPIPE, PUMP, PRV, PSV, FCV, TCV,
+                      M_PER_FT,
                       LPS_PER_CFS, PUMP_POWER_FUNC, PUMP_CUSTOM, hw_resistance, dw_resistance,
                       emitter_ke, pump_power_curve, pump_custom_curve)
 
@@ -90,7 +91,8 @@ def grid_network(seed=0, nx=10, ny=10, n_chords=20):
 
 
 def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tcv=1,
-                    n_cv=1, tank=True, n_leaks=2, tank_level_m=18.0, n_custom=0):
+                    n_cv=1, tank=True, n_leaks=2, tank_level_m=18.0, n_custom=0, n_psv=0,
+                    n_fcv=0, psv_m=(20.0, 40.0), fcv_lps=(1.0, 6.0)):
     """Grid of nx*ny junctions (elevation U[0,30] m, demand U[0.2,1.5] L/s,
     pipes 100-600 m) with: ``n_pumps`` pumps lifting from low reservoirs
     (head 5 m) into corner junctions, each with a 3-point curve (0, h0),
@@ -101,7 +103,11 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
     check-valve pipes; ``n_prv`` PRVs (setting U[25,35] m) each feeding a
     two-junction branch of its own; ``n_leaks`` leak emitters.  The first
     ``n_custom`` pumps get a 4-point CUSTOM curve instead, (0.25 qd, 0.97 h0),
-    (qd, 0.75 h0), (1.6 qd, 0.42 h0), (2.2 qd, 0.08 h0).
+    (qd, 0.75 h0), (1.6 qd, 0.42 h0), (2.2 qd, 0.08 h0).  ``n_fcv`` grid
+    pipes become FCVs (flow setting U[fcv_lps] L/s); ``n_psv`` PSVs
+    (upstream pressure setting U[psv_m] m) each let a grid junction drain
+    through a junction 5 m lower to a reservoir 10 m below it.  Control
+    valves start ACTIVE.
     headloss "H-W" (C U[90,140]) or "D-W" (roughness U[0.01,0.5] mm)."""
     g = np.random.default_rng(seed)
     nj = nx * ny
@@ -168,6 +174,33 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
         links[k].update(type=TCV, setting=g.uniform(2.0, 10.0))
     for k in grid_links[n_tcv:n_tcv + n_cv]:
         links[k].update(type=CVPIPE)
+    # control valves share no node (EPANET leaves every controlled valve out
+    # of the nodal inflow an ACTIVE valve's flow is set from, so valves on
+    # one node would not balance it): FCVs on grid pipes away from the PRV
+    # taps, PSVs on junctions that touch no other control valve
+    busy = {int(a) for a in taps}
+    fcv = []
+    for k in grid_links[n_tcv + n_cv:]:
+        if len(fcv) == n_fcv:
+            break
+        a, b = pairs[k]
+        if a in busy or b in busy:
+            continue
+        fcv.append(k)
+        busy |= {a, b}
+    for k in fcv:
+        links[k].update(type=FCV, setting=g.uniform(*fcv_lps))
+    # PSVs sustain the pressure of a grid junction that drains to a low
+    # reservoir: grid junction -> PSV -> junction (5 m lower) -> pipe ->
+    # reservoir (head 10 m below the grid junction's elevation)
+    free = [a for a in range(nj) if a not in busy]
+    for a in (g.choice(free, size=n_psv, replace=False) if n_psv else []):
+        a = int(a)
+        b = add_node(JUNCTION, elev[a] - 5.0, 0.5)
+        r = add_node(RESERVOIR, elev[a] - 10.0)
+        links.append(dict(n1=a, n2=b, type=PSV, diam_mm=150.0, setting_m=g.uniform(*psv_m)))
+        links.append(dict(n1=b, n2=r, type=PIPE, len_m=200.0, diam_mm=150.0, C=120.0,
+                          rough_mm=0.05))
     N, L = len(ntype), len(links)
     lt = np.array([lk["type"] for lk in links], dtype=np.int8)
     ln_ft = np.array([lk.get("len_m", 0.0) for lk in links]) / M_PER_FT
@@ -185,7 +218,7 @@ def feature_network(seed=0, nx=6, ny=6, headloss="H-W", n_pumps=1, n_prv=1, n_tc
     else:
         raise ValueError("headloss must be 'H-W' or 'D-W'")
     rough = np.where(pipe, rough, 0.0)
-    st = np.array([2 if t in (PRV, TCV) else 1 for t in lt], dtype=np.int8)
+    st = np.array([2 if t in (PRV, PSV, FCV, TCV) else 1 for t in lt], dtype=np.int8)
     setting = np.array([lk.get("setting_m", lk.get("setting", 0.0)) for lk in links])
     for k, _ in pumps:
         setting[k] = 1.0          # relative speed

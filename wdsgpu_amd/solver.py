@@ -83,7 +83,7 @@ class GGASolver:
         nt = np.asarray(net.node_type, dtype=np.int64)
         self.N, self.L = net.N, net.L
         self.lt_np = lt
-        allowed = {CVPIPE, PIPE, TCV, PUMP, PRV} if self.dense_status_machine \
+        allowed = {CVPIPE, PIPE, TCV, PUMP, PRV, PSV, FCV} if self.dense_status_machine \
             else {PIPE, TCV, PUMP}
         bad = set(lt.tolist()) - allowed
         if bad:
