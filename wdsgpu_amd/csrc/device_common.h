@@ -29,8 +29,20 @@ constexpr double PI_EPANET = 3.141592654;
 // (Arena mode keeps the same bound: at two wavefronts per SIMD the fused
 // forward would spill to scratch.)
 #define WDS_SCENARIO_KERNEL __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+// The fused forward runs two wavefronts per scenario (a 128-lane workgroup):
+// the second shares the link, row and node loops of the Newton iteration, and
+// a SIMD holds two wavefronts, so one hides the other's memory waits.  The
+// register budget is then half the file.
+constexpr int kSolveWaves = 2;
+#define WDS_SOLVE_KERNEL __launch_bounds__(64 * kSolveWaves) \
+    __attribute__((amdgpu_waves_per_eu(kSolveWaves, kSolveWaves)))
 
-__device__ __forceinline__ int lane_id() { return (int)threadIdx.x; }
+__device__ __forceinline__ int lane_id() { return (int)(threadIdx.x & 63); }
+// the wavefront's index in its scenario's workgroup
+__device__ __forceinline__ int wave_idx() { return (int)(threadIdx.x >> 6); }
+// barrier of the scenario's workgroup: the LDS and global writes of every
+// wavefront before it are visible to every wavefront after it
+__device__ __forceinline__ void wg_sync() { __syncthreads(); }
 
 // Make this wave's global stores visible to its own later loads.
 __device__ __forceinline__ void mem_sync() {
@@ -392,8 +404,8 @@ __device__ __forceinline__ void two_sum(double a, double b, double& s, double& e
 
 // R_j *= V[diag_pos[j]] (the reciprocal pivots) for every row, V and R in
 // LDS: the pivot positions of up to kRowG rows per lane load before use.
-constexpr int kRowG = 16;
-constexpr int kRowG8 = 8;   // rows per lane per round where each row loads several values
+constexpr int kRowG = 8;
+constexpr int kRowG8 = 4;   // rows per lane per round where each row loads several values
 // The pivot positions of the first kRowG rows per lane, loaded ahead (before
 // the forward program runs) so that the scaling after it waits on no memory.
 __device__ __forceinline__ void pivots_prefetch(const DevPlan& pl, int (&dp)[kRowG]) {
@@ -464,34 +476,53 @@ __device__ __forceinline__ void sweep_values(const lds_f64* X, const int (&cl)[k
   }
 }
 
-// dst_j = b_j - (A x)_j in double-double, x and dst in LDS, b global
+// The first row block of share i of nw over the row blocks of a row-block
+// table whose block b starts at slot blk[b] (blk[nb] = E): the smallest b
+// with blk[b] nw >= i E, so the shares hold about E / nw slots each.
+__device__ __forceinline__ int block_split(const int32_t* blk, int nb, int E, int i, int nw) {
+  if (i <= 0) return 0;
+  if (i >= nw) return nb;
+  int b = 0;
+  while (b < nb && (int64_t)blk[b] * nw < (int64_t)i * E) ++b;
+  return b;
+}
+
+// dst_j = b_j - (A x)_j in double-double, x and dst in LDS, b global; the
+// row blocks split into nw contiguous shares, share w on this wavefront
 __device__ inline void residual_all(const DevPlan& pl, const double* A, const double* x,
-                                    const double* b, double* dst) {
+                                    const double* b, double* dst, int w = 0, int nw = 1) {
   const int lane = lane_id();
   const lds_f64* X = (const lds_f64*)x;
   lds_f64* D = (lds_f64*)dst;
   const int nb = (pl.Nj + 63) >> 6;
-  const int E = pl.rs_blk[nb];
+  const int b0 = block_split(pl.rs_blk, nb, pl.rs_blk[nb], w, nw);
+  const int b1 = block_split(pl.rs_blk, nb, pl.rs_blk[nb], w + 1, nw);
+  if (b0 >= b1) return;
+  const int S0 = pl.rs_blk[b0];
+  const int E = pl.rs_blk[b1];
   const int jl = pl.Nj - 1;
+  const int rows_end = min(64 * b1, pl.Nj);
   int cl[kSweep];
   double av[kSweep];
-  sweep_load(pl, A, 0, E, cl, av);
+  sweep_load(pl, A, S0, E, cl, av);
   __builtin_amdgcn_sched_barrier(0);
-  // D = b (each lane copies and later reads only its own rows)
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG) {
+  // D = b on this share's rows (each lane copies and later reads only its
+  // own rows)
+  for (int j0 = 64 * b0 + lane; j0 < rows_end; j0 += 64 * kRowG) {
     double bv[kRowG];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u) bv[u] = b[min(j0 + 64 * u, jl)];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u)
-      if (j0 + 64 * u < pl.Nj) D[j0 + 64 * u] = bv[u];
+      if (j0 + 64 * u < rows_end) D[j0 + 64 * u] = bv[u];
   }
-  int bk = 0;
-  int be = pl.rs_blk[1];
-  int be2 = pl.rs_blk[min(2, nb)];
-  double hi = D[min(lane, jl)], lo = 0.0;
-  double hn = D[min(64 + lane, jl)];
-  for (int s0 = 0; s0 < E; s0 += kSweep) {
+  int bk = b0;
+  int be = pl.rs_blk[b0 + 1];
+  int be2 = pl.rs_blk[min(b0 + 2, nb)];
+  double hi = D[min(64 * b0 + lane, jl)], lo = 0.0;
+  // the next block's first value; past the share's end it is never used
+  double hn = D[min(64 * (b0 + 1) + lane, jl)];
+  for (int s0 = S0; s0 < E; s0 += kSweep) {
     double xv[kSweep];
     sweep_values(X, cl, av, xv);
 #pragma unroll
@@ -862,10 +893,12 @@ __device__ inline void row_gather(const DevPlan& pl, int e0, int e1, double sgn,
 }
 
 // dst[j] = a[j], or a[j] - b[j] when b is given, for j < n; loads batched
-__device__ inline void vec_copy(double* dst, const double* a, const double* b, int n) {
+__device__ inline void vec_copy(double* dst, const double* a, const double* b, int n,
+                                int w = 0, int nw = 1) {
   const int lane = lane_id();
-  // kRowG rows per lane per round: one memory round trip for up to 1024 rows
-  for (int j0 = lane; j0 < n; j0 += 64 * kRowG) {
+  // kRowG rows per lane per round: one memory round trip per round; the
+  // rounds alternate between the nw wavefronts
+  for (int j0 = lane + 64 * kRowG * w; j0 < n; j0 += 64 * kRowG * nw) {
     double av[kRowG], bv[kRowG];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u) {
@@ -883,9 +916,10 @@ __device__ inline void vec_copy(double* dst, const double* a, const double* b, i
 
 // link coefficients P, Y of every link (K1 and the polish), batched
 __device__ inline void link_coeffs_all(const DevPlan& pl, const double* q, const double* r,
-                                       const int8_t* S, double* P, double* Y) {
+                                       const int8_t* S, double* P, double* Y, int w = 0,
+                                       int nw = 1) {
   const int lane = lane_id();
-  for (int k0 = lane; k0 < pl.L; k0 += 64 * kGather) {
+  for (int k0 = lane + 64 * kGather * w; k0 < pl.L; k0 += 64 * kGather * nw) {
     double qv[kGather], rv[kGather], kmv[kGather];
     int fv[kGather];
     int8_t sv[kGather];

@@ -511,7 +511,8 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
     arena_solve(P->dev, sp, a->B, (hipStream_t)stream);
   } else {
     set_lds(k_solve, lds);
-    hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, sp);
+    hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64 * kSolveWaves), lds, (hipStream_t)stream,
+                       P->dev, sp);
   }
   P->cnt.launches += 1;
   return launch_check("k_solve");

@@ -15,7 +15,7 @@
 namespace wds {
 
 void arena_solve(const DevPlan& pl, const SolveParams& sp, int B, hipStream_t st) {
-  hipLaunchKernelGGL(k_solve_arena, dim3(B), dim3(64), 0, st, pl, sp);
+  hipLaunchKernelGGL(k_solve_arena, dim3(B), dim3(64 * kSolveWaves), 0, st, pl, sp);
 }
 
 void arena_newton_iter(const DevPlan& pl, const SolveParams& sp, int it, int32_t* ctrl, int B,

@@ -253,9 +253,9 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
 // ---------------------------------------------------------------------------
 // K1: link and emitter coefficients at the current state
 // emitter hloss / hgrad of every junction with an emitter, rows batched
-__device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c) {
+__device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
   const int lane = lane_id();
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+  for (int j0 = lane + 64 * kRowG8 * w; j0 < pl.Nj; j0 += 64 * kRowG8 * nw) {
     double kv[kRowG8], ev[kRowG8];
 #pragma unroll
     for (int u = 0; u < kRowG8; ++u) {
@@ -276,9 +276,9 @@ __device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c) {
   }
 }
 
-__device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
-  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
-  emitter_coeffs(pl, c);
+__device__ inline void k1_coeffs(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
+  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y, w, nw);
+  emitter_coeffs(pl, c, w, nw);
   mem_sync();
 }
 
@@ -290,24 +290,27 @@ __device__ inline void k1_coeffs(const DevPlan& pl, Scn& c) {
 // batches of kGather with every load issued before any is consumed, and the
 // accumulation is branch-free.  Sums keep the reference's order exactly.
 
-constexpr int kAsE = 8;   // assembly entries per row loaded per round
-constexpr int kOffG = 16; // off-diagonal entries per lane loaded per round
+constexpr int kAsE = 4;   // assembly entries per row loaded per round
+constexpr int kOffG = 8;  // off-diagonal entries per lane loaded per round
 
 template <bool kF>
-__device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
+__device__ inline void k2_assemble(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
   const int lane = lane_id();
   const int nb = (pl.Nj + 63) >> 6;
   const lds_f64* Hn = (const lds_f64*)c.Hn;
   lds_f64* Vl = (lds_f64*)c.V;
   lds_f64* Rl = (lds_f64*)c.R;
-  for (int b0 = 0; b0 < nb; b0 += 2) {
+  // the row blocks in nw contiguous shares of about equal entry counts
+  const int bw0 = block_split(pl.as_blk, nb, pl.as_blk[nb], w, nw);
+  const int bw1 = block_split(pl.as_blk, nb, pl.as_blk[nb], w + 1, nw);
+  for (int b0 = bw0; b0 < bw1; b0 += 2) {
     int e0[2], K[2], jr[2], dp[2], da[2];
     double kv[2], hg[2], hl[2], ev[2], dj[2], el[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const int bk = min(b0 + t, nb - 1);
+      const int bk = min(b0 + t, bw1 - 1);
       e0[t] = pl.as_blk[bk];
-      K[t] = b0 + t < nb ? pl.as_blk[bk + 1] - e0[t] : 0;
+      K[t] = b0 + t < bw1 ? pl.as_blk[bk + 1] - e0[t] : 0;
       jr[t] = bk * 64 + lane;
       const int j = min(jr[t], pl.Nj - 1);
       kv[t] = c.kej[j];
@@ -382,7 +385,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
           X[t] = X[t] - ev[t];
         }
       }
-      if (b0 + t < nb && jr[t] < pl.Nj) {
+      if (b0 + t < bw1 && jr[t] < pl.Nj) {
         c.A[da[t]] = dg[t];
         Vl[dp[t]] = dg[t];
         if (kF) {
@@ -393,7 +396,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
       }
     }
   }
-  for (int o0 = lane; o0 < pl.noff; o0 += 64 * kOffG) {
+  for (int o0 = lane + 64 * kOffG * w; o0 < pl.noff; o0 += 64 * kOffG * nw) {
     int4 w[kOffG];
     int2 wa[kOffG];
 #pragma unroll
@@ -423,7 +426,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
       }
     }
   }
-  for (int f0 = lane; f0 < pl.nfill; f0 += 64 * kRowG) {
+  for (int f0 = lane + 64 * kRowG * w; f0 < pl.nfill; f0 += 64 * kRowG * nw) {
     int fp[kRowG];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u) fp[u] = pl.fill_pos[min(f0 + 64 * u, pl.nfill - 1)];
@@ -431,7 +434,7 @@ __device__ inline void k2_assemble(const DevPlan& pl, Scn& c) {
     for (int u = 0; u < kRowG; ++u)
       if (f0 + 64 * u < pl.nfill) Vl[fp[u]] = 0.0;
   }
-  if (lane == 0) Vl[pl.nnzL] = 0.0;
+  if (lane == 0 && w == 0) Vl[pl.nnzL] = 0.0;
   mem_sync();
   lds_sync();
 }
@@ -544,24 +547,33 @@ __device__ inline void solve_bwd(const DevPlan& pl, Scn& c) {
 }
 
 // X = A^-1 b (b global, slot order; X in LDS, not S) with n_refine steps of
-// refinement on the double-double residual of the assembled A.
+// refinement on the double-double residual of the assembled A.  With nw > 1
+// wavefronts the copies and residuals are shared and wavefront 0 runs the
+// triangular solves (the others wait at the workgroup barriers).
 __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, double* X,
-                                int n_refine) {
+                                int n_refine, int w = 0, int nw = 1) {
   const int lane = lane_id();
   double* const S = c.R;
-  vec_copy(S, bsrc, nullptr, pl.Nj);
+  vec_copy(S, bsrc, nullptr, pl.Nj, w, nw);
   lds_sync();
-  solve_fwd(pl, c);
-  solve_bwd(pl, c);
-  for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
-  lds_sync();
-  for (int it = 0; it < n_refine; ++it) {
-    residual_all(pl, c.A, X, bsrc, S);
-    lds_sync();
+  if (nw > 1) wg_sync();
+  if (w == 0) {
     solve_fwd(pl, c);
     solve_bwd(pl, c);
-    for (int j = lane; j < pl.Nj; j += 64) X[j] += S[j];
+    for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
     lds_sync();
+  }
+  for (int it = 0; it < n_refine; ++it) {
+    if (nw > 1) wg_sync();
+    residual_all(pl, c.A, X, bsrc, S, w, nw);
+    lds_sync();
+    if (nw > 1) wg_sync();
+    if (w == 0) {
+      solve_fwd(pl, c);
+      solve_bwd(pl, c);
+      for (int j = lane; j < pl.Nj; j += 64) X[j] += S[j];
+      lds_sync();
+    }
   }
 }
 
@@ -846,7 +858,7 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
 #define WDS_TACC(c, i)                                              \
   do {                                                              \
     const long long t1_ = clock64();                                \
-    if (lane_id() == 0) (c).misc[400 + (i)] += (double)(t1_ - wds_t0_); \
+    if (threadIdx.x == 0) (c).misc[400 + (i)] += (double)(t1_ - wds_t0_); \
     wds_t0_ = t1_;                                                  \
   } while (0)
 #else
@@ -854,22 +866,43 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
 #define WDS_TACC(c, i) (void)0
 #endif
 
-// one Newton iteration; returns relerr, sets bad on a non-positive pivot
+// The scenario's LDS word that carries wavefront 0's decisions (a
+// non-positive pivot, the end of the loop) to the other wavefronts: the
+// spare slot after the factor's zero slot.
+__device__ __forceinline__ lds_f64& wave_flag(const DevPlan& pl, Scn& c) {
+  return ((lds_f64*)c.V)[pl.nnzL + 1];
+}
+
+// one Newton iteration; returns relerr (on wavefront 0), sets bad on a
+// non-positive pivot (on every wavefront).  With nw > 1 wavefronts the link
+// coefficients, the assembly, the copies and the refinement residuals are
+// shared; wavefront 0 alone applies the valve terms, factors, runs the
+// triangular solves and updates the flows.
 __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const SolveParams& sp,
-                                          bool& bad) {
+                                          bool& bad, int w = 0, int nw = 1) {
   WDS_T0();
-  k1_coeffs(pl, c);
+  k1_coeffs(pl, c, w, nw);
+  if (nw > 1) wg_sync();
   WDS_TACC(c, 0);
-  k2_assemble<true>(pl, c);
+  k2_assemble<true>(pl, c, w, nw);
+  if (nw > 1) wg_sync();
   WDS_TACC(c, 104);
-  k2_prv(pl, c);
-  WDS_TACC(c, 105);
-  bad = k3_factor(pl, c);
+  if (w == 0) {
+    k2_prv(pl, c);
+    WDS_TACC(c, 105);
+    bad = k3_factor(pl, c);
+    if (nw > 1 && lane_id() == 0) wave_flag(pl, c) = bad ? 1.0 : 0.0;
+  }
+  if (nw > 1) {
+    wg_sync();
+    bad = wave_flag(pl, c) != 0.0;
+  }
   WDS_TACC(c, 2);
   if (bad) return 0.0;
-  k4_solve(pl, c, c.F, c.Hn, sp.n_refine);
+  k4_solve(pl, c, c.F, c.Hn, sp.n_refine, w, nw);
   WDS_TACC(c, 3);
-  const double r = k5_update(pl, c, sp.hacc);
+  double r = 0.0;
+  if (w == 0) r = k5_update(pl, c, sp.hacc);
   WDS_TACC(c, 4);
   return r;
 }
@@ -1069,25 +1102,31 @@ __device__ inline void init_state(const DevPlan& pl, Scn& c, const SolveParams& 
 }
 
 // Fused forward: the whole Newton loop of scenario b, its layout at lds.
+// The workgroup's wavefronts (blockDim.x / 64) share the Newton iterations;
+// the status machine, the polish and the outputs are wavefront 0's.
 __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolveParams& sp, int b,
                                       double* lds) {
   const int lane = lane_id();
+  const int w = wave_idx(), nw = (int)(blockDim.x >> 6);
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
-  load_inputs(pl, c, sp, b);
-  init_state(pl, c, sp, b);
+  if (w == 0) {
+    load_inputs(pl, c, sp, b);
+    init_state(pl, c, sp, b);
 #ifdef WDS_PROFILE_STAGES
-  for (int i = lane; i < 112; i += 64) c.misc[400 + i] = 0.0;
-  // marks the counter block for the host that reads it back
-  if (lane == 0) c.misc[511] = 12345.0;
+    for (int i = lane; i < 112; i += 64) c.misc[400 + i] = 0.0;
+    // marks the counter block for the host that reads it back
+    if (lane == 0) c.misc[511] = 12345.0;
 #endif
-  mem_sync();
-  lds_sync();
+    mem_sync();
+    lds_sync();
+  }
+  if (nw > 1) wg_sync();
   const int maxtrials = maxtrials_of(pl, sp);
   Ctrl st{1, 0, pl.checkfreq, 0, 0};
   double relerr_out = 0.0, dratio = 0.0;
   for (int it = 1; it <= maxtrials; ++it) {
     bool bad = false;
-    const double relerr = newton_iteration(pl, c, sp, bad);
+    const double relerr = newton_iteration(pl, c, sp, bad, w, nw);
     if (bad) {
       st.info = it;
       break;
@@ -1095,13 +1134,22 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
     st.iters = it;
     relerr_out = relerr;
     WDS_T0();
-    const bool done = control_step(pl, c, sp, b, it, relerr, st);
+    bool done = false;
+    if (w == 0) done = control_step(pl, c, sp, b, it, relerr, st);
+    if (nw > 1) {
+      if (w == 0 && lane == 0) wave_flag(pl, c) = done ? 1.0 : 0.0;
+      wg_sync();
+      done = wave_flag(pl, c) != 0.0;
+    }
     WDS_TACC(c, 5);
     if (done) {
       st.active = 0;
       break;
     }
   }
+  // the other wavefronts are done: the polish, the terminal factor and the
+  // outputs are wavefront 0's (no workgroup barrier follows)
+  if (w != 0) return;
   if (st.active && st.info == 0) st.iters = maxtrials + 1;
   // the last assembled matrix is still in A (the polish assembles its own)
   if (pl.p > 0 && sp.diag_ratio) dratio = diag_ratio_now(pl, c);
@@ -1123,8 +1171,8 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
   }
 }
 
-// one scenario per workgroup (scenario blockIdx.x)
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
+// one scenario per workgroup (scenario blockIdx.x), kSolveWaves wavefronts
+__global__ WDS_SOLVE_KERNEL void WDS_K(k_solve)(DevPlan pl, SolveParams sp) {
   const int b = blockIdx.x;
   WDS_SCENARIO_MEM(sp.ws, b);
   solve_scenario(pl, sp, b, lds);
