@@ -234,3 +234,47 @@ def test_parameter_maps_refuse_cpu_tensors():
         hw_resistance(one, one, one)
     with pytest.raises(NotImplementedError, match="HIP device"):
         emitter_ke(one)
+
+
+_PSWITCH_INP = """[JUNCTIONS]
+ J1 10 5
+ J2 8 3
+[RESERVOIRS]
+ R1 60
+[TANKS]
+ T1 20 5 2 10 30 0
+[PIPES]
+ P1 R1 J1 1000 12 100 0 Open
+ P2 J1 J2 1000 10 100 0 Open
+ P3 J2 T1 1000 10 100 0 Open
+[CONTROLS]
+%s
+[OPTIONS]
+ Units GPM
+ Headloss H-W
+[END]
+"""
+
+
+@pytest.mark.parametrize("control,refused", [
+    ("LINK P2 CLOSED IF NODE J1 BELOW 20", True),
+    ("LINK P3 OPEN IF NODE J2 ABOVE 10", True),
+    ("LINK P3 CLOSED IF NODE T1 ABOVE 12", False),
+    ("LINK P2 CLOSED AT TIME 2", False),
+    ("", False),
+])
+def test_junction_head_controls_are_refused(tmp_path, control, refused):
+    # pswitch guard of the dense path (dgga/solver.py:617-644): a simple
+    # control watching a junction's head would change a link after
+    # convergence, which the batched status machine does not do
+    from wdsgpu_amd.network import read_inp
+    p = tmp_path / "ctl.inp"
+    p.write_text(_PSWITCH_INP % control)
+    net = read_inp(p)
+    if refused:
+        with pytest.raises(NotImplementedError, match="pswitch"):
+            GGASolver(net, device="cpu", dense_status_machine=True)
+        with pytest.raises(NotImplementedError, match="pswitch"):
+            GGASolver(net, device="cpu")
+    else:
+        GGASolver(net, device="cpu", dense_status_machine=True)

@@ -51,6 +51,29 @@ class GGASolver:
     the per-scenario factor and vectors in global memory, even for a network
     that fits a CU's LDS; used to test that mode)."""
 
+    def _check_pswitch(self, nt):
+        """Refuse simple controls that watch a junction's head (the guard of
+        dgga/solver.py:617-644, run by its __init__ for every dense solver,
+        :478-484).  EPANET's pswitch (hydsolver.c:268-355) re-checks them after
+        convergence and changes the link's status or setting; the batched
+        status machine has no counterpart, so solving would be silently wrong.
+        Controls on tank levels and timers belong to the extended-period
+        driver and pass."""
+        net = self.net
+        if net.ctl_node is None or len(net.ctl_node) == 0:
+            return
+        cn = np.asarray(net.ctl_node, dtype=np.int64)
+        ct = np.asarray(net.ctl_type, dtype=np.int64)
+        hit = (cn >= 0) & (ct < 2) & (nt[np.where(cn >= 0, cn, 0)] == JUNCTION)
+        if hit.any():
+            i = int(np.nonzero(hit)[0][0])
+            raise NotImplementedError(
+                "mode=dense: control #%d watches the head of junction %s (link %s); "
+                "pswitch (EPANET hydsolver.c:268-355) would change that link's status "
+                "or setting after convergence, and the batched status machine has no "
+                "counterpart: use the scalar EPANET path"
+                % (i, net.node_id[int(cn[i])], net.link_id[int(net.ctl_link[i])]))
+
     # record_schedule: status changes logged per scenario before a re-run
     # with room for all of them (None: 4 L + 64)
     sched_cap = None
@@ -96,6 +119,7 @@ class GGASolver:
             raise NotImplementedError("only demand-driven analysis (DDA)")
         if float(net.meta.get("damp_limit", 0.0)) > 0.0:
             raise NotImplementedError("DAMPLIMIT > 0 is not supported")
+        self._check_pswitch(nt)
         self.junc_nodes = np.where(nt == JUNCTION)[0]
         self.fixed_nodes = np.where(nt != JUNCTION)[0]
         self.Nj = int(self.junc_nodes.size)
