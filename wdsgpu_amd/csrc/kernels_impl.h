@@ -546,6 +546,21 @@ __device__ inline void solve_bwd(const DevPlan& pl, Scn& c) {
 #endif
 }
 
+// Stage cycle counters for a profiling build (-DWDS_PROFILE_STAGES): lane 0
+// accumulates clock64() deltas of each stage into misc[400 + stage].
+#ifdef WDS_PROFILE_STAGES
+#define WDS_T0() long long wds_t0_ = clock64()
+#define WDS_TACC(c, i)                                              \
+  do {                                                              \
+    const long long t1_ = clock64();                                \
+    if (threadIdx.x == 0) (c).misc[400 + (i)] += (double)(t1_ - wds_t0_); \
+    wds_t0_ = t1_;                                                  \
+  } while (0)
+#else
+#define WDS_T0() (void)0
+#define WDS_TACC(c, i) (void)0
+#endif
+
 // X = A^-1 b (b global, slot order; X in LDS, not S) with n_refine steps of
 // refinement on the double-double residual of the assembled A.  With nw > 1
 // wavefronts the copies and residuals are shared and wavefront 0 runs the
@@ -554,27 +569,36 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
                                 int n_refine, int w = 0, int nw = 1) {
   const int lane = lane_id();
   double* const S = c.R;
+  WDS_T0();
   vec_copy(S, bsrc, nullptr, pl.Nj, w, nw);
   lds_sync();
   if (nw > 1) wg_sync();
+  WDS_TACC(c, 28);
   if (w == 0) {
     solve_fwd(pl, c);
+    WDS_TACC(c, 29);
     solve_bwd(pl, c);
+    WDS_TACC(c, 30);
     for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
     lds_sync();
   }
   for (int it = 0; it < n_refine; ++it) {
     if (nw > 1) wg_sync();
+    WDS_TACC(c, 31);
     residual_all(pl, c.A, X, bsrc, S, w, nw);
     lds_sync();
     if (nw > 1) wg_sync();
+    WDS_TACC(c, 27);
     if (w == 0) {
       solve_fwd(pl, c);
+      WDS_TACC(c, 29);
       solve_bwd(pl, c);
+      WDS_TACC(c, 30);
       for (int j = lane; j < pl.Nj; j += 64) X[j] += S[j];
       lds_sync();
     }
   }
+  WDS_TACC(c, 31);
 }
 
 // Numeric factorisation, then the unit-factor scaling l_ij = w_ij / D_j of
@@ -851,26 +875,23 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
   }
 }
 
-// Stage cycle counters for a profiling build (-DWDS_PROFILE_STAGES): lane 0
-// accumulates clock64() deltas of each stage into misc[400 + stage].
-#ifdef WDS_PROFILE_STAGES
-#define WDS_T0() long long wds_t0_ = clock64()
-#define WDS_TACC(c, i)                                              \
-  do {                                                              \
-    const long long t1_ = clock64();                                \
-    if (threadIdx.x == 0) (c).misc[400 + (i)] += (double)(t1_ - wds_t0_); \
-    wds_t0_ = t1_;                                                  \
-  } while (0)
-#else
-#define WDS_T0() (void)0
-#define WDS_TACC(c, i) (void)0
-#endif
 
 // The scenario's LDS word that carries wavefront 0's decisions (a
 // non-positive pivot, the end of the loop) to the other wavefronts: the
 // spare slot after the factor's zero slot.
 __device__ __forceinline__ lds_f64& wave_flag(const DevPlan& pl, Scn& c) {
   return ((lds_f64*)c.V)[pl.nnzL + 1];
+}
+
+// wavefront 0's v on every wavefront of the workgroup (two workgroup
+// barriers: the flag slot is free again on return)
+__device__ inline int wave0_int(const DevPlan& pl, Scn& c, int v, int w, int nw) {
+  if (nw == 1) return v;
+  if (w == 0 && lane_id() == 0) wave_flag(pl, c) = (double)v;
+  wg_sync();
+  const int r = (int)wave_flag(pl, c);
+  wg_sync();
+  return r;
 }
 
 // one Newton iteration; returns relerr (on wavefront 0), sets bad on a
@@ -886,10 +907,10 @@ __device__ inline double newton_iteration(const DevPlan& pl, Scn& c, const Solve
   WDS_TACC(c, 0);
   k2_assemble<true>(pl, c, w, nw);
   if (nw > 1) wg_sync();
-  WDS_TACC(c, 104);
+  WDS_TACC(c, 1);
   if (w == 0) {
     k2_prv(pl, c);
-    WDS_TACC(c, 105);
+    WDS_TACC(c, 7);
     bad = k3_factor(pl, c);
     if (nw > 1 && lane_id() == 0) wave_flag(pl, c) = bad ? 1.0 : 0.0;
   }
@@ -1102,8 +1123,8 @@ __device__ inline void init_state(const DevPlan& pl, Scn& c, const SolveParams& 
 }
 
 // Fused forward: the whole Newton loop of scenario b, its layout at lds.
-// The workgroup's wavefronts (blockDim.x / 64) share the Newton iterations;
-// the status machine, the polish and the outputs are wavefront 0's.
+// The workgroup's wavefronts (blockDim.x / 64) share the Newton iterations
+// and the polish; the status machine and the outputs are wavefront 0's.
 __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolveParams& sp, int b,
                                       double* lds) {
   const int lane = lane_id();
@@ -1147,19 +1168,22 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
       break;
     }
   }
-  // the other wavefronts are done: the polish, the terminal factor and the
-  // outputs are wavefront 0's (no workgroup barrier follows)
-  if (w != 0) return;
+  // every wavefront has read the last flag before the polish writes it again
+  if (nw > 1) wg_sync();
   if (st.active && st.info == 0) st.iters = maxtrials + 1;
-  // the last assembled matrix is still in A (the polish assembles its own)
-  if (pl.p > 0 && sp.diag_ratio) dratio = diag_ratio_now(pl, c);
+  // the last assembled matrix is still in A (the polish assembles its own
+  // after a workgroup barrier)
+  if (w == 0 && pl.p > 0 && sp.diag_ratio) dratio = diag_ratio_now(pl, c);
   double resid = 0.0;
+  // st.info is uniform: a bad pivot reached every wavefront
   if (st.info == 0 && sp.polish_steps >= 0) {
     WDS_T0();
-    const int pinfo = polish_and_factor(pl, c, sp, b, resid);
+    const int pinfo = polish_and_factor(pl, c, sp, b, resid, w, nw);
     WDS_TACC(c, 6);
     if (pinfo) st.info = pinfo;
   }
+  // the outputs are wavefront 0's (no workgroup barrier follows)
+  if (w != 0) return;
   write_outputs(pl, c, sp, b);
   if (lane == 0) {
     sp.iters[b] = st.iters;

@@ -24,13 +24,17 @@ namespace wds {
 // (_adjgpu_coeffs, dgga/autodiff.py:786-875): link P/Y with pcv PRVs taking
 // their prvcoeff values (ACTIVE -> P = 0), emitter hloss/hgrad.  Writes P, Y
 // (global), hle/hge and the per-valve ACTIVE mask into sh[0..p).
-__device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
+// With nw > 1 wavefronts the link and emitter loops are shared and
+// wavefront 0 writes the valve rows after a workgroup barrier (they
+// overwrite what the link loop wrote for the valve links).
+__device__ inline void polish_coeffs(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
   const int lane = lane_id();
-  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y);
-  emitter_coeffs(pl, c);
+  link_coeffs_all(pl, c.q, c.r, c.S, c.P, c.Y, w, nw);
+  emitter_coeffs(pl, c, w, nw);
   mem_sync();
+  if (nw > 1) wg_sync();
   const int i = lane;
-  if (i < pl.p) {
+  if (w == 0 && i < pl.p) {
     const int k = pl.prv_k[i];
     const double qk = c.q[k];
     const int8_t sk = c.S[k];
@@ -48,9 +52,12 @@ __device__ inline void polish_coeffs(const DevPlan& pl, Scn& c) {
 }
 
 // Assemble A-hat = A(P) + diag(em/hge) + CBIG * ACTIVE at j2 into A and V
-__device__ inline void polish_assemble(const DevPlan& pl, Scn& c) {
-  k2_assemble<false>(pl, c);
-  if (lane_id() == 0) {
+// (rows shared by the nw wavefronts; the ACTIVE pins by wavefront 0 after a
+// workgroup barrier)
+__device__ inline void polish_assemble(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
+  k2_assemble<false>(pl, c, w, nw);
+  if (nw > 1) wg_sync();
+  if (w == 0 && lane_id() == 0) {
     for (int i = 0; i < pl.p; ++i) {
       if (c.sh[i] != 0.0) {
         const int d2 = pl.diag_pos[pl.prv_s2[i]];
@@ -215,9 +222,12 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   double* w = c.sw;            // [n]
   double* x0 = c.sx0;          // [2p]
   double* const S = c.R;
+  WDS_T0();
   vec_copy(S, rr, nullptr, pl.Nj);
   lds_sync();
+  WDS_TACC(c, 23);
   solve_fwd(pl, c);
+  WDS_TACC(c, 24);
   wood_probe(pl, c, S, x0);
   const int n = adjoint ? p : 2 * p;
   // capacitance rows, one entry per lane: row a < p is valve a's replaced
@@ -253,7 +263,9 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   lds_sync();
   if (!small_solve(M, w, n)) return false;
   wood_apply(pl, c, S, w, n);
+  WDS_TACC(c, 25);
   solve_bwd(pl, c);
+  WDS_TACC(c, 26);
   return true;
 }
 
@@ -310,9 +322,9 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   mem_sync();
   WDS_T0();
   wood_paths(pl, c);
-  WDS_TACC(c, 110);
+  WDS_TACC(c, 32);
   if (!wood_solve(pl, c, rp, false)) return false;
-  WDS_TACC(c, 111);
+  WDS_TACC(c, 33);
   for (int j = lane; j < pl.Nj; j += 64) dst[j] = c.R[j];
   lds_sync();
   // two refinement steps on the residual of the true replaced system
@@ -359,11 +371,15 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
 // link residuals r_l (into Y), PRV residuals (sprv), emitter residual
 // r_e (into v4 when want_re) and node mass m -> rhs_p in v1
 // (_adjgpu_residuals + the right-hand side of _adjgpu_polish,
-// dgga/autodiff.py:1032-1050, 1066-1075).  Returns the inf-norm residual.
-__device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_rhs) {
+// dgga/autodiff.py:1032-1050, 1066-1075).  Returns the inf-norm residual
+// (on wavefront 0).  With nw > 1 wavefronts the link loop and the mass row
+// blocks are shared (a workgroup barrier between them: the mass rows read
+// every link's residual) and the maxima meet in LDS.
+__device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_rhs, int w = 0,
+                                          int nw = 1) {
   const int lane = lane_id();
   double rmax = 0.0;
-  for (int k0 = lane; k0 < pl.L; k0 += 64 * kGather) {
+  for (int k0 = lane + 64 * kGather * w; k0 < pl.L; k0 += 64 * kGather * nw) {
     double Pv[kGather], Yv[kGather];
     int s1[kGather], s2[kGather];
 #pragma unroll
@@ -386,26 +402,30 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
     }
   }
   double* rprv = c.sprv;
-  if (lane < pl.p) {
+  if (w == 0 && lane < pl.p) {
     const double rp = (c.Hn[pl.prv_s2[lane]] - pl.prv_hset[lane]) * c.sh[lane];
     rprv[lane] = rp;
     rmax = fmax(rmax, fabs(rp));
   }
   mem_sync();
   lds_sync();
+  if (nw > 1) wg_sync();
   // mass rows over the polish row layout: a row's links with n2 == j first
   // (reference scatter order), then those with n1 == j negated; two 64-row
-  // blocks per pass, every load of a pass issued before use
+  // blocks per pass, every load of a pass issued before use; the blocks in
+  // nw contiguous shares of about equal entry counts
   const int nb = (pl.Nj + 63) >> 6;
+  const int bw0 = block_split(pl.ps_blk, nb, pl.ps_blk[nb], w, nw);
+  const int bw1 = block_split(pl.ps_blk, nb, pl.ps_blk[nb], w + 1, nw);
   const lds_f64* Hl = (const lds_f64*)c.Hn;
-  for (int b0 = 0; b0 < nb; b0 += 2) {
+  for (int b0 = bw0; b0 < bw1; b0 += 2) {
     int e0[2], K[2], len[2], jr[2];
     double kv[2], hlv[2], hgv[2], ev[2], djv[2], elv[2], hn[2];
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      const int bk = min(b0 + t, nb - 1);
+      const int bk = min(b0 + t, bw1 - 1);
       e0[t] = pl.ps_blk[bk];
-      K[t] = b0 + t < nb ? pl.ps_blk[bk + 1] - e0[t] : 0;
+      K[t] = b0 + t < bw1 ? pl.ps_blk[bk + 1] - e0[t] : 0;
       jr[t] = bk * 64 + lane;
       const int j = min(jr[t], pl.Nj - 1);
       len[t] = pl.ps_len[j];
@@ -452,7 +472,7 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
     }
 #pragma unroll
     for (int t = 0; t < 2; ++t) {
-      if (!(b0 + t < nb && jr[t] < pl.Nj)) continue;
+      if (!(b0 + t < bw1 && jr[t] < pl.Nj)) continue;
       double mm = m[t], re = 0.0, emh = 0.0;
       if (kv[t] > 0.0) {
         re = hlv[t] - (hn[t] - elv[t]);
@@ -469,7 +489,18 @@ __device__ inline double polish_residuals(const DevPlan& pl, Scn& c, bool build_
     }
   }
   mem_sync();
-  return wave_max(rmax);
+  double m = wave_max(rmax);
+  if (nw > 1) {
+    // the maxima of the other wavefronts, one at a time through the flag
+    // slot (a maximum is exact in any order)
+    for (int v = 1; v < nw; ++v) {
+      if (w == v && lane == 0) wave_flag(pl, c) = m;
+      wg_sync();
+      if (w == 0) m = fmax(m, wave_flag(pl, c));
+      wg_sync();
+    }
+  }
+  return m;
 }
 
 // A CONST_HP pump on its CBIG / RQtol clamp branch: the assembled P is
@@ -494,28 +525,41 @@ __device__ inline bool chp_on_clamp(const DevPlan& pl, const Scn& c) {
 }
 
 // Polish steps + terminal factor (_adjgpu_polish, dgga/autodiff.py:1053-1104).
-// Returns 0, or a negative info code on failure.
+// Returns 0, or a negative info code on failure (on every wavefront).  With
+// nw > 1 wavefronts the coefficient, assembly, residual and update loops are
+// shared; wavefront 0 alone factors, runs the Woodbury primal solve and
+// applies the valve rows, and its outcome reaches the others through the
+// flag slot, so every wavefront returns at the same point.
 __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolveParams& sp, int b,
-                                        double& resid) {
+                                        double& resid, int w = 0, int nw = 1) {
   const int lane = lane_id();
   double* dH = c.X2;  // LDS
   for (int step = 0; step < sp.polish_steps; ++step) {
     WDS_T0();
-    if (chp_on_clamp(pl, c)) return kInfoChpClamp;
-    polish_coeffs(pl, c);
-    polish_assemble(pl, c);
+    int fail = 0;
+    if (w == 0 && chp_on_clamp(pl, c)) fail = kInfoChpClamp;
+    fail = wave0_int(pl, c, fail, w, nw);
+    if (fail) return fail;
+    polish_coeffs(pl, c, w, nw);
+    if (nw > 1) wg_sync();
+    polish_assemble(pl, c, w, nw);
     WDS_TACC(c, 104);
-    if (k3_factor(pl, c)) return -(step + 1);
+    if (w == 0 && k3_factor(pl, c)) fail = -(step + 1);
+    fail = wave0_int(pl, c, fail, w, nw);
+    if (fail) return fail;
     WDS_TACC(c, 105);
-    polish_residuals(pl, c, true);
+    polish_residuals(pl, c, true, w, nw);
     // v4 holds r_e; keep it in F (free during the polish) before primal_solve
-    vec_copy(c.F, c.v4, nullptr, pl.Nj);
+    vec_copy(c.F, c.v4, nullptr, pl.Nj, w, nw);
     mem_sync();
+    if (nw > 1) wg_sync();
     WDS_TACC(c, 106);
-    if (!primal_solve(pl, c, dH)) return -100 - step;
+    if (w == 0 && !primal_solve(pl, c, dH)) fail = -100 - step;
+    fail = wave0_int(pl, c, fail, w, nw);
+    if (fail) return fail;
     WDS_TACC(c, 107);
     // dq = -P (r_l + dH[j2] - dH[j1]); ACTIVE PRVs from the downstream row
-    for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG8) {
+    for (int k0 = lane + 64 * kRowG8 * w; k0 < pl.L; k0 += 64 * kRowG8 * nw) {
       int fv[kRowG8], s1[kRowG8], s2[kRowG8];
       double Pv[kRowG8], Yv[kRowG8];
 #pragma unroll
@@ -547,17 +591,21 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
       // applies rhs_p - Abar dH, the opposite sign: that doubles the j2 mass
       // residual every step instead of removing it; this follows the full
       // Newton step of the CPU polish (dgga/autodiff.py:473-491).
-      matvec(pl, c, dH, c.v3);
-      if (lane == 0)
-        for (int i = 0; i < pl.p; ++i) {
-          if (c.sh[i] == 0.0) continue;
-          const int j2 = pl.prv_s2[i];
-          const double abar2 = c.v3[j2] - CBIG * dH[j2];
-          c.P[pl.prv_k[i]] = abar2 - c.v1[j2];
-        }
-      mem_sync();
+      if (nw > 1) wg_sync();
+      if (w == 0) {
+        matvec(pl, c, dH, c.v3);
+        if (lane == 0)
+          for (int i = 0; i < pl.p; ++i) {
+            if (c.sh[i] == 0.0) continue;
+            const int j2 = pl.prv_s2[i];
+            const double abar2 = c.v3[j2] - CBIG * dH[j2];
+            c.P[pl.prv_k[i]] = abar2 - c.v1[j2];
+          }
+        mem_sync();
+      }
     }
-    for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG) {
+    if (nw > 1) wg_sync();
+    for (int k0 = lane + 64 * kRowG * w; k0 < pl.L; k0 += 64 * kRowG * nw) {
       double qv[kRowG], dv[kRowG];
 #pragma unroll
       for (int u = 0; u < kRowG; ++u) {
@@ -569,7 +617,7 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
       for (int u = 0; u < kRowG; ++u)
         if (k0 + 64 * u < pl.L) c.q[k0 + 64 * u] = qv[u] + dv[u];
     }
-    for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+    for (int j0 = lane + 64 * kRowG8 * w; j0 < pl.Nj; j0 += 64 * kRowG8 * nw) {
       double kv[kRowG8], ev[kRowG8], fv[kRowG8], hgv[kRowG8];
 #pragma unroll
       for (int u = 0; u < kRowG8; ++u) {
@@ -589,19 +637,26 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
     }
     mem_sync();
     lds_sync();
+    if (nw > 1) wg_sync();
     WDS_TACC(c, 108);
   }
   // terminal state: coefficients, factor kept for the adjoint, residual
   WDS_T0();
-  if (chp_on_clamp(pl, c)) return kInfoChpClamp;
-  polish_coeffs(pl, c);
-  polish_assemble(pl, c);
-  if (k3_factor(pl, c)) return -99;
+  int fail = 0;
+  if (w == 0 && chp_on_clamp(pl, c)) fail = kInfoChpClamp;
+  fail = wave0_int(pl, c, fail, w, nw);
+  if (fail) return fail;
+  polish_coeffs(pl, c, w, nw);
+  if (nw > 1) wg_sync();
+  polish_assemble(pl, c, w, nw);
+  if (w == 0 && k3_factor(pl, c)) fail = -99;
+  fail = wave0_int(pl, c, fail, w, nw);
+  if (fail) return fail;
   if (sp.factor) {
     double* fo = sp.factor + (int64_t)b * pl.nnzL;
-    for (int t = lane; t < pl.nnzL; t += 64) fo[t] = c.V[t];
+    for (int t = lane + 64 * w; t < pl.nnzL; t += 64 * nw) fo[t] = c.V[t];
   }
-  resid = polish_residuals(pl, c, false);
+  resid = polish_residuals(pl, c, false, w, nw);
   WDS_TACC(c, 109);
   return 0;
 }
