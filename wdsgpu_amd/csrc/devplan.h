@@ -42,11 +42,20 @@ enum LinkFlag : int32_t {
 WDS_HD inline int64_t lds_sink_index(int64_t nnzL, int64_t Nj, int64_t Nf) {
   return nnzL + 2 + (Nj + Nf) + 2 * Nj + 2 * kMaxPrv;
 }
+// Emitter rows per lane that the Newton update reads from a per-scenario list
+// (rows r = lane + 64 i with an emitter, in ascending r); a scenario with more
+// in some lane walks every row instead.
+constexpr int kEmPass = 2;
+// the list: int32 [kEmPass * 64] slots (-1 past a lane's rows), then the
+// wavefront's largest count, padded to whole doubles
+constexpr int kEmDoubles = 32 * kEmPass + 2;
+
 // after the sinks: the Woodbury block sized by the plan's np = 2p valve ends
 // (valve-end paths y, z [wp_tot] each, capacitance [np^2], the small dense
-// system [np^2], its right-hand side [np], probes [np], valve residuals [p])
+// system [np^2], its right-hand side [np], probes [np], valve residuals [p]),
+// then the emitter row list
 WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
-  return 2 * wp_tot + 2 * np * np + 2 * np + np / 2;
+  return 2 * wp_tot + 2 * np * np + 2 * np + np / 2 + kEmDoubles;
 }
 
 // solve programs: the store word of a level's last step carries the level

@@ -163,6 +163,7 @@ struct Scn {
   double* sw;   // LDS [np] its right-hand side / solution
   double* sx0;  // LDS [np] probes e_u^T A-hat^-1 r
   double* sprv; // LDS [p] valve residuals (polish) / valve cotangents (adjoint)
+  int32_t* emj; // LDS [kEmPass * 64 + 1] emitter rows by lane (load_inputs)
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
   // per-scenario inputs / state
@@ -241,6 +242,8 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.sx0 = sm;
   sm += pl.np;
   c.sprv = sm;
+  sm += pl.np / 2;
+  c.emj = reinterpret_cast<int32_t*>(sm);
   c.st0 = reinterpret_cast<int8_t*>(w);
   w += ((int64_t)L + 7) / 8;
   c.misc = w;
@@ -657,24 +660,53 @@ __device__ inline double k5_update(const DevPlan& pl, Scn& c, double hacc) {
   // emitter rows in original junction order (row r = lane + 64 i), so the
   // reduction tree below is defined on file order: lane partials over links
   // k = lane + 64 i and rows r = lane + 64 i in ascending order, then an xor
-  // butterfly (m = 32 .. 1)
-  for (int r0 = lane; r0 < pl.Nj; r0 += 64 * kRowG8) {
-    int jv[kRowG8];
+  // butterfly (m = 32 .. 1).  The rows with an emitter come from the list
+  // load_inputs made (same rows, same order per lane; one memory round
+  // trip), unless some lane has more than kEmPass of them.
+  const int npass = c.emj[kEmPass * 64];
+  if (npass <= kEmPass) {
+    int jv[kEmPass];
 #pragma unroll
-    for (int u = 0; u < kRowG8; ++u) jv[u] = pl.row_slot[min(r0 + 64 * u, pl.Nj - 1)];
-    double kv[kRowG8];
+    for (int u = 0; u < kEmPass; ++u) jv[u] = c.emj[u * 64 + lane];
+    double hl[kEmPass], hg[kEmPass], ev[kEmPass], hn[kEmPass], el[kEmPass];
 #pragma unroll
-    for (int u = 0; u < kRowG8; ++u) kv[u] = c.kej[jv[u]];
+    for (int u = 0; u < kEmPass; ++u) {
+      const int j = max(jv[u], 0);
+      hl[u] = c.hle[j];
+      hg[u] = c.hge[j];
+      ev[u] = c.e[j];
+      hn[u] = c.Hn[j];
+      el[u] = pl.el_slot[j];
+    }
 #pragma unroll
-    for (int u = 0; u < kRowG8; ++u) {
-      const int j = jv[u];
-      if (r0 + 64 * u < pl.Nj && kv[u] > 0.0) {
-        const double dh = c.Hn[j] - pl.el_slot[j];
-        const double dq = (c.hle[j] - dh) / c.hge[j];
-        const double en = c.e[j] - dq;
-        dqe += fabs(dq);
-        qse += fabs(en);
-        c.e[j] = en;
+    for (int u = 0; u < kEmPass; ++u) {
+      if (jv[u] < 0) continue;
+      const double dh = hn[u] - el[u];
+      const double dq = (hl[u] - dh) / hg[u];
+      const double en = ev[u] - dq;
+      dqe += fabs(dq);
+      qse += fabs(en);
+      c.e[jv[u]] = en;
+    }
+  } else {
+    for (int r0 = lane; r0 < pl.Nj; r0 += 64 * kRowG8) {
+      int jv[kRowG8];
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) jv[u] = pl.row_slot[min(r0 + 64 * u, pl.Nj - 1)];
+      double kv[kRowG8];
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) kv[u] = c.kej[jv[u]];
+#pragma unroll
+      for (int u = 0; u < kRowG8; ++u) {
+        const int j = jv[u];
+        if (r0 + 64 * u < pl.Nj && kv[u] > 0.0) {
+          const double dh = c.Hn[j] - pl.el_slot[j];
+          const double dq = (c.hle[j] - dh) / c.hge[j];
+          const double en = c.e[j] - dq;
+          dqe += fabs(dq);
+          qse += fabs(en);
+          c.e[j] = en;
+        }
       }
     }
   }
@@ -873,6 +905,26 @@ __device__ inline void load_inputs(const DevPlan& pl, Scn& c, const SolveParams&
     const double h = rh[pl.slot_node[pl.Nj + f]];
     c.Hn[pl.Nj + f] = (h != h) ? 0.0 : h;  // NaN reads as 0 (torch.nan_to_num)
   }
+  // the emitter rows of each lane (rows r = lane + 64 i in ascending r), by
+  // slot, for k5_update; kej was stored above by this wavefront
+  int n = 0;
+  for (int r0 = lane; r0 < pl.Nj; r0 += 64 * kRowG) {
+    int jv[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) jv[u] = pl.row_slot[min(r0 + 64 * u, pl.Nj - 1)];
+    double kv[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) kv[u] = c.kej[jv[u]];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) {
+      if (r0 + 64 * u >= pl.Nj || !(kv[u] > 0.0)) continue;
+      if (n < kEmPass) c.emj[n * 64 + lane] = jv[u];
+      ++n;
+    }
+  }
+  for (int u = n; u < kEmPass; ++u) c.emj[u * 64 + lane] = -1;
+  const int nmax = (int)wave_max((double)n);
+  if (lane == 0) c.emj[kEmPass * 64] = nmax;
 }
 
 
