@@ -53,9 +53,9 @@ constexpr int kEmDoubles = 32 * kEmPass + 2;
 // after the sinks: the Woodbury block sized by the plan's np = 2p valve ends
 // (valve-end paths y, z [wp_tot] each, capacitance [np^2], the small dense
 // system [np^2], its right-hand side [np], probes [np], valve residuals [p]),
-// then the emitter row list
+// then the emitter row list, then the path nodes' slots (int32 [wp_tot])
 WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
-  return 2 * wp_tot + 2 * np * np + 2 * np + np / 2 + kEmDoubles;
+  return 2 * wp_tot + 2 * np * np + 2 * np + np / 2 + kEmDoubles + (wp_tot + 1) / 2;
 }
 
 // solve programs: the store word of a level's last step carries the level

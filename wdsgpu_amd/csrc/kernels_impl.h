@@ -164,6 +164,7 @@ struct Scn {
   double* sx0;  // LDS [np] probes e_u^T A-hat^-1 r
   double* sprv; // LDS [p] valve residuals (polish) / valve cotangents (adjoint)
   int32_t* emj; // LDS [kEmPass * 64 + 1] emitter rows by lane (load_inputs)
+  int32_t* wpn; // LDS [wp_tot] slots of the valve-end path nodes (wood_paths)
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
   // per-scenario inputs / state
@@ -244,6 +245,8 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.sprv = sm;
   sm += pl.np / 2;
   c.emj = reinterpret_cast<int32_t*>(sm);
+  sm += kEmDoubles;
+  c.wpn = reinterpret_cast<int32_t*>(sm);
   c.st0 = reinterpret_cast<int8_t*>(w);
   w += ((int64_t)L + 7) / 8;
   c.misc = w;

@@ -134,6 +134,16 @@ __device__ inline bool small_solve(double* M, double* w, int n) {
 // ancestors of u and v.
 __device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
   const int lane = lane_id();
+  // the path nodes' slots into LDS for the probes and corrections of the
+  // solves that follow (their loads overlap the first path's)
+  for (int i0 = lane; i0 < pl.wp_tot; i0 += 64 * kRowG) {
+    int nd[kRowG];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u) nd[u] = pl.wp_node[min(i0 + 64 * u, pl.wp_tot - 1)];
+#pragma unroll
+    for (int u = 0; u < kRowG; ++u)
+      if (i0 + 64 * u < pl.wp_tot) c.wpn[i0 + 64 * u] = nd[u];
+  }
   if (lane < pl.np) {
     const int i0 = pl.wp_ptr[lane], i1 = pl.wp_ptr[lane + 1];
     for (int i = i0; i < i1; ++i) {
@@ -157,18 +167,18 @@ __device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
 }
 
 // e_u^T A-hat^-1 r for every valve end u, from z = D^-1 L^-1 r in X (LDS)
-// (one lane per path; the path's node slots load kPathG at a time before
-// any is used, the sum runs in path order)
+// (one lane per path, its range [i0, i1) loaded by the caller; the path's
+// node slots come from LDS kPathG at a time, the sum runs in path order)
 constexpr int kPathG = 16;
-__device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, double* x0) {
+__device__ inline void wood_probe(const DevPlan& pl, Scn& c, const double* X, double* x0,
+                                  int i0, int i1) {
   const int lane = lane_id();
   if (lane < pl.np) {
-    const int i0 = pl.wp_ptr[lane], i1 = pl.wp_ptr[lane + 1];
     double s = 0.0;
     for (int b = i0; b < i1; b += kPathG) {
       int nd[kPathG];
 #pragma unroll
-      for (int u = 0; u < kPathG; ++u) nd[u] = pl.wp_node[min(b + u, i1 - 1)];
+      for (int u = 0; u < kPathG; ++u) nd[u] = c.wpn[min(b + u, i1 - 1)];
 #pragma unroll
       for (int u = 0; u < kPathG; ++u)
         if (b + u < i1) s = fma(c.Yp[b + u], X[nd[u]], s);
@@ -190,7 +200,7 @@ __device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const do
     // the columns then run in order from registers and LDS
     int nd[kApG];
 #pragma unroll
-    for (int u = 0; u < kApG; ++u) nd[u] = pl.wp_node[min(lane + 64 * u, pl.wp_tot - 1)];
+    for (int u = 0; u < kApG; ++u) nd[u] = c.wpn[min(lane + 64 * u, pl.wp_tot - 1)];
     for (int col = 0; col < ncol; ++col) {
       const double wc = w[col];
       const int c0 = pl.wp_ptr[col], c1 = pl.wp_ptr[col + 1];
@@ -206,7 +216,7 @@ __device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const do
   for (int col = 0; col < ncol; ++col) {
     const double wc = w[col];
     for (int i = pl.wp_ptr[col] + lane; i < pl.wp_ptr[col + 1]; i += 64)
-      X[pl.wp_node[i]] -= wc * c.Zp[i];
+      X[c.wpn[i]] -= wc * c.Zp[i];
     lds_sync();
   }
 }
@@ -223,12 +233,18 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   double* x0 = c.sx0;          // [2p]
   double* const S = c.R;
   WDS_T0();
+  // plan words of the probes and the valve rows, loaded before the copy and
+  // the forward solve so that their latency hides under them
+  const int i0 = pl.wp_ptr[min(lane, pl.np - 1)], i1 = pl.wp_ptr[min(lane, pl.np - 1) + 1];
+  const int s2v = pl.prv_s2[min(lane, p - 1)];
   vec_copy(S, rr, nullptr, pl.Nj);
   lds_sync();
+  // rr at the valves' downstream rows: S holds rr until the forward solve
+  const double r2v = S[s2v];
   WDS_TACC(c, 23);
   solve_fwd(pl, c);
   WDS_TACC(c, 24);
-  wood_probe(pl, c, S, x0);
+  wood_probe(pl, c, S, x0, i0, i1);
   const int n = adjoint ? p : 2 * p;
   // capacitance rows, one entry per lane: row a < p is valve a's replaced
   // downstream row; rows p + a (primal only) its merged upstream row
@@ -252,7 +268,7 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   if (lane < p) {
     const int a = lane;
     const double act = c.sh[a];
-    const double r2 = rr[pl.prv_s2[a]];
+    const double r2 = r2v;
     if (adjoint) {
       w[a] = (x0[a] - x0[p + a] - r2) * act;
     } else {
