@@ -331,7 +331,15 @@ __device__ __forceinline__ lds_f64& lds_at(const char* base, int byte_addr) {
 // last step (the lane's sink otherwise).  The store word of a level's last
 // step carries the level end and the lane's log2 group width in its high
 // bits.
-__device__ inline void run_dot(const ProgDev& pg, const double* arena,
+// The ring of a program's first kRing steps, loaded ahead of its run (by the
+// caller, before other work whose time covers the loads' latency); nothing
+// for an empty program.
+__device__ __forceinline__ void prefetch_prog(const ProgDev& pg, i32x4 (&ring)[kSlots]) {
+  if (pg.nsteps > 0) fill_ring(ring, (const glb_i32x4*)pg.instr + lane_id());
+}
+
+// run_dot with the ring prefetched (prefetch_prog of the same program)
+__device__ inline void run_dot(const ProgDev& pg, const double* arena, i32x4 (&ring)[kSlots],
                                double* prof = nullptr) {
   const char* ab = (const char*)arena;
   const int lane = lane_id();
@@ -339,8 +347,6 @@ __device__ inline void run_dot(const ProgDev& pg, const double* arena,
   if (ns == 0) return;
   const int sent = pg.dummy + 8 * lane;  // byte address of this lane's sink slot
   const glb_i32x4* ip = (const glb_i32x4*)pg.instr + lane;
-  i32x4 ring[kSlots];
-  fill_ring(ring, ip);
   double acc = 0.0;
   int fin = sent;  // grouped tasks: byte address of the slice's target
 #ifdef WDS_PROFILE_STAGES

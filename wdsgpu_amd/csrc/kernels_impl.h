@@ -531,24 +531,27 @@ __device__ inline void k2_prv(const DevPlan& pl, Scn& c) {
 // b is read from global (bsrc, slot order), x is left in X (LDS).
 // Solve programs on the solve vector S = c.R with the unit factor in V.
 // Forward y = L^-1 S then the pivot scaling z = D^-1 y (in place):
-__device__ inline void solve_fwd(const DevPlan& pl, Scn& c) {
+__device__ inline void solve_fwd(const DevPlan& pl, Scn& c, i32x4 (&ring)[kSlots]) {
   int dp[kRowG];
   pivots_prefetch(pl, dp);
 #ifdef WDS_PROFILE_STAGES
-  run_dot(pl.fwd, c.V, c.misc + 408 + 32);
+  run_dot(pl.fwd, c.V, ring, c.misc + 408 + 32);
 #else
-  run_dot(pl.fwd, c.V);
+  run_dot(pl.fwd, c.V, ring);
 #endif
+  // the backward program's first steps load while the pivots scale
+  prefetch_prog(pl.bwd, ring);
   scale_by_pivots(pl, c.V, c.R, dp);
   lds_sync();
 }
 
-// backward x = L^-T z in place in S
-__device__ inline void solve_bwd(const DevPlan& pl, Scn& c) {
+// backward x = L^-T z in place in S; ring holds the backward program's
+// first steps (left there by solve_fwd)
+__device__ inline void solve_bwd(const DevPlan& pl, Scn& c, i32x4 (&ring)[kSlots]) {
 #ifdef WDS_PROFILE_STAGES
-  run_dot(pl.bwd, c.V, c.misc + 408 + 64);
+  run_dot(pl.bwd, c.V, ring, c.misc + 408 + 64);
 #else
-  run_dot(pl.bwd, c.V);
+  run_dot(pl.bwd, c.V, ring);
 #endif
 }
 
@@ -576,14 +579,18 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
   const int lane = lane_id();
   double* const S = c.R;
   WDS_T0();
+  // the forward program's first steps load under the copy (and under each
+  // refinement residual)
+  i32x4 ring[kSlots];
+  if (w == 0) prefetch_prog(pl.fwd, ring);
   vec_copy(S, bsrc, nullptr, pl.Nj, w, nw);
   lds_sync();
   if (nw > 1) wg_sync();
   WDS_TACC(c, 28);
   if (w == 0) {
-    solve_fwd(pl, c);
+    solve_fwd(pl, c, ring);
     WDS_TACC(c, 29);
-    solve_bwd(pl, c);
+    solve_bwd(pl, c, ring);
     WDS_TACC(c, 30);
     for (int j = lane; j < pl.Nj; j += 64) X[j] = S[j];
     lds_sync();
@@ -591,14 +598,15 @@ __device__ inline void k4_solve(const DevPlan& pl, Scn& c, const double* bsrc, d
   for (int it = 0; it < n_refine; ++it) {
     if (nw > 1) wg_sync();
     WDS_TACC(c, 31);
+    if (w == 0) prefetch_prog(pl.fwd, ring);
     residual_all(pl, c.A, X, bsrc, S, w, nw);
     lds_sync();
     if (nw > 1) wg_sync();
     WDS_TACC(c, 27);
     if (w == 0) {
-      solve_fwd(pl, c);
+      solve_fwd(pl, c, ring);
       WDS_TACC(c, 29);
-      solve_bwd(pl, c);
+      solve_bwd(pl, c, ring);
       WDS_TACC(c, 30);
       for (int j = lane; j < pl.Nj; j += 64) X[j] += S[j];
       lds_sync();

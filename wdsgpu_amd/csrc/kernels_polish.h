@@ -237,12 +237,14 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   // the forward solve so that their latency hides under them
   const int i0 = pl.wp_ptr[min(lane, pl.np - 1)], i1 = pl.wp_ptr[min(lane, pl.np - 1) + 1];
   const int s2v = pl.prv_s2[min(lane, p - 1)];
+  i32x4 ring[kSlots];
+  prefetch_prog(pl.fwd, ring);
   vec_copy(S, rr, nullptr, pl.Nj);
   lds_sync();
   // rr at the valves' downstream rows: S holds rr until the forward solve
   const double r2v = S[s2v];
   WDS_TACC(c, 23);
-  solve_fwd(pl, c);
+  solve_fwd(pl, c, ring);
   WDS_TACC(c, 24);
   wood_probe(pl, c, S, x0, i0, i1);
   const int n = adjoint ? p : 2 * p;
@@ -280,7 +282,7 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   if (!small_solve(M, w, n)) return false;
   wood_apply(pl, c, S, w, n);
   WDS_TACC(c, 25);
-  solve_bwd(pl, c);
+  solve_bwd(pl, c, ring);
   WDS_TACC(c, 26);
   return true;
 }
