@@ -381,12 +381,18 @@ static int launch_check(const char* what) {
   return WDS_OK;
 }
 
+// dynamic LDS beyond the default 64 KiB needs the kernel's attribute raised
 template <class K>
-static void set_lds(K kernel, int64_t bytes) {
-  if (bytes > 65536)
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
+static hipError_t set_lds(K kernel, int64_t bytes) {
+  if (bytes <= 65536) return hipSuccess;
+  return hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes);
 }
+#define WDS_SET_LDS(kernel, bytes)                                          \
+  do {                                                                      \
+    const hipError_t le_ = set_lds(kernel, bytes);                          \
+    if (le_ != hipSuccess) return hip_fail(le_, "hipFuncSetAttribute(" #kernel ")"); \
+  } while (0)
 
 extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const double* csr,
                                 const double* F, double* H, int32_t n_refine, double* factor_out,
@@ -402,7 +408,7 @@ extern "C" int wds_factor_solve(wds_plan* P, void* stream, int32_t B, const doub
     arena_factor_solve(P->dev, csr, F, H, n_refine, factor_out, info, (double*)workspace, B,
                        (hipStream_t)stream);
   } else {
-    set_lds(k_factor_solve, lds);
+    WDS_SET_LDS(k_factor_solve, lds);
     hipLaunchKernelGGL(k_factor_solve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
                        F, H, n_refine, factor_out, info, (double*)workspace);
   }
@@ -424,7 +430,7 @@ extern "C" int wds_factor(wds_plan* P, void* stream, int32_t B, const double* cs
   if (P->arena) {
     arena_factor(P->dev, csr, factor_out, info, (double*)workspace, B, (hipStream_t)stream);
   } else {
-    set_lds(k_factor, lds);
+    WDS_SET_LDS(k_factor, lds);
     hipLaunchKernelGGL(k_factor, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, csr,
                        factor_out, info, (double*)workspace);
   }
@@ -447,7 +453,7 @@ extern "C" int wds_trisolve(wds_plan* P, void* stream, int32_t B, const double* 
     arena_trisolve(P->dev, factor, csr, F, H, n_refine, (double*)workspace, B,
                    (hipStream_t)stream);
   } else {
-    set_lds(k_trisolve, lds);
+    WDS_SET_LDS(k_trisolve, lds);
     hipLaunchKernelGGL(k_trisolve, dim3(B), dim3(64), lds, (hipStream_t)stream, P->dev, factor,
                        csr, F, H, n_refine, (double*)workspace);
   }
@@ -510,7 +516,7 @@ extern "C" int wds_solve(wds_plan* P, void* stream, const wds_solve_args* a) {
   if (P->arena) {
     arena_solve(P->dev, sp, a->B, (hipStream_t)stream);
   } else {
-    set_lds(k_solve, lds);
+    WDS_SET_LDS(k_solve, lds);
     hipLaunchKernelGGL(k_solve, dim3(a->B), dim3(64 * kSolveWaves), lds, (hipStream_t)stream,
                        P->dev, sp);
   }
@@ -650,7 +656,8 @@ extern "C" int wds_multi_create(wds_plan* const* plans, int32_t nnet, const int3
       delete M;
       return r;
     }
-    const bool ar = P->arena > 0 || lds_budget <= 0 || P->stats.lds_bytes > lds_budget;
+    const bool ar = P->arena > 0 || lds_budget <= 0 || P->stats.lds_bytes > lds_budget ||
+                    P->stats.lds_bytes + kItemSlotBytes > kMaxLds;
     dp[i] = P->dev;
     dp[i].arena = ar ? (P->arena > 0 ? P->arena : (P->stats.lds_bytes / 8 + 31) / 32 * 32) : 0;
     M->plans.push_back(P);
@@ -681,8 +688,9 @@ extern "C" int wds_multi_create(wds_plan* const* plans, int32_t nnet, const int3
   // one wavefront per SIMD (four per CU), fewer where the LDS layout is
   // larger than a quarter of the CU's LDS
   const int per_cu_lds =
-      M->lds_bytes > 0 ? (int)std::max<int64_t>(1, std::min<int64_t>(4, kMaxLds / M->lds_bytes))
-                       : 4;
+      M->lds_bytes > 0
+          ? (int)std::max<int64_t>(1, std::min<int64_t>(4, kMaxLds / (M->lds_bytes + kItemSlotBytes)))
+          : 4;
   M->grid_lds = std::min(M->n_lds, cus * per_cu_lds);
   M->grid_arena = std::min(M->n_arena, cus * 4);
   auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
@@ -780,9 +788,11 @@ extern "C" int wds_multi_solve(wds_multi* M, void* stream, const wds_solve_args*
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(multi params)");
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, M->d_counter);
   if (M->n_lds) {
-    set_lds(k_solve_multi, M->lds_bytes);
-    hipLaunchKernelGGL(k_solve_multi, dim3(M->grid_lds), dim3(64), M->lds_bytes, st, M->d_plans,
-                       M->d_sp, M->d_items, M->n_lds, M->d_counter);
+    // the item index passes between the wavefronts in a word after the layout
+    WDS_SET_LDS(k_solve_multi, M->lds_bytes + kItemSlotBytes);
+    hipLaunchKernelGGL(k_solve_multi, dim3(M->grid_lds), dim3(64 * kSolveWaves),
+                       M->lds_bytes + kItemSlotBytes, st, M->d_plans, M->d_sp, M->d_items,
+                       M->n_lds, M->d_counter, (int32_t)M->lds_bytes);
   }
   if (M->n_arena)
     arena_solve_multi(M->d_plans, M->d_sp, M->d_items + M->n_lds, M->n_arena,
@@ -815,7 +825,7 @@ extern "C" int wds_multi_adjoint(wds_multi* M, void* stream, const wds_adjoint_a
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(multi params)");
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, M->d_counter);
   if (M->n_lds) {
-    set_lds(k_adjoint_multi, M->lds_bytes);
+    WDS_SET_LDS(k_adjoint_multi, M->lds_bytes);
     hipLaunchKernelGGL(k_adjoint_multi, dim3(M->grid_lds), dim3(64), M->lds_bytes, st,
                        M->d_plans, M->d_ap, M->d_items, M->n_lds, M->d_counter);
   }
@@ -849,7 +859,7 @@ extern "C" int wds_newton_iter(wds_plan* P, void* stream, const wds_solve_args* 
   if (P->arena) {
     arena_newton_iter(P->dev, sp, it, ctrl, a->B, (hipStream_t)stream);
   } else {
-    set_lds(k_newton_iter, lds);
+    WDS_SET_LDS(k_newton_iter, lds);
     hipLaunchKernelGGL(k_newton_iter, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev,
                        sp, it, ctrl);
   }
@@ -875,7 +885,7 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
   if (P->arena) {
     arena_adjoint(P->dev, ap, a->B, (hipStream_t)stream);
   } else {
-    set_lds(k_adjoint, lds);
+    WDS_SET_LDS(k_adjoint, lds);
     hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, ap);
   }
   P->cnt.bwd_solve += 1;

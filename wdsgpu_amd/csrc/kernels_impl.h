@@ -5,6 +5,8 @@
 namespace wds {
 
 constexpr int64_t kMaxLds = kMaxLdsBytes;
+// dynamic LDS after the layouts of the multi-network forward (its item word)
+constexpr int64_t kItemSlotBytes = 16;
 
 // per-scenario global workspace, in doubles (wp_tot path nodes, np paths)
 __host__ __device__ inline int64_t ws_doubles(int32_t L, int32_t Nj, int32_t na,
@@ -1346,12 +1348,29 @@ __device__ __forceinline__ int next_item(int32_t* counter) {
   return __shfl(i, 0, 64);
 }
 
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_solve_multi)(const DevPlan* plans,
-                                                         const SolveParams* params,
-                                                         const int2* items, int32_t nitems,
-                                                         int32_t* counter) {
+// (the forward: kSolveWaves wavefronts per workgroup share each item; the
+// item index reaches them through an LDS word: in LDS mode the dynamic LDS
+// word at byte next_at, after every item's layout (the layouts address the
+// dynamic LDS from 0, so the kernel declares no static LDS); in arena mode a
+// static one)
+__global__ WDS_SOLVE_KERNEL void WDS_K(k_solve_multi)(const DevPlan* plans,
+                                                      const SolveParams* params,
+                                                      const int2* items, int32_t nitems,
+                                                      int32_t* counter, int32_t next_at) {
+#ifndef WDS_GLOBAL_ARENA
+  __attribute__((address_space(3))) int32_t* const next =
+      (__attribute__((address_space(3))) int32_t*)(size_t)(unsigned)next_at;
+#else
+  __shared__ int32_t next_word;
+  int32_t* const next = &next_word;
+  (void)next_at;
+#endif
   for (;;) {
-    const int i = next_item(counter);
+    if (threadIdx.x == 0) *next = atomicAdd(counter, 1);
+    wg_sync();
+    const int i = *next;
+    // every wavefront has read the index before the next item's is stored
+    wg_sync();
     if (i >= nitems) break;
     const int2 w = items[i];
     const int net = __builtin_amdgcn_readfirstlane(w.x);
