@@ -79,8 +79,14 @@ class Calibration:
         self.solver.check()
 
     def step(self):
-        """One forward + adjoint pass; returns the four gradients."""
-        d, rh, C, Ce = self.leaves()
+        """One forward + adjoint pass at the workload's parameters; returns
+        the loss and the four gradients (fresh tensors each step: the
+        persistent leaves' gradients are reset, not accumulated)."""
+        if getattr(self, "_params", None) is None:
+            self._params = self.leaves()
+        d, rh, C, Ce = self._params
+        for t in self._params:
+            t.grad = None
         L = self.loss(d, rh, C, Ce)
         L.backward()
         return L.detach(), d.grad, rh.grad, C.grad, Ce.grad
