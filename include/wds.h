@@ -119,8 +119,10 @@ int wds_plan_stats_get(const wds_plan* plan, wds_plan_stats* out);
 /* Copy a host-side plan array for inspection (tests): what = 0 perm,
  * 1 colptr, 2 rowidx, 3 parent, 4 node_slot, 5 csr indptr, 6 csr indices,
  * 7 csr_to_L, 8.. programs (8+3*p: step-major instr as int32 x4, +1 reserved,
- * +2 level_end; p = 0 factor, 1 fwd, 2 bwd), 17 row-block starts and 18
- * row-block (pos, col) pairs of the symmetric rows, 19 row_slot (slot of each
+ * +2 level_end; p = 0 factor, 1 fwd, 2 bwd), 17 the lane passes of the
+ * symmetric rows (3 x 64 x int4: first slot, end slot, first row, rows) and
+ * 18 their (pos, col) pairs in lane layout (slot t of lane l at t * 64 + l,
+ * a row's last column word with bit 30 set), 19 row_slot (slot of each
  * junction row).  Returns the element count
  * (int32 units) or a negative status; copies min(count, cap). */
 int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* dst, int64_t cap);

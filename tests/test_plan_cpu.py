@@ -145,12 +145,24 @@ def test_programs_factor_and_solve_like_dense(name):
     assert np.abs(X[slot] - xr).max() <= 1e-10 * np.abs(xr).max()
 
 
-@pytest.mark.parametrize("name", ["grid", "planar"])
-def test_row_block_layout_multiplies_like_dense(name):
-    # the 64-row-block layout of the symmetric rows (rs_blk / rs_ell) that the
-    # batched residual and matrix-vector product walk: every row's entries in
-    # column order, pads (0, -1), and A x equal to the dense product
-    net = grid_network(0) if name == "grid" else planar_network(1)
+ROW_END = 1 << 30
+
+
+@pytest.mark.parametrize("name", ["grid", "planar", "tiny"])
+def test_lane_layout_multiplies_like_dense(name):
+    # the lane layout of the symmetric rows (rs_share / rs_ell) that the
+    # batched residual and matrix-vector product walk: lane l's rows l,
+    # l + 64, ... packed in order, every row's entries in column order with
+    # the last one flagged, pads (0, -1) only after a lane's range; the two
+    # halves of a pair of wavefronts split every lane at a row boundary; and
+    # A x equal to the dense product
+    from netutil import make_net
+    if name == "tiny":
+        nodes = [(RESERVOIR, 100.0, 0.0)] + [(JUNCTION, 0.0, 0.1)] * 5
+        links = [dict(n1=i, n2=i + 1, len_ft=500.0, diam_ft=0.5, C=100.0) for i in range(5)]
+        net = make_net(nodes, links)
+    else:
+        net = grid_network(0) if name == "grid" else planar_network(1)
     m = gga.GGAModel(net)
     s = GGASolver(net, device="cpu", dense_status_machine=True)
     st = s.stats
@@ -165,27 +177,47 @@ def test_row_block_layout_multiplies_like_dense(name):
     c2l = s.plan_export(7)
     AL = np.zeros(st["nnzL"] + 1)
     AL[c2l[c2l >= 0]] = csr[0][c2l >= 0]
-    blk, ell = s.plan_export(17), s.plan_export(18).reshape(-1, 2)
-    nb = (m.Nj + 63) // 64
-    assert len(blk) == nb + 1
+    share = s.plan_export(17).reshape(3, 64, 4)
+    ell = s.plan_export(18).reshape(-1, 2)
+    assert ell.shape[0] % 64 == 0
+    T = ell.shape[0] // 64
     slot = np.argsort(s.plan_export(0))
     x = np.random.default_rng(0).normal(size=m.Nj)
     xs = np.zeros(m.Nj)
     xs[slot] = x
     ys = np.zeros(m.Nj)
-    for r in range(m.Nj):
-        b, lane = divmod(r, 64)
-        cols = []
-        for k in range(blk[b + 1] - blk[b]):
-            pos, col = ell[(blk[b] + k) * 64 + lane]
-            if col < 0:
-                assert pos == 0
-                continue
+    seen = np.zeros(m.Nj, dtype=int)
+    rowlen = np.zeros(m.Nj, dtype=int)
+    for lane in range(64):
+        t0, t1, j0, rows = share[0, lane]
+        assert t0 == 0 and j0 == lane and t1 <= T
+        assert rows == (len(range(lane, m.Nj, 64)))
+        j, cols = j0, []
+        for t in range(t1):
+            pos, cw = ell[t * 64 + lane]
+            assert cw >= 0
+            col = cw & (ROW_END - 1)
             cols.append(col)
-            ys[r] += AL[pos] * xs[col]
-        assert cols == sorted(cols) and r in cols
+            ys[j] += AL[pos] * xs[col]
+            if cw & ROW_END:
+                assert cols == sorted(cols) and j in cols
+                seen[j] += 1
+                rowlen[j] = len(cols)
+                j, cols = j + 64, []
+        assert not cols and j == lane + 64 * rows
+        for t in range(t1, T):
+            assert tuple(ell[t * 64 + lane]) == (0, -1)
+        # the halves: [0, tk) with the first k rows, [tk, t1) with the rest
+        a, b = share[1, lane], share[2, lane]
+        assert a[0] == 0 and a[1] == b[0] and b[1] == t1
+        assert a[2] == lane and b[2] == lane + 64 * a[3] and a[3] + b[3] == rows
+    assert (seen == 1).all()
     A = m.csr_to_dense(csr)[0]
     assert np.allclose(ys[slot], A @ x, rtol=1e-13, atol=1e-12 * np.abs(A).max())
+    # no lane holds more slots than 64-row blocks as wide as their longest
+    # row would give it
+    blocks = [rowlen[b:b + 64].max() for b in range(0, m.Nj, 64)]
+    assert T <= sum(blocks)
 
 
 def test_minimum_degree_beats_natural_ordering():

@@ -441,39 +441,42 @@ __device__ inline void scale_by_pivots(const DevPlan& pl, const double* V, doubl
   }
 }
 
-// Every junction row at once over the row-block layout rs_blk / rs_ell, in
-// whose slots A itself is stored.  A lane's slots form one flat sequence:
-// block b (rows 64 b + lane) owns slots rs_blk[b] .. rs_blk[b + 1] - 1, and
-// the block boundaries are the same in every lane.  A sweep loads kSweep
-// slots at a time (column words and matrix values, unconditionally from
-// clamped slots, so that no load waits for another), then walks them in
-// order, closing a row and opening the next at each block boundary with a
-// uniform branch; the next boundary and the next row's first value are read
-// one block ahead (every block holds at least one slot: each row has its
-// diagonal).  Entries past a row's end (column -1) contribute exact
-// zeros (p = 0: two_sum(hi, -0) returns hi and error 0), and the entries are
-// taken in each row's column order.
+// Every junction row at once over the lane layout rs_ell, in whose slots A
+// itself is stored.  Lane l walks its rows l, l + 64, ... as one packed
+// sequence of slots (each row's entries in column order, the last one with
+// kRowEndFlag in its column word), so a lane carries no padding between its
+// rows; a pass (all of a lane's rows, or one of the two halves a pair of
+// wavefronts takes, rs_share) runs to the longest lane's end, rs_len.  A
+// sweep loads kSweep slots at a time (column words and matrix values,
+// unconditionally from clamped slots, so that no load waits for another),
+// then walks them in order, closing a row at its flagged entry with the
+// next row's first value read one row ahead.  Slots past a lane's range
+// read as column -1 and contribute exact zeros (p = 0: two_sum(hi, -0)
+// returns hi and error 0).
 constexpr int kSweep = 16;
 
-// column words and matrix values of slots s0 .. s0 + kSweep - 1 (clamped to
-// the last slot E - 1)
-__device__ __forceinline__ void sweep_load(const DevPlan& pl, const double* A, int s0, int E,
-                                           int (&cl)[kSweep], double (&av)[kSweep]) {
+// column words and matrix values of slots t0 + s0 .. t0 + s0 + kSweep - 1 of
+// this lane (column -1 from t1 on; the loads come from slots clamped into
+// the lane's range)
+__device__ __forceinline__ void sweep_load(const DevPlan& pl, const double* A, int t0, int s0,
+                                           int t1, int (&cl)[kSweep], double (&av)[kSweep]) {
   const int lane = lane_id();
   const int* cw = (const int*)pl.rs_ell;
 #pragma unroll
   for (int u = 0; u < kSweep; ++u) {
-    const int slot = min(s0 + u, E - 1) * 64 + lane;
-    cl[u] = cw[2 * slot + 1];
+    const int t = t0 + s0 + u;
+    const int slot = max(min(t, t1 - 1), 0) * 64 + lane;
+    const int w = cw[2 * slot + 1];
+    cl[u] = t < t1 ? w : -1;
     av[u] = A[slot];
   }
 }
 
-// vector entries of a loaded chunk; entries past a row's end read as zeros
+// vector entries of a loaded chunk; entries past a lane's range read as zeros
 __device__ __forceinline__ void sweep_values(const lds_f64* X, const int (&cl)[kSweep],
                                              double (&av)[kSweep], double (&xv)[kSweep]) {
 #pragma unroll
-  for (int u = 0; u < kSweep; ++u) xv[u] = X[max(cl[u], 0)];
+  for (int u = 0; u < kSweep; ++u) xv[u] = X[cl[u] >= 0 ? (cl[u] & kColMask) : 0];
 #pragma unroll
   for (int u = 0; u < kSweep; ++u) {
     const bool ok = cl[u] >= 0;
@@ -481,6 +484,9 @@ __device__ __forceinline__ void sweep_values(const lds_f64* X, const int (&cl)[k
     xv[u] = ok ? xv[u] : 0.0;
   }
 }
+
+// the last entry of a row (never a slot past the lane's range)
+__device__ __forceinline__ bool row_end(int cl) { return cl >= 0 && (cl & kRowEndFlag); }
 
 // The first row block of share i of nw over the row blocks of a row-block
 // table whose block b starts at slot blk[b] (blk[nb] = E): the smallest b
@@ -493,69 +499,56 @@ __device__ __forceinline__ int block_split(const int32_t* blk, int nb, int E, in
   return b;
 }
 
-// dst_j = b_j - (A x)_j in double-double, x and dst in LDS, b global; the
-// row blocks split into nw contiguous shares, share w on this wavefront
+// dst_j = b_j - (A x)_j in double-double, x and dst in LDS, b global; with
+// nw = 2 wavefronts each takes one half of every lane's rows
 __device__ inline void residual_all(const DevPlan& pl, const double* A, const double* x,
                                     const double* b, double* dst, int w = 0, int nw = 1) {
   const int lane = lane_id();
   const lds_f64* X = (const lds_f64*)x;
   lds_f64* D = (lds_f64*)dst;
-  const int nb = (pl.Nj + 63) >> 6;
-  const int b0 = block_split(pl.rs_blk, nb, pl.rs_blk[nb], w, nw);
-  const int b1 = block_split(pl.rs_blk, nb, pl.rs_blk[nb], w + 1, nw);
-  if (b0 >= b1) return;
-  const int S0 = pl.rs_blk[b0];
-  const int E = pl.rs_blk[b1];
+  const int c = nw == 1 ? 0 : 1 + w;
+  const int4 sh = pl.rs_share[c * 64 + lane];  // (t0, t1, first row, rows)
+  const int T = pl.rs_len[c];
   const int jl = pl.Nj - 1;
-  const int rows_end = min(64 * b1, pl.Nj);
   int cl[kSweep];
   double av[kSweep];
-  sweep_load(pl, A, S0, E, cl, av);
+  sweep_load(pl, A, sh.x, 0, sh.y, cl, av);
   __builtin_amdgcn_sched_barrier(0);
-  // D = b on this share's rows (each lane copies and later reads only its
-  // own rows)
-  for (int j0 = 64 * b0 + lane; j0 < rows_end; j0 += 64 * kRowG) {
+  // D = b on this lane's rows of the pass (each lane copies and later reads
+  // only its own rows)
+  for (int r0 = 0; r0 < sh.w; r0 += kRowG) {
     double bv[kRowG];
 #pragma unroll
-    for (int u = 0; u < kRowG; ++u) bv[u] = b[min(j0 + 64 * u, jl)];
+    for (int u = 0; u < kRowG; ++u) bv[u] = b[min(sh.z + 64 * (r0 + u), jl)];
 #pragma unroll
     for (int u = 0; u < kRowG; ++u)
-      if (j0 + 64 * u < rows_end) D[j0 + 64 * u] = bv[u];
+      if (r0 + u < sh.w) D[sh.z + 64 * (r0 + u)] = bv[u];
   }
-  int bk = b0;
-  int be = pl.rs_blk[b0 + 1];
-  int be2 = pl.rs_blk[min(b0 + 2, nb)];
-  double hi = D[min(64 * b0 + lane, jl)], lo = 0.0;
-  // the next block's first value; past the share's end it is never used
-  double hn = D[min(64 * (b0 + 1) + lane, jl)];
-  for (int s0 = S0; s0 < E; s0 += kSweep) {
+  int j = sh.z;
+  double hi = D[min(j, jl)], lo = 0.0;
+  // the next row's first value; past the lane's rows it is never used
+  double hn = D[min(j + 64, jl)];
+  for (int s0 = 0; s0 < T; s0 += kSweep) {
     double xv[kSweep];
     sweep_values(X, cl, av, xv);
 #pragma unroll
     for (int u = 0; u < kSweep; ++u) {
-      const int s = s0 + u;
-      if (s >= E) continue;
-      if (s == be) {
-        const int j = bk * 64 + lane;
-        if (j < pl.Nj) D[j] = hi + lo;
-        ++bk;
-        be = be2;
-        be2 = pl.rs_blk[min(bk + 2, nb)];
-        hi = hn;
-        lo = 0.0;
-        hn = D[min((bk + 1) * 64 + lane, jl)];
-      }
       const double p = av[u] * xv[u];
       const double pe = fma(av[u], xv[u], -p);
       double sm, err;
       two_sum(hi, -p, sm, err);
       hi = sm;
       lo += err - pe;
+      if (row_end(cl[u])) {
+        D[j] = hi + lo;
+        j += 64;
+        hi = hn;
+        lo = 0.0;
+        hn = D[min(j + 64, jl)];
+      }
     }
-    if (s0 + kSweep < E) sweep_load(pl, A, s0 + kSweep, E, cl, av);
+    if (s0 + kSweep < T) sweep_load(pl, A, sh.x, s0 + kSweep, sh.y, cl, av);
   }
-  const int j = bk * 64 + lane;
-  if (j < pl.Nj) D[j] = hi + lo;
 }
 
 // dst = A x, x in LDS, dst in global memory
@@ -564,36 +557,27 @@ __device__ inline void spmv_all(const DevPlan& pl, const double* A, const double
   const int lane = lane_id();
   const lds_f64* X = (const lds_f64*)x;
   __attribute__((address_space(1))) double* D = (__attribute__((address_space(1))) double*)dst;
-  const int nb = (pl.Nj + 63) >> 6;
-  const int E = pl.rs_blk[nb];
+  const int4 sh = pl.rs_share[lane];
+  const int T = pl.rs_len[0];
   int cl[kSweep];
   double av[kSweep];
-  sweep_load(pl, A, 0, E, cl, av);
-  int bk = 0;
-  int be = pl.rs_blk[1];
-  int be2 = pl.rs_blk[min(2, nb)];
+  sweep_load(pl, A, sh.x, 0, sh.y, cl, av);
+  int j = sh.z;
   double sum = 0.0;
-  for (int s0 = 0; s0 < E; s0 += kSweep) {
+  for (int s0 = 0; s0 < T; s0 += kSweep) {
     double xv[kSweep];
     sweep_values(X, cl, av, xv);
 #pragma unroll
     for (int u = 0; u < kSweep; ++u) {
-      const int s = s0 + u;
-      if (s >= E) continue;
-      if (s == be) {
-        const int j = bk * 64 + lane;
-        if (j < pl.Nj) D[j] = sum;
-        ++bk;
-        be = be2;
-        be2 = pl.rs_blk[min(bk + 2, nb)];
+      sum = fma(av[u], xv[u], sum);
+      if (row_end(cl[u])) {
+        D[j] = sum;
+        j += 64;
         sum = 0.0;
       }
-      sum = fma(av[u], xv[u], sum);
     }
-    if (s0 + kSweep < E) sweep_load(pl, A, s0 + kSweep, E, cl, av);
+    if (s0 + kSweep < T) sweep_load(pl, A, sh.x, s0 + kSweep, sh.y, cl, av);
   }
-  const int j = bk * 64 + lane;
-  if (j < pl.Nj) D[j] = sum;
 }
 
 // ---------------------------------------------------------------------------

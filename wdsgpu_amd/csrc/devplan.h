@@ -60,6 +60,8 @@ WDS_HD inline int64_t lds_small_doubles(int64_t wp_tot, int64_t np) {
 
 // solve programs: the store word of a level's last step carries the level
 // end and log2 of the lane's group width above the LDS byte address
+constexpr int32_t kRowEndFlag = 1 << 30;   // column word of a row's last entry
+constexpr int32_t kColMask = kRowEndFlag - 1;
 constexpr int32_t kDotAddrMask = (1 << 27) - 1;   // byte offsets below 128 MiB
 constexpr int32_t kDotLevelEnd = 1 << 30;
 constexpr int kDotWidthShift = 27;
@@ -132,10 +134,15 @@ struct DevPlan {
   // symmetric SpMV rows over A (slot order): (L position, column slot)
   const int32_t* spmv_ptr;    // [Nj+1]
   const int2* spmv_ent;       // [..]
-  // the symmetric rows in 64-row blocks, entry-major (ELL): entry e of row
-  // 64 b + lane is rs_ell[(rs_blk[b] + e) * 64 + lane], pads (0, -1)
-  const int32_t* rs_blk;      // [ceil(Nj / 64) + 1]
+  // the symmetric rows in lane sequences (lane l: rows l, l + 64, ..., each
+  // row's entries in column order, packed; slot t of lane l is
+  // rs_ell[t * 64 + l]; a row's last entry has kRowEndFlag in its column
+  // word; pads (0, -1)); rs_share[c * 64 + l] = (t0, t1, first row, rows)
+  // of lane l's pass c (0: all its rows; 1, 2: the halves of a pair of
+  // wavefronts), rs_len[c] the pass's longest slot range
+  const int4* rs_share;       // [3 * 64]
   const int2* rs_ell;
+  int32_t rs_len[3];
   // assembly rows in the same blocks: (link | side bit, -1) for the row's
   // links in jadj order, then (link, fixed slot) for its fixed-head terms in
   // jg order; pads (0, -2)

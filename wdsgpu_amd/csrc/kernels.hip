@@ -120,7 +120,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   size_t o_fap = bb.add(P->fadj_ptr), o_fa = bb.add(P->fadj);
   size_t o_op = bb.add(P->off_pos), o_optr = bb.add(P->off_ptr), o_ol = bb.add(P->off_link);
   size_t o_fp = bb.add(P->fill_pos), o_sp = bb.add(P->spmv_ptr), o_se = bb.add(P->spmv_ent);
-  size_t o_rb = bb.add(P->rs_blk), o_re = bb.add(P->rs_ell);
+  size_t o_rb = bb.add(P->rs_share), o_re = bb.add(P->rs_ell);
   size_t o_ab = bb.add(P->as_blk), o_ae = bb.add(P->as_ell), o_o4 = bb.add(P->off4);
   size_t o_pb = bb.add(P->ps_blk), o_pe = bb.add(P->ps_ell), o_pln = bb.add(P->ps_len);
   size_t o_dae = bb.add(P->diag_ae), o_oae = bb.add(P->off_ae), o_pae = bb.add(P->prv_ae),
@@ -205,8 +205,9 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.fill_pos = at<int32_t>(d, o_fp);
   D.spmv_ptr = at<int32_t>(d, o_sp);
   D.spmv_ent = at<int2>(d, o_se);
-  D.rs_blk = at<int32_t>(d, o_rb);
+  D.rs_share = at<int4>(d, o_rb);
   D.rs_ell = at<int2>(d, o_re);
+  for (int i = 0; i < 3; ++i) D.rs_len[i] = P->rs_len[i];
   D.as_blk = at<int32_t>(d, o_ab);
   D.as_ell = at<int2>(d, o_ae);
   D.off4 = at<int4>(d, o_o4);

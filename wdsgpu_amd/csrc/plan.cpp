@@ -247,23 +247,53 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     P->spmv_ptr[s + 1] = P->spmv_ptr[s] + (int32_t)r.size();
   }
   {
-    const int nb = (Nj + kLanes - 1) / kLanes;
-    P->rs_blk.assign(nb + 1, 0);
-    for (int b = 0; b < nb; ++b) {
-      int32_t k = 0;
-      for (int s = b * kLanes; s < std::min(Nj, (b + 1) * kLanes); ++s)
-        k = std::max(k, P->spmv_ptr[s + 1] - P->spmv_ptr[s]);
-      P->rs_blk[b + 1] = P->rs_blk[b] + k;
+    // lane sequences: lane l's rows l, l + 64, ... packed in order
+    std::vector<int32_t> tl(kLanes, 0);
+    P->rs_t0.assign(Nj, 0);
+    for (int s = 0; s < Nj; ++s) {
+      P->rs_t0[s] = tl[s % kLanes];
+      tl[s % kLanes] += P->spmv_ptr[s + 1] - P->spmv_ptr[s];
     }
-    P->rs_ell.assign((size_t)P->rs_blk[nb] * kLanes * 2, 0);
+    const int32_t T = *std::max_element(tl.begin(), tl.end());
+    P->rs_ell.assign((size_t)std::max<int32_t>(T, 1) * kLanes * 2, 0);
     for (size_t i = 1; i < P->rs_ell.size(); i += 2) P->rs_ell[i] = -1;
     for (int s = 0; s < Nj; ++s) {
-      const int b = s / kLanes, lane = s % kLanes;
-      for (int32_t e = 0; e < P->spmv_ptr[s + 1] - P->spmv_ptr[s]; ++e) {
-        const size_t x = ((size_t)(P->rs_blk[b] + e) * kLanes + lane) * 2;
+      const int lane = s % kLanes;
+      const int32_t len = P->spmv_ptr[s + 1] - P->spmv_ptr[s];
+      for (int32_t e = 0; e < len; ++e) {
+        const size_t x = ((size_t)(P->rs_t0[s] + e) * kLanes + lane) * 2;
+        const int32_t col = P->spmv_ent[2 * (size_t)(P->spmv_ptr[s] + e) + 1];
         P->rs_ell[x] = P->spmv_ent[2 * (size_t)(P->spmv_ptr[s] + e)];
-        P->rs_ell[x + 1] = P->spmv_ent[2 * (size_t)(P->spmv_ptr[s] + e) + 1];
+        P->rs_ell[x + 1] = col | (e == len - 1 ? kRowEndFlag : 0);
       }
+    }
+    // passes: all of a lane's rows, and two halves split at the row boundary
+    // nearest the middle of the lane's slots
+    P->rs_share.assign(3 * kLanes * 4, 0);
+    for (int i = 0; i < 3; ++i) P->rs_len[i] = 0;
+    auto put = [&](int c, int lane, int32_t t0, int32_t t1, int32_t j0, int32_t rows) {
+      int32_t* q = &P->rs_share[4 * ((size_t)c * kLanes + lane)];
+      q[0] = t0;
+      q[1] = t1;
+      q[2] = j0;
+      q[3] = rows;
+      P->rs_len[c] = std::max(P->rs_len[c], t1 - t0);
+    };
+    for (int lane = 0; lane < kLanes; ++lane) {
+      const int32_t nrows = lane < Nj ? (Nj - 1 - lane) / kLanes + 1 : 0;
+      put(0, lane, 0, tl[lane], lane, nrows);
+      int32_t k = 0, best = tl[lane];  // rows in the first half, its imbalance
+      for (int32_t r = 0; r <= nrows; ++r) {
+        const int32_t t = r < nrows ? P->rs_t0[lane + kLanes * r] : tl[lane];
+        const int32_t imb = std::abs(2 * t - tl[lane]);
+        if (imb < best) {
+          best = imb;
+          k = r;
+        }
+      }
+      const int32_t tk = k < nrows ? P->rs_t0[lane + kLanes * k] : tl[lane];
+      put(1, lane, 0, tk, lane, k);
+      put(2, lane, tk, tl[lane], lane + kLanes * k, nrows - k);
     }
   }
   {
@@ -477,15 +507,16 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
     P->csr_to_L[e] = (x == y) ? S.colptr[sx] : (sx > sy ? (int32_t)S.pos(sx, sy) : -1);
   }
   for (int r = 0; r < Nj; ++r) P->csr_indptr[r + 1] += P->csr_indptr[r];
-  // slot of each CSR entry in the row-block layout of A
+  // slot of each CSR entry in the lane layout of A
   P->csr_to_ell.assign(P->csr_indices.size(), -1);
   for (int r = 0; r < Nj; ++r) {
-    const int sr = P->row_slot[r], b = sr / kLanes, lane = sr % kLanes;
+    const int sr = P->row_slot[r], lane = sr % kLanes;
+    const int32_t t0 = P->rs_t0[sr], t1 = t0 + P->spmv_ptr[sr + 1] - P->spmv_ptr[sr];
     for (int32_t e = P->csr_indptr[r]; e < P->csr_indptr[r + 1]; ++e) {
       const int sc = P->row_slot[P->csr_indices[e]];
-      for (int32_t k = P->rs_blk[b]; k < P->rs_blk[b + 1]; ++k) {
+      for (int32_t k = t0; k < t1; ++k) {
         const size_t x = (size_t)k * kLanes + lane;
-        if (P->rs_ell[2 * x + 1] == sc) {
+        if ((P->rs_ell[2 * x + 1] & kColMask) == sc) {
           P->csr_to_ell[e] = (int32_t)x;
           break;
         }
@@ -607,7 +638,7 @@ extern "C" int64_t wds_plan_export(const wds_plan* plan, int32_t what, int32_t* 
     case 5: return copy_out(plan->csr_indptr, dst, cap);
     case 6: return copy_out(plan->csr_indices, dst, cap);
     case 7: return copy_out(plan->csr_to_L, dst, cap);
-    case 17: return copy_out(plan->rs_blk, dst, cap);
+    case 17: return copy_out(plan->rs_share, dst, cap);
     case 18: return copy_out(plan->rs_ell, dst, cap);
     case 19: return copy_out(plan->row_slot, dst, cap);
     default: break;

@@ -24,9 +24,16 @@ struct wds_plan {
       spmv_ptr, prv_k, prv_s1, prv_s2, prv_pos_off, prv_kind, tl_k, tl_tank, tl_other,
       tl_bits, csr_indptr, csr_indices, csr_to_L, diag_pos, row_slot;
   std::vector<int32_t> spmv_ent;  // pairs (pos, col)
-  // the same rows in 64-row blocks, entry-major within a block (ELL): entry e
-  // of row 64 b + lane is pair (rs_blk[b] + e) * 64 + lane; pads are (0, -1)
-  std::vector<int32_t> rs_blk, rs_ell;
+  // the same rows in lane sequences: lane l walks rows l, l + 64, ... in
+  // order, each row's entries in column order, packed; slot t of lane l is
+  // pair t * 64 + l, the column word of a row's last entry carries
+  // kRowEndFlag, pads are (0, -1).  rs_share: per lane, the slot range and
+  // first row of a pass over all its rows (entries 0..63) and of the two
+  // halves a pair of wavefronts takes (64..127, 128..191): int4 (t0, t1,
+  // first row, rows); rs_len: the longest range of each of those three
+  // passes.  rs_t0: the first slot of each row (host only).
+  std::vector<int32_t> rs_share, rs_ell, rs_t0;
+  int32_t rs_len[3] = {0, 0, 0};
   // assembly rows in the same blocks: a row's links in jadj order (link |
   // side bit, -1), then its fixed-head terms in jg order (link, fixed slot);
   // pads (0, -2).  off4: per off-diagonal entry (L position, first link,
