@@ -887,7 +887,8 @@ extern "C" int wds_adjoint(wds_plan* P, void* stream, const wds_adjoint_args* a)
     arena_adjoint(P->dev, ap, a->B, (hipStream_t)stream);
   } else {
     WDS_SET_LDS(k_adjoint, lds);
-    hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64), lds, (hipStream_t)stream, P->dev, ap);
+    hipLaunchKernelGGL(k_adjoint, dim3(a->B), dim3(64 * kSolveWaves), lds, (hipStream_t)stream,
+                       P->dev, ap);
   }
   P->cnt.bwd_solve += 1;
   P->cnt.bwd_reuse += 1;

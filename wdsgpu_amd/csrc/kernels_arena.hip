@@ -42,7 +42,7 @@ void arena_trisolve(const DevPlan& pl, const double* factor, const double* csr, 
 }
 
 void arena_adjoint(const DevPlan& pl, const AdjParams& ap, int B, hipStream_t st) {
-  hipLaunchKernelGGL(k_adjoint_arena, dim3(B), dim3(64), 0, st, pl, ap);
+  hipLaunchKernelGGL(k_adjoint_arena, dim3(B), dim3(64 * kSolveWaves), 0, st, pl, ap);
 }
 
 void arena_solve_multi(const DevPlan* plans, const SolveParams* params, const int2* items,

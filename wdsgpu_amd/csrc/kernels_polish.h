@@ -682,15 +682,19 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
 // Node adjoint M lam = r (_adjgpu_node_solve, dgga/autodiff.py:1107-1154):
 // r in v1, gQ of the valves in sprv; result in dst (LDS, not the solve
 // vector).
-__device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_ref) {
+// With nw > 1 wavefronts the plain solve shares its copies and residuals;
+// the Woodbury solve is wavefront 0's (the caller broadcasts its outcome).
+__device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_ref,
+                                  int w = 0, int nw = 1) {
   const int lane = lane_id();
   const int p = pl.p;
   bool any_act = false;
   for (int i = 0; i < p; ++i) any_act |= c.sh[i] != 0.0;
   if (!any_act) {
-    k4_solve(pl, c, c.v1, dst, n_ref);
+    k4_solve(pl, c, c.v1, dst, n_ref, w, nw);
     return true;
   }
+  if (w != 0) return true;
   double* r2 = c.v3;
   double* wk = c.v4;
   const double* gqp = c.sprv;
@@ -721,10 +725,15 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
 }
 
 // Backward of the implicit solve (ImplicitGGASolveGPU.backward,
-// dgga/autodiff.py:1245-1312), one wavefront per scenario.
+// dgga/autodiff.py:1245-1312), one workgroup per scenario: its wavefronts
+// (blockDim.x / 64) share the state loads, the coefficients and assembly,
+// the factor copy, the right-hand side, the plain solve's residuals and the
+// contractions; the Woodbury solve and the fixed-node sums are wavefront
+// 0's.
 __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjParams& ap, int b,
                                         double* lds) {
   const int lane = lane_id();
+  const int w = wave_idx(), nw = (int)(blockDim.x >> 6);
   Scn c = make_scn(pl, lds, ap.ws, b, const_cast<double*>(ap.flow),
                    const_cast<int8_t*>(ap.status), ap.r, ap.r_stride);
   const double* ke = ap.ke + (int64_t)b * ap.ke_stride;
@@ -738,7 +747,7 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
   auto GE = [&](int n) { return gE ? gE[n] : 0.0; };
   auto GQ = [&](int k) { return gQ ? gQ[k] : 0.0; };
   const int ns = pl.Nj + pl.Nf;
-  for (int s0 = lane; s0 < ns; s0 += 64 * kRowG8) {
+  for (int s0 = lane + 64 * kRowG8 * w; s0 < ns; s0 += 64 * kRowG8 * nw) {
     int nd[kRowG8];
 #pragma unroll
     for (int u = 0; u < kRowG8; ++u) nd[u] = pl.slot_node[min(s0 + 64 * u, ns - 1)];
@@ -762,17 +771,20 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
   }
   mem_sync();
   lds_sync();
+  if (nw > 1) wg_sync();
   // coefficients at the converged state; assemble A-hat (values for the
   // refinement mat-vec) and load the forward's terminal factor into V
-  polish_coeffs(pl, c);
-  polish_assemble(pl, c);
+  polish_coeffs(pl, c, w, nw);
+  if (nw > 1) wg_sync();
+  polish_assemble(pl, c, w, nw);
+  if (nw > 1) wg_sync();
   const double* fac = ap.factor + (int64_t)b * pl.nnzL;
-  for (int t = lane; t < pl.nnzL; t += 64) c.V[t] = fac[t];
-  if (lane == 0) c.V[pl.nnzL] = 0.0;
+  for (int t = lane + 64 * w; t < pl.nnzL; t += 64 * nw) c.V[t] = fac[t];
+  if (w == 0 && lane == 0) c.V[pl.nnzL] = 0.0;
   lds_sync();
   // r = B^T P gQ - em gE / hge - gH (n2 side first, reference scatter order);
   // rows in groups of kRowG8 per lane with their loads issued together
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+  for (int j0 = lane + 64 * kRowG8 * w; j0 < pl.Nj; j0 += 64 * kRowG8 * nw) {
     int nd[kRowG8];
     double kv[kRowG8], hgv[kRowG8];
 #pragma unroll
@@ -806,12 +818,14 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
       c.v1[j] = r;
     }
   }
-  if (lane < pl.p) c.sprv[lane] = GQ(pl.prv_k[lane]);
+  if (w == 0 && lane < pl.p) c.sprv[lane] = GQ(pl.prv_k[lane]);
   mem_sync();
   lds_sync();
+  if (nw > 1) wg_sync();
   double* lam = c.X2;  // LDS
-  if (!node_solve(pl, c, lam, ap.n_refine)) {
-    if (lane == 0) ap.info[b] = -1;
+  const bool solved = node_solve(pl, c, lam, ap.n_refine, w, nw);
+  if (!wave0_int(pl, c, solved ? 1 : 0, w, nw)) {
+    if (w == 0 && lane == 0) ap.info[b] = -1;
     return;
   }
   // back-substitution and theta contraction
@@ -820,7 +834,7 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
   double* grh = ap.grh + (int64_t)b * pl.N;
   double* gr = ap.gr + (int64_t)b * pl.L;
   double* laml = c.Y;  // lambda_l per link
-  for (int k0 = lane; k0 < pl.L; k0 += 64 * kRowG8) {
+  for (int k0 = lane + 64 * kRowG8 * w; k0 < pl.L; k0 += 64 * kRowG8 * nw) {
     int fv[kRowG8], s1[kRowG8], s2[kRowG8];
     int8_t sv[kRowG8];
     double Pv[kRowG8], qv[kRowG8], rv[kRowG8], gq[kRowG8];
@@ -869,7 +883,7 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
       }
     }
   }
-  for (int j0 = lane; j0 < pl.Nj; j0 += 64 * kRowG8) {
+  for (int j0 = lane + 64 * kRowG8 * w; j0 < pl.Nj; j0 += 64 * kRowG8 * nw) {
     int nd[kRowG8];
     double kv[kRowG8], hgv[kRowG8], ev[kRowG8];
 #pragma unroll
@@ -900,6 +914,9 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
     }
   }
   mem_sync();
+  // the fixed-node sums read every link's lambda
+  if (nw > 1) wg_sync();
+  if (w != 0) return;
   for (int f = lane; f < pl.Nf; f += 64) {
     double g = 0.0;
     for (int e = pl.fadj_ptr[f]; e < pl.fadj_ptr[f + 1]; ++e) {
@@ -918,7 +935,8 @@ __device__ __forceinline__ void adjoint_scenario(const DevPlan& pl, const AdjPar
   if (lane == 0) ap.info[b] = 0;
 }
 
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
+// one scenario per workgroup, kSolveWaves wavefronts
+__global__ WDS_SOLVE_KERNEL void WDS_K(k_adjoint)(DevPlan pl, AdjParams ap) {
   const int b = blockIdx.x;
   WDS_SCENARIO_MEM(ap.ws, b);
   adjoint_scenario(pl, ap, b, lds);
