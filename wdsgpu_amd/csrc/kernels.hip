@@ -826,9 +826,10 @@ extern "C" int wds_multi_adjoint(wds_multi* M, void* stream, const wds_adjoint_a
   if (e != hipSuccess) return hip_fail(e, "hipMemcpyAsync(multi params)");
   hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, M->d_counter);
   if (M->n_lds) {
-    WDS_SET_LDS(k_adjoint_multi, M->lds_bytes);
-    hipLaunchKernelGGL(k_adjoint_multi, dim3(M->grid_lds), dim3(64), M->lds_bytes, st,
-                       M->d_plans, M->d_ap, M->d_items, M->n_lds, M->d_counter);
+    WDS_SET_LDS(k_adjoint_multi, M->lds_bytes + kItemSlotBytes);
+    hipLaunchKernelGGL(k_adjoint_multi, dim3(M->grid_lds), dim3(64 * kSolveWaves),
+                       M->lds_bytes + kItemSlotBytes, st, M->d_plans, M->d_ap, M->d_items,
+                       M->n_lds, M->d_counter, (int32_t)M->lds_bytes);
   }
   if (M->n_arena)
     arena_adjoint_multi(M->d_plans, M->d_ap, M->d_items + M->n_lds, M->n_arena,

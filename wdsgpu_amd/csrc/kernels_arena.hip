@@ -53,8 +53,8 @@ void arena_solve_multi(const DevPlan* plans, const SolveParams* params, const in
 
 void arena_adjoint_multi(const DevPlan* plans, const AdjParams* params, const int2* items,
                          int nitems, int32_t* counter, int grid, hipStream_t st) {
-  hipLaunchKernelGGL(k_adjoint_multi_arena, dim3(grid), dim3(64), 0, st, plans, params, items,
-                     nitems, counter);
+  hipLaunchKernelGGL(k_adjoint_multi_arena, dim3(grid), dim3(64 * kSolveWaves), 0, st, plans,
+                     params, items, nitems, counter, 0);
 }
 
 }  // namespace wds

@@ -1337,22 +1337,15 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams
 // ---------------------------------------------------------------------------
 // Ragged batches: many networks, each with its own plan and arrays, in one
 // persistent launch.  The work items (network, scenario) are in the order the
-// host chose (longest first); every wavefront takes the next item from a
+// host chose (longest first); every workgroup takes the next item from a
 // global counter, solves it with the network's plan, and exits when the list
 // is exhausted.  In LDS mode the dynamic LDS is the largest layout of the
 // group and every item reuses it; in arena mode each item's layout is its
-// arena.
-__device__ __forceinline__ int next_item(int32_t* counter) {
-  int i = 0;
-  if (lane_id() == 0) i = atomicAdd(counter, 1);
-  return __shfl(i, 0, 64);
-}
-
-// (the forward: kSolveWaves wavefronts per workgroup share each item; the
-// item index reaches them through an LDS word: in LDS mode the dynamic LDS
-// word at byte next_at, after every item's layout (the layouts address the
-// dynamic LDS from 0, so the kernel declares no static LDS); in arena mode a
-// static one)
+// arena.  The kSolveWaves wavefronts of a workgroup share each item; the item
+// index reaches them through an LDS word: in LDS mode the dynamic LDS word at
+// byte next_at, after every item's layout (the layouts address the dynamic
+// LDS from 0, so the kernels declare no static LDS); in arena mode a static
+// one.
 __global__ WDS_SOLVE_KERNEL void WDS_K(k_solve_multi)(const DevPlan* plans,
                                                       const SolveParams* params,
                                                       const int2* items, int32_t nitems,
@@ -1384,12 +1377,23 @@ __global__ WDS_SOLVE_KERNEL void WDS_K(k_solve_multi)(const DevPlan* plans,
   }
 }
 
-__global__ WDS_SCENARIO_KERNEL void WDS_K(k_adjoint_multi)(const DevPlan* plans,
-                                                           const AdjParams* params,
-                                                           const int2* items, int32_t nitems,
-                                                           int32_t* counter) {
+__global__ WDS_SOLVE_KERNEL void WDS_K(k_adjoint_multi)(const DevPlan* plans,
+                                                        const AdjParams* params,
+                                                        const int2* items, int32_t nitems,
+                                                        int32_t* counter, int32_t next_at) {
+#ifndef WDS_GLOBAL_ARENA
+  __attribute__((address_space(3))) int32_t* const next =
+      (__attribute__((address_space(3))) int32_t*)(size_t)(unsigned)next_at;
+#else
+  __shared__ int32_t next_word;
+  int32_t* const next = &next_word;
+  (void)next_at;
+#endif
   for (;;) {
-    const int i = next_item(counter);
+    if (threadIdx.x == 0) *next = atomicAdd(counter, 1);
+    wg_sync();
+    const int i = *next;
+    wg_sync();
     if (i >= nitems) break;
     const int2 w = items[i];
     const int net = __builtin_amdgcn_readfirstlane(w.x);
