@@ -166,6 +166,7 @@ struct Scn {
   double* sx0;  // LDS [np] probes e_u^T A-hat^-1 r
   double* sprv; // LDS [p] valve residuals (polish) / valve cotangents (adjoint)
   int32_t* emj; // LDS [kEmPass * 64 + 1] emitter rows by lane (load_inputs)
+  bool em_list; // emj holds this scenario's list (set once load_inputs ran)
   int32_t* wpn; // LDS [wp_tot] slots of the valve-end path nodes (wood_paths)
   int8_t* st0;  // [L]
   double* misc; // [512] small dense systems (Woodbury)
@@ -247,6 +248,7 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
   c.sprv = sm;
   sm += pl.np / 2;
   c.emj = reinterpret_cast<int32_t*>(sm);
+  c.em_list = false;
   sm += kEmDoubles;
   c.wpn = reinterpret_cast<int32_t*>(sm);
   c.st0 = reinterpret_cast<int8_t*>(w);
@@ -263,6 +265,25 @@ __device__ inline Scn make_scn(const DevPlan& pl, double* lds, double* ws_all, i
 // emitter hloss / hgrad of every junction with an emitter, rows batched
 __device__ inline void emitter_coeffs(const DevPlan& pl, Scn& c, int w = 0, int nw = 1) {
   const int lane = lane_id();
+  if (c.em_list) {
+    const int npass = c.emj[kEmPass * 64];
+    if (npass <= kEmPass) {
+      // the rows with an emitter from load_inputs' list, pass u on
+      // wavefront u % nw: one round trip instead of a walk over every row
+      for (int u = w; u < npass; u += nw) {
+        const int j = c.emj[u * 64 + lane];
+        const int jj = max(j, 0);
+        const double kv = c.kej[jj], ev = c.e[jj];
+        if (j >= 0) {
+          double hl, hg;
+          emitter_hl(pl, ev, kv, hl, hg);
+          c.hle[j] = hl;
+          c.hge[j] = hg;
+        }
+      }
+      return;
+    }
+  }
   for (int j0 = lane + 64 * kRowG8 * w; j0 < pl.Nj; j0 += 64 * kRowG8 * nw) {
     double kv[kRowG8], ev[kRowG8];
 #pragma unroll
@@ -1207,6 +1228,7 @@ __device__ __forceinline__ void solve_scenario(const DevPlan& pl, const SolvePar
     lds_sync();
   }
   if (nw > 1) wg_sync();
+  c.em_list = true;  // load_inputs built the emitter row list
   const int maxtrials = maxtrials_of(pl, sp);
   Ctrl st{1, 0, pl.checkfreq, 0, 0};
   double relerr_out = 0.0, dratio = 0.0;
@@ -1277,6 +1299,7 @@ __global__ WDS_SCENARIO_KERNEL void WDS_K(k_newton_iter)(DevPlan pl, SolveParams
   int32_t* cb = ctrl + 4 * b;
   Scn c = make_scn(pl, lds, sp.ws, b, sp.flow, sp.status, sp.r, sp.r_stride);
   load_inputs(pl, c, sp, b);
+  c.em_list = true;
   const double* ke = sp.ke + (int64_t)b * sp.ke_stride;
   double* H = sp.head + (int64_t)b * pl.N;
   double* E = sp.emitter + (int64_t)b * pl.N;
