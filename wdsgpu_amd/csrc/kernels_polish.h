@@ -78,26 +78,42 @@ __device__ inline void matvec(const DevPlan& pl, Scn& c, const double* x, double
   mem_sync();
 }
 
-// Solve the n x n system M y = w (row-major, LDS) by Gaussian elimination
-// with partial pivoting (largest magnitude, lowest row on ties), rows
-// eliminated in parallel, one lane per row; y overwrites w.  Returns false
-// if singular.
+// one stage of the (largest value, lowest index) reduction within rows of 16
+// lanes; the combination is commutative and associative, so after the four
+// stages (xor 1, xor 2, half mirror, mirror) every lane of a row of 16 holds
+// its row's result
+template <int CTRL>
+__device__ __forceinline__ void argmax_stage(double& v, int& idx) {
+  const double ov = dpp_f64<CTRL>(v);
+  const int oi = __builtin_amdgcn_update_dpp(0, idx, CTRL, 0xf, 0xf, false);
+  if (ov > v || (ov == v && oi < idx)) {
+    v = ov;
+    idx = oi;
+  }
+}
+
+// Solve the n x n system M y = w (row-major, LDS; n <= 2 kMaxPrv = 16) by
+// Gaussian elimination with partial pivoting (largest magnitude, lowest row
+// on ties), rows eliminated in parallel, one lane per row; y overwrites w.
+// The pivot search runs within the first 16 lanes (DPP) and reaches every
+// lane from lane 0.  Returns false if singular.
 __device__ inline bool small_solve(double* M, double* w, int n) {
   const int lane = lane_id();
   for (int col = 0; col < n; ++col) {
     double v = (lane >= col && lane < n) ? fabs(M[lane * n + col]) : -1.0;
     int idx = lane;
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) {
-      const double ov = __shfl_xor(v, m, 64);
-      const int oi = __shfl_xor(idx, m, 64);
-      if (ov > v || (ov == v && oi < idx)) {
-        v = ov;
-        idx = oi;
-      }
+    argmax_stage<0xB1>(v, idx);   // quad_perm [1, 0, 3, 2]
+    argmax_stage<0x4E>(v, idx);   // quad_perm [2, 3, 0, 1]
+    argmax_stage<0x141>(v, idx);  // row_half_mirror
+    argmax_stage<0x140>(v, idx);  // row_mirror
+    {
+      const long long vb = __double_as_longlong(v);
+      const int lo = __builtin_amdgcn_readfirstlane((int)vb);
+      const int hi = __builtin_amdgcn_readfirstlane((int)(vb >> 32));
+      v = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
     }
-    if (!(v > 0.0)) return false;  // uniform: every lane holds the maximum
-    const int piv = idx;
+    if (!(v > 0.0)) return false;  // uniform: lane 0's maximum
+    const int piv = __builtin_amdgcn_readfirstlane(idx);
     if (piv != col) {
       if (lane < n) {
         const double t = M[col * n + lane];
