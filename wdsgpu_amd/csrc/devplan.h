@@ -46,6 +46,9 @@ WDS_HD inline int64_t lds_sink_index(int64_t nnzL, int64_t Nj, int64_t Nf) {
 // (rows r = lane + 64 i with an emitter, in ascending r); a scenario with more
 // in some lane walks every row instead.
 constexpr int kEmPass = 2;
+// fixed-stride valve-end path terms: nodes per path, terms per node
+constexpr int kPathMax = 16;
+constexpr int kTermMax = 8;
 // the list: int32 [kEmPass * 64] slots (-1 past a lane's rows), then the
 // wavefront's largest count, padded to whole doubles
 constexpr int kEmDoubles = 32 * kEmPass + 2;
@@ -183,6 +186,12 @@ struct DevPlan {
   const int32_t* wt_ptr;      // [wp_tot+1] path node -> range of its terms
   const int2* wt;             // [..] (V position of w(node, m), path-node index of m)
   const int4* wpair;          // [np*np] common suffix of paths u, v: (start u, start v, length)
+  // the terms at a fixed stride (wq_ok: every path has at most kPathMax
+  // nodes and every node at most kTermMax terms): entry k of path u at
+  // wq[(u * kPathMax + k) * kTermMax + m], wq_n[u * kPathMax + k] terms
+  int32_t wq_ok;
+  const int2* wq;
+  const int32_t* wq_n;
   // tank links (tankstatus)
   const int32_t* tl_k;        // [ntl]
   const int32_t* tl_tank;     // [ntl] slot of the tank

@@ -132,7 +132,8 @@ extern "C" int wds_plan_upload(wds_plan* P) {
          o_tb = bb.add(P->tl_bits), o_tx = bb.add(P->tl_hmax), o_tn = bb.add(P->tl_hmin);
   size_t o_jgp = bb.add(P->jg_ptr), o_jg = bb.add(P->jg), o_jm = bb.add(P->jadj_mid);
   size_t o_wpp = bb.add(P->wp_ptr), o_wpn = bb.add(P->wp_node), o_wpd = bb.add(P->wp_diag),
-         o_wtp = bb.add(P->wt_ptr), o_wt = bb.add(P->wt), o_wpr = bb.add(P->wpair);
+         o_wtp = bb.add(P->wt_ptr), o_wt = bb.add(P->wt), o_wpr = bb.add(P->wpair),
+         o_wq = bb.add(P->wq), o_wqn = bb.add(P->wq_n);
   size_t o_c2l = bb.add(P->csr_to_L), o_dp = bb.add(P->diag_pos), o_rs = bb.add(P->row_slot);
   size_t fi, fs, fl, wi, ws, bi, bs;
   prog_vectors(S.factor, fi, fs, fl);
@@ -237,6 +238,9 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.wt_ptr = at<int32_t>(d, o_wtp);
   D.wt = at<int2>(d, o_wt);
   D.wpair = at<int4>(d, o_wpr);
+  D.wq_ok = P->wq_ok;
+  D.wq = at<int2>(d, o_wq);
+  D.wq_n = at<int32_t>(d, o_wqn);
   D.tl_k = at<int32_t>(d, o_tk);
   D.tl_tank = at<int32_t>(d, o_tt);
   D.tl_other = at<int32_t>(d, o_to);

@@ -160,7 +160,34 @@ __device__ inline void wood_paths(const DevPlan& pl, Scn& c) {
     for (int u = 0; u < kRowG; ++u)
       if (i0 + 64 * u < pl.wp_tot) c.wpn[i0 + 64 * u] = nd[u];
   }
-  if (lane < pl.np) {
+  if (lane < pl.np && pl.wq_ok) {
+    // fixed-stride terms: the next node's words load while this node's
+    // sum runs, so the chain waits on LDS only (same terms, same order)
+    const int i0 = pl.wp_ptr[lane], len = pl.wp_ptr[lane + 1] - i0;
+    const int2* q = pl.wq + lane * kPathMax * kTermMax;
+    const int32_t* qn = pl.wq_n + lane * kPathMax;
+    int2 tm[kTermMax];
+#pragma unroll
+    for (int m = 0; m < kTermMax; ++m) tm[m] = q[m];
+    int n = qn[0], dg = pl.wp_diag[i0];
+    for (int k = 0; k < len; ++k) {
+      const int k1 = min(k + 1, len - 1);
+      int2 tn[kTermMax];
+#pragma unroll
+      for (int m = 0; m < kTermMax; ++m) tn[m] = q[k1 * kTermMax + m];
+      const int nn = qn[k1], dgn = pl.wp_diag[i0 + k1];
+      double acc = (k == 0) ? 1.0 : 0.0;
+#pragma unroll
+      for (int m = 0; m < kTermMax; ++m)
+        if (m < n) acc = fma(-c.V[tm[m].x], c.Yp[tm[m].y], acc);  // V: the unit factor L
+      c.Yp[i0 + k] = acc;
+      c.Zp[i0 + k] = acc * c.V[dg];
+#pragma unroll
+      for (int m = 0; m < kTermMax; ++m) tm[m] = tn[m];
+      n = nn;
+      dg = dgn;
+    }
+  } else if (lane < pl.np) {
     const int i0 = pl.wp_ptr[lane], i1 = pl.wp_ptr[lane + 1];
     for (int i = i0; i < i1; ++i) {
       double acc = (i == i0) ? 1.0 : 0.0;

@@ -444,6 +444,26 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       paths.push_back(std::move(path));
     }
     const int np = (int)ends.size();
+    // the fixed-stride copy of the terms, when every path fits it
+    P->wq_ok = 1;
+    for (int u = 0; u < np; ++u) {
+      if (P->wp_ptr[u + 1] - P->wp_ptr[u] > kPathMax) P->wq_ok = 0;
+      for (int32_t i = P->wp_ptr[u]; i < P->wp_ptr[u + 1]; ++i)
+        if (P->wt_ptr[i + 1] - P->wt_ptr[i] > kTermMax) P->wq_ok = 0;
+    }
+    P->wq.assign(P->wq_ok ? (size_t)np * kPathMax * kTermMax * 2 : 0, 0);
+    P->wq_n.assign(P->wq_ok ? (size_t)np * kPathMax : 0, 0);
+    if (P->wq_ok)
+      for (int u = 0; u < np; ++u)
+        for (int32_t i = P->wp_ptr[u]; i < P->wp_ptr[u + 1]; ++i) {
+          const int32_t k = i - P->wp_ptr[u];
+          P->wq_n[(size_t)u * kPathMax + k] = P->wt_ptr[i + 1] - P->wt_ptr[i];
+          for (int32_t t = P->wt_ptr[i]; t < P->wt_ptr[i + 1]; ++t) {
+            const size_t x = (((size_t)u * kPathMax + k) * kTermMax + (t - P->wt_ptr[i])) * 2;
+            P->wq[x] = P->wt[2 * (size_t)t];
+            P->wq[x + 1] = P->wt[2 * (size_t)t + 1];
+          }
+        }
     P->wpair.clear();
     for (int u = 0; u < np; ++u)
       for (int v = 0; v < np; ++v) {

@@ -50,6 +50,11 @@ struct wds_plan {
   // the slot of each entry of the reference CSR pattern
   std::vector<int32_t> ae_pos, diag_ae, off_ae, prv_ae, csr_to_ell;
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
+  // the same terms at a fixed stride: entry k of path u holds its terms at
+  // pairs ((u * kPathMax + k) * kTermMax + m), wq_n[u * kPathMax + k] of them;
+  // empty unless every path fits (wq_ok)
+  std::vector<int32_t> wq, wq_n;
+  int32_t wq_ok = 0;
   std::vector<int32_t> jg_ptr, jg;  // jg: pairs (link, fixed slot)
   std::vector<int32_t> l_scale, jadj_mid;
   std::vector<double> prv_hset, prv_km, tl_hmax, tl_hmin;
