@@ -339,6 +339,16 @@ int wds_emitter_ke(void* stream, int64_t n, const double* C, double qexp, double
 int wds_emitter_ke_grad(void* stream, int64_t n, const double* C, const double* ke,
                         double qexp, const double* g_ke, double* g_C);
 
+/* Column sums over the batch, src[i] [B][n[i]] -> dst[i] [n[i]] for up to
+ * four arrays in one launch (each column in 16 consecutive batch slices,
+ * each slice in batch order, the slice sums in slice order): the
+ * gradients of the batch-shared inputs of an implicit solve (reservoir
+ * heads, emitter coefficients, resistances, pump parameters), which the
+ * reference leaves to autograd's expand backward (a sum over dim 0 per input,
+ * dgga/autodiff.py:1305-1312 returning per-scenario gradients). */
+int wds_batch_sums(void* stream, int32_t B, int32_t narr, const double* const* src,
+                   const int64_t* n, double* const* dst);
+
 /* The deterministic power laws the kernels use in place of libm pow/log
  * (the reference's calls at dgga/solver.py:58-69, 2176-2377), elementwise on
  * device arrays: op 0 out = x^y, op 1 out = log x (y unused).  For checking

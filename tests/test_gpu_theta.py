@@ -44,3 +44,31 @@ def test_emitter_coefficient_and_gradient(cuda):
     (ke * w.to(cuda)).sum().backward()
     assert torch.allclose(ke.detach().cpu(), ref.detach(), rtol=1e-14, atol=0)
     assert torch.allclose(Cg.grad.cpu(), Cc.grad, rtol=1e-13, atol=0)
+
+
+@pytest.mark.parametrize("B", [1, 17, 1024])
+def test_batch_sums_match_column_sums(cuda, B):
+    # the one-launch column sums of the batch-shared gradients: four arrays
+    # of ragged widths (not multiples of the 16-column block), against
+    # float64 sums on the host, and the same bits on a second call
+    import ctypes as C
+    from wdsgpu_amd import _lib
+    lib = _lib.load()
+    rng = np.random.default_rng(B)
+    widths = [5, 784, 908, 33]
+    src = [torch.tensor(rng.normal(size=(B, w)), device=cuda) for w in widths]
+    dst = [torch.empty(w, dtype=torch.float64, device=cuda) for w in widths]
+    arr = C.c_void_p * 4
+    n = (C.c_int64 * 4)(*widths)
+
+    def run():
+        _lib.check(lib.wds_batch_sums(None, B, 4, arr(*[_lib.ptr(t) for t in src]), n,
+                                      arr(*[_lib.ptr(t) for t in dst])), "wds_batch_sums")
+        torch.cuda.synchronize()
+        return [d.clone() for d in dst]
+
+    first = run()
+    for t, d in zip(src, first):
+        ref = t.cpu().numpy().sum(axis=0)
+        assert np.allclose(d.cpu().numpy(), ref, rtol=1e-13, atol=1e-13 * B)
+    assert all(torch.equal(a, b) for a, b in zip(first, run()))

@@ -123,6 +123,8 @@ SIGNATURES = {
                                          C.c_void_p, C.c_void_p]),
     "wds_emitter_ke": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_double, C.c_double,
                                  C.c_void_p]),
+    "wds_batch_sums": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.POINTER(C.c_void_p),
+                                 C.POINTER(C.c_int64), C.POINTER(C.c_void_p)]),
     "wds_emitter_ke_grad": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p,
                                       C.c_double, C.c_void_p, C.c_void_p]),
     "wds_detmath_eval": (C.c_int, [C.c_void_p, C.c_int32, C.c_int64, C.c_void_p, C.c_void_p,

@@ -187,16 +187,24 @@ def _backward(ctx, gH, gQ, gE, want_pump):
         else s._pinned_info(B)
     binfo.copy_(info, non_blocking=True)
     dm, dv = ctx.dims, ctx.in_devs
-
-    def _fit(g, dim, dev_):
-        g = g.to(device=dev_)
-        return g.sum(0) if dim == 1 else g
-
-    # the batch reductions are queued behind the adjoint before the host
-    # waits for its status
-    grads = (_fit(gd, dm[0], dv[0]), _fit(grh, dm[1], dv[1]), _fit(gke, dm[2], dv[2]),
-             _fit(gr, dm[3], dv[3]))
-    pump = tuple(gp.sum(0).unbind(0)) if want_pump else (None, None, None)
+    # the batch reductions (inputs shared by the batch) are queued behind the
+    # adjoint, in one launch, before the host waits for its status
+    per = [gd, grh, gke, gr]
+    sums = [i for i in range(4) if dm[i] == 1]
+    outs = {}
+    src = [per[i] for i in sums] + ([gp.reshape(B, -1)] if want_pump else [])
+    if src:
+        dst = [torch.empty(t.shape[1:], dtype=s.dtype, device=dev) for t in src]
+        arr = C.c_void_p * len(src)
+        n = (C.c_int64 * len(src))(*[t.shape[1] for t in src])
+        with torch.cuda.device(dev):
+            _lib.check(s._lib.wds_batch_sums(s._stream(), B, len(src),
+                                              arr(*[_lib.ptr(t) for t in src]), n,
+                                              arr(*[_lib.ptr(t) for t in dst])),
+                       "wds_batch_sums")
+        outs = dict(zip(sums, dst))
+    grads = tuple((outs[i] if i in outs else per[i]).to(device=dv[i]) for i in range(4))
+    pump = tuple(dst[-1].view(3, s.L).unbind(0)) if want_pump else (None, None, None)
     fwd = ctx.fwd_check
     ctx.fwd_check = None
     if s.lazy_checks:

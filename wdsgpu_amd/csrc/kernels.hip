@@ -1050,6 +1050,85 @@ extern "C" int wds_hw_resistance_grad(void* stream, int64_t n, const double* C, 
   return launch_check("k_hw_resistance_grad");
 }
 
+struct BatchSums {
+  const double* src[4];
+  double* dst[4];
+  int64_t n[4];
+  int32_t narr, B;
+};
+
+// a block sums 16 columns of the concatenated arrays: thread (column c =
+// t % 16, slice k = t / 16) adds rows k * chunk .. (k + 1) * chunk - 1 in
+// order, then the 16 slice sums of a column add in slice order (a fixed
+// tree: the same bits on every run)
+__global__ void k_batch_sums(BatchSums a, int64_t tot) {
+  __shared__ double part[16][16];
+  const int c = threadIdx.x & 15, k = threadIdx.x >> 4;
+  const int chunk = (a.B + 15) / 16;
+  for (int64_t g0 = (int64_t)blockIdx.x * 16; g0 < tot; g0 += (int64_t)gridDim.x * 16) {
+    int64_t col = g0 + c;
+    int arr = -1;
+    for (int i = 0; i < a.narr && col < tot; ++i) {
+      if (col < a.n[i]) {
+        arr = i;
+        break;
+      }
+      col -= a.n[i];
+    }
+    double acc = 0.0;
+    if (arr >= 0) {
+      const double* sp = a.src[arr] + col;
+      const int b1 = min(a.B, (k + 1) * chunk);
+      const int64_t st = a.n[arr];
+      int b = k * chunk;
+      // sixteen rows' loads in flight at a time, added in row order
+      for (; b + 16 <= b1; b += 16) {
+        double v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = sp[(int64_t)(b + u) * st];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc += v[u];
+      }
+      for (; b < b1; ++b) acc += sp[(int64_t)b * st];
+    }
+    part[k][c] = acc;
+    __syncthreads();
+    if (k == 0 && arr >= 0) {
+      double s = 0.0;
+      for (int q = 0; q < 16; ++q) s += part[q][c];
+      a.dst[arr][col] = s;
+    }
+    __syncthreads();
+  }
+}
+
+extern "C" int wds_batch_sums(void* stream, int32_t B, int32_t narr, const double* const* src,
+                              const int64_t* n, double* const* dst) {
+  if (B < 0 || narr < 0 || narr > 4 || (narr > 0 && (!src || !n || !dst))) {
+    set_error("wds_batch_sums: bad argument");
+    return WDS_EINVAL;
+  }
+  BatchSums a{};
+  int64_t tot = 0;
+  for (int i = 0; i < narr; ++i) {
+    if (n[i] < 0 || (n[i] > 0 && (!src[i] || !dst[i]))) {
+      set_error("wds_batch_sums: bad array " + std::to_string(i));
+      return WDS_EINVAL;
+    }
+    a.src[i] = src[i];
+    a.dst[i] = dst[i];
+    a.n[i] = n[i];
+    tot += n[i];
+  }
+  a.narr = narr;
+  a.B = B;
+  if (tot == 0) return WDS_OK;
+  const int64_t blocks = std::min<int64_t>((tot + 15) / 16, 4096);
+  hipLaunchKernelGGL(k_batch_sums, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, a,
+                     tot);
+  return launch_check("k_batch_sums");
+}
+
 extern "C" int wds_emitter_ke(void* stream, int64_t n, const double* C, double qexp,
                               double coef, double* ke) {
   if (n < 0 || (n > 0 && (!C || !ke))) {
