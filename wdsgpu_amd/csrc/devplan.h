@@ -165,6 +165,7 @@ struct DevPlan {
   const int32_t* diag_ae;     // [Nj]
   const int2* off_ae;         // [noff]
   const int4* prv_ae;         // [p]
+  const int4* prv_rt;         // [p] downstream row in the lane layout (t0, entries, lane)
   const int32_t* csr_to_ell;  // [nnz_csr]
   // control valves with a setting (PRV, PSV, FCV; file order)
   const int32_t* prv_k;       // [p]

@@ -123,6 +123,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   size_t o_rb = bb.add(P->rs_share), o_re = bb.add(P->rs_ell);
   size_t o_ab = bb.add(P->as_blk), o_ae = bb.add(P->as_ell), o_o4 = bb.add(P->off4);
   size_t o_pb = bb.add(P->ps_blk), o_pe = bb.add(P->ps_ell), o_pln = bb.add(P->ps_len);
+  size_t o_prt = bb.add(P->prv_rt);
   size_t o_dae = bb.add(P->diag_ae), o_oae = bb.add(P->off_ae), o_pae = bb.add(P->prv_ae),
          o_cae = bb.add(P->csr_to_ell);
   size_t o_pk = bb.add(P->prv_k), o_p1 = bb.add(P->prv_s1), o_p2 = bb.add(P->prv_s2),
@@ -219,6 +220,7 @@ extern "C" int wds_plan_upload(wds_plan* P) {
   D.diag_ae = at<int32_t>(d, o_dae);
   D.off_ae = at<int2>(d, o_oae);
   D.prv_ae = at<int4>(d, o_pae);
+  D.prv_rt = at<int4>(d, o_prt);
   D.csr_to_ell = at<int32_t>(d, o_cae);
   D.prv_k = at<int32_t>(d, o_pk);
   D.prv_kind = at<int32_t>(d, o_pkd);

@@ -652,18 +652,22 @@ __device__ inline int polish_and_factor(const DevPlan& pl, Scn& c, const SolvePa
       // applies rhs_p - Abar dH, the opposite sign: that doubles the j2 mass
       // residual every step instead of removing it; this follows the full
       // Newton step of the CPU polish (dgga/autodiff.py:473-491).
+      // (A dH)[j2] of each ACTIVE valve's downstream row only, one lane per
+      // valve, its entries in the order of the full product
       if (nw > 1) wg_sync();
-      if (w == 0) {
-        matvec(pl, c, dH, c.v3);
-        if (lane == 0)
-          for (int i = 0; i < pl.p; ++i) {
-            if (c.sh[i] == 0.0) continue;
-            const int j2 = pl.prv_s2[i];
-            const double abar2 = c.v3[j2] - CBIG * dH[j2];
-            c.P[pl.prv_k[i]] = abar2 - c.v1[j2];
-          }
-        mem_sync();
+      if (w == 0 && lane < pl.p && c.sh[lane] != 0.0) {
+        const int4 rt = pl.prv_rt[lane];
+        const int* cw = (const int*)pl.rs_ell;
+        double sum = 0.0;
+        for (int e = 0; e < rt.y; ++e) {
+          const int slot = (rt.x + e) * 64 + rt.z;
+          sum = fma(c.A[slot], dH[cw[2 * slot + 1] & kColMask], sum);
+        }
+        const int j2 = pl.prv_s2[lane];
+        const double abar2 = sum - CBIG * dH[j2];
+        c.P[pl.prv_k[lane]] = abar2 - c.v1[j2];
       }
+      mem_sync();
     }
     if (nw > 1) wg_sync();
     for (int k0 = lane + 64 * kRowG * w; k0 < pl.L; k0 += 64 * kRowG * nw) {

@@ -409,6 +409,15 @@ static void build(const wds_network_desc* d, int ordering, wds_plan* P) {
       P->prv_ae[4 * i + 2] = P->diag_ae[P->prv_s1[i]];
       P->prv_ae[4 * i + 3] = P->diag_ae[P->prv_s2[i]];
     }
+    // the downstream row of each valve in the lane layout: (first slot,
+    // entries, lane)
+    P->prv_rt.assign(4 * (size_t)P->p, 0);
+    for (int i = 0; i < P->p; ++i) {
+      const int32_t j2 = P->prv_s2[i];
+      P->prv_rt[4 * i] = P->rs_t0[j2];
+      P->prv_rt[4 * i + 1] = P->spmv_ptr[j2 + 1] - P->spmv_ptr[j2];
+      P->prv_rt[4 * i + 2] = j2 % kLanes;
+    }
   }
   // ---- elimination-tree paths of the valve ends ----
   // The Woodbury corrections of the polish and adjoint need A-hat^-1 e_u only

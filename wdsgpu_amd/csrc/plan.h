@@ -49,6 +49,7 @@ struct wds_plan {
   // off-diagonal entries and the PRV positions (off pair, d1, d2), csr_to_ell
   // the slot of each entry of the reference CSR pattern
   std::vector<int32_t> ae_pos, diag_ae, off_ae, prv_ae, csr_to_ell;
+  std::vector<int32_t> prv_rt;  // per valve: its downstream row (t0, entries, lane)
   std::vector<int32_t> wp_ptr, wp_node, wp_diag, wt_ptr, wt, wpair;
   // the same terms at a fixed stride: entry k of path u holds its terms at
   // pairs ((u * kPathMax + k) * kTermMax + m), wq_n[u * kPathMax + k] of them;
