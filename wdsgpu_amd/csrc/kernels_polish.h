@@ -268,7 +268,9 @@ __device__ inline void wood_apply(const DevPlan& pl, Scn& c, double* X, const do
 // result is left in the solve vector S.  adjoint = false: primal M_p (rows
 // j2 -> dH[j2] = rr[j2], rows j1 merged; 2p columns e_j2, e_j1); adjoint =
 // true: node system M (rows j2 -> lam[j2] - lam[j1] = rr[j2]; p columns e_j2).
-__device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, bool adjoint) {
+// (in_S: the right-hand side is already in S, rr unused)
+__device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, bool adjoint,
+                                  bool in_S = false) {
   const int lane = lane_id();
   const int p = pl.p;
   double* M = c.sM;            // [n x n], n = 2p or p
@@ -282,7 +284,7 @@ __device__ inline bool wood_solve(const DevPlan& pl, Scn& c, const double* rr, b
   const int s2v = pl.prv_s2[min(lane, p - 1)];
   i32x4 ring[kSlots];
   prefetch_prog(pl.fwd, ring);
-  vec_copy(S, rr, nullptr, pl.Nj);
+  if (!in_S) vec_copy(S, rr, nullptr, pl.Nj);
   lds_sync();
   // rr at the valves' downstream rows: S holds rr until the forward solve
   const double r2v = S[s2v];
@@ -391,9 +393,10 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
   // two refinement steps on the residual of the true replaced system
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);  // base = A-hat dH
-    double* res = c.v5;
+    // the residual goes straight into the solve vector S
+    double* res = c.R;
     vec_copy(res, rp, wk, pl.Nj);
-    mem_sync();
+    lds_sync();
     if (lane == 0) {
       // the replaced rows of the active valves, stored in valve order (all
       // loads first: none reads res)
@@ -421,8 +424,8 @@ __device__ inline bool primal_solve(const DevPlan& pl, Scn& c, double* dst) {
         res[s2[i]] = p2[i] - d2;
       }
     }
-    mem_sync();
-    if (!wood_solve(pl, c, res, false)) return false;
+    lds_sync();
+    if (!wood_solve(pl, c, nullptr, false, true)) return false;
     for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.R[j];
     lds_sync();
   }
@@ -756,15 +759,16 @@ __device__ inline bool node_solve(const DevPlan& pl, Scn& c, double* dst, int n_
   lds_sync();
   for (int ref = 0; ref < 2; ++ref) {
     matvec(pl, c, dst, wk);
-    double* res = c.v5;
+    // the residual goes straight into the solve vector S
+    double* res = c.R;
     vec_copy(res, r2, wk, pl.Nj);
-    mem_sync();
+    lds_sync();
     if (lane < p && c.sh[lane] != 0.0) {
       const int j1 = pl.prv_s1[lane], j2 = pl.prv_s2[lane];
       res[j2] = r2[j2] - (dst[j2] - dst[j1]);
     }
-    mem_sync();
-    if (!wood_solve(pl, c, res, true)) return false;
+    lds_sync();
+    if (!wood_solve(pl, c, nullptr, true, true)) return false;
     for (int j = lane; j < pl.Nj; j += 64) dst[j] += c.R[j];
     lds_sync();
   }
